@@ -1,0 +1,156 @@
+/*
+ * rasterizer_amd.h -- C ABI of the MI355X occlusion-culling rasterizer.
+ *
+ * Drop-in GPU replacement for the hot path of rawrunprotected/rasterizer
+ * (SoftwareRasterizer/Rasterizer.h, Rasterizer.cpp, Occluder.h, Occluder.cpp).
+ * Every entry point below cites the reference interface it replaces.  The C++
+ * class API with the reference's exact signatures (Rasterizer, Occluder) sits
+ * on top of this ABI in include/rasterizer_amd/Rasterizer.h and Occluder.h.
+ *
+ * Conventions
+ *  - Every function returns an int status: RZ_OK (0) or a negative RZ_ERR_*.
+ *  - Handles are opaque and destroyed by the caller.
+ *  - A context is one Rasterizer: one depth buffer + hiZ on one GPU, bound to
+ *    one HIP stream it creates.  A context is not thread-safe; independent
+ *    contexts are.
+ *  - rz_rasterize / rz_draw_frame / rz_clear / rz_set_mvp are stream-ordered
+ *    and do not block the host.  rz_query*, rz_readback_* and rz_sync block
+ *    until the stream has drained, and report any device-side error raised by
+ *    earlier asynchronous calls.
+ *  - Vectors are 4 floats (x, y, z, w), as the reference's __m128.
+ *  - Nothing here falls back to the CPU: without a usable GPU, rz_create fails.
+ */
+#ifndef RASTERIZER_AMD_H
+#define RASTERIZER_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RZ_OK 0
+#define RZ_ERR_INVALID_ARGUMENT (-1)
+#define RZ_ERR_OUT_OF_MEMORY (-2)
+#define RZ_ERR_DEVICE (-3)
+#define RZ_ERR_TIMEOUT (-4)
+#define RZ_ERR_UNSUPPORTED_HOST (-5)
+#define RZ_ERR_TOO_LARGE (-6)
+
+typedef struct rz_context rz_context;
+typedef struct rz_occluder rz_occluder;
+
+/* Human-readable text for a status code (static storage). */
+const char* rz_status_string(int status);
+
+/* Rasterizer::Rasterizer(uint32_t width, uint32_t height)
+ * (Rasterizer.h:13, Rasterizer.cpp:66-74, table build :547-604).
+ * width and height must be non-zero multiples of 8.  Builds the 4096-entry
+ * coverage table and the host CPU's RCPPS table on the host, uploads both, and
+ * allocates depth (2 B/pixel, 8x8-block-linear) and hiZ (1 u16 per block + 8).
+ * The depth buffer starts at 0 and hiZ at 0, as the reference's vectors do. */
+int rz_create(uint32_t width, uint32_t height, rz_context** out);
+int rz_destroy(rz_context* ctx);
+
+/* Rasterizer::setModelViewProjection(const float* matrix)
+ * (Rasterizer.h:15, Rasterizer.cpp:76-105).  16 floats, row-major for row
+ * vectors (a DirectXMath XMMATRIX memcpy'd to float[16]). */
+int rz_set_mvp(rz_context* ctx, const float matrix[16]);
+
+/* Rasterizer::clear() (Rasterizer.h:17, Rasterizer.cpp:107-121).  Sets every
+ * hiZ entry to 1 (the "cleared" flag); depth is left as it is. */
+int rz_clear(rz_context* ctx);
+
+/* Device copy of a baked Occluder (Occluder.h:7-21): its packets
+ * (m_vertexData, 4 x 8 words per 8-quad batch), m_refMin/m_refMax and
+ * m_boundsMin/m_boundsMax.  packetCount must be a multiple of 4.  The packets
+ * are copied; the host array may be freed afterwards. */
+int rz_occluder_create(rz_context* ctx, const uint32_t* packets, uint32_t packetCount, const float refMin[4],
+                       const float refMax[4], const float boundsMin[4], const float boundsMax[4],
+                       rz_occluder** out);
+int rz_occluder_destroy(rz_occluder* occluder);
+
+/* template<bool possiblyNearClipped> Rasterizer::rasterize(const Occluder&)
+ * (Rasterizer.h:19-20, Rasterizer.cpp:606-1295; instantiations :1297-1299).
+ * Enqueued on the context stream; the occluder must outlive the call's
+ * completion (rz_sync or any blocking call). */
+int rz_rasterize(rz_context* ctx, const rz_occluder* occluder, int possiblyNearClipped);
+
+/* The query-gated draw loop of Main.cpp:192-206, on the device in one pass:
+ * for i in order: if queryVisibility(bounds_i, needsClipping) then
+ * rasterize<needsClipping>(occluder_i).  Results are identical to issuing
+ * those calls one by one.  statusOut (optional, n bytes, written when the
+ * stream reaches it; valid after rz_sync): 0 not drawn, 1 drawn with
+ * rasterize<false>, 2 drawn with rasterize<true>. */
+int rz_draw_frame(rz_context* ctx, uint32_t n, const rz_occluder* const* occluders, uint8_t* statusOut);
+
+/* Rasterizer::queryVisibility(__m128 boundsMin, __m128 boundsMax, bool& needsClipping)
+ * (Rasterizer.h:22, Rasterizer.cpp:123-281).  Blocks.  *needsClipping is
+ * written only when the box passes the frustum test, as in the reference. */
+int rz_query(rz_context* ctx, const float boundsMin[4], const float boundsMax[4], int* visible,
+             int* needsClipping);
+
+/* Rasterizer::query2D(minX, maxX, minY, maxY, maxZ) (Rasterizer.h:24,
+ * Rasterizer.cpp:283-349).  Blocks.  Pixel bounds are inclusive, maxX <
+ * width, maxY < height. */
+int rz_query2d(rz_context* ctx, uint32_t minX, uint32_t maxX, uint32_t minY, uint32_t maxY, uint32_t maxZ,
+               int* visible);
+
+/* Batched queryVisibility over n boxes, host memory: bounds is n x 8 floats
+ * (min xyzw, max xyzw); visible and needsClipping are n bytes each
+ * (needsClipping is 0 when the box was frustum-culled).  Blocks. */
+int rz_query_batch(rz_context* ctx, uint32_t n, const float* bounds, uint8_t* visible, uint8_t* needsClipping);
+
+/* Same on device-resident arrays (device pointers), enqueued without
+ * blocking. */
+int rz_query_batch_device(rz_context* ctx, uint32_t n, const float* d_bounds, uint8_t* d_visible,
+                          uint8_t* d_needsClipping);
+
+/* Rasterizer::readBackDepth(void* target) (Rasterizer.h:26,
+ * Rasterizer.cpp:351-399).  width*height*4 bytes, row-major.  Blocks. */
+int rz_readback_depth(rz_context* ctx, void* rgba);
+
+/* Raw buffer access (the reference's private m_depthBuffer / m_hiZ):
+ * depth blocksX*blocksY*64 u16, hiz blocksX*blocksY + 8 u16; either may be
+ * NULL.  rz_upload_raw replaces the buffers.  Blocking. */
+int rz_readback_raw(rz_context* ctx, uint16_t* depth, uint16_t* hiz);
+int rz_upload_raw(rz_context* ctx, const uint16_t* depth, const uint16_t* hiz);
+
+/* Waits for the context stream and returns the first device-side error
+ * raised since the last blocking call (RZ_OK if none). */
+int rz_sync(rz_context* ctx);
+
+/* The HIP stream (hipStream_t) the context enqueues on. */
+int rz_get_stream(rz_context* ctx, void** stream);
+
+/* Context facts: width, height, blocksX, blocksY, tile width and height in
+ * blocks used by the frame kernel. */
+int rz_get_info(rz_context* ctx, uint32_t info[6]);
+
+/* The two host-built tables as uploaded to the device: the 4096-entry u64
+ * coverage table (Rasterizer.cpp:547-604) and the 4096-entry RCPPS table of
+ * the host CPU (mantissa index = top 12 bits, value = RCPPS(1.m)). */
+int rz_get_tables(rz_context* ctx, uint64_t* coverage4096, uint32_t* rcp4096);
+
+/* Prebaked MVP state as setModelViewProjection stores it
+ * (m_modelViewProjection, m_modelViewProjectionRaw). */
+int rz_get_mvp(rz_context* ctx, float prebaked[16], float raw[16]);
+
+/* Occluder::bake(const std::vector<__m128>& vertices, __m128 refMin, __m128 refMax)
+ * (Occluder.h:9, Occluder.cpp:7-181), host side.  vertices: n x 4 floats, 4
+ * per quad, n a multiple of 32 (8 quads).  packets: n words out.  bounds and
+ * center as m_boundsMin, m_boundsMax, m_center. */
+int rz_bake(const float* vertices, uint32_t n, const float refMin[4], const float refMax[4], uint32_t* packets,
+            float boundsMin[4], float boundsMax[4], float center[4]);
+
+/* Timing helpers on the context stream (HIP events): rz_timer_start records
+ * a start event, rz_timer_stop a stop event and returns (after waiting for
+ * it) the elapsed milliseconds between them. */
+int rz_timer_start(rz_context* ctx);
+int rz_timer_stop(rz_context* ctx, float* ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,278 @@
+"""MI355X occlusion-culling rasterizer: Python binding of the C ABI.
+
+The classes mirror the reference's C++ interface (rawrunprotected/rasterizer,
+SoftwareRasterizer/Rasterizer.h:10-61 and Occluder.h:7-21) so that tests and
+benchmarks read like calls into the reference.  All work runs in
+``_lib/librasterizer_amd.so`` (hand-written gfx950 kernels); there is no CPU
+fallback: if the library or a GPU is missing, construction raises.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+from pathlib import Path
+
+import numpy as np
+
+_PKG = Path(__file__).resolve().parent
+LIB_PATH = _PKG / "_lib" / "librasterizer_amd.so"
+
+RZ_OK = 0
+_STATUS = {
+    -1: "invalid argument",
+    -2: "out of memory",
+    -3: "HIP runtime or device error",
+    -4: "device-side wait timed out",
+    -5: "host CPU RCPPS has an unsupported structure",
+    -6: "buffer too large for one co-resident frame launch",
+}
+
+
+class RasterizerError(RuntimeError):
+    def __init__(self, status: int, what: str):
+        super().__init__(f"{what}: {_STATUS.get(status, 'unknown status')} ({status})")
+        self.status = status
+
+
+_lib = None
+
+_P = ctypes.c_void_p
+_U32 = ctypes.c_uint32
+_I32P = ctypes.POINTER(ctypes.c_int)
+
+_SIGNATURES = {
+    "rz_status_string": ([ctypes.c_int], ctypes.c_char_p),
+    "rz_create": ([_U32, _U32, ctypes.POINTER(_P)], ctypes.c_int),
+    "rz_destroy": ([_P], ctypes.c_int),
+    "rz_set_mvp": ([_P, _P], ctypes.c_int),
+    "rz_clear": ([_P], ctypes.c_int),
+    "rz_occluder_create": ([_P, _P, _U32, _P, _P, _P, _P, ctypes.POINTER(_P)], ctypes.c_int),
+    "rz_occluder_destroy": ([_P], ctypes.c_int),
+    "rz_rasterize": ([_P, _P, ctypes.c_int], ctypes.c_int),
+    "rz_draw_frame": ([_P, _U32, _P, _P], ctypes.c_int),
+    "rz_query": ([_P, _P, _P, _I32P, _I32P], ctypes.c_int),
+    "rz_query2d": ([_P, _U32, _U32, _U32, _U32, _U32, _I32P], ctypes.c_int),
+    "rz_query_batch": ([_P, _U32, _P, _P, _P], ctypes.c_int),
+    "rz_query_batch_device": ([_P, _U32, _P, _P, _P], ctypes.c_int),
+    "rz_readback_depth": ([_P, _P], ctypes.c_int),
+    "rz_readback_raw": ([_P, _P, _P], ctypes.c_int),
+    "rz_upload_raw": ([_P, _P, _P], ctypes.c_int),
+    "rz_sync": ([_P], ctypes.c_int),
+    "rz_get_stream": ([_P, ctypes.POINTER(_P)], ctypes.c_int),
+    "rz_get_info": ([_P, _P], ctypes.c_int),
+    "rz_get_tables": ([_P, _P, _P], ctypes.c_int),
+    "rz_get_mvp": ([_P, _P, _P], ctypes.c_int),
+    "rz_bake": ([_P, _U32, _P, _P, _P, _P, _P, _P], ctypes.c_int),
+    "rz_timer_start": ([_P], ctypes.c_int),
+    "rz_timer_stop": ([_P, ctypes.POINTER(ctypes.c_float)], ctypes.c_int),
+}
+
+
+def lib() -> ctypes.CDLL:
+    """Loads the HIP library (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(
+                f"{LIB_PATH} is missing: build it with `python -m rasterizer_amd.build` "
+                "(the rasterizer has no CPU fallback)")
+        handle = ctypes.CDLL(str(LIB_PATH))
+        for name, (args, res) in _SIGNATURES.items():
+            fn = getattr(handle, name)
+            fn.argtypes = args
+            fn.restype = res
+        _lib = handle
+    return _lib
+
+
+def _check(status: int, what: str) -> None:
+    if status != RZ_OK:
+        raise RasterizerError(status, what)
+
+
+def _ptr(a: np.ndarray) -> ctypes.c_void_p:
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+def _vec4(v) -> np.ndarray:
+    a = np.ascontiguousarray(np.asarray(v, dtype=np.float32).reshape(4))
+    return a
+
+
+class Occluder:
+    """Baked occluder (Occluder.h:7-21): packets plus its reference box and bounds."""
+
+    def __init__(self, packets: np.ndarray, ref_min, ref_max, bounds_min, bounds_max, center=None):
+        self.packets = np.ascontiguousarray(packets, dtype=np.uint32)
+        if self.packets.size % 32:
+            raise ValueError("packets must hold whole 8-quad batches (32 words each)")
+        self.m_refMin = _vec4(ref_min)
+        self.m_refMax = _vec4(ref_max)
+        self.m_boundsMin = _vec4(bounds_min)
+        self.m_boundsMax = _vec4(bounds_max)
+        self.m_center = _vec4(center if center is not None else (self.m_boundsMin + self.m_boundsMax) * 0.5)
+        self._device = {}
+
+    @property
+    def m_packetCount(self) -> int:
+        return self.packets.size // 8
+
+    @property
+    def quad_count(self) -> int:
+        return self.packets.size // 4
+
+    @staticmethod
+    def bake(vertices, ref_min, ref_max) -> "Occluder":
+        """Occluder::bake (Occluder.cpp:7-181), run by the library's host code."""
+        v = np.ascontiguousarray(np.asarray(vertices, dtype=np.float32).reshape(-1, 4))
+        n = v.shape[0]
+        if n == 0 or n % 32:
+            raise ValueError("vertex count must be a positive multiple of 32 (whole 8-quad batches)")
+        packets = np.zeros(n, dtype=np.uint32)
+        lo, hi = _vec4(ref_min), _vec4(ref_max)
+        bmin, bmax, cen = (np.zeros(4, np.float32) for _ in range(3))
+        _check(lib().rz_bake(_ptr(v), n, _ptr(lo), _ptr(hi), _ptr(packets), _ptr(bmin), _ptr(bmax), _ptr(cen)),
+               "Occluder.bake")
+        return Occluder(packets, lo, hi, bmin, bmax, cen)
+
+    def device(self, ctx) -> ctypes.c_void_p:
+        key = ctx.value
+        h = self._device.get(key)
+        if h is None:
+            h = ctypes.c_void_p()
+            _check(lib().rz_occluder_create(ctx, _ptr(self.packets), self.m_packetCount, _ptr(self.m_refMin),
+                                            _ptr(self.m_refMax), _ptr(self.m_boundsMin), _ptr(self.m_boundsMax),
+                                            ctypes.byref(h)), "rz_occluder_create")
+            self._device[key] = h
+        return h
+
+    def release(self) -> None:
+        if _lib is not None:
+            for h in self._device.values():
+                _lib.rz_occluder_destroy(h)
+        self._device = {}
+
+    def __del__(self):
+        try:
+            self.release()
+        except Exception:
+            pass
+
+
+class Rasterizer:
+    """One depth buffer + hiZ on the GPU (Rasterizer.h:10-61)."""
+
+    def __init__(self, width: int, height: int):
+        self._ctx = ctypes.c_void_p()
+        _check(lib().rz_create(width, height, ctypes.byref(self._ctx)), "Rasterizer")
+        self.width, self.height = width, height
+        self.blocks_x, self.blocks_y = width // 8, height // 8
+        info = np.zeros(6, np.uint32)
+        _check(lib().rz_get_info(self._ctx, _ptr(info)), "rz_get_info")
+        self.tile = (int(info[4]), int(info[5]))
+
+    @property
+    def ctx(self) -> ctypes.c_void_p:
+        return self._ctx
+
+    def close(self) -> None:
+        if self._ctx and _lib is not None:
+            _lib.rz_destroy(self._ctx)
+        self._ctx = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def setModelViewProjection(self, matrix) -> None:
+        m = np.ascontiguousarray(np.asarray(matrix, dtype=np.float32).reshape(16))
+        _check(lib().rz_set_mvp(self._ctx, _ptr(m)), "setModelViewProjection")
+
+    def clear(self) -> None:
+        _check(lib().rz_clear(self._ctx), "clear")
+
+    def rasterize(self, occluder: Occluder, possiblyNearClipped: bool) -> None:
+        _check(lib().rz_rasterize(self._ctx, occluder.device(self._ctx), 1 if possiblyNearClipped else 0),
+               "rasterize")
+
+    def queryVisibility(self, bounds_min, bounds_max, needs_clipping: bool = False):
+        """Returns (visible, needsClipping); needsClipping keeps the passed value
+        when the box is frustum-culled, as the reference leaves it unwritten."""
+        lo, hi = _vec4(bounds_min), _vec4(bounds_max)
+        vis = ctypes.c_int(0)
+        clip = ctypes.c_int(1 if needs_clipping else 0)
+        _check(lib().rz_query(self._ctx, _ptr(lo), _ptr(hi), ctypes.byref(vis), ctypes.byref(clip)),
+               "queryVisibility")
+        return bool(vis.value), bool(clip.value)
+
+    def query2D(self, min_x, max_x, min_y, max_y, max_z) -> bool:
+        vis = ctypes.c_int(0)
+        _check(lib().rz_query2d(self._ctx, min_x, max_x, min_y, max_y, max_z, ctypes.byref(vis)), "query2D")
+        return bool(vis.value)
+
+    def readBackDepth(self) -> np.ndarray:
+        out = np.zeros((self.height, self.width, 4), np.uint8)
+        _check(lib().rz_readback_depth(self._ctx, _ptr(out)), "readBackDepth")
+        return out
+
+    # Extensions beyond the reference class.
+    def draw_frame(self, occluders, want_status: bool = True):
+        """Main.cpp:192-206's query-gated loop on the device; returns per-occluder
+        status (0 not drawn, 1 rasterize<false>, 2 rasterize<true>)."""
+        n = len(occluders)
+        handles = (ctypes.c_void_p * max(n, 1))(*[o.device(self._ctx).value for o in occluders])
+        status = np.zeros(max(n, 1), np.uint8)
+        _check(lib().rz_draw_frame(self._ctx, n, ctypes.cast(handles, ctypes.c_void_p),
+                                   _ptr(status) if want_status else None), "draw_frame")
+        if want_status:
+            self.sync()
+        return status[:n]
+
+    def query_batch(self, bounds: np.ndarray):
+        b = np.ascontiguousarray(np.asarray(bounds, dtype=np.float32).reshape(-1, 8))
+        n = b.shape[0]
+        vis = np.zeros(max(n, 1), np.uint8)
+        clip = np.zeros(max(n, 1), np.uint8)
+        _check(lib().rz_query_batch(self._ctx, n, _ptr(b), _ptr(vis), _ptr(clip)), "query_batch")
+        return vis[:n].astype(bool), clip[:n].astype(bool)
+
+    def raw(self):
+        depth = np.zeros(self.blocks_x * self.blocks_y * 64, np.uint16)
+        hiz = np.zeros(self.blocks_x * self.blocks_y + 8, np.uint16)
+        _check(lib().rz_readback_raw(self._ctx, _ptr(depth), _ptr(hiz)), "readback_raw")
+        return depth, hiz
+
+    def upload_raw(self, depth=None, hiz=None) -> None:
+        d = None if depth is None else np.ascontiguousarray(depth, dtype=np.uint16)
+        h = None if hiz is None else np.ascontiguousarray(hiz, dtype=np.uint16)
+        _check(lib().rz_upload_raw(self._ctx, _ptr(d) if d is not None else None,
+                                   _ptr(h) if h is not None else None), "upload_raw")
+
+    def tables(self):
+        cov = np.zeros(4096, np.uint64)
+        rcp = np.zeros(4096, np.uint32)
+        _check(lib().rz_get_tables(self._ctx, _ptr(cov), _ptr(rcp)), "rz_get_tables")
+        return cov, rcp
+
+    def mvp_state(self):
+        pre = np.zeros(16, np.float32)
+        raw = np.zeros(16, np.float32)
+        _check(lib().rz_get_mvp(self._ctx, _ptr(pre), _ptr(raw)), "rz_get_mvp")
+        return pre, raw
+
+    def sync(self) -> None:
+        _check(lib().rz_sync(self._ctx), "rz_sync")
+
+    def timer_start(self) -> None:
+        _check(lib().rz_timer_start(self._ctx), "rz_timer_start")
+
+    def timer_stop(self) -> float:
+        ms = ctypes.c_float(0.0)
+        _check(lib().rz_timer_stop(self._ctx, ctypes.byref(ms)), "rz_timer_stop")
+        return float(ms.value)
+
+
+__all__ = ["Rasterizer", "Occluder", "RasterizerError", "lib", "LIB_PATH"]

@@ -1,0 +1,607 @@
+// C ABI of the MI355X rasterizer (include/rasterizer_amd.h): contexts,
+// device buffers, and the enqueue logic around the gfx950 kernels.
+#include "rasterizer_amd.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <mutex>
+#include <new>
+#include <vector>
+
+#include "rz_host.h"
+#include "rz_kernels.h"
+
+using namespace rzd;
+
+namespace {
+
+constexpr int kStagingSlots = 8;
+constexpr uint64_t kTimeoutTicks = 200000000ull;  // 2 s of the 100 MHz s_memrealtime clock
+
+struct Tables {
+  bool built = false;
+  int status = RZ_OK;
+  uint64_t cov[4096];
+  uint32_t rcp[4096];
+};
+
+// Both tables depend only on this host (libm, RCPPS); build them once.
+const Tables& host_tables() {
+  static Tables t;
+  static std::once_flag once;
+  std::call_once(once, [] {
+    if (!rzh::build_coverage_table(t.cov)) t.status = RZ_ERR_UNSUPPORTED_HOST;
+    else if (rzh::build_rcp_table(t.rcp) != 0) t.status = RZ_ERR_UNSUPPORTED_HOST;
+    t.built = true;
+  });
+  return t;
+}
+
+template <typename T>
+hipError_t dev_alloc(T** p, size_t count) {
+  *p = nullptr;
+  if (count == 0) return hipSuccess;
+  return hipMalloc(reinterpret_cast<void**>(p), count * sizeof(T));
+}
+
+}  // namespace
+
+struct rz_occluder {
+  rz_context* ctx;
+  uint32_t* d_packets;
+  uint32_t packetCount;
+  float refMin[4], refMax[4], boundsMin[4], boundsMax[4];
+};
+
+struct rz_context {
+  uint32_t width = 0, height = 0, bX = 0, bY = 0;
+  int tileW = 8, tilesX = 0, tilesY = 0;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  uint16_t* d_depth = nullptr;
+  uint16_t* d_hiz = nullptr;
+  uint64_t* d_cov = nullptr;
+  uint32_t* d_rcp = nullptr;
+  uint32_t* d_err = nullptr;
+  Mvp mvp{};
+  // per-draw buffers
+  uint32_t capItems = 0, capQuads = 0;
+  ItemDesc* d_items = nullptr;
+  ItemState* d_state = nullptr;
+  uint32_t* d_done = nullptr;
+  uint32_t* d_visible = nullptr;
+  uint8_t* d_status = nullptr;
+  QuadRec* d_recs = nullptr;
+  uint64_t* d_bbox = nullptr;
+  // host staging ring for item tables
+  ItemDesc* h_items[kStagingSlots] = {};
+  uint32_t h_itemsCap[kStagingSlots] = {};
+  hipEvent_t ev_items[kStagingSlots] = {};
+  int slot = 0;
+  // query staging
+  float* d_qbounds = nullptr;
+  uint8_t* d_qvis = nullptr;
+  uint8_t* d_qclip = nullptr;
+  uint32_t capQueries = 0;
+  uint32_t* d_rgba = nullptr;
+  // pending frame status copy
+  uint8_t* h_status = nullptr;
+  uint32_t h_statusCap = 0;
+  uint8_t* pendingStatusDst = nullptr;
+  uint32_t pendingStatusN = 0;
+  hipEvent_t t0 = nullptr, t1 = nullptr;
+  int stickyError = RZ_OK;
+};
+
+#define RZ_HIP(call)                           \
+  do {                                         \
+    if ((call) != hipSuccess) return RZ_ERR_DEVICE; \
+  } while (0)
+
+extern "C" const char* rz_status_string(int s) {
+  switch (s) {
+    case RZ_OK: return "ok";
+    case RZ_ERR_INVALID_ARGUMENT: return "invalid argument";
+    case RZ_ERR_OUT_OF_MEMORY: return "out of device or host memory";
+    case RZ_ERR_DEVICE: return "HIP runtime or device error";
+    case RZ_ERR_TIMEOUT: return "device-side wait timed out";
+    case RZ_ERR_UNSUPPORTED_HOST: return "host CPU RCPPS has an unsupported structure";
+    case RZ_ERR_TOO_LARGE: return "buffer too large for one co-resident frame launch";
+    default: return "unknown status";
+  }
+}
+
+static void free_context(rz_context* c) {
+  if (!c) return;
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  hipFree(c->d_depth);
+  hipFree(c->d_hiz);
+  hipFree(c->d_cov);
+  hipFree(c->d_rcp);
+  hipFree(c->d_err);
+  hipFree(c->d_items);
+  hipFree(c->d_state);
+  hipFree(c->d_done);
+  hipFree(c->d_visible);
+  hipFree(c->d_status);
+  hipFree(c->d_recs);
+  hipFree(c->d_bbox);
+  hipFree(c->d_qbounds);
+  hipFree(c->d_qvis);
+  hipFree(c->d_qclip);
+  hipFree(c->d_rgba);
+  for (int i = 0; i < kStagingSlots; ++i) {
+    if (c->h_items[i]) (void)hipHostFree(c->h_items[i]);
+    if (c->ev_items[i]) (void)hipEventDestroy(c->ev_items[i]);
+  }
+  if (c->h_status) (void)hipHostFree(c->h_status);
+  if (c->t0) (void)hipEventDestroy(c->t0);
+  if (c->t1) (void)hipEventDestroy(c->t1);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+extern "C" int rz_create(uint32_t width, uint32_t height, rz_context** out) {
+  if (!out) return RZ_ERR_INVALID_ARGUMENT;
+  *out = nullptr;
+  if (width == 0 || height == 0 || (width % 8) || (height % 8) || width > 65528 || height > 65528)
+    return RZ_ERR_INVALID_ARGUMENT;
+  if (uint64_t(width / 8) * (height / 8) >= (1u << 24)) return RZ_ERR_TOO_LARGE;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return RZ_ERR_DEVICE;
+  const Tables& tabs = host_tables();
+  if (tabs.status != RZ_OK) return tabs.status;
+
+  rz_context* c = new (std::nothrow) rz_context();
+  if (!c) return RZ_ERR_OUT_OF_MEMORY;
+  c->width = width;
+  c->height = height;
+  c->bX = width / 8;
+  c->bY = height / 8;
+  auto fail = [&](int code) {
+    free_context(c);
+    return code;
+  };
+  if (hipGetDevice(&c->device) != hipSuccess) return fail(RZ_ERR_DEVICE);
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return fail(RZ_ERR_DEVICE);
+  const size_t nBlocks = size_t(c->bX) * c->bY;
+  if (dev_alloc(&c->d_depth, nBlocks * 64) != hipSuccess || dev_alloc(&c->d_hiz, nBlocks + 8) != hipSuccess ||
+      dev_alloc(&c->d_cov, 4096) != hipSuccess || dev_alloc(&c->d_rcp, 4096) != hipSuccess ||
+      dev_alloc(&c->d_err, 1) != hipSuccess)
+    return fail(RZ_ERR_OUT_OF_MEMORY);
+  if (hipMemset(c->d_depth, 0, nBlocks * 64 * sizeof(uint16_t)) != hipSuccess ||
+      hipMemset(c->d_hiz, 0, (nBlocks + 8) * sizeof(uint16_t)) != hipSuccess ||
+      hipMemset(c->d_err, 0, sizeof(uint32_t)) != hipSuccess ||
+      hipMemcpy(c->d_cov, tabs.cov, sizeof tabs.cov, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(c->d_rcp, tabs.rcp, sizeof tabs.rcp, hipMemcpyHostToDevice) != hipSuccess)
+    return fail(RZ_ERR_DEVICE);
+
+  // Pick the smallest tile whose grid fits co-resident on this device.
+  bool placed = false;
+  for (int tw : {8, 16}) {
+    int maxGroups = 0;
+    if (frame_max_groups(tw, &maxGroups) != hipSuccess) return fail(RZ_ERR_DEVICE);
+    const int tx = int((c->bX + tw - 1) / tw), ty = int((c->bY + tw - 1) / tw);
+    if (tx * ty <= maxGroups) {
+      c->tileW = tw;
+      c->tilesX = tx;
+      c->tilesY = ty;
+      placed = true;
+      break;
+    }
+  }
+  if (!placed) return fail(RZ_ERR_TOO_LARGE);
+  for (int i = 0; i < kStagingSlots; ++i)
+    if (hipEventCreateWithFlags(&c->ev_items[i], hipEventDisableTiming) != hipSuccess) return fail(RZ_ERR_DEVICE);
+  if (hipEventCreate(&c->t0) != hipSuccess || hipEventCreate(&c->t1) != hipSuccess) return fail(RZ_ERR_DEVICE);
+  // Identity MVP until the caller sets one.
+  const float ident[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
+  rzh::prebake_mvp(width, height, ident, c->mvp.prebaked, c->mvp.raw);
+  *out = c;
+  return RZ_OK;
+}
+
+extern "C" int rz_destroy(rz_context* ctx) {
+  if (!ctx) return RZ_ERR_INVALID_ARGUMENT;
+  free_context(ctx);
+  return RZ_OK;
+}
+
+extern "C" int rz_set_mvp(rz_context* ctx, const float matrix[16]) {
+  if (!ctx || !matrix) return RZ_ERR_INVALID_ARGUMENT;
+  rzh::prebake_mvp(ctx->width, ctx->height, matrix, ctx->mvp.prebaked, ctx->mvp.raw);
+  return RZ_OK;
+}
+
+extern "C" int rz_clear(rz_context* ctx) {
+  if (!ctx) return RZ_ERR_INVALID_ARGUMENT;
+  RZ_HIP(launch_fill_u16(ctx->d_hiz, ctx->bX * ctx->bY + 8, 1, ctx->stream));
+  return RZ_OK;
+}
+
+extern "C" int rz_occluder_create(rz_context* ctx, const uint32_t* packets, uint32_t packetCount,
+                                  const float refMin[4], const float refMax[4], const float boundsMin[4],
+                                  const float boundsMax[4], rz_occluder** out) {
+  if (!ctx || !out || !refMin || !refMax || !boundsMin || !boundsMax || (packetCount % 4) != 0 ||
+      (packetCount && !packets))
+    return RZ_ERR_INVALID_ARGUMENT;
+  *out = nullptr;
+  rz_occluder* o = new (std::nothrow) rz_occluder();
+  if (!o) return RZ_ERR_OUT_OF_MEMORY;
+  o->ctx = ctx;
+  o->packetCount = packetCount;
+  std::memcpy(o->refMin, refMin, 16);
+  std::memcpy(o->refMax, refMax, 16);
+  std::memcpy(o->boundsMin, boundsMin, 16);
+  std::memcpy(o->boundsMax, boundsMax, 16);
+  o->d_packets = nullptr;
+  if (packetCount) {
+    if (hipMalloc(reinterpret_cast<void**>(&o->d_packets), size_t(packetCount) * 32) != hipSuccess) {
+      delete o;
+      return RZ_ERR_OUT_OF_MEMORY;
+    }
+    if (hipMemcpy(o->d_packets, packets, size_t(packetCount) * 32, hipMemcpyHostToDevice) != hipSuccess) {
+      hipFree(o->d_packets);
+      delete o;
+      return RZ_ERR_DEVICE;
+    }
+  }
+  *out = o;
+  return RZ_OK;
+}
+
+extern "C" int rz_occluder_destroy(rz_occluder* o) {
+  if (!o) return RZ_ERR_INVALID_ARGUMENT;
+  // Any context on this device may still have a draw in flight that reads it.
+  (void)hipDeviceSynchronize();
+  hipFree(o->d_packets);
+  delete o;
+  return RZ_OK;
+}
+
+static int ensure_capacity(rz_context* c, uint32_t nItems, uint32_t nQuads) {
+  if (nItems > c->capItems || nQuads > c->capQuads) {
+    RZ_HIP(hipStreamSynchronize(c->stream));
+  }
+  if (nItems > c->capItems) {
+    const uint32_t cap = std::max<uint32_t>(nItems, std::max<uint32_t>(64, c->capItems * 2));
+    hipFree(c->d_items);
+    hipFree(c->d_state);
+    hipFree(c->d_done);
+    hipFree(c->d_visible);
+    hipFree(c->d_status);
+    c->capItems = 0;
+    if (dev_alloc(&c->d_items, cap) != hipSuccess || dev_alloc(&c->d_state, cap) != hipSuccess ||
+        dev_alloc(&c->d_done, cap) != hipSuccess || dev_alloc(&c->d_visible, cap) != hipSuccess ||
+        dev_alloc(&c->d_status, cap) != hipSuccess)
+      return RZ_ERR_OUT_OF_MEMORY;
+    c->capItems = cap;
+  }
+  if (nQuads > c->capQuads) {
+    const uint32_t cap = std::max<uint32_t>(nQuads, std::max<uint32_t>(1024, c->capQuads + c->capQuads / 2));
+    hipFree(c->d_recs);
+    hipFree(c->d_bbox);
+    c->capQuads = 0;
+    if (dev_alloc(&c->d_recs, cap) != hipSuccess || dev_alloc(&c->d_bbox, cap) != hipSuccess)
+      return RZ_ERR_OUT_OF_MEMORY;
+    c->capQuads = cap;
+  }
+  return RZ_OK;
+}
+
+static int flush_pending_status(rz_context* c) {
+  if (!c->pendingStatusDst) return RZ_OK;
+  RZ_HIP(hipStreamSynchronize(c->stream));
+  std::memcpy(c->pendingStatusDst, c->h_status, c->pendingStatusN);
+  c->pendingStatusDst = nullptr;
+  c->pendingStatusN = 0;
+  return RZ_OK;
+}
+
+// Enqueues prep, setup and frame kernels for `n` items of the given kinds.
+static int enqueue_draw(rz_context* c, uint32_t n, const rz_occluder* const* occ, const uint32_t* kinds,
+                        uint8_t* statusOut) {
+  if (n == 0) return RZ_OK;
+  uint64_t totalQuads = 0;
+  for (uint32_t i = 0; i < n; ++i) {
+    if (!occ[i]) return RZ_ERR_INVALID_ARGUMENT;
+    totalQuads += uint64_t(occ[i]->packetCount / 4) * 8;
+  }
+  if (totalQuads >= (1ull << 31)) return RZ_ERR_TOO_LARGE;
+  if (statusOut) {
+    int s = flush_pending_status(c);
+    if (s != RZ_OK) return s;
+  }
+  int s = ensure_capacity(c, n, uint32_t(totalQuads));
+  if (s != RZ_OK) return s;
+
+  // Host staging: wait until this ring slot's previous upload has completed.
+  const int slot = c->slot;
+  c->slot = (c->slot + 1) % kStagingSlots;
+  RZ_HIP(hipEventSynchronize(c->ev_items[slot]));
+  if (c->h_itemsCap[slot] < n) {
+    if (c->h_items[slot]) (void)hipHostFree(c->h_items[slot]);
+    c->h_items[slot] = nullptr;
+    c->h_itemsCap[slot] = 0;
+    const uint32_t cap = std::max<uint32_t>(n, 64);
+    if (hipHostMalloc(reinterpret_cast<void**>(&c->h_items[slot]), cap * sizeof(ItemDesc)) != hipSuccess)
+      return RZ_ERR_OUT_OF_MEMORY;
+    c->h_itemsCap[slot] = cap;
+  }
+  ItemDesc* items = c->h_items[slot];
+  uint32_t base = 0;
+  for (uint32_t i = 0; i < n; ++i) {
+    const rz_occluder* o = occ[i];
+    ItemDesc& d = items[i];
+    d.packets = o->d_packets;
+    d.quadBase = base;
+    d.quadCount = (o->packetCount / 4) * 8;
+    d.kind = kinds[i];
+    d.pad = 0;
+    std::memcpy(d.refMin, o->refMin, 16);
+    std::memcpy(d.refMax, o->refMax, 16);
+    std::memcpy(d.boundsMin, o->boundsMin, 16);
+    std::memcpy(d.boundsMax, o->boundsMax, 16);
+    base += d.quadCount;
+  }
+  RZ_HIP(hipMemcpyAsync(c->d_items, items, n * sizeof(ItemDesc), hipMemcpyHostToDevice, c->stream));
+  RZ_HIP(hipEventRecord(c->ev_items[slot], c->stream));
+
+  PrepParams pp{};
+  pp.items = c->d_items;
+  pp.state = c->d_state;
+  pp.nItems = n;
+  pp.width = c->width;
+  pp.height = c->height;
+  pp.tileW = uint32_t(c->tileW);
+  pp.tileH = uint32_t(c->tileW);
+  pp.rcp = c->d_rcp;
+  pp.done = c->d_done;
+  pp.visible = c->d_visible;
+  pp.mvp = c->mvp;
+  RZ_HIP(launch_prep(pp, c->stream));
+
+  SetupParams sp{};
+  sp.items = c->d_items;
+  sp.state = c->d_state;
+  sp.nItems = n;
+  sp.nQuads = uint32_t(totalQuads);
+  sp.bX = c->bX;
+  sp.bY = c->bY;
+  sp.rcp = c->d_rcp;
+  sp.recs = c->d_recs;
+  sp.bbox = c->d_bbox;
+  RZ_HIP(launch_setup(sp, c->stream));
+
+  FrameParams fp{};
+  fp.items = c->d_items;
+  fp.state = c->d_state;
+  fp.nItems = n;
+  fp.bX = c->bX;
+  fp.bY = c->bY;
+  fp.tilesX = uint32_t(c->tilesX);
+  fp.tilesY = uint32_t(c->tilesY);
+  fp.recs = c->d_recs;
+  fp.bbox = c->d_bbox;
+  fp.done = c->d_done;
+  fp.visible = c->d_visible;
+  fp.depth = c->d_depth;
+  fp.hiz = c->d_hiz;
+  fp.cov = c->d_cov;
+  fp.err = c->d_err;
+  fp.timeoutTicks = kTimeoutTicks;
+  RZ_HIP(launch_frame(fp, c->tileW, c->stream));
+
+  if (statusOut) {
+    if (c->h_statusCap < n) {
+      if (c->h_status) (void)hipHostFree(c->h_status);
+      c->h_status = nullptr;
+      c->h_statusCap = 0;
+      if (hipHostMalloc(reinterpret_cast<void**>(&c->h_status), std::max<uint32_t>(n, 64)) != hipSuccess)
+        return RZ_ERR_OUT_OF_MEMORY;
+      c->h_statusCap = std::max<uint32_t>(n, 64);
+    }
+    RZ_HIP(launch_status(c->d_state, c->d_visible, n, c->d_status, c->stream));
+    RZ_HIP(hipMemcpyAsync(c->h_status, c->d_status, n, hipMemcpyDeviceToHost, c->stream));
+    c->pendingStatusDst = statusOut;
+    c->pendingStatusN = n;
+  }
+  return RZ_OK;
+}
+
+extern "C" int rz_rasterize(rz_context* ctx, const rz_occluder* occluder, int possiblyNearClipped) {
+  if (!ctx || !occluder) return RZ_ERR_INVALID_ARGUMENT;
+  const uint32_t kind = possiblyNearClipped ? kItemForceClip : kItemForceNoClip;
+  return enqueue_draw(ctx, 1, &occluder, &kind, nullptr);
+}
+
+extern "C" int rz_draw_frame(rz_context* ctx, uint32_t n, const rz_occluder* const* occluders, uint8_t* statusOut) {
+  if (!ctx || (n && !occluders)) return RZ_ERR_INVALID_ARGUMENT;
+  std::vector<uint32_t> kinds(n, kItemQuery);
+  return enqueue_draw(ctx, n, occluders, kinds.data(), statusOut);
+}
+
+// Waits for the stream, then reports and clears the device error word.
+extern "C" int rz_sync(rz_context* ctx) {
+  if (!ctx) return RZ_ERR_INVALID_ARGUMENT;
+  RZ_HIP(hipStreamSynchronize(ctx->stream));
+  int s = flush_pending_status(ctx);
+  if (s != RZ_OK) return s;
+  uint32_t err = 0;
+  RZ_HIP(hipMemcpy(&err, ctx->d_err, sizeof err, hipMemcpyDeviceToHost));
+  if (err) {
+    RZ_HIP(hipMemset(ctx->d_err, 0, sizeof(uint32_t)));
+    return (err & kErrTimeout) ? RZ_ERR_TIMEOUT : RZ_ERR_DEVICE;
+  }
+  return RZ_OK;
+}
+
+static int ensure_queries(rz_context* c, uint32_t n) {
+  if (n <= c->capQueries) return RZ_OK;
+  RZ_HIP(hipStreamSynchronize(c->stream));
+  hipFree(c->d_qbounds);
+  hipFree(c->d_qvis);
+  hipFree(c->d_qclip);
+  c->capQueries = 0;
+  const uint32_t cap = std::max<uint32_t>(n, 1024);
+  if (dev_alloc(&c->d_qbounds, size_t(cap) * 8) != hipSuccess || dev_alloc(&c->d_qvis, cap) != hipSuccess ||
+      dev_alloc(&c->d_qclip, cap) != hipSuccess)
+    return RZ_ERR_OUT_OF_MEMORY;
+  c->capQueries = cap;
+  return RZ_OK;
+}
+
+static QueryParams query_params(rz_context* c, const float* bounds, uint32_t n, uint8_t* vis, uint8_t* clip,
+                                uint8_t* code) {
+  QueryParams q{};
+  q.bounds = bounds;
+  q.n = n;
+  q.width = c->width;
+  q.height = c->height;
+  q.bX = c->bX;
+  q.depth = c->d_depth;
+  q.hiz = c->d_hiz;
+  q.rcp = c->d_rcp;
+  q.visible = vis;
+  q.needsClipping = clip;
+  q.code = code;
+  q.mvp = c->mvp;
+  return q;
+}
+
+extern "C" int rz_query(rz_context* ctx, const float boundsMin[4], const float boundsMax[4], int* visible,
+                        int* needsClipping) {
+  if (!ctx || !boundsMin || !boundsMax || !visible || !needsClipping) return RZ_ERR_INVALID_ARGUMENT;
+  int s = ensure_queries(ctx, 1);
+  if (s != RZ_OK) return s;
+  float b[8];
+  std::memcpy(b, boundsMin, 16);
+  std::memcpy(b + 4, boundsMax, 16);
+  RZ_HIP(hipMemcpyAsync(ctx->d_qbounds, b, sizeof b, hipMemcpyHostToDevice, ctx->stream));
+  QueryParams q = query_params(ctx, ctx->d_qbounds, 1, ctx->d_qvis, nullptr, ctx->d_qclip);
+  RZ_HIP(launch_query(q, ctx->stream));
+  uint8_t code = 0;
+  RZ_HIP(hipMemcpyAsync(&code, ctx->d_qclip, 1, hipMemcpyDeviceToHost, ctx->stream));
+  s = rz_sync(ctx);
+  if (s != RZ_OK) return s;
+  *visible = code & 1;
+  if (code & 4) *needsClipping = (code >> 1) & 1;
+  return RZ_OK;
+}
+
+extern "C" int rz_query2d(rz_context* ctx, uint32_t minX, uint32_t maxX, uint32_t minY, uint32_t maxY,
+                          uint32_t maxZ, int* visible) {
+  if (!ctx || !visible || minX > maxX || minY > maxY || maxX >= ctx->width || maxY >= ctx->height)
+    return RZ_ERR_INVALID_ARGUMENT;
+  int s = ensure_queries(ctx, 1);
+  if (s != RZ_OK) return s;
+  RZ_HIP(launch_query2d(minX, maxX, minY, maxY, maxZ, ctx->d_depth, ctx->d_hiz, ctx->bX, ctx->d_qvis, ctx->stream));
+  uint8_t v = 0;
+  RZ_HIP(hipMemcpyAsync(&v, ctx->d_qvis, 1, hipMemcpyDeviceToHost, ctx->stream));
+  s = rz_sync(ctx);
+  if (s != RZ_OK) return s;
+  *visible = v;
+  return RZ_OK;
+}
+
+extern "C" int rz_query_batch(rz_context* ctx, uint32_t n, const float* bounds, uint8_t* visible,
+                              uint8_t* needsClipping) {
+  if (!ctx || (n && (!bounds || !visible))) return RZ_ERR_INVALID_ARGUMENT;
+  if (n == 0) return RZ_OK;
+  int s = ensure_queries(ctx, n);
+  if (s != RZ_OK) return s;
+  RZ_HIP(hipMemcpyAsync(ctx->d_qbounds, bounds, size_t(n) * 32, hipMemcpyHostToDevice, ctx->stream));
+  QueryParams q = query_params(ctx, ctx->d_qbounds, n, ctx->d_qvis, ctx->d_qclip, nullptr);
+  RZ_HIP(launch_query(q, ctx->stream));
+  RZ_HIP(hipMemcpyAsync(visible, ctx->d_qvis, n, hipMemcpyDeviceToHost, ctx->stream));
+  if (needsClipping) RZ_HIP(hipMemcpyAsync(needsClipping, ctx->d_qclip, n, hipMemcpyDeviceToHost, ctx->stream));
+  return rz_sync(ctx);
+}
+
+extern "C" int rz_query_batch_device(rz_context* ctx, uint32_t n, const float* d_bounds, uint8_t* d_visible,
+                                     uint8_t* d_needsClipping) {
+  if (!ctx || (n && (!d_bounds || !d_visible))) return RZ_ERR_INVALID_ARGUMENT;
+  QueryParams q = query_params(ctx, d_bounds, n, d_visible, d_needsClipping, nullptr);
+  RZ_HIP(launch_query(q, ctx->stream));
+  return RZ_OK;
+}
+
+extern "C" int rz_readback_depth(rz_context* ctx, void* rgba) {
+  if (!ctx || !rgba) return RZ_ERR_INVALID_ARGUMENT;
+  const size_t px = size_t(ctx->width) * ctx->height;
+  if (!ctx->d_rgba && dev_alloc(&ctx->d_rgba, px) != hipSuccess) return RZ_ERR_OUT_OF_MEMORY;
+  RZ_HIP(launch_readback(ctx->d_depth, ctx->d_hiz, ctx->width, ctx->height, ctx->bX, ctx->d_rgba, ctx->stream));
+  RZ_HIP(hipMemcpyAsync(rgba, ctx->d_rgba, px * 4, hipMemcpyDeviceToHost, ctx->stream));
+  return rz_sync(ctx);
+}
+
+extern "C" int rz_readback_raw(rz_context* ctx, uint16_t* depth, uint16_t* hiz) {
+  if (!ctx) return RZ_ERR_INVALID_ARGUMENT;
+  const size_t nb = size_t(ctx->bX) * ctx->bY;
+  if (depth) RZ_HIP(hipMemcpyAsync(depth, ctx->d_depth, nb * 128, hipMemcpyDeviceToHost, ctx->stream));
+  if (hiz) RZ_HIP(hipMemcpyAsync(hiz, ctx->d_hiz, (nb + 8) * 2, hipMemcpyDeviceToHost, ctx->stream));
+  return rz_sync(ctx);
+}
+
+extern "C" int rz_upload_raw(rz_context* ctx, const uint16_t* depth, const uint16_t* hiz) {
+  if (!ctx) return RZ_ERR_INVALID_ARGUMENT;
+  const size_t nb = size_t(ctx->bX) * ctx->bY;
+  if (depth) RZ_HIP(hipMemcpyAsync(ctx->d_depth, depth, nb * 128, hipMemcpyHostToDevice, ctx->stream));
+  if (hiz) RZ_HIP(hipMemcpyAsync(ctx->d_hiz, hiz, (nb + 8) * 2, hipMemcpyHostToDevice, ctx->stream));
+  return rz_sync(ctx);
+}
+
+extern "C" int rz_get_stream(rz_context* ctx, void** stream) {
+  if (!ctx || !stream) return RZ_ERR_INVALID_ARGUMENT;
+  *stream = ctx->stream;
+  return RZ_OK;
+}
+
+extern "C" int rz_get_info(rz_context* ctx, uint32_t info[6]) {
+  if (!ctx || !info) return RZ_ERR_INVALID_ARGUMENT;
+  info[0] = ctx->width;
+  info[1] = ctx->height;
+  info[2] = ctx->bX;
+  info[3] = ctx->bY;
+  info[4] = uint32_t(ctx->tileW);
+  info[5] = uint32_t(ctx->tileW);
+  return RZ_OK;
+}
+
+extern "C" int rz_get_tables(rz_context* ctx, uint64_t* coverage4096, uint32_t* rcp4096) {
+  if (!ctx) return RZ_ERR_INVALID_ARGUMENT;
+  if (coverage4096) RZ_HIP(hipMemcpy(coverage4096, ctx->d_cov, 4096 * 8, hipMemcpyDeviceToHost));
+  if (rcp4096) RZ_HIP(hipMemcpy(rcp4096, ctx->d_rcp, 4096 * 4, hipMemcpyDeviceToHost));
+  return RZ_OK;
+}
+
+extern "C" int rz_get_mvp(rz_context* ctx, float prebaked[16], float raw[16]) {
+  if (!ctx) return RZ_ERR_INVALID_ARGUMENT;
+  if (prebaked) std::memcpy(prebaked, ctx->mvp.prebaked, 64);
+  if (raw) std::memcpy(raw, ctx->mvp.raw, 64);
+  return RZ_OK;
+}
+
+extern "C" int rz_bake(const float* vertices, uint32_t n, const float refMin[4], const float refMax[4],
+                       uint32_t* packets, float boundsMin[4], float boundsMax[4], float center[4]) {
+  if (!vertices || !refMin || !refMax || !packets || !boundsMin || !boundsMax || !center)
+    return RZ_ERR_INVALID_ARGUMENT;
+  return rzh::bake(vertices, n, refMin, refMax, packets, boundsMin, boundsMax, center) == 0 ? RZ_OK
+                                                                                             : RZ_ERR_INVALID_ARGUMENT;
+}
+
+extern "C" int rz_timer_start(rz_context* ctx) {
+  if (!ctx) return RZ_ERR_INVALID_ARGUMENT;
+  RZ_HIP(hipEventRecord(ctx->t0, ctx->stream));
+  return RZ_OK;
+}
+
+extern "C" int rz_timer_stop(rz_context* ctx, float* ms) {
+  if (!ctx || !ms) return RZ_ERR_INVALID_ARGUMENT;
+  RZ_HIP(hipEventRecord(ctx->t1, ctx->stream));
+  RZ_HIP(hipEventSynchronize(ctx->t1));
+  RZ_HIP(hipEventElapsedTime(ms, ctx->t0, ctx->t1));
+  return RZ_OK;
+}

@@ -1,0 +1,962 @@
+// gfx950 kernels of the occlusion-culling rasterizer.
+//
+// Pipeline of one draw (a frame of items, or a single rasterize call):
+//   rz_prep_kernel   one thread per item: rasterize prologue (matrix, c0, c1)
+//                    and the depth-independent half of queryVisibility.
+//   rz_setup_kernel  one thread per quad: transform, cull, bounding rectangle,
+//                    depth plane, edge normalization (Rasterizer.cpp:657-1081).
+//   rz_frame_kernel  one work-group per screen tile, all tiles co-resident:
+//                    the tile's depth and hiZ live in LDS; items are walked in
+//                    draw order; a query2D is reduced across the tiles that
+//                    overlap its rectangle with device atomics, so each item
+//                    is drawn exactly when the reference would draw it.  One
+//                    wave64 works on one 8x8 block at a time with lane = pixel
+//                    (Rasterizer.cpp:1095-1292).
+#include "rz_kernels.h"
+
+#include <hip/hip_runtime.h>
+
+#include "rz_x86.h"
+
+extern "C" __device__ uint32_t __ockl_wfred_min_u32(uint32_t);
+extern "C" __device__ uint32_t __ockl_wfred_max_u32(uint32_t);
+
+namespace rzd {
+
+constexpr uint32_t kModeTriangle0 = 1, kModeTriangle1 = 2, kModeConcaveRight = 3, kModeConcaveLeft = 4,
+                   kModeConcaveCenter = 5, kModeConvex = 6;
+
+// modeTable of Rasterizer.cpp:30-64: primitive mode for each 8-bit
+// configuration (bits 0-3: triangle area <= 0, bits 4-7: w < 0).
+__constant__ uint8_t c_modeTable[256] = {
+    6, 2, 4, 2, 1, 0, 1, 0, 3, 2, 0, 2, 1, 0, 1, 0, 6, 2, 4, 2, 1, 0, 1, 0, 0, 2, 5, 2, 1, 0, 1, 0,
+    6, 0, 4, 2, 1, 0, 1, 0, 3, 2, 5, 2, 1, 0, 0, 0, 6, 2, 4, 0, 1, 0, 1, 0, 3, 2, 5, 2, 0, 0, 1, 0,
+    6, 2, 0, 2, 1, 0, 1, 0, 3, 2, 5, 2, 1, 0, 1, 0, 0, 2, 5, 2, 1, 0, 1, 0, 5, 2, 5, 2, 1, 0, 1, 0,
+    6, 2, 4, 2, 1, 0, 0, 0, 3, 0, 5, 2, 1, 0, 1, 0, 2, 2, 2, 2, 0, 0, 0, 0, 2, 2, 2, 0, 0, 0, 0, 0,
+    6, 2, 4, 2, 0, 0, 1, 0, 3, 2, 5, 0, 1, 0, 1, 0, 6, 2, 4, 2, 1, 0, 0, 0, 3, 0, 5, 2, 1, 0, 1, 0,
+    0, 2, 4, 2, 1, 0, 1, 0, 3, 2, 5, 2, 1, 0, 1, 0, 3, 2, 0, 2, 1, 0, 1, 0, 3, 2, 5, 2, 1, 0, 1, 0,
+    6, 2, 4, 0, 1, 0, 1, 0, 3, 2, 5, 2, 0, 0, 1, 0, 1, 0, 1, 0, 1, 0, 1, 0, 1, 0, 1, 0, 1, 0, 0, 0,
+    4, 2, 4, 2, 1, 0, 1, 0, 0, 2, 5, 2, 1, 0, 1, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0,
+};
+
+constexpr uint64_t kEmptyBox = 0x000000000000FFFFull;  // min column 0xFFFF > max column 0
+
+// ---------------------------------------------------------------------------
+// Depth-independent half of queryVisibility (Rasterizer.cpp:125-273).
+// ---------------------------------------------------------------------------
+struct QueryPre {
+  uint32_t kind;  // 0 frustum-culled, 1 near the camera plane, 2 empty rectangle, 3 rectangle test
+  uint32_t minX, maxX, minY, maxY, maxZ;
+};
+
+__device__ QueryPre query_pre(const Mvp& M, const float* bmin, const float* bmax, uint32_t width, uint32_t height,
+                              const uint32_t* __restrict__ rcpTab) {
+  QueryPre out = {0, 0, 0, 0, 0, 0};
+  float ext[4], cen[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    ext[j] = bmax[j] - bmin[j];
+    cen[j] = bmax[j] + bmin[j];
+  }
+  // Six frustum planes from the raw rows; the box corner is chosen by the
+  // plane's sign bits (:135-167).  A set sign bit on any distance culls.
+  const float* R = M.raw;
+  uint32_t signs = 0;
+#pragma unroll
+  for (int pl = 0; pl < 6; ++pl) {
+    const int rr = pl >> 1;
+    float plane[4], off[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      plane[j] = (pl & 1) ? (R[12 + j] - R[4 * rr + j]) : (R[12 + j] + R[4 * rr + j]);
+      off[j] = cen[j] + xor_sign(ext[j], fbits(plane[j]) & 0x80000000u);
+    }
+    signs |= fbits(dp4_x86(plane[0], plane[1], plane[2], plane[3], off[0], off[1], off[2], off[3]));
+  }
+  if (signs & 0x80000000u) return out;
+
+  // Eight corners with the prebaked columns (:169-198).
+  const float* C = M.prebaked;
+  float c[8][4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const float e0 = C[0 + j] * ext[0];
+    const float e1 = C[4 + j] * ext[1];
+    const float e2 = C[8 + j] * ext[2];
+    c[0][j] = fmaf(C[0 + j], bmin[0], fmaf(C[4 + j], bmin[1], fmaf(C[8 + j], bmin[2], C[12 + j])));
+    c[1][j] = c[0][j] + e0;
+    c[2][j] = c[0][j] + e1;
+    c[4][j] = c[0][j] + e2;
+    c[3][j] = c[1][j] + e1;
+    c[5][j] = c[4][j] + e0;
+    c[6][j] = c[2][j] + e2;
+    c[7][j] = c[6][j] + e0;
+  }
+  // Per-lane near-plane epsilon (:204-213): lane j compares corners j and j+4.
+  float me1[4], me2[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) me1[j] = max_x86(ext[j], ext[j ^ 2]);
+#pragma unroll
+  for (int j = 0; j < 4; ++j) me2[j] = max_x86(me1[j], me1[j ^ 1]);
+  bool close = false;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const float eps = me2[j] * 0.001f;
+    close |= (c[j][3] < eps) || (c[4 + j][3] < eps);
+  }
+  if (close) {
+    out.kind = 1;
+    return out;
+  }
+  // Projection, screen rectangle (:217-264).
+  float px[8], py[8], pz[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    const float rw = rcp_x86(c[k][3], rcpTab);
+    px[k] = c[k][0] * rw;
+    py[k] = c[k][1] * rw;
+    pz[k] = c[k][2] * rw;
+  }
+  float minsX[4], maxsX[4], minsY[4], maxsY[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    minsX[j] = min_x86(px[j], px[4 + j]);
+    maxsX[j] = max_x86(px[j], px[4 + j]);
+    minsY[j] = min_x86(py[j], py[4 + j]);
+    maxsY[j] = max_x86(py[j], py[4 + j]);
+  }
+  float mXY[4] = {min_x86(minsX[0], minsX[2]), min_x86(minsY[0], minsY[2]), min_x86(minsX[1], minsX[3]),
+                  min_x86(minsY[1], minsY[3])};
+  float MXY[4] = {max_x86(maxsX[0], maxsX[2]), max_x86(maxsY[0], maxsY[2]), max_x86(maxsX[1], maxsX[3]),
+                  max_x86(maxsY[1], maxsY[3])};
+  const float lim[4] = {float(width - 1), float(height - 1), float(width - 1), float(height - 1)};
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    mXY[j] = max_x86(mXY[j], 0.0f);
+    MXY[j] = -min_x86(MXY[j], lim[j]);
+  }
+  const float bf[4] = {min_x86(mXY[0], mXY[2]), min_x86(MXY[0], MXY[2]), min_x86(mXY[1], mXY[3]),
+                       min_x86(MXY[1], MXY[3])};
+  int32_t bi[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) bi[j] = cvtt_x86(floorf(bf[j]));
+  bi[1] = int32_t(0u - uint32_t(bi[1]));
+  bi[3] = int32_t(0u - uint32_t(bi[3]));
+  if (bi[0] >= bi[1] || bi[2] >= bi[3]) {
+    out.kind = 2;
+    return out;
+  }
+  uint32_t maxZ = 0;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) maxZ = max(maxZ, pack_depth(pz[k]));
+  out.kind = 3;
+  out.minX = uint32_t(bi[0]);
+  out.maxX = uint32_t(bi[1]);
+  out.minY = uint32_t(bi[2]);
+  out.maxY = uint32_t(bi[3]);
+  out.maxZ = maxZ;
+  return out;
+}
+
+// query2D (Rasterizer.cpp:283-349) on the buffers in device memory.
+__device__ bool query2d_global(uint32_t minX, uint32_t maxX, uint32_t minY, uint32_t maxY, uint32_t maxZ,
+                               const uint16_t* __restrict__ depth, const uint16_t* __restrict__ hiz, uint32_t bX) {
+  const uint32_t bMinX = minX >> 3, bMaxX = maxX >> 3, bMinY = minY >> 3, bMaxY = maxY >> 3;
+  for (uint32_t by = bMinY; by <= bMaxY; ++by) {
+    const int32_t sy = int32_t(minY - 8 * by), ey = int32_t(maxY - 8 * by);
+    const uint32_t startY = uint32_t(sy > 0 ? sy : 0), endY = uint32_t(ey < 7 ? ey : 7);
+    const bool interiorLine = startY == 0 && endY == 7;
+    for (uint32_t bx = bMinX; bx <= bMaxX; ++bx) {
+      const uint32_t b = by * bX + bx;
+      if (maxZ <= hiz[b]) continue;
+      const int32_t sx = int32_t(minX - 8 * bx), ex = int32_t(maxX - 8 * bx);
+      const uint32_t startX = uint32_t(sx > 0 ? sx : 0), endX = uint32_t(ex < 7 ? ex : 7);
+      if (interiorLine && startX == 0 && endX == 7) return true;
+      const uint4* rows = reinterpret_cast<const uint4*>(depth + size_t(b) * 64);
+      for (uint32_t y = startY; y <= endY; ++y) {
+        const uint4 r = rows[y];
+        const uint32_t w[4] = {r.x, r.y, r.z, r.w};
+        for (uint32_t x = startX; x <= endX; ++x) {
+          const uint32_t v = (w[x >> 1] >> (16 * (x & 1))) & 0xFFFFu;
+          if (v < maxZ) return true;
+        }
+      }
+    }
+  }
+  return false;
+}
+
+// ---------------------------------------------------------------------------
+// Prep: rasterize prologue per item (Rasterizer.cpp:616-655) and the item's
+// status for the draw loop.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) rz_prep_kernel(PrepParams p) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= p.nItems) return;
+  const ItemDesc& D = p.items[i];
+  ItemState& S = p.state[i];
+  const float* P = p.mvp.prebaked;
+  float ext[3];
+#pragma unroll
+  for (int j = 0; j < 3; ++j) ext[j] = D.refMax[j] - D.refMin[j];
+  const float k0 = 1.0f / float(2047ull << 21);
+  const float k1 = 1.0f / float(2047 << 10);
+  const float k2 = 1.0f / float(1023);
+  float m0[4], m1[4], m2[4], m3[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    m3[j] = fmaf(P[0 + j], D.refMin[0], fmaf(P[4 + j], D.refMin[1], fmaf(P[8 + j], D.refMin[2], P[12 + j])));
+    m0[j] = P[0 + j] * (ext[0] * k0);
+    m1[j] = P[4 + j] * (ext[1] * k1);
+    m2[j] = P[8 + j] * (ext[2] * k2);
+  }
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    m3[j] = fmaf(m0[j], 2147483648.0f, m3[j]);
+    m1[j] = m1[j] - m0[j];
+    m2[j] = m2[j] - m0[j];
+  }
+#pragma unroll
+  for (int o = 0; o < 4; ++o) {
+    S.T[o][0] = m0[o];
+    S.T[o][1] = m1[o];
+    S.T[o][2] = m2[o];
+    S.T[o][3] = m3[o];
+  }
+  const float Za = m3[2], Zb = dp4_x86(m0[2], m1[2], m2[2], m3[2], 2097152.0f, 1024.0f, 1.0f, 1.0f);
+  const float Wa = m3[3], Wb = dp4_x86(m0[3], m1[3], m2[3], m3[3], 2097152.0f, 1024.0f, 1.0f, 1.0f);
+  const float q = (Za - Zb) / (Wa - Wb);
+  S.c0 = q;
+  S.c1 = fmaf(-q, Wa, Za);
+
+  S.fp[0] = 0xFFFFFFFFu;
+  S.fp[1] = 0u;
+  S.fp[2] = 0xFFFFFFFFu;
+  S.fp[3] = 0u;
+  S.expected = 0;
+  S.qMinX = S.qMaxX = S.qMinY = S.qMaxY = S.qMaxZ = 0;
+  p.done[i] = 0;
+  p.visible[i] = 0;
+  uint32_t status;
+  if (D.kind == kItemForceNoClip) {
+    status = kStatusNoClip;
+  } else if (D.kind == kItemForceClip) {
+    status = kStatusClip;
+  } else {
+    const QueryPre Q = query_pre(p.mvp, D.boundsMin, D.boundsMax, p.width, p.height, p.rcp);
+    if (Q.kind == 1) {
+      status = kStatusClip;
+    } else if (Q.kind == 3) {
+      status = kStatusTest;
+      S.qMinX = Q.minX;
+      S.qMaxX = Q.maxX;
+      S.qMinY = Q.minY;
+      S.qMaxY = Q.maxY;
+      S.qMaxZ = Q.maxZ;
+      const uint32_t tx0 = (Q.minX >> 3) / p.tileW, tx1 = (Q.maxX >> 3) / p.tileW;
+      const uint32_t ty0 = (Q.minY >> 3) / p.tileH, ty1 = (Q.maxY >> 3) / p.tileH;
+      S.expected = (tx1 - tx0 + 1) * (ty1 - ty0 + 1);
+    } else {
+      status = kStatusSkip;
+    }
+  }
+  S.status = status;
+}
+
+// ---------------------------------------------------------------------------
+// Setup: one quad per thread (Rasterizer.cpp:660-1063).
+// ---------------------------------------------------------------------------
+struct SetupOut {
+  bool valid;
+  QuadRec rec;
+};
+
+__device__ void setup_quad(const ItemState& S, const uint32_t I[4], bool clip, uint32_t bX, uint32_t bY,
+                           const uint32_t* __restrict__ rcpTab, SetupOut& out) {
+  const float maxInvW = 1.84467430e+19f;  // sqrt(FLT_MAX) rounded to float (Rasterizer.cpp:11)
+  float x[4], y[4], invW[4];
+  uint32_t ws[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const float Xf = float(int32_t(I[k]));
+    const float Yf = float(int32_t(I[k] & (2047u << 10)));
+    const float Zf = float(int32_t(I[k] & 1023u));
+    const float X = fmaf(Xf, S.T[0][0], fmaf(Yf, S.T[0][1], fmaf(Zf, S.T[0][2], S.T[0][3])));
+    const float Y = fmaf(Xf, S.T[1][0], fmaf(Yf, S.T[1][1], fmaf(Zf, S.T[1][2], S.T[1][3])));
+    const float W = fmaf(Xf, S.T[3][0], fmaf(Yf, S.T[3][1], fmaf(Zf, S.T[3][2], S.T[3][3])));
+    const float r = rcp_x86(W, rcpTab);
+    invW[k] = clip ? min_x86(maxInvW, max_x86(-maxInvW, r)) : r;
+    x[k] = rintf(X * invW[k]) * 0.125f;
+    y[k] = rintf(Y * invW[k]) * 0.125f;
+    ws[k] = clip ? (fbits(invW[k]) & 0x80000000u) : 0u;
+  }
+  float eX[5], eY[5];
+  eX[0] = y[1] - y[0];
+  eX[1] = y[2] - y[1];
+  eX[2] = y[3] - y[2];
+  eX[3] = y[0] - y[3];
+  eY[0] = x[0] - x[1];
+  eY[1] = x[1] - x[2];
+  eY[2] = x[2] - x[3];
+  eY[3] = x[3] - x[0];
+  const float area0 = fmaf(eX[0], eY[1], -(eX[1] * eY[0]));
+  const float area1 = fmaf(eX[1], eY[2], -(eX[2] * eY[1]));
+  const float area2 = fmaf(eX[2], eY[3], -(eX[3] * eY[2]));
+  const float area3 = (area0 + area2) - area1;
+  // Area signs with zero counted as negative; flipped per w < 0 vertex when
+  // clipping (:775-803).
+  const uint32_t s0 = xor_sign(xor_sign(area0, ws[0]), ws[1] ^ ws[2]) <= 0.0f;
+  const uint32_t s1 = xor_sign(xor_sign(area1, ws[1]), ws[2] ^ ws[3]) <= 0.0f;
+  const uint32_t s2 = xor_sign(xor_sign(area2, ws[0]), ws[2] ^ ws[3]) <= 0.0f;
+  const uint32_t s3 = xor_sign(xor_sign(area3, ws[1]), ws[0] ^ ws[3]) <= 0.0f;
+  const uint32_t config = s0 | (s1 << 1) | (s2 << 2) | (s3 << 3) | (ws[0] >> 27) | (ws[1] >> 26) | (ws[2] >> 25) |
+                          (ws[3] >> 24);
+  const uint32_t mode = c_modeTable[config];
+  out.valid = false;
+  if (mode == 0) return;
+
+  float minFx, minFy, maxFx, maxFy;
+  if (clip) {
+    // Clipless bounding box from the intervals of w > 0 and w < 0 vertices
+    // (:820-909).
+    const float infP = 10000.0f, infN = -10000.0f;
+    float minPx[4], minPy[4], maxPx[4], maxPy[4], minNx[4], minNy[4], maxNx[4], maxNy[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const bool neg = ws[k] != 0u;
+      minPx[k] = neg ? infP : x[k];
+      minPy[k] = neg ? infP : y[k];
+      maxPx[k] = xor_sign(minPx[k], ws[k]);
+      maxPy[k] = xor_sign(minPy[k], ws[k]);
+      minNx[k] = neg ? x[k] : infP;
+      minNy[k] = neg ? y[k] : infP;
+      maxNx[k] = neg ? x[k] : infN;
+      maxNy[k] = neg ? y[k] : infN;
+    }
+    // Reductions in the reference's pairing: op(op(v0, v1), op(v2, v3)).
+    const float mPx = min_x86(min_x86(minPx[0], minPx[1]), min_x86(minPx[2], minPx[3]));
+    const float mPy = min_x86(min_x86(minPy[0], minPy[1]), min_x86(minPy[2], minPy[3]));
+    const float MPx = max_x86(max_x86(maxPx[0], maxPx[1]), max_x86(maxPx[2], maxPx[3]));
+    const float MPy = max_x86(max_x86(maxPy[0], maxPy[1]), max_x86(maxPy[2], maxPy[3]));
+    const float mNx = min_x86(min_x86(minNx[0], minNx[1]), min_x86(minNx[2], minNx[3]));
+    const float mNy = min_x86(min_x86(minNy[0], minNy[1]), min_x86(minNy[2], minNy[3]));
+    const float MNx = max_x86(max_x86(maxNx[0], maxNx[1]), max_x86(maxNx[2], maxNx[3]));
+    const float MNy = max_x86(max_x86(maxNy[0], maxNy[1]), max_x86(maxNy[2], maxNy[3]));
+    const float incAx = (MNx > mPx) ? infN : mPx;
+    const float incAy = (MNy > mPy) ? infN : mPy;
+    const float incBx = (MPx > mNx) ? infP : MPx;
+    const float incBy = (MPy > mNy) ? infP : MPy;
+    minFx = min_x86(incAx, incBx);
+    minFy = min_x86(incAy, incBy);
+    maxFx = max_x86(incAx, incBx);
+    maxFy = max_x86(incAy, incBy);
+  } else {
+    minFx = min_x86(min_x86(x[0], x[1]), min_x86(x[2], x[3]));
+    minFy = min_x86(min_x86(y[0], y[1]), min_x86(y[2], y[3]));
+    maxFx = max_x86(max_x86(x[0], x[1]), max_x86(x[2], x[3]));
+    maxFy = max_x86(max_x86(y[0], y[1]), max_x86(y[2], y[3]));
+  }
+  int32_t minX = cvtt_x86(minFx + 0.6249875f);
+  int32_t minY = cvtt_x86(minFy + 0.6249875f);
+  int32_t maxX = cvtt_x86(maxFx + 1.375f);
+  int32_t maxY = cvtt_x86(maxFy + 1.375f);
+  minX = max(minX, 0);
+  minY = max(minY, 0);
+  maxX = min(maxX, int32_t(bX));
+  maxY = min(maxY, int32_t(bY));
+  if (!(maxX > minX && maxY > minY)) return;
+
+  float z[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) z[k] = fmaf(invW[k], S.c1, S.c0);
+  float maxZ = max_x86(max_x86(z[0], z[1]), max_x86(z[2], z[3]));
+  if ((ws[0] | ws[1] | ws[2] | ws[3]) != 0u) maxZ = 1.0f;
+
+  // Depth plane from the larger triangle (:963-999).
+  const bool tri0 = mode == kModeTriangle0, tri1 = mode == kModeTriangle1;
+  const bool greater = !tri0 && (tri1 || (__builtin_fabsf(area0) < __builtin_fabsf(area2)));
+  const float selArea = greater ? area2 : area0;
+  const float invArea = clip ? 1.0f / selArea : rcp_x86(selArea, rcpTab);
+  const float z12 = z[1] - z[2], z20 = z[2] - z[0], z30 = z[3] - z[0];
+  eX[4] = y[0] - y[2];
+  eY[4] = x[2] - x[0];
+  const float dpx = greater ? fmaf(-z20, eX[3], z30 * eX[4]) : fmaf(z20, eX[1], -(z12 * eX[4]));
+  const float dpy = greater ? fmaf(-z20, eY[3], z30 * eY[4]) : fmaf(z20, eY[1], -(z12 * eY[4]));
+  const float dp1 = invArea * dpx;
+  const float dp2 = invArea * dpy;
+  const float x0 = x[0] - float(minX);
+  const float y0 = y[0] - float(minY);
+  const float dp0 = fmaf(-x0, dp1, fmaf(-y0, dp2, z[0]));
+
+  // Triangle modes swap in the diagonal edge (:1001-1005); edges facing w < 0
+  // are flipped (:1007-1022); normalization to 70 / (|nx| + |ny|) (:452-468).
+  if (tri0) {
+    eX[2] = eX[4];
+    eY[2] = eY[4];
+  }
+  if (tri1) {
+    eX[0] = -eX[4];
+    eY[0] = -eY[4];
+  }
+  uint32_t flip[4] = {0u, 0u, 0u, 0u};
+  if (clip) {
+    flip[0] = ws[0] ^ (tri1 ? ws[2] : ws[1]);
+    flip[1] = ws[1] ^ ws[2];
+    flip[2] = ws[2] ^ (tri0 ? ws[0] : ws[3]);
+    flip[3] = ws[0] ^ ws[3];
+  }
+  const float edgeScale = 63.0f / (0.45f - -0.45f);
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    float invLen = rcp_x86(__builtin_fabsf(eX[e]) + __builtin_fabsf(eY[e]), rcpTab);
+    const float mul = clip ? xor_sign(edgeScale, flip[e]) : edgeScale;
+    invLen = mul * invLen;
+    eX[e] = eX[e] * invLen;
+    eY[e] = eY[e] * invLen;
+  }
+  // Edge offsets at the rectangle origin (:1030-1045).
+  const float add = 0.5f - -0.45f * 63.0f / (0.45f - -0.45f);
+  const float fMinX = float(minX), fMinY = float(minY);
+  float off0 = fmaf(-x0, eX[0], fmaf(-y0, eY[0], add));
+  float off1 = fmaf(-x[1], eX[1], fmaf(-y[1], eY[1], add));
+  float off2 = fmaf(-x[2], eX[2], fmaf(-y[2], eY[2], add));
+  float off3 = fmaf(-x[3], eX[3], fmaf(-y[3], eY[3], add));
+  off1 = fmaf(fMinX, eX[1], off1);
+  off2 = fmaf(fMinX, eX[2], off2);
+  off3 = fmaf(fMinX, eX[3], off3);
+  off1 = fmaf(fMinY, eY[1], off1);
+  off2 = fmaf(fMinY, eY[2], off2);
+  off3 = fmaf(fMinY, eY[3], off3);
+  // Slope buckets with ny <= 0 selecting the odd bucket (:482-493).
+  const float slopeMul = 15.5f / edgeScale;
+  uint32_t slope[4];
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    const int32_t q = cvtt_x86(fmaf(eX[e], slopeMul, 16.0f));
+    slope[e] = ((uint32_t(q) << 1) + (eY[e] <= 0.0f ? 1u : 0u)) << 6;
+  }
+
+  QuadRec& R = out.rec;
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    R.nx[e] = eX[e];
+    R.ny[e] = eY[e];
+  }
+  R.off[0] = off0;
+  R.off[1] = off1;
+  R.off[2] = off2;
+  R.off[3] = off3;
+  R.dp0 = dp0;
+  R.dDx = dp1;
+  R.dDy = dp2;
+  // 16-bit lane multiply of the first block index (:1054).
+  R.fb = ((uint32_t(minY) * bX) & 0xFFFFu) + uint32_t(minX);
+  R.range = uint32_t(maxX - minX) | (uint32_t(maxY - minY) << 16);
+  R.modeZ = mode | (pack_depth(maxZ) << 16);
+  R.slope01 = (slope[0] & 0xFFFFu) | (slope[1] << 16);
+  R.slope23 = (slope[2] & 0xFFFFu) | (slope[3] << 16);
+  out.valid = true;
+}
+
+// Conservative screen-block box of a quad's blocks fb + r*bX + c,
+// 0 <= r < rangeY, 0 <= c < rangeX (exact unless the 16-bit multiply
+// wrapped the row index, when it spans whole rows).
+__device__ uint64_t quad_box(uint32_t fb, uint32_t rangeX, uint32_t rangeY, uint32_t bX, uint32_t bY) {
+  const uint32_t col = fb % bX, row = fb / bX;
+  uint32_t c0, c1, r0 = row, r1;
+  if (col + rangeX <= bX) {
+    c0 = col;
+    c1 = col + rangeX - 1;
+    r1 = row + rangeY - 1;
+  } else {
+    c0 = 0;
+    c1 = bX - 1;
+    r1 = row + rangeY;
+  }
+  r1 = min(r1, bY - 1);
+  return uint64_t(c0) | (uint64_t(c1) << 16) | (uint64_t(r0) << 32) | (uint64_t(r1) << 48);
+}
+
+__global__ void __launch_bounds__(256) rz_setup_kernel(SetupParams p) {
+  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool inRange = g < p.nQuads;
+  uint32_t item = 0;
+  bool valid = false;
+  uint64_t box = kEmptyBox;
+  if (inRange) {
+    uint32_t lo = 0, hi = p.nItems - 1;
+    while (lo < hi) {
+      const uint32_t mid = (lo + hi + 1) >> 1;
+      if (p.items[mid].quadBase <= g) lo = mid;
+      else hi = mid - 1;
+    }
+    item = lo;
+    const ItemDesc& D = p.items[item];
+    const ItemState& S = p.state[item];
+    const uint32_t st = S.status;
+    if (st != kStatusSkip) {
+      const uint32_t q = g - D.quadBase;
+      const uint32_t* pk = D.packets + (q >> 3) * 32 + (q & 7);
+      const uint32_t I[4] = {pk[0], pk[8], pk[16], pk[24]};
+      SetupOut o;
+      setup_quad(S, I, st == kStatusClip, p.bX, p.bY, p.rcp, o);
+      if (o.valid) {
+        valid = true;
+        p.recs[g] = o.rec;
+        box = quad_box(o.rec.fb, o.rec.range & 0xFFFFu, o.rec.range >> 16, p.bX, p.bY);
+      }
+    }
+    p.bbox[g] = box;
+  }
+  // Footprint of each item's valid quads: one atomic per wave when the whole
+  // wave works on one item, else one per valid lane.
+  const uint32_t c0 = valid ? uint32_t(box & 0xFFFF) : 0xFFFFFFFFu;
+  const uint32_t c1 = valid ? uint32_t((box >> 16) & 0xFFFF) : 0u;
+  const uint32_t r0 = valid ? uint32_t((box >> 32) & 0xFFFF) : 0xFFFFFFFFu;
+  const uint32_t r1 = valid ? uint32_t(box >> 48) : 0u;
+  const uint32_t firstItem = __builtin_amdgcn_readfirstlane(item);
+  const bool sameItem = __ballot(inRange && item != firstItem) == 0ull;
+  if (sameItem) {
+    const uint32_t a = __ockl_wfred_min_u32(c0), b = __ockl_wfred_max_u32(c1);
+    const uint32_t c = __ockl_wfred_min_u32(r0), d = __ockl_wfred_max_u32(r1);
+    if ((threadIdx.x & 63u) == 0u && a != 0xFFFFFFFFu) {
+      uint32_t* fp = p.state[firstItem].fp;
+      atomicMin(&fp[0], a);
+      atomicMax(&fp[1], b);
+      atomicMin(&fp[2], c);
+      atomicMax(&fp[3], d);
+    }
+  } else if (valid) {
+    uint32_t* fp = p.state[item].fp;
+    atomicMin(&fp[0], c0);
+    atomicMax(&fp[1], c1);
+    atomicMin(&fp[2], r0);
+    atomicMax(&fp[3], r1);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Frame kernel.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float readlane_f(float v, int l) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+
+__device__ __forceinline__ int div_bx(int d, int bX, float invBX) {
+  int q = int(float(d) * invBX);
+  if (q * bX > d) --q;
+  else if ((q + 1) * bX <= d) ++q;
+  return q;
+}
+
+template <int TBX, int TBY>
+struct TileSmem {
+  uint64_t cov[4096];
+  uint16_t depth[TBX * TBY * 64];
+  uint16_t hiz[TBX * TBY];
+  uint32_t list[256];
+  uint32_t wcnt[4];
+  uint32_t part[2][4];
+  uint32_t vis[2];
+};
+
+// Draws one quad's blocks that lie in this wave's rows of the tile
+// (Rasterizer.cpp:1095-1292).  Blocks are independent, so the order among a
+// quad's blocks does not matter; the order among quads is the caller's.
+template <int TBX, int TBY>
+__device__ void raster_quad_wave(const QuadRec* __restrict__ recp, TileSmem<TBX, TBY>& sm, int tbx0, int tbx1,
+                                 int tby0, int tby1, int bX, float invBX, uint32_t wave, uint32_t lane) {
+  const QuadRec R = *recp;
+  const int fb = int(R.fb);
+  const int rangeX = int(R.range & 0xFFFFu), rangeY = int(R.range >> 16);
+  const uint32_t mode = R.modeZ & 0xFFu, maxZ = R.modeZ >> 16;
+  const uint32_t sl0 = R.slope01 & 0xFFFFu, sl1 = R.slope01 >> 16, sl2 = R.slope23 & 0xFFFFu, sl3 = R.slope23 >> 16;
+
+  // Lanes 0-3 fold the four edge offsets, lanes 4-11 the eight depth samples
+  // of rows 0/1 (x = 0..3 at sample positions -0.4375 + 0.125 k).
+  float base, stepY, stepX;
+  if (lane < 4) {
+    base = lane == 0 ? R.off[0] : lane == 1 ? R.off[1] : lane == 2 ? R.off[2] : R.off[3];
+    stepY = lane == 0 ? R.ny[0] : lane == 1 ? R.ny[1] : lane == 2 ? R.ny[2] : R.ny[3];
+    stepX = lane == 0 ? R.nx[0] : lane == 1 ? R.nx[1] : lane == 2 ? R.nx[2] : R.nx[3];
+  } else {
+    const uint32_t k = (lane - 4) & 7u;
+    const float spx = -0.4375f + 0.125f * float(k & 3u);
+    const float spy = -0.4375f + 0.125f * float(k >> 2);
+    base = fmaf(R.dDx, spx, fmaf(R.dDy, spy, R.dp0));
+    stepY = R.dDy;
+    stepX = R.dDx;
+  }
+  const uint32_t px = lane & 7u, py = lane >> 3;
+  const int srcLane = 4 + int((px & 3u) + 4u * (py & 1u));
+  const uint32_t j = py >> 1;
+  const uint32_t maskBit = 8u * px + ((py & 1u) ? (py >> 1) : 4u + (py >> 1));
+
+  float line = base;
+  int cur = 0;
+  for (int lr = int(wave); lr < TBY; lr += 4) {
+    const int by = tby0 + lr;
+    if (by > tby1) break;
+    const int rowL = by * bX;
+    int d0 = rowL + tbx0 - fb;
+    const int d1 = rowL + tbx1 - fb;
+    if (d1 < 0) continue;
+    if (d0 < 0) d0 = 0;
+    const int r0 = div_bx(d0, bX, invBX), r1 = div_bx(d1, bX, invBX);
+    for (int r = r0; r <= r1 && r < rangeY; ++r) {
+      const int cLo = max(d0 - r * bX, 0);
+      const int cHi = min(d1 - r * bX, rangeX - 1);
+      if (cLo > cHi) continue;
+      while (cur < r) {
+        line = line + stepY;
+        ++cur;
+      }
+      float s = line;
+      for (int c = 0; c < cLo; ++c) s = stepX + s;
+      for (int c = cLo; c <= cHi; ++c, s = stepX + s) {
+        const int bx = fb + r * bX + c - rowL;
+        const int lb = lr * TBX + (bx - tbx0);
+        const uint32_t hz = sm.hiz[lb];
+        if (hz >= maxZ) continue;
+        const float o0 = readlane_f(s, 0), o1 = readlane_f(s, 1), o2 = readlane_f(s, 2), o3 = readlane_f(s, 3);
+        uint64_t mask;
+        if (mode == kModeConvex) {
+          if (o0 >= 63.0f || o1 >= 63.0f || o2 >= 63.0f || o3 >= 63.0f) continue;
+          const uint32_t i0 = (sl0 | uint32_t(max(cvtt_x86(o0), 0))) & 4095u;
+          const uint32_t i1 = (sl1 | uint32_t(max(cvtt_x86(o1), 0))) & 4095u;
+          const uint32_t i2 = (sl2 | uint32_t(max(cvtt_x86(o2), 0))) & 4095u;
+          const uint32_t i3 = (sl3 | uint32_t(max(cvtt_x86(o3), 0))) & 4095u;
+          mask = (sm.cov[i0] & sm.cov[i1]) & (sm.cov[i2] & sm.cov[i3]);
+        } else {
+          const uint32_t i0 = sl0 | uint32_t(min(max(cvtt_x86(o0), 0), 63));
+          const uint32_t i1 = sl1 | uint32_t(min(max(cvtt_x86(o1), 0), 63));
+          const uint32_t i2 = sl2 | uint32_t(min(max(cvtt_x86(o2), 0), 63));
+          const uint32_t i3 = sl3 | uint32_t(min(max(cvtt_x86(o3), 0), 63));
+          const uint64_t A = sm.cov[i0 & 4095u], B = sm.cov[i1 & 4095u], C = sm.cov[i2 & 4095u],
+                         D = sm.cov[i3 & 4095u];
+          if (mode == kModeTriangle0) mask = A & B & C;
+          else if (mode == kModeTriangle1) mask = A & C & D;
+          else if (mode == kModeConcaveRight) mask = (A | D) & (B & C);
+          else if (mode == kModeConcaveCenter) mask = (A & B) | (C & D);
+          else mask = (A & D) & (B | C);
+          if (mask == 0) continue;
+        }
+        // Depth: rows 0/1 and 8/9 from the plane, rows 2-7 by rounding-up
+        // averages of the packed values (:1241-1254).
+        float v = __shfl(s, srcLane);
+        if (px >= 4u) v = fmaf(R.dDx, 0.5f, v);
+        const uint32_t lo = pack_depth(v);
+        const uint32_t hi = pack_depth(R.dDy + v);
+        const uint32_t mid = avg_u16(lo, hi);
+        uint32_t d = j == 0 ? lo : (j == 1 ? avg_u16(lo, mid) : (j == 2 ? mid : avg_u16(mid, hi)));
+        if (!((mask >> maskBit) & 1ull)) d = 0;
+        uint16_t* dp = &sm.depth[lb * 64 + int(lane)];
+        if (hz != 1u) d = max(d, uint32_t(*dp));
+        *dp = uint16_t(d);
+        const uint32_t mn = __ockl_wfred_min_u32(d);
+        if (lane == 0) sm.hiz[lb] = uint16_t(mn);
+      }
+    }
+  }
+}
+
+// query2D restricted to this wave's rows of the tile (Rasterizer.cpp:283-349).
+template <int TBX, int TBY>
+__device__ bool query_wave(const ItemState& S, TileSmem<TBX, TBY>& sm, int tbx0, int tbx1, int tby0, int tby1,
+                           uint32_t wave, uint32_t lane) {
+  const int qbx0 = int(S.qMinX >> 3), qbx1 = int(S.qMaxX >> 3), qby0 = int(S.qMinY >> 3), qby1 = int(S.qMaxY >> 3);
+  const uint32_t maxZ = S.qMaxZ;
+  const int px = int(lane & 7u), py = int(lane >> 3);
+  for (int lr = int(wave); lr < TBY; lr += 4) {
+    const int by = tby0 + lr;
+    if (by > tby1) break;
+    if (by < qby0 || by > qby1) continue;
+    const int sy = int(S.qMinY) - 8 * by, ey = int(S.qMaxY) - 8 * by;
+    const int startY = max(sy, 0), endY = min(ey, 7);
+    const int cx0 = max(tbx0, qbx0), cx1 = min(tbx1, qbx1);
+    for (int bx = cx0; bx <= cx1; ++bx) {
+      const int lb = lr * TBX + (bx - tbx0);
+      if (maxZ <= uint32_t(sm.hiz[lb])) continue;
+      const int startX = max(int(S.qMinX) - 8 * bx, 0), endX = min(int(S.qMaxX) - 8 * bx, 7);
+      if (startY == 0 && endY == 7 && startX == 0 && endX == 7) return true;
+      const bool mine = py >= startY && py <= endY && px >= startX && px <= endX &&
+                        uint32_t(sm.depth[lb * 64 + int(lane)]) < maxZ;
+      if (__ballot(mine) != 0ull) return true;
+    }
+  }
+  return false;
+}
+
+template <int TBX, int TBY>
+__global__ void __launch_bounds__(256) rz_frame_kernel(FrameParams p) {
+  __shared__ TileSmem<TBX, TBY> sm;
+  const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+  const uint32_t tX = blockIdx.x % p.tilesX, tY = blockIdx.x / p.tilesX;
+  const int bX = int(p.bX), bY = int(p.bY);
+  const int tbx0 = int(tX) * TBX, tby0 = int(tY) * TBY;
+  const int tbx1 = min(tbx0 + TBX, bX) - 1, tby1 = min(tby0 + TBY, bY) - 1;
+  const float invBX = 1.0f / float(bX);
+
+  // Does any item touch this tile, through its query rectangle or its quads?
+  int involved = 0;
+  for (uint32_t i = tid; i < p.nItems; i += 256) {
+    const ItemState& S = p.state[i];
+    if (S.status == kStatusSkip) continue;
+    bool q = false;
+    if (S.status == kStatusTest) {
+      q = tX >= (S.qMinX >> 3) / TBX && tX <= (S.qMaxX >> 3) / TBX && tY >= (S.qMinY >> 3) / TBY &&
+          tY <= (S.qMaxY >> 3) / TBY;
+    }
+    const bool f = S.fp[0] <= S.fp[1] && int(S.fp[0]) <= tbx1 && int(S.fp[1]) >= tbx0 && int(S.fp[2]) <= tby1 &&
+                   int(S.fp[3]) >= tby0;
+    involved |= (q || f) ? 1 : 0;
+  }
+  if (!__syncthreads_or(involved)) return;
+
+  for (uint32_t i = tid; i < 4096; i += 256) sm.cov[i] = p.cov[i];
+  constexpr int NB = TBX * TBY;
+  for (int rrow = int(tid); rrow < NB * 8; rrow += 256) {
+    const int lb = rrow >> 3, y = rrow & 7;
+    const int gx = tbx0 + lb % TBX, gy = tby0 + lb / TBX;
+    uint4 v = make_uint4(0, 0, 0, 0);
+    if (gx <= tbx1 && gy <= tby1)
+      v = *reinterpret_cast<const uint4*>(p.depth + (size_t(gy) * bX + gx) * 64 + y * 8);
+    *reinterpret_cast<uint4*>(&sm.depth[lb * 64 + y * 8]) = v;
+  }
+  for (int lb = int(tid); lb < NB; lb += 256) {
+    const int gx = tbx0 + lb % TBX, gy = tby0 + lb / TBX;
+    sm.hiz[lb] = (gx <= tbx1 && gy <= tby1) ? p.hiz[size_t(gy) * bX + gx] : uint16_t(0xFFFF);
+  }
+  __syncthreads();
+
+  uint32_t qPar = 0, vPar = 0;
+  for (uint32_t i = 0; i < p.nItems; ++i) {
+    const ItemState& S = p.state[i];
+    const uint32_t st = S.status;
+    if (st == kStatusSkip) continue;
+    const bool qov = st == kStatusTest && tX >= (S.qMinX >> 3) / TBX && tX <= (S.qMaxX >> 3) / TBX &&
+                     tY >= (S.qMinY >> 3) / TBY && tY <= (S.qMaxY >> 3) / TBY;
+    const bool fov = S.fp[0] <= S.fp[1] && int(S.fp[0]) <= tbx1 && int(S.fp[1]) >= tbx0 &&
+                     int(S.fp[2]) <= tby1 && int(S.fp[3]) >= tby0;
+    if (!qov && !fov) continue;
+    if (qov) {
+      const bool found = query_wave<TBX, TBY>(S, sm, tbx0, tbx1, tby0, tby1, wave, lane);
+      if (lane == 0) sm.part[qPar][wave] = found ? 1u : 0u;
+      __syncthreads();
+      if (tid == 0) {
+        const uint32_t part = sm.part[qPar][0] | sm.part[qPar][1] | sm.part[qPar][2] | sm.part[qPar][3];
+        if (part) __hip_atomic_fetch_or(&p.visible[i], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_fetch_add(&p.done[i], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      }
+      qPar ^= 1u;
+    }
+    if (!fov) continue;
+    if (st == kStatusTest) {
+      if (tid == 0) {
+        const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
+        const uint32_t expected = S.expected;
+        while (__hip_atomic_load(&p.done[i], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < expected) {
+          if (__builtin_amdgcn_s_memrealtime() - t0 > p.timeoutTicks) {
+            __hip_atomic_fetch_or(p.err, uint32_t(kErrTimeout), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            break;
+          }
+          __builtin_amdgcn_s_sleep(1);
+        }
+        sm.vis[vPar] = __hip_atomic_load(&p.visible[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+      __syncthreads();
+      const bool vis = sm.vis[vPar] != 0u;
+      vPar ^= 1u;
+      if (!vis) continue;
+    }
+    const ItemDesc& D = p.items[i];
+    const uint32_t qb = D.quadBase, qc = D.quadCount;
+    for (uint32_t base = 0; base < qc; base += 256) {
+      const uint32_t g = qb + base + tid;
+      bool hit = false;
+      if (base + tid < qc) {
+        const uint64_t bb = p.bbox[g];
+        const int c0 = int(bb & 0xFFFF), c1 = int((bb >> 16) & 0xFFFF), r0 = int((bb >> 32) & 0xFFFF),
+                  r1 = int(bb >> 48);
+        hit = c0 <= c1 && c0 <= tbx1 && c1 >= tbx0 && r0 <= tby1 && r1 >= tby0;
+      }
+      const uint64_t m = __ballot(hit);
+      const uint32_t pre = __builtin_amdgcn_mbcnt_hi(uint32_t(m >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(m), 0u));
+      if (lane == 0) sm.wcnt[wave] = uint32_t(__popcll(m));
+      __syncthreads();
+      uint32_t off = 0, total = 0;
+#pragma unroll
+      for (uint32_t w = 0; w < 4; ++w) {
+        off += w < wave ? sm.wcnt[w] : 0u;
+        total += sm.wcnt[w];
+      }
+      if (hit) sm.list[off + pre] = g;
+      __syncthreads();
+      for (uint32_t k = 0; k < total; ++k) {
+        const uint32_t gq = __builtin_amdgcn_readfirstlane(sm.list[k]);
+        raster_quad_wave<TBX, TBY>(p.recs + gq, sm, tbx0, tbx1, tby0, tby1, bX, invBX, wave, lane);
+      }
+      __syncthreads();
+    }
+  }
+
+  // Write the tile back.
+  for (int rrow = int(tid); rrow < NB * 8; rrow += 256) {
+    const int lb = rrow >> 3, y = rrow & 7;
+    const int gx = tbx0 + lb % TBX, gy = tby0 + lb / TBX;
+    if (gx <= tbx1 && gy <= tby1)
+      *reinterpret_cast<uint4*>(p.depth + (size_t(gy) * bX + gx) * 64 + y * 8) =
+          *reinterpret_cast<const uint4*>(&sm.depth[lb * 64 + y * 8]);
+  }
+  for (int lb = int(tid); lb < NB; lb += 256) {
+    const int gx = tbx0 + lb % TBX, gy = tby0 + lb / TBX;
+    if (gx <= tbx1 && gy <= tby1) p.hiz[size_t(gy) * bX + gx] = sm.hiz[lb];
+  }
+}
+
+// Per-item draw status of a frame: 0 not drawn, 1 rasterize<false>,
+// 2 rasterize<true>.
+__global__ void rz_status_kernel(const ItemState* __restrict__ state, const uint32_t* __restrict__ visible,
+                                 uint32_t n, uint8_t* __restrict__ out) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t st = state[i].status;
+  out[i] = uint8_t(st == kStatusTest ? (visible[i] ? 1u : 0u) : st);
+}
+
+// ---------------------------------------------------------------------------
+// Queries, readback, clear.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) rz_query_kernel(QueryParams p) {
+  const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= p.n) return;
+  const float* b = p.bounds + 8 * size_t(q);
+  const float bmin[4] = {b[0], b[1], b[2], b[3]};
+  const float bmax[4] = {b[4], b[5], b[6], b[7]};
+  const QueryPre Q = query_pre(p.mvp, bmin, bmax, p.width, p.height, p.rcp);
+  uint32_t vis = 0, clip = 0;
+  if (Q.kind == 1) {
+    vis = 1;
+    clip = 1;
+  } else if (Q.kind == 3) {
+    vis = query2d_global(Q.minX, Q.maxX, Q.minY, Q.maxY, Q.maxZ, p.depth, p.hiz, p.bX) ? 1u : 0u;
+  }
+  p.visible[q] = uint8_t(vis);
+  if (p.needsClipping) p.needsClipping[q] = uint8_t(clip);
+  // bit 0 visible, bit 1 needsClipping, bit 2 needsClipping was written
+  if (p.code) p.code[q] = uint8_t(vis | (clip << 1) | (Q.kind != 0 ? 4u : 0u));
+}
+
+__global__ void rz_query2d_kernel(uint32_t minX, uint32_t maxX, uint32_t minY, uint32_t maxY, uint32_t maxZ,
+                                  const uint16_t* depth, const uint16_t* hiz, uint32_t bX, uint8_t* out) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *out = query2d_global(minX, maxX, minY, maxY, maxZ, depth, hiz, bX) ? 1 : 0;
+}
+
+// readBackDepth (Rasterizer.cpp:351-399), one pixel per thread.
+__global__ void __launch_bounds__(256) rz_readback_kernel(const uint16_t* __restrict__ depth,
+                                                          const uint16_t* __restrict__ hiz, uint32_t width,
+                                                          uint32_t height, uint32_t bX, uint32_t* __restrict__ rgba) {
+  const uint32_t px = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t py = blockIdx.y;
+  if (px >= width || py >= height) return;
+  const uint32_t b = (py >> 3) * bX + (px >> 3);
+  uint32_t out = 0;
+  if (hiz[b] != 1u) {
+    const uint32_t v = depth[size_t(b) * 64 + (py & 7u) * 8 + (px & 7u)];
+    const float f = bitsf(v << 12) * 3.9623753e+28f;
+    const float l = 0.5f / (1000.25f - (1.0f - f) * 999.75f);
+    const uint32_t d = uint32_t(25600.0f * l);
+    out = (d / 100u & 0xFFu) | ((d % 256u) << 8) | 0xFF000000u;
+  }
+  rgba[size_t(py) * width + px] = out;
+}
+
+__global__ void rz_fill_u16_kernel(uint16_t* p, uint32_t n, uint16_t v) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+// ---------------------------------------------------------------------------
+// Launchers.
+// ---------------------------------------------------------------------------
+static inline uint32_t groups(uint32_t n, uint32_t per) { return (n + per - 1) / per; }
+
+hipError_t launch_prep(const PrepParams& p, hipStream_t stream) {
+  if (p.nItems == 0) return hipSuccess;
+  hipLaunchKernelGGL(rz_prep_kernel, dim3(groups(p.nItems, 256)), dim3(256), 0, stream, p);
+  return hipGetLastError();
+}
+
+hipError_t launch_setup(const SetupParams& p, hipStream_t stream) {
+  if (p.nQuads == 0 || p.nItems == 0) return hipSuccess;
+  hipLaunchKernelGGL(rz_setup_kernel, dim3(groups(p.nQuads, 256)), dim3(256), 0, stream, p);
+  return hipGetLastError();
+}
+
+hipError_t launch_frame(const FrameParams& p, int tileW, hipStream_t stream) {
+  if (p.nItems == 0) return hipSuccess;
+  FrameParams local = p;
+  void* args[] = {&local};
+  const dim3 grid(p.tilesX * p.tilesY), block(256);
+  if (tileW == 8)
+    return hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&rz_frame_kernel<8, 8>), grid, block, args, 0,
+                                      stream);
+  if (tileW == 16)
+    return hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&rz_frame_kernel<16, 16>), grid, block, args, 0,
+                                      stream);
+  return hipErrorInvalidValue;
+}
+
+hipError_t frame_max_groups(int tileW, int* maxGroups) {
+  int dev = 0, cus = 0, perCU = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  if (e != hipSuccess) return e;
+  if (tileW == 8)
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, reinterpret_cast<const void*>(&rz_frame_kernel<8, 8>),
+                                                     256, 0);
+  else if (tileW == 16)
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, reinterpret_cast<const void*>(&rz_frame_kernel<16, 16>),
+                                                     256, 0);
+  else
+    return hipErrorInvalidValue;
+  if (e != hipSuccess) return e;
+  *maxGroups = perCU * cus;
+  return hipSuccess;
+}
+
+hipError_t launch_status(const ItemState* state, const uint32_t* visible, uint32_t n, uint8_t* out,
+                         hipStream_t stream) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(rz_status_kernel, dim3(groups(n, 256)), dim3(256), 0, stream, state, visible, n, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_query(const QueryParams& p, hipStream_t stream) {
+  if (p.n == 0) return hipSuccess;
+  hipLaunchKernelGGL(rz_query_kernel, dim3(groups(p.n, 256)), dim3(256), 0, stream, p);
+  return hipGetLastError();
+}
+
+hipError_t launch_query2d(uint32_t minX, uint32_t maxX, uint32_t minY, uint32_t maxY, uint32_t maxZ,
+                          const uint16_t* depth, const uint16_t* hiz, uint32_t bX, uint8_t* out, hipStream_t stream) {
+  hipLaunchKernelGGL(rz_query2d_kernel, dim3(1), dim3(64), 0, stream, minX, maxX, minY, maxY, maxZ, depth, hiz, bX,
+                     out);
+  return hipGetLastError();
+}
+
+hipError_t launch_readback(const uint16_t* depth, const uint16_t* hiz, uint32_t width, uint32_t height, uint32_t bX,
+                           uint32_t* rgba, hipStream_t stream) {
+  hipLaunchKernelGGL(rz_readback_kernel, dim3(groups(width, 256), height), dim3(256), 0, stream, depth, hiz, width,
+                     height, bX, rgba);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill_u16(uint16_t* p, uint32_t n, uint16_t v, hipStream_t stream) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(rz_fill_u16_kernel, dim3(groups(n, 256)), dim3(256), 0, stream, p, n, v);
+  return hipGetLastError();
+}
+
+}  // namespace rzd
