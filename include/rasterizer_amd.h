@@ -150,6 +150,23 @@ int rz_bake(const float* vertices, uint32_t n, const float refMin[4], const floa
 int rz_timer_start(rz_context* ctx);
 int rz_timer_stop(rz_context* ctx, float* ms);
 
+/* Per-kernel timing with HIP events recorded around each launch on the
+ * context stream.  rz_profile_enable(ctx, 1) starts (and clears) the
+ * accumulation, 0 stops it.  rz_profile_read blocks and returns the summed
+ * milliseconds of the prep, setup and frame kernels of all draws and of the
+ * query-batch kernel, and the numbers of draws and query batches timed. */
+int rz_profile_enable(rz_context* ctx, int enable);
+int rz_profile_read(rz_context* ctx, double ms[4], uint32_t counts[2]);
+
+/* Device memory for callers without an allocator of their own (e.g. the
+ * inputs of rz_query_batch_device): rz_device_alloc allocates `bytes` on the
+ * context's device and copies `init` into it when non-NULL; rz_device_read
+ * copies back after the context stream has drained; rz_device_free releases
+ * (after a device-wide synchronization). */
+int rz_device_alloc(rz_context* ctx, size_t bytes, const void* init, void** devptr);
+int rz_device_read(rz_context* ctx, const void* devptr, size_t bytes, void* dst);
+int rz_device_free(void* devptr);
+
 #ifdef __cplusplus
 }
 #endif

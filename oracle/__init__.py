@@ -52,6 +52,7 @@ def lib() -> ctypes.CDLL:
             "or_rasterize": ([P, P, U, P, P, ctypes.c_int], None),
             "or_query": ([P, P, P, ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
             "or_query2d": ([P, U, U, U, U, U], ctypes.c_int),
+            "or_query_batch": ([P, U, P, P, P], None),
             "or_readback": ([P, P], None),
             "or_get_raw": ([P, P, P], None),
             "or_set_raw": ([P, P, P], None),
@@ -134,12 +135,11 @@ class OracleRasterizer:
         return bool(lib().or_query2d(self._r, min_x, max_x, min_y, max_y, max_z))
 
     def query_batch(self, bounds: np.ndarray):
-        b = np.asarray(bounds, np.float32).reshape(-1, 8)
-        vis = np.zeros(b.shape[0], bool)
-        clip = np.zeros(b.shape[0], bool)
-        for i in range(b.shape[0]):
-            vis[i], clip[i] = self.queryVisibility(b[i, :4], b[i, 4:], False)
-        return vis, clip
+        b = np.ascontiguousarray(np.asarray(bounds, np.float32).reshape(-1, 8))
+        vis = np.zeros(max(b.shape[0], 1), np.uint8)
+        clip = np.zeros(max(b.shape[0], 1), np.uint8)
+        lib().or_query_batch(self._r, b.shape[0], _ptr(b), _ptr(vis), _ptr(clip))
+        return vis[: b.shape[0]].astype(bool), clip[: b.shape[0]].astype(bool)
 
     def readBackDepth(self) -> np.ndarray:
         out = np.zeros((self.height, self.width, 4), np.uint8)

@@ -722,6 +722,14 @@ int or_query(OrRaster* r, const float bmin[4], const float bmax[4], int* needsCl
   return query_impl(r, bmin, bmax, needsClipping);
 }
 
+void or_query_batch(const OrRaster* r, uint32_t n, const float* bounds, uint8_t* visible, uint8_t* needsClipping) {
+  for (uint32_t i = 0; i < n; ++i) {
+    int clip = 0;
+    visible[i] = (uint8_t)query_impl(r, bounds + 8 * (size_t)i, bounds + 8 * (size_t)i + 4, &clip);
+    if (needsClipping) needsClipping[i] = (uint8_t)clip;
+  }
+}
+
 /* readBackDepth (Rasterizer.cpp:351-399). */
 void or_readback(const OrRaster* r, uint8_t* rgba) {
   const float bias = 3.9623753e+28f;

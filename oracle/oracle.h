@@ -49,6 +49,9 @@ void or_rasterize(OrRaster* r, const uint32_t* packets, uint32_t packetCount,
 /* queryVisibility (Rasterizer.cpp:123-281).  Returns visible; *needsClipping
  * is written only when the box survives the frustum test (as the reference). */
 int or_query(OrRaster* r, const float boundsMin[4], const float boundsMax[4], int* needsClipping);
+/* queryVisibility over n boxes (8 floats each: min xyzw, max xyzw);
+ * needsClipping is 0 for frustum-culled boxes. */
+void or_query_batch(const OrRaster* r, uint32_t n, const float* bounds, uint8_t* visible, uint8_t* needsClipping);
 /* query2D (Rasterizer.cpp:283-349). */
 int or_query2d(const OrRaster* r, uint32_t minX, uint32_t maxX, uint32_t minY, uint32_t maxY, uint32_t maxZ);
 

@@ -66,6 +66,11 @@ _SIGNATURES = {
     "rz_bake": ([_P, _U32, _P, _P, _P, _P, _P, _P], ctypes.c_int),
     "rz_timer_start": ([_P], ctypes.c_int),
     "rz_timer_stop": ([_P, ctypes.POINTER(ctypes.c_float)], ctypes.c_int),
+    "rz_profile_enable": ([_P, ctypes.c_int], ctypes.c_int),
+    "rz_profile_read": ([_P, _P, _P], ctypes.c_int),
+    "rz_device_alloc": ([_P, ctypes.c_size_t, _P, ctypes.POINTER(_P)], ctypes.c_int),
+    "rz_device_read": ([_P, _P, ctypes.c_size_t, _P], ctypes.c_int),
+    "rz_device_free": ([_P], ctypes.c_int),
 }
 
 
@@ -160,6 +165,38 @@ class Occluder:
             pass
 
 
+class DeviceBuffer:
+    """Device memory owned through the library (for device-pointer entry points)."""
+
+    def __init__(self, rast: "Rasterizer", nbytes: int, init: np.ndarray | None = None):
+        self.ptr = ctypes.c_void_p()
+        self.nbytes = int(nbytes)
+        src = None
+        if init is not None:
+            self._init = np.ascontiguousarray(init)
+            if self._init.nbytes != self.nbytes:
+                raise ValueError("init size does not match nbytes")
+            src = _ptr(self._init)
+        _check(lib().rz_device_alloc(rast.ctx, self.nbytes, src, ctypes.byref(self.ptr)), "rz_device_alloc")
+        self._rast = rast
+
+    def read(self, dtype, count: int) -> np.ndarray:
+        out = np.zeros(count, dtype)
+        _check(lib().rz_device_read(self._rast.ctx, self.ptr, out.nbytes, _ptr(out)), "rz_device_read")
+        return out
+
+    def free(self) -> None:
+        if self.ptr and _lib is not None:
+            _lib.rz_device_free(self.ptr)
+        self.ptr = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
 class Rasterizer:
     """One depth buffer + hiZ on the GPU (Rasterizer.h:10-61)."""
 
@@ -231,6 +268,18 @@ class Rasterizer:
             self.sync()
         return status[:n]
 
+    def draw_frame_handles(self, handles, n: int) -> None:
+        """rz_draw_frame on a prepared ctypes array of device handles (no status)."""
+        _check(lib().rz_draw_frame(self._ctx, n, ctypes.cast(handles, ctypes.c_void_p), None), "draw_frame")
+
+    def handle_array(self, occluders):
+        return (ctypes.c_void_p * max(len(occluders), 1))(*[o.device(self._ctx).value for o in occluders])
+
+    def query_batch_device(self, n: int, d_bounds: "DeviceBuffer", d_visible: "DeviceBuffer",
+                           d_clip: "DeviceBuffer | None" = None) -> None:
+        _check(lib().rz_query_batch_device(self._ctx, n, d_bounds.ptr, d_visible.ptr,
+                                           d_clip.ptr if d_clip is not None else None), "query_batch_device")
+
     def query_batch(self, bounds: np.ndarray):
         b = np.ascontiguousarray(np.asarray(bounds, dtype=np.float32).reshape(-1, 8))
         n = b.shape[0]
@@ -269,10 +318,20 @@ class Rasterizer:
     def timer_start(self) -> None:
         _check(lib().rz_timer_start(self._ctx), "rz_timer_start")
 
+    def profile(self, enable: bool) -> None:
+        _check(lib().rz_profile_enable(self._ctx, 1 if enable else 0), "rz_profile_enable")
+
+    def profile_read(self) -> dict:
+        ms = np.zeros(4, np.float64)
+        counts = np.zeros(2, np.uint32)
+        _check(lib().rz_profile_read(self._ctx, _ptr(ms), _ptr(counts)), "rz_profile_read")
+        return {"prep_ms": ms[0], "setup_ms": ms[1], "frame_ms": ms[2], "query_ms": ms[3],
+                "draws": int(counts[0]), "query_batches": int(counts[1])}
+
     def timer_stop(self) -> float:
         ms = ctypes.c_float(0.0)
         _check(lib().rz_timer_stop(self._ctx, ctypes.byref(ms)), "rz_timer_stop")
         return float(ms.value)
 
 
-__all__ = ["Rasterizer", "Occluder", "RasterizerError", "lib", "LIB_PATH"]
+__all__ = ["Rasterizer", "Occluder", "DeviceBuffer", "RasterizerError", "lib", "LIB_PATH"]

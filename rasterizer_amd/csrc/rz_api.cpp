@@ -18,6 +18,7 @@ using namespace rzd;
 namespace {
 
 constexpr int kStagingSlots = 8;
+constexpr int kProfSlots = 256;
 constexpr uint64_t kTimeoutTicks = 200000000ull;  // 2 s of the 100 MHz s_memrealtime clock
 
 struct Tables {
@@ -92,7 +93,14 @@ struct rz_context {
   uint8_t* pendingStatusDst = nullptr;
   uint32_t pendingStatusN = 0;
   hipEvent_t t0 = nullptr, t1 = nullptr;
-  int stickyError = RZ_OK;
+  // per-kernel profiling: a ring of event quadruples (prep, setup, frame,
+  // end) for draws and pairs for query batches
+  bool profiling = false;
+  std::vector<hipEvent_t> profEvents;  // kProfSlots * 4, created on first enable
+  int profKind[kProfSlots] = {};       // 0 draw, 1 query batch
+  int profUsed = 0;
+  double profMs[4] = {0, 0, 0, 0};
+  uint32_t profCounts[2] = {0, 0};
 };
 
 #define RZ_HIP(call)                           \
@@ -137,6 +145,7 @@ static void free_context(rz_context* c) {
     if (c->ev_items[i]) (void)hipEventDestroy(c->ev_items[i]);
   }
   if (c->h_status) (void)hipHostFree(c->h_status);
+  for (hipEvent_t e : c->profEvents) (void)hipEventDestroy(e);
   if (c->t0) (void)hipEventDestroy(c->t0);
   if (c->t1) (void)hipEventDestroy(c->t1);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -261,6 +270,37 @@ extern "C" int rz_occluder_destroy(rz_occluder* o) {
   return RZ_OK;
 }
 
+// Folds every recorded profiling slot into the running sums.
+static int profile_flush(rz_context* c) {
+  for (int i = 0; i < c->profUsed; ++i) {
+    hipEvent_t* ev = &c->profEvents[size_t(i) * 4];
+    const int last = c->profKind[i] == 0 ? 3 : 1;
+    RZ_HIP(hipEventSynchronize(ev[last]));
+    float ms = 0.0f;
+    if (c->profKind[i] == 0) {
+      for (int k = 0; k < 3; ++k) {
+        RZ_HIP(hipEventElapsedTime(&ms, ev[k], ev[k + 1]));
+        c->profMs[k] += ms;
+      }
+      c->profCounts[0]++;
+    } else {
+      RZ_HIP(hipEventElapsedTime(&ms, ev[0], ev[1]));
+      c->profMs[3] += ms;
+      c->profCounts[1]++;
+    }
+  }
+  c->profUsed = 0;
+  return RZ_OK;
+}
+
+// Next profiling slot (or nullptr when profiling is off).
+static hipEvent_t* profile_slot(rz_context* c, int kind) {
+  if (!c->profiling) return nullptr;
+  if (c->profUsed == kProfSlots && profile_flush(c) != RZ_OK) return nullptr;
+  c->profKind[c->profUsed] = kind;
+  return &c->profEvents[size_t(c->profUsed++) * 4];
+}
+
 static int ensure_capacity(rz_context* c, uint32_t nItems, uint32_t nQuads) {
   if (nItems > c->capItems || nQuads > c->capQuads) {
     RZ_HIP(hipStreamSynchronize(c->stream));
@@ -348,6 +388,8 @@ static int enqueue_draw(rz_context* c, uint32_t n, const rz_occluder* const* occ
   }
   RZ_HIP(hipMemcpyAsync(c->d_items, items, n * sizeof(ItemDesc), hipMemcpyHostToDevice, c->stream));
   RZ_HIP(hipEventRecord(c->ev_items[slot], c->stream));
+  hipEvent_t* prof = profile_slot(c, 0);
+  if (prof) RZ_HIP(hipEventRecord(prof[0], c->stream));
 
   PrepParams pp{};
   pp.items = c->d_items;
@@ -362,6 +404,7 @@ static int enqueue_draw(rz_context* c, uint32_t n, const rz_occluder* const* occ
   pp.visible = c->d_visible;
   pp.mvp = c->mvp;
   RZ_HIP(launch_prep(pp, c->stream));
+  if (prof) RZ_HIP(hipEventRecord(prof[1], c->stream));
 
   SetupParams sp{};
   sp.items = c->d_items;
@@ -374,6 +417,7 @@ static int enqueue_draw(rz_context* c, uint32_t n, const rz_occluder* const* occ
   sp.recs = c->d_recs;
   sp.bbox = c->d_bbox;
   RZ_HIP(launch_setup(sp, c->stream));
+  if (prof) RZ_HIP(hipEventRecord(prof[2], c->stream));
 
   FrameParams fp{};
   fp.items = c->d_items;
@@ -393,6 +437,7 @@ static int enqueue_draw(rz_context* c, uint32_t n, const rz_occluder* const* occ
   fp.err = c->d_err;
   fp.timeoutTicks = kTimeoutTicks;
   RZ_HIP(launch_frame(fp, c->tileW, c->stream));
+  if (prof) RZ_HIP(hipEventRecord(prof[3], c->stream));
 
   if (statusOut) {
     if (c->h_statusCap < n) {
@@ -514,7 +559,10 @@ extern "C" int rz_query_batch(rz_context* ctx, uint32_t n, const float* bounds, 
   if (s != RZ_OK) return s;
   RZ_HIP(hipMemcpyAsync(ctx->d_qbounds, bounds, size_t(n) * 32, hipMemcpyHostToDevice, ctx->stream));
   QueryParams q = query_params(ctx, ctx->d_qbounds, n, ctx->d_qvis, ctx->d_qclip, nullptr);
+  hipEvent_t* prof = profile_slot(ctx, 1);
+  if (prof) RZ_HIP(hipEventRecord(prof[0], ctx->stream));
   RZ_HIP(launch_query(q, ctx->stream));
+  if (prof) RZ_HIP(hipEventRecord(prof[1], ctx->stream));
   RZ_HIP(hipMemcpyAsync(visible, ctx->d_qvis, n, hipMemcpyDeviceToHost, ctx->stream));
   if (needsClipping) RZ_HIP(hipMemcpyAsync(needsClipping, ctx->d_qclip, n, hipMemcpyDeviceToHost, ctx->stream));
   return rz_sync(ctx);
@@ -524,7 +572,10 @@ extern "C" int rz_query_batch_device(rz_context* ctx, uint32_t n, const float* d
                                      uint8_t* d_needsClipping) {
   if (!ctx || (n && (!d_bounds || !d_visible))) return RZ_ERR_INVALID_ARGUMENT;
   QueryParams q = query_params(ctx, d_bounds, n, d_visible, d_needsClipping, nullptr);
+  hipEvent_t* prof = profile_slot(ctx, 1);
+  if (prof) RZ_HIP(hipEventRecord(prof[0], ctx->stream));
   RZ_HIP(launch_query(q, ctx->stream));
+  if (prof) RZ_HIP(hipEventRecord(prof[1], ctx->stream));
   return RZ_OK;
 }
 
@@ -603,5 +654,57 @@ extern "C" int rz_timer_stop(rz_context* ctx, float* ms) {
   RZ_HIP(hipEventRecord(ctx->t1, ctx->stream));
   RZ_HIP(hipEventSynchronize(ctx->t1));
   RZ_HIP(hipEventElapsedTime(ms, ctx->t0, ctx->t1));
+  return RZ_OK;
+}
+
+extern "C" int rz_profile_enable(rz_context* ctx, int enable) {
+  if (!ctx) return RZ_ERR_INVALID_ARGUMENT;
+  if (enable && ctx->profEvents.empty()) {
+    ctx->profEvents.resize(size_t(kProfSlots) * 4, nullptr);
+    for (hipEvent_t& e : ctx->profEvents) RZ_HIP(hipEventCreate(&e));
+  }
+  if (enable) {
+    int s = profile_flush(ctx);
+    if (s != RZ_OK) return s;
+    std::fill(ctx->profMs, ctx->profMs + 4, 0.0);
+    ctx->profCounts[0] = ctx->profCounts[1] = 0;
+  }
+  ctx->profiling = enable != 0;
+  return RZ_OK;
+}
+
+extern "C" int rz_profile_read(rz_context* ctx, double ms[4], uint32_t counts[2]) {
+  if (!ctx || !ms || !counts) return RZ_ERR_INVALID_ARGUMENT;
+  int s = profile_flush(ctx);
+  if (s != RZ_OK) return s;
+  std::copy(ctx->profMs, ctx->profMs + 4, ms);
+  counts[0] = ctx->profCounts[0];
+  counts[1] = ctx->profCounts[1];
+  return RZ_OK;
+}
+
+extern "C" int rz_device_alloc(rz_context* ctx, size_t bytes, const void* init, void** devptr) {
+  if (!ctx || !devptr || bytes == 0) return RZ_ERR_INVALID_ARGUMENT;
+  *devptr = nullptr;
+  if (hipMalloc(devptr, bytes) != hipSuccess) return RZ_ERR_OUT_OF_MEMORY;
+  const hipError_t e = init ? hipMemcpy(*devptr, init, bytes, hipMemcpyHostToDevice) : hipMemset(*devptr, 0, bytes);
+  if (e != hipSuccess) {
+    hipFree(*devptr);
+    *devptr = nullptr;
+    return RZ_ERR_DEVICE;
+  }
+  return RZ_OK;
+}
+
+extern "C" int rz_device_read(rz_context* ctx, const void* devptr, size_t bytes, void* dst) {
+  if (!ctx || !devptr || !dst) return RZ_ERR_INVALID_ARGUMENT;
+  RZ_HIP(hipMemcpyAsync(dst, devptr, bytes, hipMemcpyDeviceToHost, ctx->stream));
+  return rz_sync(ctx);
+}
+
+extern "C" int rz_device_free(void* devptr) {
+  if (!devptr) return RZ_ERR_INVALID_ARGUMENT;
+  RZ_HIP(hipDeviceSynchronize());
+  RZ_HIP(hipFree(devptr));
   return RZ_OK;
 }

@@ -247,6 +247,174 @@ def config1(seed: int = 1, n_boxes: int = 1024, n_queries: int = 4096):
     return scene
 
 
+def _mix64(h: np.ndarray) -> np.ndarray:
+    """SplitMix64 finalizer on a uint64 array (wrapping arithmetic)."""
+    with np.errstate(over="ignore"):
+        h = (h ^ (h >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        h = (h ^ (h >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        return h ^ (h >> np.uint64(31))
+
+
+def _lattice(ix: np.ndarray, iz: np.ndarray, seed: int) -> np.ndarray:
+    with np.errstate(over="ignore"):
+        h = (ix.astype(np.int64).astype(np.uint64) * np.uint64(0x9E3779B97F4A7C15)) ^ \
+            (iz.astype(np.int64).astype(np.uint64) * np.uint64(0xC2B2AE3D27D4EB4F)) ^ np.uint64(seed & _M64)
+    return (_mix64(h) >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+
+
+def fbm(x: np.ndarray, z: np.ndarray, seed: int, octaves: int = 4, base_period: float = 256.0) -> np.ndarray:
+    """Fractal sum of smoothstep-interpolated value noise in [-1, 1]."""
+    total = np.zeros(np.broadcast(x, z).shape)
+    amp, norm, freq = 1.0, 0.0, 1.0 / base_period
+    for o in range(octaves):
+        fx, fz = x * freq, z * freq
+        ix, iz = np.floor(fx), np.floor(fz)
+        tx, tz = fx - ix, fz - iz
+        sx, sz = tx * tx * (3 - 2 * tx), tz * tz * (3 - 2 * tz)
+        s = seed + 7919 * o
+        v00, v10 = _lattice(ix, iz, s), _lattice(ix + 1, iz, s)
+        v01, v11 = _lattice(ix, iz + 1, s), _lattice(ix + 1, iz + 1, s)
+        v = (v00 * (1 - sx) + v10 * sx) * (1 - sz) + (v01 * (1 - sx) + v11 * sx) * sz
+        total += amp * (2.0 * v - 1.0)
+        norm += amp
+        amp *= 0.5
+        freq *= 2.0
+    return total / norm
+
+
+class Heightfield:
+    """n x n vertices at `spacing`, y = amplitude * fbm(x, z)."""
+
+    def __init__(self, n: int, spacing: float, amplitude: float, seed: int):
+        self.n, self.spacing = n, spacing
+        g = np.arange(n) * spacing
+        X, Z = np.meshgrid(g, g, indexing="ij")
+        self.h = (amplitude * fbm(X, Z, seed)).astype(np.float32)  # h[i, j] at x = i*s, z = j*s
+
+    def height(self, x, z):
+        """Bilinear height at world (x, z), clamped to the grid."""
+        s, n = self.spacing, self.n
+        fx = np.clip(np.asarray(x, np.float64) / s, 0, n - 1 - 1e-9)
+        fz = np.clip(np.asarray(z, np.float64) / s, 0, n - 1 - 1e-9)
+        i, j = np.floor(fx).astype(int), np.floor(fz).astype(int)
+        tx, tz = fx - i, fz - j
+        h = self.h.astype(np.float64)
+        return (h[i, j] * (1 - tx) + h[i + 1, j] * tx) * (1 - tz) + (h[i, j + 1] * (1 - tx) + h[i + 1, j + 1] * tx) * tz
+
+    def quads(self) -> np.ndarray:
+        """Triangulated grid (cell diagonal (i,j)-(i+1,j+1)) paired back into
+        quads where both triangles lie within 0.5 units of each other's plane
+        (the merge test of QuadDecomposition.cpp:326-343); other triangles
+        become degenerate quads (a, b, c, a).  Up-facing surfaces get positive
+        screen-space area."""
+        n, s = self.n, self.spacing
+        g = np.arange(n, dtype=np.float64) * s
+        P = np.zeros((n, n, 4))
+        P[..., 0] = g[:, None]
+        P[..., 2] = g[None, :]
+        P[..., 1] = self.h
+        P[..., 3] = 1.0
+        p00, p10 = P[:-1, :-1].reshape(-1, 4), P[1:, :-1].reshape(-1, 4)
+        p11, p01 = P[1:, 1:].reshape(-1, 4), P[:-1, 1:].reshape(-1, 4)
+
+        def unit_normal(a, b, c):
+            v = np.cross(b[:, :3] - a[:, :3], c[:, :3] - a[:, :3])
+            return v / np.linalg.norm(v, axis=1, keepdims=True)
+
+        n0 = unit_normal(p00, p10, p11)
+        n2 = unit_normal(p11, p01, p00)
+        diag = p10[:, :3] - p01[:, :3]
+        planar = (np.abs((n0 * diag).sum(1)) <= 0.5) & (np.abs((n2 * diag).sum(1)) <= 0.5)
+        out = []
+        quad = np.stack([p00, p10, p11, p01], axis=1)
+        out.append(quad[planar])
+        np_ = ~planar
+        out.append(np.stack([p00[np_], p10[np_], p11[np_], p00[np_]], axis=1))
+        out.append(np.stack([p00[np_], p11[np_], p01[np_], p00[np_]], axis=1))
+        return np.concatenate(out, axis=0).astype(np.float32)
+
+
+def catmull_rom(points: np.ndarray, samples: int):
+    """Uniform Catmull-Rom through points[1..-2]; returns positions and tangents."""
+    P = np.asarray(points, np.float64)
+    segs = P.shape[0] - 3
+    t = np.linspace(0.0, segs, samples, endpoint=False)
+    k = np.minimum(t.astype(int), segs - 1)
+    u = (t - k)[:, None]
+    p0, p1, p2, p3 = P[k], P[k + 1], P[k + 2], P[k + 3]
+    pos = 0.5 * ((2 * p1) + (-p0 + p2) * u + (2 * p0 - 5 * p1 + 4 * p2 - p3) * u ** 2 + (-p0 + 3 * p1 - 3 * p2 + p3) * u ** 3)
+    tan = 0.5 * ((-p0 + p2) + 2 * (2 * p0 - 5 * p1 + 4 * p2 - p3) * u + 3 * (-p0 + 3 * p1 - 3 * p2 + p3) * u ** 2)
+    return pos, tan
+
+
+def config2(seed: int = 1, frames: int = 256, n_queries: int = 65536, grid: int = 64, terrain_n: int = 512,
+            width: int = 1920, height: int = 1080):
+    """Procedural city on terrain: grid x grid box buildings (footprint 8x8 on
+    a 16-unit pitch, heights log-U[5,80]) on a terrain_n^2 fBm heightfield
+    (amplitude 20, 4 octaves, 2-unit spacing), ~286k quads at the defaults; a
+    seeded Catmull-Rom path along the streets at eye height 2 (near plane 1,
+    so terrain quads cross it); query boxes are the building bounds plus
+    small props on the terrain."""
+    rng = SplitMix64(seed)
+    spacing = 2.0
+    hf = Heightfield(terrain_n, spacing, 20.0, seed=(seed * 0x2545F4914F6CDD1D) & _M64)
+    extent = (terrain_n - 1) * spacing
+    pitch = extent / grid
+    heights = rng.log_uniform(grid * grid, 5.0, 80.0)
+    bq, bb = [], []
+    for i in range(grid):
+        for j in range(grid):
+            cx, cz = (i + 0.5) * pitch, (j + 0.5) * pitch
+            xs = np.array([cx - 4, cx + 4, cx - 4, cx + 4, cx])
+            zs = np.array([cz - 4, cz - 4, cz + 4, cz + 4, cz])
+            base = float(hf.height(xs, zs).min()) - 1.0
+            hgt = float(heights[i * grid + j])
+            c = np.array([cx, base + 0.5 * hgt, cz])
+            h = np.array([4.0, 0.5 * hgt, 4.0])
+            bq.append(box_quads(c, h))
+            bb.append((c, h))
+    terrain = hf.quads()
+    quads = np.concatenate([terrain] + bq, axis=0)
+    scene = bake_scene(quads)
+
+    # Query boxes: building bounds, then props on the terrain.
+    nb = len(bb)
+    n_props = max(n_queries - nb, 0)
+    px = rng.uniform_range(n_props, 0.0, extent)
+    pz = rng.uniform_range(n_props, 0.0, extent)
+    ph = np.stack([rng.log_uniform(n_props, 0.25, 1.5) for _ in range(3)], axis=1)
+    py = (hf.height(px, pz) + ph[:, 1]).astype(np.float32)
+    centers = np.concatenate([np.array([c for c, _ in bb], np.float32), np.stack([px, py, pz], axis=1)])
+    halves = np.concatenate([np.array([h for _, h in bb], np.float32), ph])
+    queries = aabb_bounds(centers, halves)[:n_queries]
+
+    # Camera path: a walk over street intersections, smoothed.
+    pts = []
+    ci, cj = 1 + int(rng.uniform(1)[0] * (grid - 2)), 1 + int(rng.uniform(1)[0] * (grid - 2))
+    for step in range(12):
+        pts.append((ci * pitch, cj * pitch))
+        d = 2 + int(rng.uniform(1)[0] * 5)
+        sgn = 1 if rng.uniform(1)[0] < 0.5 else -1
+        if step % 2 == 0:
+            ci = int(np.clip(ci + sgn * d, 1, grid - 1))
+        else:
+            cj = int(np.clip(cj + sgn * d, 1, grid - 1))
+    pos2, tan2 = catmull_rom(np.array(pts), frames)
+    eyes = np.zeros((frames, 3), np.float32)
+    eyes[:, 0], eyes[:, 2] = pos2[:, 0], pos2[:, 1]
+    eyes[:, 1] = hf.height(pos2[:, 0], pos2[:, 1]) + 2.0
+    mvps = np.zeros((frames, 16), np.float32)
+    for f in range(frames):
+        d = np.array([tan2[f, 0], 0.0, tan2[f, 1]])
+        nrm = np.linalg.norm(d)
+        d = d / nrm if nrm > 1e-9 else np.array([1.0, 0.0, 0.0])
+        d[1] = -0.08
+        mvps[f] = view_projection(eyes[f], d, (0, 1, 0), math.pi / 3, width / height)
+    scene.meta.update(width=width, height=height, eyes=eyes, mvps=mvps, queries=queries, seed=seed,
+                      terrain_quads=int(terrain.shape[0]), building_quads=int(6 * nb))
+    return scene
+
+
 def front_to_back(occluders, eye) -> np.ndarray:
     """Draw order of Main.cpp:184-190: ascending squared distance from the eye
     to each occluder's centre (float32, DPPS order), ties by index."""

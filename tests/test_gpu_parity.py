@@ -76,3 +76,36 @@ def test_config1_rasterize_all(oracle_mod, scene1, report, clip):
         g.rasterize(occ, clip)
         o.rasterize(occ, clip)
     _assert_same_buffers(g, o, report, f"rasterize<{clip}> all")
+
+
+@pytest.fixture(scope="module")
+def scene2():
+    return scenes.config2(seed=1)
+
+
+def test_config2_camera_path(oracle_mod, scene2, report):
+    """All 256 frames of the city path on one reused buffer (stale depth
+    between frames, as the reference's clear leaves it), 1920x1080."""
+    m = scene2.meta
+    g, o = _pair(oracle_mod, m["width"], m["height"], m["mvps"][0])
+    nf = m["mvps"].shape[0]
+    drawn = clipped = 0
+    for f in range(nf):
+        for r in (g, o):
+            r.clear()
+            r.setModelViewProjection(m["mvps"][f])
+        occ = [scene2.occluders[i] for i in scenes.front_to_back(scene2.occluders, m["eyes"][f])]
+        st_o = o.draw_frame(occ)
+        st_g = g.draw_frame(occ)
+        assert np.array_equal(st_g, st_o), f"frame {f}: statuses differ at {np.nonzero(st_g != st_o)[0][:10]}"
+        drawn += int((st_o > 0).sum())
+        clipped += int((st_o == 2).sum())
+        _assert_same_buffers(g, o, report, f"config2 frame {f}")
+        if f % 64 == 63:
+            vg, cg = g.query_batch(m["queries"])
+            vo, co = o.query_batch(m["queries"])
+            assert np.array_equal(vg, vo), f"frame {f}: {int((vg != vo).sum())} query bits differ"
+            assert np.array_equal(cg, co), f"frame {f}: needsClipping bits differ"
+            assert np.array_equal(g.readBackDepth(), o.readBackDepth()), f"frame {f}: readback differs"
+    print(f"config2: {nf} frames, {drawn} occluder draws, {clipped} with rasterize<true>", flush=True)
+    assert clipped > 0, "the path should exercise the near-clipped raster path"
