@@ -158,6 +158,18 @@ int rz_timer_stop(rz_context* ctx, float* ms);
 int rz_profile_enable(rz_context* ctx, int enable);
 int rz_profile_read(rz_context* ctx, double ms[4], uint32_t counts[2]);
 
+/* Per-tile counters of the frame kernel for the most recent draw
+ * (observability).  rz_frame_stats_enable(ctx, 1) switches draws to the
+ * counting variant of the kernel.  rz_frame_stats_read blocks and copies
+ * tilesX*tilesY records of 16 u64 into out (when out is non-NULL and *count
+ * is large enough) and sets *count to the number of records: total ticks,
+ * ticks waiting on other tiles, waits, ticks in local query2D work, query2D
+ * contributions, ticks drawing (scan + draw), quads drawn, items touching the
+ * tile, scan ticks, scan passes, tile load ticks, item-loop ticks, draw
+ * ticks, 3 reserved (ticks of the 100 MHz s_memrealtime clock). */
+int rz_frame_stats_enable(rz_context* ctx, int enable);
+int rz_frame_stats_read(rz_context* ctx, uint64_t* out, uint32_t* count);
+
 /* Device memory for callers without an allocator of their own (e.g. the
  * inputs of rz_query_batch_device): rz_device_alloc allocates `bytes` on the
  * context's device and copies `init` into it when non-NULL; rz_device_read

@@ -9,13 +9,16 @@ imports it.
 from __future__ import annotations
 
 import ctypes
+import os
 import subprocess
 from pathlib import Path
 
 import numpy as np
 
 HERE = Path(__file__).resolve().parent
-LIB_PATH = HERE / "_build" / "liboracle.so"
+# RZ_ORACLE_LIB points the checker at another build of oracle.c (used to
+# confirm that the pinning tests fail on a deliberately altered oracle).
+LIB_PATH = Path(os.environ["RZ_ORACLE_LIB"]) if os.environ.get("RZ_ORACLE_LIB") else HERE / "_build" / "liboracle.so"
 
 _lib = None
 
@@ -34,7 +37,7 @@ class _Occ(ctypes.Structure):
 def build() -> Path:
     """Compiles the oracle with its Makefile (gcc, -ffp-contract=off)."""
     subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
-    return LIB_PATH
+    return HERE / "_build" / "liboracle.so"
 
 
 def lib() -> ctypes.CDLL:
@@ -60,6 +63,7 @@ def lib() -> ctypes.CDLL:
             "or_get_mvp": ([P, P, P], None),
             "or_get_stats": ([P, P], None),
             "or_reset_stats": ([P], None),
+            "or_block_stats": ([P, P], None),
             "or_frame": ([P, U, P, P], None),
             "or_bake": ([P, U, P, P, P, P, P, P], ctypes.c_int),
             "or_x86_rcp": ([ctypes.c_float], ctypes.c_float),
@@ -176,6 +180,16 @@ class OracleRasterizer:
 
     def reset_stats(self) -> None:
         lib().or_reset_stats(self._r)
+
+    def block_stats(self, enable_only: bool = False):
+        """Per-block (visited, written) counts since the last reset, shape
+        (blocksY, blocksX, 2); the first call switches counting on."""
+        if enable_only:
+            lib().or_block_stats(self._r, None)
+            return None
+        out = np.zeros((self.blocks_y, self.blocks_x, 2), np.uint32)
+        lib().or_block_stats(self._r, _ptr(out))
+        return out
 
     def draw_frame(self, occluders) -> np.ndarray:
         arr, keep = occluder_structs(occluders)

@@ -98,6 +98,7 @@ struct OrRaster {
   uint16_t* depth;    /* blocksX*blocksY*64 */
   uint16_t* hiz;      /* blocksX*blocksY + 8 */
   uint64_t stats[7];
+  uint32_t* blockVisits; /* optional per-block counters: [2*b] visited, [2*b+1] written */
 };
 
 /* transposeMask (Rasterizer.cpp:527-545): pixel (x, y), sampled as bit
@@ -175,6 +176,7 @@ OrRaster* or_create(uint32_t width, uint32_t height) {
 
 void or_destroy(OrRaster* r) {
   if (!r) return;
+  free(r->blockVisits);
   free(r->depth);
   free(r->hiz);
   free(r);
@@ -202,7 +204,9 @@ void or_set_mvp(OrRaster* r, const float m[16]) {
 
 /* clear (Rasterizer.cpp:107-121). */
 void or_clear(OrRaster* r) {
-  size_t n = (size_t)r->blocksX * r->blocksY + 8;
+  /* Whole 8-entry stores over size / 8 groups (Rasterizer.cpp:115-120): the
+   * last (blocks + 8) % 8 padding entries keep their value. */
+  size_t n = (((size_t)r->blocksX * r->blocksY + 8) / 8) * 8;
   for (size_t i = 0; i < n; ++i) r->hiz[i] = 1;
 }
 
@@ -477,6 +481,7 @@ static void traverse_quad(OrRaster* r, const QuadSetup* Q) {
       uint16_t* out = &r->depth[(size_t)blk * 64];
       uint16_t hz = *hizp;
       r->stats[0]++;
+      if (r->blockVisits) r->blockVisits[2 * blk]++;
       uint64_t mask = 0;
       int skip = 0;
       if (hz >= Q->maxZ) {
@@ -552,6 +557,7 @@ static void traverse_quad(OrRaster* r, const QuadSetup* Q) {
           }
         *hizp = mn;
         r->stats[4]++;
+        if (r->blockVisits) r->blockVisits[2 * blk + 1]++;
       }
       for (int k = 0; k < 8; ++k) depth[k] = dDx + depth[k];
       for (int e = 0; e < 4; ++e) offset[e] = Q->nx[e] + offset[e];
@@ -559,6 +565,26 @@ static void traverse_quad(OrRaster* r, const QuadSetup* Q) {
     for (int k = 0; k < 8; ++k) lineDepth[k] = lineDepth[k] + dDy;
     for (int e = 0; e < 4; ++e) lineOffset[e] = lineOffset[e] + Q->ny[e];
   }
+}
+
+int or_setup_dump(const OrRaster* r, const uint32_t* packets, uint32_t packetCount, const float refMin[4],
+                  const float refMax[4], int clip, float* out) {
+  Prologue P;
+  prologue(r, refMin, refMax, &P);
+  int n = 0;
+  for (uint32_t p = 0; p + 4 <= packetCount; p += 4)
+    for (int lane = 0; lane < 8; ++lane) {
+      uint32_t I[4];
+      for (int k = 0; k < 4; ++k) I[k] = packets[(size_t)(p + k) * 8 + lane];
+      QuadSetup Q;
+      setup_quad(r, &P, I, clip ? 1 : 0, &Q);
+      float* o = out + 24 * n++;
+      o[0] = (float)Q.valid; o[1] = (float)Q.mode; o[2] = (float)Q.minX; o[3] = (float)Q.minY;
+      o[4] = (float)Q.rangeX; o[5] = (float)Q.rangeY; o[6] = (float)Q.maxZ; o[7] = Q.dp0;
+      for (int e = 0; e < 4; ++e) { o[8 + e] = Q.off[e]; o[12 + e] = Q.nx[e]; o[16 + e] = Q.ny[e]; }
+      o[20] = Q.dpDx; o[21] = Q.dpDy; o[22] = (float)Q.firstBlock; o[23] = 0;
+    }
+  return n;
 }
 
 void or_rasterize(OrRaster* r, const uint32_t* packets, uint32_t packetCount, const float refMin[4],
@@ -772,7 +798,16 @@ void or_get_mvp(const OrRaster* r, float prebaked[16], float raw[16]) {
   if (raw) memcpy(raw, r->mvpRaw, 64);
 }
 void or_get_stats(const OrRaster* r, uint64_t s[7]) { memcpy(s, r->stats, sizeof r->stats); }
-void or_reset_stats(OrRaster* r) { memset(r->stats, 0, sizeof r->stats); }
+void or_reset_stats(OrRaster* r) {
+  memset(r->stats, 0, sizeof r->stats);
+  if (r->blockVisits) memset(r->blockVisits, 0, sizeof(uint32_t) * 2 * r->blocksX * r->blocksY);
+}
+
+void or_block_stats(OrRaster* r, uint32_t* out) {
+  size_t nb = (size_t)r->blocksX * r->blocksY;
+  if (!r->blockVisits) r->blockVisits = (uint32_t*)calloc(2 * nb, sizeof(uint32_t));
+  if (out && r->blockVisits) memcpy(out, r->blockVisits, sizeof(uint32_t) * 2 * nb);
+}
 
 /* Main.cpp:192-206. */
 void or_frame(OrRaster* r, uint32_t n, const OrOccluder* occ, uint8_t* status) {

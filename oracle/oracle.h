@@ -46,6 +46,12 @@ void or_clear(OrRaster* r);
 void or_rasterize(OrRaster* r, const uint32_t* packets, uint32_t packetCount,
                   const float refMin[4], const float refMax[4], int possiblyNearClipped);
 
+/* Debug: per-quad setup of rasterize<clip> (24 floats per quad: valid, mode,
+ * minX, minY, rangeX, rangeY, maxZ, dp0, off[4], nx[4], ny[4], dDx, dDy,
+ * firstBlock, 0).  Returns the number of quads. */
+int or_setup_dump(const OrRaster* r, const uint32_t* packets, uint32_t packetCount, const float refMin[4],
+                  const float refMax[4], int clip, float* out);
+
 /* queryVisibility (Rasterizer.cpp:123-281).  Returns visible; *needsClipping
  * is written only when the box survives the frustum test (as the reference). */
 int or_query(OrRaster* r, const float boundsMin[4], const float boundsMax[4], int* needsClipping);
@@ -71,6 +77,9 @@ void or_get_mvp(const OrRaster* r, float prebaked[16], float raw[16]);
  * [5] quads set up, [6] quads valid after cull. */
 void or_get_stats(const OrRaster* r, uint64_t stats[7]);
 void or_reset_stats(OrRaster* r);
+/* Per-block counters (2 u32 per block: visited, written); the first call
+ * enables counting and may pass out = NULL. */
+void or_block_stats(OrRaster* r, uint32_t* out);
 
 /* One occluder of a frame: its baked packets, ref box and bounds. */
 typedef struct OrOccluder {

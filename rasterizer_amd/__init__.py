@@ -68,6 +68,8 @@ _SIGNATURES = {
     "rz_timer_stop": ([_P, ctypes.POINTER(ctypes.c_float)], ctypes.c_int),
     "rz_profile_enable": ([_P, ctypes.c_int], ctypes.c_int),
     "rz_profile_read": ([_P, _P, _P], ctypes.c_int),
+    "rz_frame_stats_enable": ([_P, ctypes.c_int], ctypes.c_int),
+    "rz_frame_stats_read": ([_P, _P, ctypes.POINTER(_U32)], ctypes.c_int),
     "rz_device_alloc": ([_P, ctypes.c_size_t, _P, ctypes.POINTER(_P)], ctypes.c_int),
     "rz_device_read": ([_P, _P, ctypes.c_size_t, _P], ctypes.c_int),
     "rz_device_free": ([_P], ctypes.c_int),
@@ -327,6 +329,19 @@ class Rasterizer:
         _check(lib().rz_profile_read(self._ctx, _ptr(ms), _ptr(counts)), "rz_profile_read")
         return {"prep_ms": ms[0], "setup_ms": ms[1], "frame_ms": ms[2], "query_ms": ms[3],
                 "draws": int(counts[0]), "query_batches": int(counts[1])}
+
+    def frame_stats(self, enable: bool) -> None:
+        _check(lib().rz_frame_stats_enable(self._ctx, 1 if enable else 0), "rz_frame_stats_enable")
+
+    def frame_stats_read(self) -> np.ndarray:
+        """Per-tile counters of the last draw, shape (tiles, 16): total ticks, wait
+        ticks, waits, query ticks, queries, raster ticks, quads, items, scan
+        ticks, scan passes, load ticks, item-loop ticks, draw ticks, 3 spare."""
+        n = _U32(0)
+        _check(lib().rz_frame_stats_read(self._ctx, None, ctypes.byref(n)), "rz_frame_stats_read")
+        out = np.zeros((max(n.value, 1), 16), np.uint64)
+        _check(lib().rz_frame_stats_read(self._ctx, _ptr(out), ctypes.byref(n)), "rz_frame_stats_read")
+        return out[: n.value]
 
     def timer_stop(self) -> float:
         ms = ctypes.c_float(0.0)

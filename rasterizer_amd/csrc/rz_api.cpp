@@ -63,6 +63,8 @@ struct rz_context {
   hipStream_t stream = nullptr;
   uint16_t* d_depth = nullptr;
   uint16_t* d_hiz = nullptr;
+  uint16_t* d_tileMin = nullptr;  // min hiZ per 8x8-block tile, kept equal to hiZ's
+  uint32_t pyrX = 0, pyrY = 0;
   uint64_t* d_cov = nullptr;
   uint32_t* d_rcp = nullptr;
   uint32_t* d_err = nullptr;
@@ -76,6 +78,7 @@ struct rz_context {
   uint8_t* d_status = nullptr;
   QuadRec* d_recs = nullptr;
   uint64_t* d_bbox = nullptr;
+  uint64_t* d_groupBox = nullptr;
   // host staging ring for item tables
   ItemDesc* h_items[kStagingSlots] = {};
   uint32_t h_itemsCap[kStagingSlots] = {};
@@ -101,6 +104,7 @@ struct rz_context {
   int profUsed = 0;
   double profMs[4] = {0, 0, 0, 0};
   uint32_t profCounts[2] = {0, 0};
+  uint64_t* d_frameStats = nullptr;  // per-tile frame-kernel counters when enabled
 };
 
 #define RZ_HIP(call)                           \
@@ -126,6 +130,7 @@ static void free_context(rz_context* c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   hipFree(c->d_depth);
   hipFree(c->d_hiz);
+  hipFree(c->d_tileMin);
   hipFree(c->d_cov);
   hipFree(c->d_rcp);
   hipFree(c->d_err);
@@ -136,10 +141,12 @@ static void free_context(rz_context* c) {
   hipFree(c->d_status);
   hipFree(c->d_recs);
   hipFree(c->d_bbox);
+  hipFree(c->d_groupBox);
   hipFree(c->d_qbounds);
   hipFree(c->d_qvis);
   hipFree(c->d_qclip);
   hipFree(c->d_rgba);
+  hipFree(c->d_frameStats);
   for (int i = 0; i < kStagingSlots; ++i) {
     if (c->h_items[i]) (void)hipHostFree(c->h_items[i]);
     if (c->ev_items[i]) (void)hipEventDestroy(c->ev_items[i]);
@@ -176,12 +183,16 @@ extern "C" int rz_create(uint32_t width, uint32_t height, rz_context** out) {
   if (hipGetDevice(&c->device) != hipSuccess) return fail(RZ_ERR_DEVICE);
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return fail(RZ_ERR_DEVICE);
   const size_t nBlocks = size_t(c->bX) * c->bY;
+  c->pyrX = (c->bX + 7) / 8;
+  c->pyrY = (c->bY + 7) / 8;
   if (dev_alloc(&c->d_depth, nBlocks * 64) != hipSuccess || dev_alloc(&c->d_hiz, nBlocks + 8) != hipSuccess ||
+      dev_alloc(&c->d_tileMin, size_t(c->pyrX) * c->pyrY) != hipSuccess ||
       dev_alloc(&c->d_cov, 4096) != hipSuccess || dev_alloc(&c->d_rcp, 4096) != hipSuccess ||
       dev_alloc(&c->d_err, 1) != hipSuccess)
     return fail(RZ_ERR_OUT_OF_MEMORY);
   if (hipMemset(c->d_depth, 0, nBlocks * 64 * sizeof(uint16_t)) != hipSuccess ||
       hipMemset(c->d_hiz, 0, (nBlocks + 8) * sizeof(uint16_t)) != hipSuccess ||
+      hipMemset(c->d_tileMin, 0, size_t(c->pyrX) * c->pyrY * sizeof(uint16_t)) != hipSuccess ||
       hipMemset(c->d_err, 0, sizeof(uint32_t)) != hipSuccess ||
       hipMemcpy(c->d_cov, tabs.cov, sizeof tabs.cov, hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(c->d_rcp, tabs.rcp, sizeof tabs.rcp, hipMemcpyHostToDevice) != hipSuccess)
@@ -226,7 +237,10 @@ extern "C" int rz_set_mvp(rz_context* ctx, const float matrix[16]) {
 
 extern "C" int rz_clear(rz_context* ctx) {
   if (!ctx) return RZ_ERR_INVALID_ARGUMENT;
-  RZ_HIP(launch_fill_u16(ctx->d_hiz, ctx->bX * ctx->bY + 8, 1, ctx->stream));
+  // Whole 8-entry groups only (Rasterizer.cpp:115-120): the last
+  // (blocks + 8) % 8 padding entries keep their value.
+  RZ_HIP(launch_fill_u16(ctx->d_hiz, ((ctx->bX * ctx->bY + 8) / 8) * 8, 1, ctx->stream));
+  RZ_HIP(launch_fill_u16(ctx->d_tileMin, ctx->pyrX * ctx->pyrY, 1, ctx->stream));
   return RZ_OK;
 }
 
@@ -323,8 +337,10 @@ static int ensure_capacity(rz_context* c, uint32_t nItems, uint32_t nQuads) {
     const uint32_t cap = std::max<uint32_t>(nQuads, std::max<uint32_t>(1024, c->capQuads + c->capQuads / 2));
     hipFree(c->d_recs);
     hipFree(c->d_bbox);
+    hipFree(c->d_groupBox);
     c->capQuads = 0;
-    if (dev_alloc(&c->d_recs, cap) != hipSuccess || dev_alloc(&c->d_bbox, cap) != hipSuccess)
+    if (dev_alloc(&c->d_recs, cap) != hipSuccess || dev_alloc(&c->d_bbox, cap) != hipSuccess ||
+        dev_alloc(&c->d_groupBox, (cap + 63) / 64) != hipSuccess)
       return RZ_ERR_OUT_OF_MEMORY;
     c->capQuads = cap;
   }
@@ -416,6 +432,7 @@ static int enqueue_draw(rz_context* c, uint32_t n, const rz_occluder* const* occ
   sp.rcp = c->d_rcp;
   sp.recs = c->d_recs;
   sp.bbox = c->d_bbox;
+  sp.groupBox = c->d_groupBox;
   RZ_HIP(launch_setup(sp, c->stream));
   if (prof) RZ_HIP(hipEventRecord(prof[2], c->stream));
 
@@ -429,6 +446,7 @@ static int enqueue_draw(rz_context* c, uint32_t n, const rz_occluder* const* occ
   fp.tilesY = uint32_t(c->tilesY);
   fp.recs = c->d_recs;
   fp.bbox = c->d_bbox;
+  fp.groupBox = c->d_groupBox;
   fp.done = c->d_done;
   fp.visible = c->d_visible;
   fp.depth = c->d_depth;
@@ -436,6 +454,9 @@ static int enqueue_draw(rz_context* c, uint32_t n, const rz_occluder* const* occ
   fp.cov = c->d_cov;
   fp.err = c->d_err;
   fp.timeoutTicks = kTimeoutTicks;
+  fp.tileMin = c->d_tileMin;
+  fp.pyrX = c->pyrX;
+  fp.stats = c->d_frameStats;
   RZ_HIP(launch_frame(fp, c->tileW, c->stream));
   if (prof) RZ_HIP(hipEventRecord(prof[3], c->stream));
 
@@ -448,7 +469,7 @@ static int enqueue_draw(rz_context* c, uint32_t n, const rz_occluder* const* occ
         return RZ_ERR_OUT_OF_MEMORY;
       c->h_statusCap = std::max<uint32_t>(n, 64);
     }
-    RZ_HIP(launch_status(c->d_state, c->d_visible, n, c->d_status, c->stream));
+    RZ_HIP(launch_status(c->d_state, c->d_done, n, c->d_status, c->stream));
     RZ_HIP(hipMemcpyAsync(c->h_status, c->d_status, n, hipMemcpyDeviceToHost, c->stream));
     c->pendingStatusDst = statusOut;
     c->pendingStatusN = n;
@@ -508,6 +529,9 @@ static QueryParams query_params(rz_context* c, const float* bounds, uint32_t n, 
   q.bX = c->bX;
   q.depth = c->d_depth;
   q.hiz = c->d_hiz;
+  q.tileMin = c->d_tileMin;
+  q.bY = c->bY;
+  q.pyrX = c->pyrX;
   q.rcp = c->d_rcp;
   q.visible = vis;
   q.needsClipping = clip;
@@ -542,7 +566,8 @@ extern "C" int rz_query2d(rz_context* ctx, uint32_t minX, uint32_t maxX, uint32_
     return RZ_ERR_INVALID_ARGUMENT;
   int s = ensure_queries(ctx, 1);
   if (s != RZ_OK) return s;
-  RZ_HIP(launch_query2d(minX, maxX, minY, maxY, maxZ, ctx->d_depth, ctx->d_hiz, ctx->bX, ctx->d_qvis, ctx->stream));
+  QueryParams q = query_params(ctx, nullptr, 0, ctx->d_qvis, nullptr, nullptr);
+  RZ_HIP(launch_query2d(minX, maxX, minY, maxY, maxZ, q, ctx->d_qvis, ctx->stream));
   uint8_t v = 0;
   RZ_HIP(hipMemcpyAsync(&v, ctx->d_qvis, 1, hipMemcpyDeviceToHost, ctx->stream));
   s = rz_sync(ctx);
@@ -600,7 +625,10 @@ extern "C" int rz_upload_raw(rz_context* ctx, const uint16_t* depth, const uint1
   if (!ctx) return RZ_ERR_INVALID_ARGUMENT;
   const size_t nb = size_t(ctx->bX) * ctx->bY;
   if (depth) RZ_HIP(hipMemcpyAsync(ctx->d_depth, depth, nb * 128, hipMemcpyHostToDevice, ctx->stream));
-  if (hiz) RZ_HIP(hipMemcpyAsync(ctx->d_hiz, hiz, (nb + 8) * 2, hipMemcpyHostToDevice, ctx->stream));
+  if (hiz) {
+    RZ_HIP(hipMemcpyAsync(ctx->d_hiz, hiz, (nb + 8) * 2, hipMemcpyHostToDevice, ctx->stream));
+    RZ_HIP(launch_tile_min(ctx->d_hiz, ctx->bX, ctx->bY, ctx->d_tileMin, ctx->pyrX, ctx->pyrY, ctx->stream));
+  }
   return rz_sync(ctx);
 }
 
@@ -706,5 +734,36 @@ extern "C" int rz_device_free(void* devptr) {
   if (!devptr) return RZ_ERR_INVALID_ARGUMENT;
   RZ_HIP(hipDeviceSynchronize());
   RZ_HIP(hipFree(devptr));
+  return RZ_OK;
+}
+
+extern "C" int rz_frame_stats_enable(rz_context* ctx, int enable) {
+  if (!ctx) return RZ_ERR_INVALID_ARGUMENT;
+  if (enable && !ctx->d_frameStats) {
+    const size_t n = size_t(ctx->tilesX) * ctx->tilesY * kFrameStatWords;
+    if (dev_alloc(&ctx->d_frameStats, n) != hipSuccess) return RZ_ERR_OUT_OF_MEMORY;
+    RZ_HIP(hipMemset(ctx->d_frameStats, 0, n * sizeof(uint64_t)));
+  } else if (!enable && ctx->d_frameStats) {
+    RZ_HIP(hipStreamSynchronize(ctx->stream));
+    hipFree(ctx->d_frameStats);
+    ctx->d_frameStats = nullptr;
+  }
+  return RZ_OK;
+}
+
+extern "C" int rz_frame_stats_read(rz_context* ctx, uint64_t* out, uint32_t* count) {
+  if (!ctx || !count) return RZ_ERR_INVALID_ARGUMENT;
+  const uint32_t tiles = uint32_t(ctx->tilesX * ctx->tilesY);
+  if (!ctx->d_frameStats) {
+    *count = 0;
+    return RZ_OK;
+  }
+  if (out && *count >= tiles) {
+    RZ_HIP(hipMemcpyAsync(out, ctx->d_frameStats, size_t(tiles) * kFrameStatWords * sizeof(uint64_t),
+                          hipMemcpyDeviceToHost, ctx->stream));
+    int s = rz_sync(ctx);
+    if (s != RZ_OK) return s;
+  }
+  *count = tiles;
   return RZ_OK;
 }

@@ -158,29 +158,49 @@ __device__ QueryPre query_pre(const Mvp& M, const float* bmin, const float* bmax
   return out;
 }
 
-// query2D (Rasterizer.cpp:283-349) on the buffers in device memory.
-__device__ bool query2d_global(uint32_t minX, uint32_t maxX, uint32_t minY, uint32_t maxY, uint32_t maxZ,
-                               const uint16_t* __restrict__ depth, const uint16_t* __restrict__ hiz, uint32_t bX) {
-  const uint32_t bMinX = minX >> 3, bMaxX = maxX >> 3, bMinY = minY >> 3, bMaxY = maxY >> 3;
-  for (uint32_t by = bMinY; by <= bMaxY; ++by) {
-    const int32_t sy = int32_t(minY - 8 * by), ey = int32_t(maxY - 8 * by);
-    const uint32_t startY = uint32_t(sy > 0 ? sy : 0), endY = uint32_t(ey < 7 ? ey : 7);
-    const bool interiorLine = startY == 0 && endY == 7;
-    for (uint32_t bx = bMinX; bx <= bMaxX; ++bx) {
-      const uint32_t b = by * bX + bx;
-      if (maxZ <= hiz[b]) continue;
-      const int32_t sx = int32_t(minX - 8 * bx), ex = int32_t(maxX - 8 * bx);
-      const uint32_t startX = uint32_t(sx > 0 ? sx : 0), endX = uint32_t(ex < 7 ? ex : 7);
-      if (interiorLine && startX == 0 && endX == 7) return true;
-      const uint4* rows = reinterpret_cast<const uint4*>(depth + size_t(b) * 64);
-      for (uint32_t y = startY; y <= endY; ++y) {
-        const uint4 r = rows[y];
-        const uint32_t w[4] = {r.x, r.y, r.z, r.w};
-        for (uint32_t x = startX; x <= endX; ++x) {
-          const uint32_t v = (w[x >> 1] >> (16 * (x & 1))) & 0xFFFFu;
-          if (v < maxZ) return true;
+// query2D (Rasterizer.cpp:283-349) by one wave on the buffers in device
+// memory.  The result is the OR over the rectangle's blocks of "maxZ > hiZ
+// and (block interior to the rectangle, or a pixel in range nearer than
+// maxZ)", so blocks may be visited in any order: lanes take 8x8-block tiles
+// whose minimum hiZ is below maxZ (other tiles hold no candidate block), then
+// one block per lane inside each candidate tile.
+__device__ bool query2d_wave(uint32_t minX, uint32_t maxX, uint32_t minY, uint32_t maxY, uint32_t maxZ,
+                             const uint16_t* __restrict__ depth, const uint16_t* __restrict__ hiz,
+                             const uint16_t* __restrict__ tileMin, uint32_t bX, uint32_t pyrX, uint32_t lane) {
+  const uint32_t bx0 = minX >> 3, bx1 = maxX >> 3, by0 = minY >> 3, by1 = maxY >> 3;
+  const uint32_t tx0 = bx0 >> 3, tx1 = bx1 >> 3, ty0 = by0 >> 3, ty1 = by1 >> 3;
+  const uint32_t ntx = tx1 - tx0 + 1, nt = ntx * (ty1 - ty0 + 1);
+  for (uint32_t tb = 0; tb < nt; tb += 64) {
+    const uint32_t t = tb + lane;
+    const bool cand = t < nt && uint32_t(tileMin[(ty0 + t / ntx) * pyrX + tx0 + t % ntx]) < maxZ;
+    uint64_t cm = __ballot(cand);
+    while (cm) {
+      const uint32_t k = uint32_t(__builtin_ctzll(cm));
+      cm &= cm - 1;
+      const uint32_t tt = tb + k;
+      const uint32_t bx = (tx0 + tt % ntx) * 8 + (lane & 7u), by = (ty0 + tt / ntx) * 8 + (lane >> 3);
+      bool v = false;
+      if (bx >= bx0 && bx <= bx1 && by >= by0 && by <= by1) {
+        const uint32_t b = by * bX + bx;
+        if (maxZ > uint32_t(hiz[b])) {
+          const int32_t sy = int32_t(minY - 8 * by), ey = int32_t(maxY - 8 * by);
+          const int32_t sx = int32_t(minX - 8 * bx), ex = int32_t(maxX - 8 * bx);
+          const uint32_t startY = uint32_t(sy > 0 ? sy : 0), endY = uint32_t(ey < 7 ? ey : 7);
+          const uint32_t startX = uint32_t(sx > 0 ? sx : 0), endX = uint32_t(ex < 7 ? ex : 7);
+          if (startY == 0 && endY == 7 && startX == 0 && endX == 7) {
+            v = true;
+          } else {
+            const uint4* rows = reinterpret_cast<const uint4*>(depth + size_t(b) * 64);
+            for (uint32_t y = startY; y <= endY && !v; ++y) {
+              const uint4 r = rows[y];
+              const uint32_t w[4] = {r.x, r.y, r.z, r.w};
+              for (uint32_t x = startX; x <= endX; ++x)
+                v |= ((w[x >> 1] >> (16 * (x & 1))) & 0xFFFFu) < maxZ;
+            }
+          }
         }
       }
+      if (__ballot(v) != 0ull) return true;
     }
   }
   return false;
@@ -506,6 +526,7 @@ __global__ void __launch_bounds__(256) rz_setup_kernel(SetupParams p) {
         box = quad_box(o.rec.fb, o.rec.range & 0xFFFFu, o.rec.range >> 16, p.bX, p.bY);
       }
     }
+    if (!valid) p.recs[g].range = 0u;  // an empty rectangle marks the quad as not drawn
     p.bbox[g] = box;
   }
   // Footprint of each item's valid quads: one atomic per wave when the whole
@@ -516,9 +537,15 @@ __global__ void __launch_bounds__(256) rz_setup_kernel(SetupParams p) {
   const uint32_t r1 = valid ? uint32_t(box >> 48) : 0u;
   const uint32_t firstItem = __builtin_amdgcn_readfirstlane(item);
   const bool sameItem = __ballot(inRange && item != firstItem) == 0ull;
+  const uint32_t a = __ockl_wfred_min_u32(c0), b = __ockl_wfred_max_u32(c1);
+  const uint32_t c = __ockl_wfred_min_u32(r0), d = __ockl_wfred_max_u32(r1);
+  // Box of this aligned group of 64 quads (the wave's), for the frame
+  // kernel's coarse overlap test.
+  if ((threadIdx.x & 63u) == 0u && inRange)
+    p.groupBox[g >> 6] = a != 0xFFFFFFFFu
+                             ? uint64_t(a) | (uint64_t(b) << 16) | (uint64_t(c) << 32) | (uint64_t(d) << 48)
+                             : kEmptyBox;
   if (sameItem) {
-    const uint32_t a = __ockl_wfred_min_u32(c0), b = __ockl_wfred_max_u32(c1);
-    const uint32_t c = __ockl_wfred_min_u32(r0), d = __ockl_wfred_max_u32(r1);
     if ((threadIdx.x & 63u) == 0u && a != 0xFFFFFFFFu) {
       uint32_t* fp = p.state[firstItem].fp;
       atomicMin(&fp[0], a);
@@ -549,121 +576,202 @@ __device__ __forceinline__ int div_bx(int d, int bX, float invBX) {
   return q;
 }
 
+// What a tile needs of one item, gathered once per item window.
+struct ItemLocal {
+  uint32_t index;
+  uint32_t flags;  // bit 0: query rectangle overlaps the tile, bit 1: footprint does, bits 2-3: status
+  uint32_t qMinX, qMaxX, qMinY, qMaxY, qMaxZ;
+  uint32_t expected;
+  uint32_t quadBase, quadCount;
+  uint64_t groups;  // bit k: aligned 64-quad group (quadBase >> 6) + k may touch the tile
+};
+
+constexpr uint32_t kItemWindow = 128;
+constexpr uint32_t kQuadChunk = 64;   // records staged in LDS at a time
+constexpr uint32_t kQuadScan = 256;   // quads tested for overlap per pass (one per thread)
+
 template <int TBX, int TBY>
 struct TileSmem {
-  uint64_t cov[4096];
-  uint16_t depth[TBX * TBY * 64];
+  uint64_t cov[4096];              // coverage table (Rasterizer.cpp:547-604)
+  uint16_t depth[TBX * TBY * 64];  // the tile's depth, block-linear
   uint16_t hiz[TBX * TBY];
-  uint32_t list[256];
+  ItemLocal items[kItemWindow];    // this window's items that touch the tile
+  QuadRec recs[kQuadChunk];        // setup records of the quads being drawn
   uint32_t wcnt[4];
   uint32_t part[2][4];
   uint32_t vis[2];
 };
 
-// Draws one quad's blocks that lie in this wave's rows of the tile
-// (Rasterizer.cpp:1095-1292).  Blocks are independent, so the order among a
-// quad's blocks does not matter; the order among quads is the caller's.
+// Wave that owns tile-local block (lbx, lby) while a chunk of quads is drawn:
+// diagonals, so that any horizontal or vertical run of blocks is shared by all
+// four waves.
+__device__ __forceinline__ uint32_t block_owner(int lbx, int lby) { return uint32_t(lbx + lby) & 3u; }
+
+// Adds `step` to `s` n times (uniform n), each add rounded: the reference's
+// per-row and per-block folds (Rasterizer.cpp:1130-1131, 1145-1146).  IEEE
+// addition commutes, so step + s and s + step are the same operation.
+__device__ __forceinline__ float fold_n(float s, float step, int n) {
+  for (; n >= 8; n -= 8) {
+    s = s + step; s = s + step; s = s + step; s = s + step;
+    s = s + step; s = s + step; s = s + step; s = s + step;
+  }
+  for (; n > 0; --n) s = s + step;
+  return s;
+}
+
+struct QuadConst {
+  uint32_t mode, maxZ;
+  uint32_t sl[4];
+  float dDx, dDy;
+};
+
+// One block of one quad (Rasterizer.cpp:1148-1290): hiZ reject, coverage
+// mask from the four edge lookups (Convex: any offset >= 63 rejects the block;
+// other modes: an empty mask does), depth rows 0/1 and 8/9 from the plane
+// with rows 2-7 as rounding-up averages, max-merge unless the block is
+// fast-cleared (hiZ == 1), hiZ = min.  `s` holds the folded edge offsets and
+// depth samples in lanes 12g .. 12g+11 (offsets first).  The rejections come
+// first and each only waits for the LDS reads it needs: most visited blocks
+// end at the hiZ test.
 template <int TBX, int TBY>
-__device__ void raster_quad_wave(const QuadRec* __restrict__ recp, TileSmem<TBX, TBY>& sm, int tbx0, int tbx1,
-                                 int tby0, int tby1, int bX, float invBX, uint32_t wave, uint32_t lane) {
-  const QuadRec R = *recp;
+__device__ __forceinline__ void raster_block(TileSmem<TBX, TBY>& sm, int lb, float s, int g, uint32_t lane,
+                                             int kLane, const QuadConst& Q) {
+  const uint32_t hz = sm.hiz[lb];
+  if (hz >= Q.maxZ) return;
+  float o[4];
+  uint32_t idx[4];
+  bool outside = false;
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    o[e] = readlane_f(s, 12 * g + e);
+    const int32_t t = cvtt_x86(o[e]);
+    if (Q.mode == kModeConvex) {
+      outside |= o[e] >= 63.0f;
+      idx[e] = (Q.sl[e] | uint32_t(max(t, 0))) & 4095u;
+    } else {
+      idx[e] = (Q.sl[e] | uint32_t(min(max(t, 0), 63))) & 4095u;
+    }
+  }
+  if (outside) return;
+  const uint64_t A = sm.cov[idx[0]], B = sm.cov[idx[1]], C = sm.cov[idx[2]], D = sm.cov[idx[3]];
+  uint64_t mask;
+  if (Q.mode == kModeConvex) mask = (A & B) & (C & D);
+  else if (Q.mode == kModeTriangle0) mask = A & B & C;
+  else if (Q.mode == kModeTriangle1) mask = A & C & D;
+  else if (Q.mode == kModeConcaveRight) mask = (A | D) & (B & C);
+  else if (Q.mode == kModeConcaveCenter) mask = (A & B) | (C & D);
+  else mask = (A & D) & (B | C);
+  if (Q.mode != kModeConvex && mask == 0) return;
+
+  const uint32_t old = sm.depth[lb * 64 + int(lane)];
+  float v = __shfl(s, 12 * g + 4 + kLane);
+  const uint32_t px = lane & 7u, py = lane >> 3;
+  if (px >= 4u) v = fmaf(Q.dDx, 0.5f, v);
+  const uint32_t lo = pack_depth(v);
+  const uint32_t hi = pack_depth(Q.dDy + v);
+  const uint32_t mid = avg_u16(lo, hi);
+  const uint32_t q1 = avg_u16(lo, mid), q3 = avg_u16(mid, hi);
+  const uint32_t j = py >> 1;
+  const uint32_t a01 = (j & 1u) ? q1 : lo, a23 = (j & 1u) ? q3 : mid;
+  uint32_t d = (j & 2u) ? a23 : a01;
+  const uint32_t maskBit = 8u * px + ((py & 1u) ? (py >> 1) : 4u + (py >> 1));
+  d = ((mask >> maskBit) & 1ull) ? d : 0u;
+  if (hz != 1u) d = max(d, old);
+  sm.depth[lb * 64 + int(lane)] = uint16_t(d);
+  const uint32_t mn = __ockl_wfred_min_u32(d);
+  if (lane == 0) sm.hiz[lb] = uint16_t(mn);
+}
+
+// Draws the blocks of one quad that this wave owns in the tile
+// (Rasterizer.cpp:1095-1292).  Blocks are independent, so the order among a
+// quad's blocks does not matter; the order among quads is the caller's.  The
+// per-block values are the reference's folds: the edge offsets and the eight
+// depth samples of rows 0/1 start at the rectangle origin, add the row step
+// once per block row, then the column step once per block column, each add
+// rounded.  Lane 12g + e carries component e (4 offsets, 8 depth samples) for
+// the g-th of up to five rows handled together.
+template <int TBX, int TBY>
+__device__ __forceinline__ void raster_quad_wave(const QuadRec& R, TileSmem<TBX, TBY>& sm, int tbx0, int tbx1,
+                                                 int tby0, int tby1, int bX, float invBX, uint32_t wave,
+                                                 uint32_t lane) {
   const int fb = int(R.fb);
   const int rangeX = int(R.range & 0xFFFFu), rangeY = int(R.range >> 16);
-  const uint32_t mode = R.modeZ & 0xFFu, maxZ = R.modeZ >> 16;
-  const uint32_t sl0 = R.slope01 & 0xFFFFu, sl1 = R.slope01 >> 16, sl2 = R.slope23 & 0xFFFFu, sl3 = R.slope23 >> 16;
+  QuadConst Q;
+  Q.mode = R.modeZ & 0xFFu;
+  Q.maxZ = R.modeZ >> 16;
+  Q.sl[0] = R.slope01 & 0xFFFFu;
+  Q.sl[1] = R.slope01 >> 16;
+  Q.sl[2] = R.slope23 & 0xFFFFu;
+  Q.sl[3] = R.slope23 >> 16;
+  Q.dDx = R.dDx;
+  Q.dDy = R.dDy;
+  const int minY = div_bx(fb, bX, invBX), minX = fb - minY * bX;
+  const int kLane = int(((lane & 7u) & 3u) + 4u * ((lane >> 3) & 1u));
 
-  // Lanes 0-3 fold the four edge offsets, lanes 4-11 the eight depth samples
-  // of rows 0/1 (x = 0..3 at sample positions -0.4375 + 0.125 k).
+  const uint32_t g = lane / 12u, e = lane - 12u * g;
   float base, stepY, stepX;
-  if (lane < 4) {
-    base = lane == 0 ? R.off[0] : lane == 1 ? R.off[1] : lane == 2 ? R.off[2] : R.off[3];
-    stepY = lane == 0 ? R.ny[0] : lane == 1 ? R.ny[1] : lane == 2 ? R.ny[2] : R.ny[3];
-    stepX = lane == 0 ? R.nx[0] : lane == 1 ? R.nx[1] : lane == 2 ? R.nx[2] : R.nx[3];
+  if (e < 4) {
+    base = e == 0 ? R.off[0] : e == 1 ? R.off[1] : e == 2 ? R.off[2] : R.off[3];
+    stepY = e == 0 ? R.ny[0] : e == 1 ? R.ny[1] : e == 2 ? R.ny[2] : R.ny[3];
+    stepX = e == 0 ? R.nx[0] : e == 1 ? R.nx[1] : e == 2 ? R.nx[2] : R.nx[3];
   } else {
-    const uint32_t k = (lane - 4) & 7u;
+    const uint32_t k = e - 4;
     const float spx = -0.4375f + 0.125f * float(k & 3u);
     const float spy = -0.4375f + 0.125f * float(k >> 2);
-    base = fmaf(R.dDx, spx, fmaf(R.dDy, spy, R.dp0));
-    stepY = R.dDy;
-    stepX = R.dDx;
+    base = fmaf(Q.dDx, spx, fmaf(Q.dDy, spy, R.dp0));
+    stepY = Q.dDy;
+    stepX = Q.dDx;
   }
-  const uint32_t px = lane & 7u, py = lane >> 3;
-  const int srcLane = 4 + int((px & 3u) + 4u * (py & 1u));
-  const uint32_t j = py >> 1;
-  const uint32_t maskBit = 8u * px + ((py & 1u) ? (py >> 1) : 4u + (py >> 1));
 
-  float line = base;
-  int cur = 0;
-  for (int lr = int(wave); lr < TBY; lr += 4) {
-    const int by = tby0 + lr;
-    if (by > tby1) break;
-    const int rowL = by * bX;
-    int d0 = rowL + tbx0 - fb;
-    const int d1 = rowL + tbx1 - fb;
-    if (d1 < 0) continue;
-    if (d0 < 0) d0 = 0;
-    const int r0 = div_bx(d0, bX, invBX), r1 = div_bx(d1, bX, invBX);
-    for (int r = r0; r <= r1 && r < rangeY; ++r) {
-      const int cLo = max(d0 - r * bX, 0);
-      const int cHi = min(d1 - r * bX, rangeX - 1);
-      if (cLo > cHi) continue;
-      while (cur < r) {
-        line = line + stepY;
-        ++cur;
+  if (minX + rangeX > bX) {
+    // The 16-bit multiply of the first block index wrapped (Rasterizer.cpp:1054):
+    // the quad's blocks fb + r*bX + c no longer form a screen rectangle, so
+    // each owned block finds its (r, c) and folds from the origin.
+    for (int by = tby0; by <= tby1; ++by)
+      for (int bx = tbx0; bx <= tbx1; ++bx) {
+        if (block_owner(bx - tbx0, by - tby0) != wave) continue;
+        const int d = by * bX + bx - fb;
+        if (d < 0) continue;
+        const int r = div_bx(d, bX, invBX), c = d - r * bX;
+        if (r >= rangeY || c >= rangeX) continue;
+        const float s = fold_n(fold_n(base, stepY, r), stepX, c);
+        raster_block<TBX, TBY>(sm, (by - tby0) * TBX + (bx - tbx0), s, 0, lane, kLane, Q);
       }
-      float s = line;
-      for (int c = 0; c < cLo; ++c) s = stepX + s;
-      for (int c = cLo; c <= cHi; ++c, s = stepX + s) {
-        const int bx = fb + r * bX + c - rowL;
-        const int lb = lr * TBX + (bx - tbx0);
-        const uint32_t hz = sm.hiz[lb];
-        if (hz >= maxZ) continue;
-        const float o0 = readlane_f(s, 0), o1 = readlane_f(s, 1), o2 = readlane_f(s, 2), o3 = readlane_f(s, 3);
-        uint64_t mask;
-        if (mode == kModeConvex) {
-          if (o0 >= 63.0f || o1 >= 63.0f || o2 >= 63.0f || o3 >= 63.0f) continue;
-          const uint32_t i0 = (sl0 | uint32_t(max(cvtt_x86(o0), 0))) & 4095u;
-          const uint32_t i1 = (sl1 | uint32_t(max(cvtt_x86(o1), 0))) & 4095u;
-          const uint32_t i2 = (sl2 | uint32_t(max(cvtt_x86(o2), 0))) & 4095u;
-          const uint32_t i3 = (sl3 | uint32_t(max(cvtt_x86(o3), 0))) & 4095u;
-          mask = (sm.cov[i0] & sm.cov[i1]) & (sm.cov[i2] & sm.cov[i3]);
-        } else {
-          const uint32_t i0 = sl0 | uint32_t(min(max(cvtt_x86(o0), 0), 63));
-          const uint32_t i1 = sl1 | uint32_t(min(max(cvtt_x86(o1), 0), 63));
-          const uint32_t i2 = sl2 | uint32_t(min(max(cvtt_x86(o2), 0), 63));
-          const uint32_t i3 = sl3 | uint32_t(min(max(cvtt_x86(o3), 0), 63));
-          const uint64_t A = sm.cov[i0 & 4095u], B = sm.cov[i1 & 4095u], C = sm.cov[i2 & 4095u],
-                         D = sm.cov[i3 & 4095u];
-          if (mode == kModeTriangle0) mask = A & B & C;
-          else if (mode == kModeTriangle1) mask = A & C & D;
-          else if (mode == kModeConcaveRight) mask = (A | D) & (B & C);
-          else if (mode == kModeConcaveCenter) mask = (A & B) | (C & D);
-          else mask = (A & D) & (B | C);
-          if (mask == 0) continue;
-        }
-        // Depth: rows 0/1 and 8/9 from the plane, rows 2-7 by rounding-up
-        // averages of the packed values (:1241-1254).
-        float v = __shfl(s, srcLane);
-        if (px >= 4u) v = fmaf(R.dDx, 0.5f, v);
-        const uint32_t lo = pack_depth(v);
-        const uint32_t hi = pack_depth(R.dDy + v);
-        const uint32_t mid = avg_u16(lo, hi);
-        uint32_t d = j == 0 ? lo : (j == 1 ? avg_u16(lo, mid) : (j == 2 ? mid : avg_u16(mid, hi)));
-        if (!((mask >> maskBit) & 1ull)) d = 0;
-        uint16_t* dp = &sm.depth[lb * 64 + int(lane)];
-        if (hz != 1u) d = max(d, uint32_t(*dp));
-        *dp = uint16_t(d);
-        const uint32_t mn = __ockl_wfred_min_u32(d);
-        if (lane == 0) sm.hiz[lb] = uint16_t(mn);
+    return;
+  }
+  const int x0 = max(tbx0, minX), x1 = min(tbx1, minX + rangeX - 1);
+  const int y0 = max(tby0, minY), y1 = min(tby1, minY + rangeY - 1);
+  if (x0 > x1 || y0 > y1) return;
+  // Skip quads none of whose blocks in the tile belong to this wave.
+  if ((x1 - x0) + (y1 - y0) < 3) {
+    bool mine = false;
+    for (int by = y0; by <= y1; ++by)
+      for (int bx = x0; bx <= x1; ++bx) mine |= block_owner(bx - tbx0, by - tby0) == wave;
+    if (!mine) return;
+  }
+  const int cLo = x0 - minX;
+  // Lane group g starts at quad row (y0 - minY) + g.
+  float line = fold_n(base, stepY, y0 - minY);
+#pragma unroll
+  for (uint32_t i = 0; i < 4; ++i)
+    if (i < g) line = line + stepY;
+  for (int py = y0; py <= y1; py += 5) {
+    float s = fold_n(line, stepX, cLo);
+    for (int bx = x0; bx <= x1; ++bx, s = stepX + s) {
+      for (int gg = 0; gg < 5 && py + gg <= y1; ++gg) {
+        const int lbx = bx - tbx0, lby = py + gg - tby0;
+        if (block_owner(lbx, lby) != wave) continue;
+        raster_block<TBX, TBY>(sm, lby * TBX + lbx, s, gg, lane, kLane, Q);
       }
     }
+    line = fold_n(line, stepY, 5);
   }
 }
 
 // query2D restricted to this wave's rows of the tile (Rasterizer.cpp:283-349).
 template <int TBX, int TBY>
-__device__ bool query_wave(const ItemState& S, TileSmem<TBX, TBY>& sm, int tbx0, int tbx1, int tby0, int tby1,
-                           uint32_t wave, uint32_t lane) {
+__device__ __forceinline__ bool query_wave(const ItemLocal& S, TileSmem<TBX, TBY>& sm, int tbx0, int tbx1,
+                                           int tby0, int tby1, uint32_t wave, uint32_t lane) {
   const int qbx0 = int(S.qMinX >> 3), qbx1 = int(S.qMaxX >> 3), qby0 = int(S.qMinY >> 3), qby1 = int(S.qMaxY >> 3);
   const uint32_t maxZ = S.qMaxZ;
   const int px = int(lane & 7u), py = int(lane >> 3);
@@ -687,9 +795,84 @@ __device__ bool query_wave(const ItemState& S, TileSmem<TBX, TBY>& sm, int tbx0,
   return false;
 }
 
+__device__ __forceinline__ bool box_touches(uint64_t bb, int tbx0, int tbx1, int tby0, int tby1) {
+  const int c0 = int(bb & 0xFFFF), c1 = int((bb >> 16) & 0xFFFF), r0 = int((bb >> 32) & 0xFFFF), r1 = int(bb >> 48);
+  return c0 <= c1 && c0 <= tbx1 && c1 >= tbx0 && r0 <= tby1 && r1 >= tby0;
+}
+
+// True when no pixel of the tile can be covered by the quad because some edge
+// is outside for every block of the tile.  The block loop folds each edge
+// offset as off + nY + ... + nY (row steps) + nX + ... + nX (column steps) in
+// float; after k steps the fold differs from the exact off + r*nY + c*nX by at
+// most k * max|partial sum| * 2^-24 (each rounding is within half an ulp), so
+// an exact minimum at least 63 + that bound proves the offset test
+// (Rasterizer.cpp:1158) or the clamped lookup of bucket 63, whose table entry
+// is empty (:599), rejects every block.  For the non-convex modes the offset
+// must also stay below 2^31, where the truncation would wrap to bucket 0.
+// Quads whose block index wrapped in the 16-bit multiply are never rejected.
+__device__ bool quad_misses_tile(const QuadRec& R, int tbx0, int tbx1, int tby0, int tby1, int bX) {
+  const int fb = int(R.fb), rangeX = int(R.range & 0xFFFFu), rangeY = int(R.range >> 16);
+  const int minX = fb % bX, minY = fb / bX;
+  if (minX + rangeX > bX) return false;
+  const int c0 = max(tbx0, minX) - minX, c1 = min(tbx1, minX + rangeX - 1) - minX;
+  const int r0 = max(tby0, minY) - minY, r1 = min(tby1, minY + rangeY - 1) - minY;
+  if (c0 > c1 || r0 > r1) return true;
+  const uint32_t mode = R.modeZ & 0xFFu;
+  bool out[4];
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    const double o = R.off[e], nx = R.nx[e], ny = R.ny[e];
+    const double v00 = o + r0 * ny + c0 * nx, v01 = o + r0 * ny + c1 * nx;
+    const double v10 = o + r1 * ny + c0 * nx, v11 = o + r1 * ny + c1 * nx;
+    const double lo = fmin(fmin(v00, v01), fmin(v10, v11));
+    const double hi = fmax(fmax(v00, v01), fmax(v10, v11));
+    const double steps = double(r1 + c1);
+    const double bound = fabs(o) + r1 * fabs(ny) + c1 * fabs(nx);
+    const double err = steps * (bound * 0x1p-23 + 0x1p-126);
+    const bool finite = isfinite(o) && isfinite(nx) && isfinite(ny) && isfinite(bound);
+    const bool below31 = hi + err < 2147483648.0;
+    out[e] = finite && lo - err >= 63.0 && (mode == kModeConvex || below31);
+  }
+  switch (mode) {
+    case kModeConvex: return out[0] || out[1] || out[2] || out[3];
+    case kModeTriangle0: return out[0] || out[1] || out[2];
+    case kModeTriangle1: return out[0] || out[2] || out[3];
+    case kModeConcaveRight: return out[1] || out[2] || (out[0] && out[3]);
+    case kModeConcaveLeft: return out[0] || out[3] || (out[1] && out[2]);
+    case kModeConcaveCenter: return (out[0] || out[1]) && (out[2] || out[3]);
+    default: return false;
+  }
+}
+
+// Tile-overlap tests shared by the prep kernel's tile count and the frame
+// kernel (they must agree exactly).
 template <int TBX, int TBY>
+__device__ __forceinline__ bool query_overlaps_tile(uint32_t qMinX, uint32_t qMaxX, uint32_t qMinY, uint32_t qMaxY,
+                                                   uint32_t tX, uint32_t tY) {
+  return tX >= (qMinX >> 3) / TBX && tX <= (qMaxX >> 3) / TBX && tY >= (qMinY >> 3) / TBY &&
+         tY <= (qMaxY >> 3) / TBY;
+}
+
+// done[i] packs, over the tiles of item i's query rectangle, the number of
+// tiles that have contributed (bits 0-15) and how many found a visible pixel
+// (bits 16-31).
+constexpr uint32_t kDoneCountMask = 0xFFFFu;
+constexpr uint32_t kDoneVisibleOne = 0x10000u;
+
+template <int TBX, int TBY, bool STATS>
 __global__ void __launch_bounds__(256) rz_frame_kernel(FrameParams p) {
   __shared__ TileSmem<TBX, TBY> sm;
+  // Observability counters, kept by thread 0 (compiled out unless STATS).
+  uint64_t ctr[kFrameStatWords] = {};
+  const uint64_t tEntry = STATS ? __builtin_amdgcn_s_memrealtime() : 0;
+  auto stamp = []() -> uint64_t { return STATS ? __builtin_amdgcn_s_memrealtime() : 0; };
+  auto flushStats = [&]() {
+    if (STATS && threadIdx.x == 0) {
+      ctr[kStatTotalTicks] = __builtin_amdgcn_s_memrealtime() - tEntry;
+      for (int k = 0; k < kFrameStatWords; ++k) p.stats[size_t(blockIdx.x) * kFrameStatWords + k] = ctr[k];
+    }
+  };
+
   const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
   const uint32_t tX = blockIdx.x % p.tilesX, tY = blockIdx.x / p.tilesX;
   const int bX = int(p.bX), bY = int(p.bY);
@@ -697,22 +880,26 @@ __global__ void __launch_bounds__(256) rz_frame_kernel(FrameParams p) {
   const int tbx1 = min(tbx0 + TBX, bX) - 1, tby1 = min(tby0 + TBY, bY) - 1;
   const float invBX = 1.0f / float(bX);
 
-  // Does any item touch this tile, through its query rectangle or its quads?
-  int involved = 0;
-  for (uint32_t i = tid; i < p.nItems; i += 256) {
+  // Relevance of one item to this tile: bit 0 the query rectangle overlaps
+  // it, bit 1 some valid quad's blocks do; status in bits 2-3.
+  auto relevance = [&](uint32_t i) -> uint32_t {
     const ItemState& S = p.state[i];
-    if (S.status == kStatusSkip) continue;
-    bool q = false;
-    if (S.status == kStatusTest) {
-      q = tX >= (S.qMinX >> 3) / TBX && tX <= (S.qMaxX >> 3) / TBX && tY >= (S.qMinY >> 3) / TBY &&
-          tY <= (S.qMaxY >> 3) / TBY;
-    }
+    const uint32_t status = S.status;
+    if (status == kStatusSkip) return 0u;
+    const bool q = status == kStatusTest && query_overlaps_tile<TBX, TBY>(S.qMinX, S.qMaxX, S.qMinY, S.qMaxY, tX, tY);
     const bool f = S.fp[0] <= S.fp[1] && int(S.fp[0]) <= tbx1 && int(S.fp[1]) >= tbx0 && int(S.fp[2]) <= tby1 &&
                    int(S.fp[3]) >= tby0;
-    involved |= (q || f) ? 1 : 0;
-  }
-  if (!__syncthreads_or(involved)) return;
+    return (q || f) ? ((q ? 1u : 0u) | (f ? 2u : 0u) | (status << 2)) : 0u;
+  };
 
+  int involved = 0;
+  for (uint32_t i = tid; i < p.nItems && !involved; i += 256) involved = relevance(i) != 0u;
+  if (!__syncthreads_or(involved)) {
+    flushStats();
+    return;
+  }
+
+  const uint64_t tLoad = stamp();
   for (uint32_t i = tid; i < 4096; i += 256) sm.cov[i] = p.cov[i];
   constexpr int NB = TBX * TBY;
   for (int rrow = int(tid); rrow < NB * 8; rrow += 256) {
@@ -727,79 +914,164 @@ __global__ void __launch_bounds__(256) rz_frame_kernel(FrameParams p) {
     const int gx = tbx0 + lb % TBX, gy = tby0 + lb / TBX;
     sm.hiz[lb] = (gx <= tbx1 && gy <= tby1) ? p.hiz[size_t(gy) * bX + gx] : uint16_t(0xFFFF);
   }
-  __syncthreads();
 
   uint32_t qPar = 0, vPar = 0;
-  for (uint32_t i = 0; i < p.nItems; ++i) {
-    const ItemState& S = p.state[i];
-    const uint32_t st = S.status;
-    if (st == kStatusSkip) continue;
-    const bool qov = st == kStatusTest && tX >= (S.qMinX >> 3) / TBX && tX <= (S.qMaxX >> 3) / TBX &&
-                     tY >= (S.qMinY >> 3) / TBY && tY <= (S.qMaxY >> 3) / TBY;
-    const bool fov = S.fp[0] <= S.fp[1] && int(S.fp[0]) <= tbx1 && int(S.fp[1]) >= tbx0 &&
-                     int(S.fp[2]) <= tby1 && int(S.fp[3]) >= tby0;
-    if (!qov && !fov) continue;
-    if (qov) {
-      const bool found = query_wave<TBX, TBY>(S, sm, tbx0, tbx1, tby0, tby1, wave, lane);
-      if (lane == 0) sm.part[qPar][wave] = found ? 1u : 0u;
-      __syncthreads();
-      if (tid == 0) {
-        const uint32_t part = sm.part[qPar][0] | sm.part[qPar][1] | sm.part[qPar][2] | sm.part[qPar][3];
-        if (part) __hip_atomic_fetch_or(&p.visible[i], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_fetch_add(&p.done[i], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-      }
-      qPar ^= 1u;
-    }
-    if (!fov) continue;
-    if (st == kStatusTest) {
-      if (tid == 0) {
-        const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
-        const uint32_t expected = S.expected;
-        while (__hip_atomic_load(&p.done[i], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < expected) {
-          if (__builtin_amdgcn_s_memrealtime() - t0 > p.timeoutTicks) {
-            __hip_atomic_fetch_or(p.err, uint32_t(kErrTimeout), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            break;
-          }
-          __builtin_amdgcn_s_sleep(1);
-        }
-        sm.vis[vPar] = __hip_atomic_load(&p.visible[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      }
-      __syncthreads();
-      const bool vis = sm.vis[vPar] != 0u;
-      vPar ^= 1u;
-      if (!vis) continue;
-    }
-    const ItemDesc& D = p.items[i];
-    const uint32_t qb = D.quadBase, qc = D.quadCount;
-    for (uint32_t base = 0; base < qc; base += 256) {
-      const uint32_t g = qb + base + tid;
-      bool hit = false;
-      if (base + tid < qc) {
-        const uint64_t bb = p.bbox[g];
-        const int c0 = int(bb & 0xFFFF), c1 = int((bb >> 16) & 0xFFFF), r0 = int((bb >> 32) & 0xFFFF),
-                  r1 = int(bb >> 48);
-        hit = c0 <= c1 && c0 <= tbx1 && c1 >= tbx0 && r0 <= tby1 && r1 >= tby0;
-      }
-      const uint64_t m = __ballot(hit);
-      const uint32_t pre = __builtin_amdgcn_mbcnt_hi(uint32_t(m >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(m), 0u));
-      if (lane == 0) sm.wcnt[wave] = uint32_t(__popcll(m));
-      __syncthreads();
-      uint32_t off = 0, total = 0;
+  if (STATS) {
+    __syncthreads();
+    ctr[kStatLoadTicks] = stamp() - tLoad;
+  }
+  const uint64_t tLoop = stamp();
+  for (uint32_t wbase = 0; wbase < p.nItems; wbase += kItemWindow) {
+    // Gather this window's items that touch the tile, in draw order.
+    const uint32_t i = wbase + tid;
+    const uint32_t rel = (tid < kItemWindow && i < p.nItems) ? relevance(i) : 0u;
+    const uint64_t m = __ballot(rel != 0u);
+    const uint32_t pre = __builtin_amdgcn_mbcnt_hi(uint32_t(m >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(m), 0u));
+    if (lane == 0) sm.wcnt[wave] = uint32_t(__popcll(m));
+    __syncthreads();
+    uint32_t off = 0, nRel = 0;
 #pragma unroll
-      for (uint32_t w = 0; w < 4; ++w) {
-        off += w < wave ? sm.wcnt[w] : 0u;
-        total += sm.wcnt[w];
-      }
-      if (hit) sm.list[off + pre] = g;
-      __syncthreads();
-      for (uint32_t k = 0; k < total; ++k) {
-        const uint32_t gq = __builtin_amdgcn_readfirstlane(sm.list[k]);
-        raster_quad_wave<TBX, TBY>(p.recs + gq, sm, tbx0, tbx1, tby0, tby1, bX, invBX, wave, lane);
-      }
-      __syncthreads();
+    for (uint32_t w = 0; w < 4; ++w) {
+      off += w < wave ? sm.wcnt[w] : 0u;
+      nRel += sm.wcnt[w];
     }
+    if (rel) {
+      const ItemState& S = p.state[i];
+      ItemLocal& L = sm.items[off + pre];
+      L.index = i;
+      L.flags = rel;
+      L.qMinX = S.qMinX;
+      L.qMaxX = S.qMaxX;
+      L.qMinY = S.qMinY;
+      L.qMaxY = S.qMaxY;
+      L.qMaxZ = S.qMaxZ;
+      L.expected = S.expected;
+      const uint32_t qb = p.items[i].quadBase, qc = p.items[i].quadCount;
+      L.quadBase = qb;
+      L.quadCount = qc;
+      uint64_t groups = ~0ull;
+      if ((rel & 2u) && qc > 0) {
+        const uint32_t g0 = qb >> 6, g1 = (qb + qc - 1) >> 6;
+        if (g1 - g0 < 64) {
+          groups = 0;
+          for (uint32_t gi = g0; gi <= g1; ++gi)
+            if (box_touches(p.groupBox[gi], tbx0, tbx1, tby0, tby1)) groups |= 1ull << (gi - g0);
+        }
+      }
+      L.groups = groups;
+    }
+    __syncthreads();
+
+    for (uint32_t k = 0; k < nRel; ++k) {
+      const ItemLocal& L = sm.items[k];
+      const uint32_t item = L.index, flags = L.flags, status = flags >> 2;
+      if (STATS) ctr[kStatItems]++;
+      if (flags & 1u) {
+        // This tile's share of query2D for the item, at its exact turn.
+        const uint64_t tq = stamp();
+        const bool found = query_wave<TBX, TBY>(L, sm, tbx0, tbx1, tby0, tby1, wave, lane);
+        if (lane == 0) sm.part[qPar][wave] = found ? 1u : 0u;
+        __syncthreads();
+        if (tid == 0) {
+          const uint32_t part = sm.part[qPar][0] | sm.part[qPar][1] | sm.part[qPar][2] | sm.part[qPar][3];
+          __hip_atomic_fetch_add(&p.done[item], 1u + (part ? kDoneVisibleOne : 0u), __ATOMIC_RELEASE,
+                                 __HIP_MEMORY_SCOPE_AGENT);
+        }
+        qPar ^= 1u;
+        if (STATS) {
+          ctr[kStatQueryTicks] += stamp() - tq;
+          ctr[kStatQueries]++;
+        }
+      }
+      if (!(flags & 2u)) continue;
+      if (status == kStatusTest) {
+        // Wait until every tile of the rectangle has answered, or one of them
+        // has already seen a visible pixel.
+        if (tid == 0) {
+          const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
+          uint32_t d;
+          while (true) {
+            d = __hip_atomic_load(&p.done[item], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+            if ((d & kDoneCountMask) >= L.expected || d >= kDoneVisibleOne) break;
+            if (__builtin_amdgcn_s_memrealtime() - t0 > p.timeoutTicks) {
+              __hip_atomic_fetch_or(p.err, uint32_t(kErrTimeout), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+              break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+          }
+          sm.vis[vPar] = d >= kDoneVisibleOne ? 1u : 0u;
+          if (STATS) {
+            ctr[kStatWaitTicks] += stamp() - t0;
+            ctr[kStatWaits]++;
+          }
+        }
+        __syncthreads();
+        const bool vis = sm.vis[vPar] != 0u;
+        vPar ^= 1u;
+        if (!vis) continue;
+      }
+      // Draw the item's quads that touch the tile, in submission order: each
+      // pass tests 256 quads (one per thread; passes and waves whose 64-quad
+      // groups miss the tile are skipped), each hit keeping its setup record
+      // in registers until it is staged into LDS and drawn, 64 at a time.
+      const uint64_t tr = stamp();
+      const uint32_t gBase = L.quadBase >> 6;
+      for (uint32_t base = 0; base < L.quadCount; base += kQuadScan) {
+        // groups overlapping this pass, relative to gBase
+        const uint32_t pg0 = ((L.quadBase + base) >> 6) - gBase;
+        const uint32_t pg1 = ((L.quadBase + min(base + kQuadScan, L.quadCount) - 1) >> 6) - gBase;
+        const uint64_t passMask = (pg1 - pg0 >= 63) ? ~0ull : (((2ull << (pg1 - pg0)) - 1) << pg0);
+        if (pg0 < 64 && (L.groups & passMask) == 0) continue;
+        const uint64_t tScan = stamp();
+        if (STATS) ctr[kStatPasses]++;
+        const uint32_t waveFirst = base + wave * 64u;
+        bool hit = false;
+        QuadRec rec;
+        if (waveFirst < L.quadCount) {
+          const uint32_t wg0 = ((L.quadBase + waveFirst) >> 6) - gBase;
+          const uint32_t wg1 = ((L.quadBase + min(waveFirst + 63u, L.quadCount - 1u)) >> 6) - gBase;
+          const bool groupHit = wg1 >= 64 || ((L.groups >> wg0) & 1ull) || ((L.groups >> wg1) & 1ull);
+          if (groupHit && base + tid < L.quadCount) {
+            rec = p.recs[L.quadBase + base + tid];
+            const uint32_t rx = rec.range & 0xFFFFu, ry = rec.range >> 16;
+            hit = rx != 0u && box_touches(quad_box(rec.fb, rx, ry, p.bX, p.bY), tbx0, tbx1, tby0, tby1) &&
+                  !quad_misses_tile(rec, tbx0, tbx1, tby0, tby1, bX);
+          }
+        }
+        const uint64_t hm = __ballot(hit);
+        const uint32_t hpre =
+            __builtin_amdgcn_mbcnt_hi(uint32_t(hm >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(hm), 0u));
+        if (lane == 0) sm.wcnt[wave] = uint32_t(__popcll(hm));
+        __syncthreads();
+        uint32_t hoff = 0, total = 0;
+#pragma unroll
+        for (uint32_t w = 0; w < 4; ++w) {
+          hoff += w < wave ? sm.wcnt[w] : 0u;
+          total += sm.wcnt[w];
+        }
+        const uint32_t pos = hoff + hpre;
+        if (STATS) ctr[kStatScanTicks] += stamp() - tScan;
+        // The first 64 hits are staged from registers; later batches (rare)
+        // reload their records.
+        if (hit && pos < kQuadChunk) sm.recs[pos] = rec;
+        for (uint32_t k0 = 0; k0 < total; k0 += kQuadChunk) {
+          const uint32_t n = min(kQuadChunk, total - k0);
+          if (k0 > 0 && hit && pos >= k0 && pos < k0 + n) sm.recs[pos - k0] = p.recs[L.quadBase + base + tid];
+          __syncthreads();
+          const uint64_t tDraw = stamp();
+          for (uint32_t q = 0; q < n; ++q)
+            raster_quad_wave<TBX, TBY>(sm.recs[q], sm, tbx0, tbx1, tby0, tby1, bX, invBX, wave, lane);
+          if (STATS) ctr[kStatQuads] += n;
+          __syncthreads();
+          if (STATS) ctr[kStatDrawTicks] += stamp() - tDraw;
+        }
+      }
+      if (STATS) ctr[kStatRasterTicks] += stamp() - tr;
+    }
+    __syncthreads();
   }
 
+  if (STATS) ctr[kStatLoopTicks] = stamp() - tLoop;
   // Write the tile back.
   for (int rrow = int(tid); rrow < NB * 8; rrow += 256) {
     const int lb = rrow >> 3, y = rrow & 7;
@@ -812,6 +1084,17 @@ __global__ void __launch_bounds__(256) rz_frame_kernel(FrameParams p) {
     const int gx = tbx0 + lb % TBX, gy = tby0 + lb / TBX;
     if (gx <= tbx1 && gy <= tby1) p.hiz[size_t(gy) * bX + gx] = sm.hiz[lb];
   }
+  // Min hiZ of each 8x8-block sub-tile for the query kernel (blocks outside
+  // the buffer hold 0xFFFF and do not lower it).
+  constexpr int kSubX = TBX / 8, kSubY = TBY / 8;
+  if (int(wave) < kSubX * kSubY) {
+    const int sx = int(wave) % kSubX, sy = int(wave) / kSubX;
+    const int lb = (sy * 8 + int(lane >> 3)) * TBX + sx * 8 + int(lane & 7u);
+    const uint32_t mn = __ockl_wfred_min_u32(uint32_t(sm.hiz[lb]));
+    const uint32_t px = tX * kSubX + uint32_t(sx), py = tY * kSubY + uint32_t(sy);
+    if (lane == 0 && px < p.pyrX && py * 8 < p.bY) p.tileMin[py * p.pyrX + px] = uint16_t(mn);
+  }
+  flushStats();
 }
 
 // Per-item draw status of a frame: 0 not drawn, 1 rasterize<false>,
@@ -821,14 +1104,16 @@ __global__ void rz_status_kernel(const ItemState* __restrict__ state, const uint
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const uint32_t st = state[i].status;
-  out[i] = uint8_t(st == kStatusTest ? (visible[i] ? 1u : 0u) : st);
+  out[i] = uint8_t(st == kStatusTest ? ((visible[i] >> 16) != 0u ? 1u : 0u) : st);
 }
 
 // ---------------------------------------------------------------------------
 // Queries, readback, clear.
 // ---------------------------------------------------------------------------
+// queryVisibility (Rasterizer.cpp:123-281), one wave per box.
 __global__ void __launch_bounds__(256) rz_query_kernel(QueryParams p) {
-  const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t q = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   if (q >= p.n) return;
   const float* b = p.bounds + 8 * size_t(q);
   const float bmin[4] = {b[0], b[1], b[2], b[3]};
@@ -839,17 +1124,33 @@ __global__ void __launch_bounds__(256) rz_query_kernel(QueryParams p) {
     vis = 1;
     clip = 1;
   } else if (Q.kind == 3) {
-    vis = query2d_global(Q.minX, Q.maxX, Q.minY, Q.maxY, Q.maxZ, p.depth, p.hiz, p.bX) ? 1u : 0u;
+    vis = query2d_wave(Q.minX, Q.maxX, Q.minY, Q.maxY, Q.maxZ, p.depth, p.hiz, p.tileMin, p.bX, p.pyrX, lane) ? 1u
+                                                                                                              : 0u;
   }
-  p.visible[q] = uint8_t(vis);
-  if (p.needsClipping) p.needsClipping[q] = uint8_t(clip);
-  // bit 0 visible, bit 1 needsClipping, bit 2 needsClipping was written
-  if (p.code) p.code[q] = uint8_t(vis | (clip << 1) | (Q.kind != 0 ? 4u : 0u));
+  if (lane == 0) {
+    p.visible[q] = uint8_t(vis);
+    if (p.needsClipping) p.needsClipping[q] = uint8_t(clip);
+    // bit 0 visible, bit 1 needsClipping, bit 2 needsClipping was written
+    if (p.code) p.code[q] = uint8_t(vis | (clip << 1) | (Q.kind != 0 ? 4u : 0u));
+  }
 }
 
 __global__ void rz_query2d_kernel(uint32_t minX, uint32_t maxX, uint32_t minY, uint32_t maxY, uint32_t maxZ,
-                                  const uint16_t* depth, const uint16_t* hiz, uint32_t bX, uint8_t* out) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) *out = query2d_global(minX, maxX, minY, maxY, maxZ, depth, hiz, bX) ? 1 : 0;
+                                  QueryParams p, uint8_t* out) {
+  const bool v = query2d_wave(minX, maxX, minY, maxY, maxZ, p.depth, p.hiz, p.tileMin, p.bX, p.pyrX, threadIdx.x);
+  if (threadIdx.x == 0) *out = v ? 1 : 0;
+}
+
+// Min of hiZ over each 8x8-block tile, one wave per tile.
+__global__ void rz_tile_min_kernel(const uint16_t* __restrict__ hiz, uint32_t bX, uint32_t bY,
+                                   uint16_t* __restrict__ tileMin, uint32_t pyrX, uint32_t pyrY) {
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t t = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  if (t >= pyrX * pyrY) return;
+  const uint32_t bx = (t % pyrX) * 8 + (lane & 7u), by = (t / pyrX) * 8 + (lane >> 3);
+  const uint32_t v = (bx < bX && by < bY) ? uint32_t(hiz[by * bX + bx]) : 0xFFFFu;
+  const uint32_t mn = __ockl_wfred_min_u32(v);
+  if (lane == 0) tileMin[t] = uint16_t(mn);
 }
 
 // readBackDepth (Rasterizer.cpp:351-399), one pixel per thread.
@@ -898,30 +1199,41 @@ hipError_t launch_frame(const FrameParams& p, int tileW, hipStream_t stream) {
   FrameParams local = p;
   void* args[] = {&local};
   const dim3 grid(p.tilesX * p.tilesY), block(256);
+  const void* fn = nullptr;
   if (tileW == 8)
-    return hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&rz_frame_kernel<8, 8>), grid, block, args, 0,
-                                      stream);
-  if (tileW == 16)
-    return hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&rz_frame_kernel<16, 16>), grid, block, args, 0,
-                                      stream);
-  return hipErrorInvalidValue;
+    fn = p.stats ? reinterpret_cast<const void*>(&rz_frame_kernel<8, 8, true>)
+                 : reinterpret_cast<const void*>(&rz_frame_kernel<8, 8, false>);
+  else if (tileW == 16)
+    fn = p.stats ? reinterpret_cast<const void*>(&rz_frame_kernel<16, 16, true>)
+                 : reinterpret_cast<const void*>(&rz_frame_kernel<16, 16, false>);
+  else
+    return hipErrorInvalidValue;
+  return hipLaunchCooperativeKernel(fn, grid, block, args, 0, stream);
 }
 
 hipError_t frame_max_groups(int tileW, int* maxGroups) {
-  int dev = 0, cus = 0, perCU = 0;
+  int dev = 0, cus = 0;
   hipError_t e = hipGetDevice(&dev);
   if (e != hipSuccess) return e;
   e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
   if (e != hipSuccess) return e;
-  if (tileW == 8)
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, reinterpret_cast<const void*>(&rz_frame_kernel<8, 8>),
-                                                     256, 0);
-  else if (tileW == 16)
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, reinterpret_cast<const void*>(&rz_frame_kernel<16, 16>),
-                                                     256, 0);
-  else
+  const void* variants[2];
+  if (tileW == 8) {
+    variants[0] = reinterpret_cast<const void*>(&rz_frame_kernel<8, 8, false>);
+    variants[1] = reinterpret_cast<const void*>(&rz_frame_kernel<8, 8, true>);
+  } else if (tileW == 16) {
+    variants[0] = reinterpret_cast<const void*>(&rz_frame_kernel<16, 16, false>);
+    variants[1] = reinterpret_cast<const void*>(&rz_frame_kernel<16, 16, true>);
+  } else {
     return hipErrorInvalidValue;
-  if (e != hipSuccess) return e;
+  }
+  int perCU = 1 << 30;
+  for (const void* fn : variants) {
+    int n = 0;
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fn, 256, 0);
+    if (e != hipSuccess) return e;
+    perCU = n < perCU ? n : perCU;
+  }
   *maxGroups = perCU * cus;
   return hipSuccess;
 }
@@ -935,14 +1247,20 @@ hipError_t launch_status(const ItemState* state, const uint32_t* visible, uint32
 
 hipError_t launch_query(const QueryParams& p, hipStream_t stream) {
   if (p.n == 0) return hipSuccess;
-  hipLaunchKernelGGL(rz_query_kernel, dim3(groups(p.n, 256)), dim3(256), 0, stream, p);
+  hipLaunchKernelGGL(rz_query_kernel, dim3(groups(p.n, 4)), dim3(256), 0, stream, p);
   return hipGetLastError();
 }
 
 hipError_t launch_query2d(uint32_t minX, uint32_t maxX, uint32_t minY, uint32_t maxY, uint32_t maxZ,
-                          const uint16_t* depth, const uint16_t* hiz, uint32_t bX, uint8_t* out, hipStream_t stream) {
-  hipLaunchKernelGGL(rz_query2d_kernel, dim3(1), dim3(64), 0, stream, minX, maxX, minY, maxY, maxZ, depth, hiz, bX,
-                     out);
+                          const QueryParams& p, uint8_t* out, hipStream_t stream) {
+  hipLaunchKernelGGL(rz_query2d_kernel, dim3(1), dim3(64), 0, stream, minX, maxX, minY, maxY, maxZ, p, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_tile_min(const uint16_t* hiz, uint32_t bX, uint32_t bY, uint16_t* tileMin, uint32_t pyrX,
+                           uint32_t pyrY, hipStream_t stream) {
+  hipLaunchKernelGGL(rz_tile_min_kernel, dim3(groups(pyrX * pyrY, 4)), dim3(256), 0, stream, hiz, bX, bY, tileMin,
+                     pyrX, pyrY);
   return hipGetLastError();
 }
 

@@ -231,7 +231,7 @@ def aabb_bounds(centers: np.ndarray, half: np.ndarray) -> np.ndarray:
     return b
 
 
-def config1(seed: int = 1, n_boxes: int = 1024, n_queries: int = 4096):
+def config1(seed: int = 1, n_boxes: int = 1024, n_queries: int = 4096, width: int = 1280, height: int = 720):
     """1,024 box occluders (centres U[-50,50]^3, half-extents log-U[0.5,5]),
     camera at the origin looking +z (60 deg vertical FOV, near 1, far 5000,
     1280x720), then 4,096 query boxes (half-extents log-U[0.1,2])."""
@@ -240,7 +240,6 @@ def config1(seed: int = 1, n_boxes: int = 1024, n_queries: int = 4096):
     quads = np.concatenate([box_quads(c[i], h[i]) for i in range(n_boxes)], axis=0)
     scene = bake_scene(quads)
     qc, qh = random_boxes(rng, n_queries, -50.0, 50.0, 0.1, 2.0)
-    width, height = 1280, 720
     eye = np.zeros(3, np.float32)
     mvp = view_projection(eye, (0, 0, 1), (0, 1, 0), math.pi / 3, width / height)
     scene.meta.update(width=width, height=height, eye=eye, mvp=mvp, queries=aabb_bounds(qc, qh), seed=seed)
