@@ -13,8 +13,10 @@ frame's buffer.  The draw order of each frame is computed before timing.
 Both phases run on inputs already resident in device memory.
 
 Also reports the roofline of the dominant kernel (the frame kernel), timed
-with HIP events on its own stream, and the CPU oracle (a scalar port of the
-reference) on this host's cores as the CPU baseline.
+with HIP events on its own stream, the CPU oracle (a scalar port of the
+reference) on this host's cores as the CPU baseline, and under "batched" the
+multi-view workload (64 views) through the batched persistent launch and
+through the plain single-view path.
 """
 
 from __future__ import annotations
@@ -79,6 +81,97 @@ def cpu_baseline(scene, orders, threads: int, frames: int, query_reps: int):
     }
 
 
+def views_bench(steps: int, warmup: int, seed: int = 3) -> dict:
+    """The multi-view workload (64 views of mixed resolution, per-view
+    occluder subsets of 1k-200k quads, 16,384 queries per view) two ways:
+    batched -- one graph replay per step of the persistent multi-view launch
+    (classify, draw, query every view) -- and plain -- the same views one by
+    one through the single-view path (clear, setModelViewProjection,
+    query-gated rz_draw_frame, then a query batch).  Units: submitted real
+    quads per second and queries per second, wall time over the timed steps."""
+    from rasterizer_amd import DeviceBuffer, Rasterizer, Views, scenes
+
+    t0 = time.perf_counter()
+    sc = scenes.config3(seed=seed)
+    views = sc.meta["views"]
+    nv = len(views)
+    total_quads = sum(v["quads"] for v in views)
+    total_q = sum(v["queries"].shape[0] for v in views)
+    pixels = sum(v["size"][0] * v["size"][1] for v in views)
+    log(f"views: {nv} views, {pixels:,} px, {total_quads:,} quads, {total_q:,} queries "
+        f"({time.perf_counter() - t0:.1f} s)")
+    n_mvp = 16
+    mvps = [[scenes.view_mvp(v, s) for v in views] for s in range(n_mvp)]
+
+    owner = Rasterizer(64, 64)
+    V = Views(owner, [v["size"] for v in views])
+    for k, v in enumerate(views):
+        V.set_occluders(k, v["occluders"])
+        V.set_queries(k, v["queries"])
+
+    def batched_step(s: int) -> None:
+        for k in range(nv):
+            V.set_mvp(k, mvps[s % n_mvp][k])
+        V.run(graph=True)
+
+    for s in range(warmup):
+        batched_step(s)
+    V.sync()
+    t0 = time.perf_counter()
+    for s in range(steps):
+        batched_step(s)
+    V.sync()
+    t_b = time.perf_counter() - t0
+    gpu_ms = []
+    for s in range(min(steps, 8)):
+        batched_step(s)
+        gpu_ms.append(V.sync())
+    info = V.info()
+    log(f"batched: {t_b * 1e3 / steps:.3f} ms/step (GPU {np.median(gpu_ms):.3f} ms), {info}")
+
+    # Plain: one context per view, the single-view frame path.
+    ctxs = [Rasterizer(*v["size"]) for v in views]
+    handles = [ctxs[k].handle_array(v["occluders"]) for k, v in enumerate(views)]
+    dq = [(DeviceBuffer(ctxs[k], v["queries"].nbytes, v["queries"]), DeviceBuffer(ctxs[k], v["queries"].shape[0]),
+           DeviceBuffer(ctxs[k], v["queries"].shape[0])) for k, v in enumerate(views)]
+
+    def plain_step(s: int) -> None:
+        for k, v in enumerate(views):
+            R = ctxs[k]
+            R.clear()
+            R.setModelViewProjection(mvps[s % n_mvp][k])
+            R.draw_frame_handles(handles[k], len(v["occluders"]))
+            R.query_batch_device(v["queries"].shape[0], *dq[k])
+
+    plain_steps = max(2, steps // 4)
+    for s in range(min(warmup, 2)):
+        plain_step(s)
+    for R in ctxs:
+        R.sync()
+    t0 = time.perf_counter()
+    for s in range(plain_steps):
+        plain_step(s)
+    for R in ctxs:
+        R.sync()
+    t_p = time.perf_counter() - t0
+    log(f"plain per-view: {t_p * 1e3 / plain_steps:.3f} ms/step")
+    return {
+        "workload": (f"{nv} views ({pixels:,} px: 16x2048^2, 24x1024^2, 16x512^2, 8x1920x1080), "
+                     f"{total_quads:,} quads submitted per step from a 240,000-quad box scene, "
+                     f"{total_q:,} AABB queries per step"),
+        "batched_quads_per_s": steps * total_quads / t_b,
+        "batched_queries_per_s": steps * total_q / t_b,
+        "batched_ms_per_step": t_b * 1e3 / steps,
+        "batched_gpu_ms_per_step": float(np.median(gpu_ms)),
+        "plain_quads_per_s": plain_steps * total_quads / t_p,
+        "plain_ms_per_step": t_p * 1e3 / plain_steps,
+        "speedup_vs_plain": (steps * total_quads / t_b) / (plain_steps * total_quads / t_p),
+        "steps": steps,
+        "plain_steps": plain_steps,
+        "learned_order": None,
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -87,6 +180,8 @@ def main() -> int:
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-frames", type=int, default=1024)
+    ap.add_argument("--no-views", action="store_true", help="skip the batched multi-view measurement")
+    ap.add_argument("--views-steps", type=int, default=32)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for {args.gpus}).",
@@ -194,8 +289,10 @@ def main() -> int:
         },
         "cpu_baseline": None,
     }
+    if not args.no_views:
+        out["batched"] = views_bench(args.views_steps, min(args.warmup, 4))
     if not args.no_cpu:
-        threads = min(os.cpu_count() or 1, 16)
+        threads = min(len(os.sched_getaffinity(0)), 16)
         t0 = time.perf_counter()
         out["cpu_baseline"] = cpu_baseline(scene, orders, threads, args.cpu_frames, 4)
         log(f"cpu baseline {time.perf_counter() - t0:.1f} s")

@@ -179,6 +179,61 @@ int rz_device_alloc(rz_context* ctx, size_t bytes, const void* init, void** devp
 int rz_device_read(rz_context* ctx, const void* devptr, size_t bytes, void* dst);
 int rz_device_free(void* devptr);
 
+/* ---------------------------------------------------------------------------
+ * Batched multi-view launches.  No counterpart in the reference class: they
+ * run the reference's per-view sequence for many views in one step.  For each
+ * view: clear (Rasterizer.cpp:107-121); then, per occluder in the view's
+ * order, the depth-independent part of queryVisibility (Rasterizer.cpp:123-
+ * 264: frustum cull, near-plane test, empty screen rectangle) and
+ * rasterize<needsClipping> (:606-1295) of the survivors, with no occlusion
+ * test between draws; then queryVisibility (:123-281) of the view's boxes
+ * against the finished buffer.  Each view's buffer, statuses and query bits
+ * equal that sequence on one Rasterizer.
+ * ------------------------------------------------------------------------- */
+typedef struct rz_views rz_views;
+
+/* nViews buffers of widths[i] x heights[i] (non-zero multiples of 8) on the
+ * current device, each zero-initialised with an identity matrix. */
+int rz_views_create(uint32_t nViews, const uint32_t* widths, const uint32_t* heights, rz_views** out);
+int rz_views_destroy(rz_views* views);
+/* setModelViewProjection of one view (takes effect at the next run). */
+int rz_views_set_mvp(rz_views* views, uint32_t view, const float matrix[16]);
+/* The view's occluders in draw order (handles from rz_occluder_create on a
+ * context of the same device; they must outlive their use here). */
+int rz_views_set_occluders(rz_views* views, uint32_t view, uint32_t n, const rz_occluder* const* occluders);
+/* The view's query boxes (n x 8 floats: min xyzw, max xyzw), copied. */
+int rz_views_set_queries(rz_views* views, uint32_t view, uint32_t n, const float* bounds);
+/* Enqueues one step of every view.  useGraph != 0 replays a hipGraph of the
+ * step, captured at the first run after an occluder list or query count
+ * changed; 0 launches the same kernels directly. */
+int rz_views_run(rz_views* views, int useGraph);
+/* Waits for the last step; *ms (optional) is its GPU time between events. */
+int rz_views_sync(rz_views* views, float* ms);
+/* Results of the last step for one view (block until it has finished):
+ * raw depth + hiZ as rz_readback_raw; query bits as rz_query_batch; per
+ * occluder status 0 skipped, 1 rasterize<false>, 2 rasterize<true>. */
+int rz_views_read_raw(rz_views* views, uint32_t view, uint16_t* depth, uint16_t* hiz);
+int rz_views_read_queries(rz_views* views, uint32_t view, uint8_t* visible, uint8_t* needsClipping);
+int rz_views_read_status(rz_views* views, uint32_t view, uint8_t* status);
+/* info[0..7]: views, items, quads, (view, tile) units, queries, raster
+ * work-groups, graph captured, plan out of date. */
+int rz_views_get_info(rz_views* views, uint32_t info[8]);
+
+/* Draw-order learning (no counterpart in the reference).  With labels on,
+ * every step also computes, per 8-quad batch (the k-th batch of a view is the
+ * k-th in its occluder order): 16 features before the draw and, after it,
+ * the label = number of blocks in which the batch keeps some pixel of the
+ * finished buffer (each quad sampled alone as the block loop samples it,
+ * without hiZ test or merge; a pixel keeps the earliest batch whose sample
+ * equals its non-zero depth in a written block). */
+int rz_views_enable_labels(rz_views* views, int enable);
+/* Draws the view's batches in the given order (a permutation of its batch
+ * numbers 0..n-1), each with its occluder's classification; n = 0 restores
+ * occluder order. */
+int rz_views_set_batch_order(rz_views* views, uint32_t view, uint32_t n, const uint32_t* order);
+/* Labels (u32 per batch) and features (16 floats per batch) of the last step. */
+int rz_views_read_labels(rz_views* views, uint32_t view, uint32_t* labels, float* features);
+
 #ifdef __cplusplus
 }
 #endif

@@ -65,6 +65,9 @@ def lib() -> ctypes.CDLL:
             "or_reset_stats": ([P], None),
             "or_block_stats": ([P, P], None),
             "or_frame": ([P, U, P, P], None),
+            "or_frame_classified": ([P, U, P, P], None),
+            "or_labels": ([P, U, P, P, P], None),
+            "or_setup_dump": ([P, P, U, P, P, ctypes.c_int, P], ctypes.c_int),
             "or_bake": ([P, U, P, P, P, P, P, P], ctypes.c_int),
             "or_x86_rcp": ([ctypes.c_float], ctypes.c_float),
             "or_x86_cvtt": ([ctypes.c_float], ctypes.c_int32),
@@ -191,10 +194,32 @@ class OracleRasterizer:
         lib().or_block_stats(self._r, _ptr(out))
         return out
 
-    def draw_frame(self, occluders) -> np.ndarray:
+    def setup_dump(self, occ, clip: bool) -> np.ndarray:
+        """Per-quad setup of rasterize<clip> for one occluder (24 floats per
+        quad: valid, mode, minX, minY, rangeX, rangeY, maxZ, ...)."""
+        pk = np.ascontiguousarray(occ.packets, dtype=np.uint32)
+        out = np.zeros((pk.size // 4, 24), np.float32)
+        n = lib().or_setup_dump(self._r, _ptr(pk), pk.size // 8, _ptr(_f4(occ.m_refMin)), _ptr(_f4(occ.m_refMax)),
+                                1 if clip else 0, _ptr(out))
+        return out[:n]
+
+    def labels(self, occluders, status) -> np.ndarray:
+        """Per-batch draw-order labels of the current buffer (see oracle.h)."""
+        arr, keep = occluder_structs(occluders)
+        st = np.ascontiguousarray(status, np.uint8)
+        nb = sum(int(np.asarray(o.packets).size) // 32 for o in occluders)
+        out = np.zeros(max(nb, 1), np.uint32)
+        lib().or_labels(self._r, len(occluders), ctypes.cast(arr, ctypes.c_void_p), _ptr(st), _ptr(out))
+        del keep
+        return out[:nb]
+
+    def draw_frame(self, occluders, gated: bool = True) -> np.ndarray:
+        """Main.cpp:192-206 (gated) or the batched path's classify-then-draw
+        sequence (gated=False); returns per-occluder status 0/1/2."""
         arr, keep = occluder_structs(occluders)
         status = np.zeros(max(len(occluders), 1), np.uint8)
-        lib().or_frame(self._r, len(occluders), ctypes.cast(arr, ctypes.c_void_p), _ptr(status))
+        fn = lib().or_frame if gated else lib().or_frame_classified
+        fn(self._r, len(occluders), ctypes.cast(arr, ctypes.c_void_p), _ptr(status))
         del keep
         return status[: len(occluders)]
 

@@ -99,6 +99,10 @@ struct OrRaster {
   uint16_t* hiz;      /* blocksX*blocksY + 8 */
   uint64_t stats[7];
   uint32_t* blockVisits; /* optional per-block counters: [2*b] visited, [2*b+1] written */
+  /* Label mode (or_labels): quads are compared with the buffer instead of
+   * drawn; winner[pixel] keeps the smallest batch index that matches. */
+  uint32_t* winner;
+  uint32_t curBatch;
 };
 
 /* transposeMask (Rasterizer.cpp:527-545): pixel (x, y), sampled as bit
@@ -484,7 +488,7 @@ static void traverse_quad(OrRaster* r, const QuadSetup* Q) {
       if (r->blockVisits) r->blockVisits[2 * blk]++;
       uint64_t mask = 0;
       int skip = 0;
-      if (hz >= Q->maxZ) {
+      if (!r->winner && hz >= Q->maxZ) {
         r->stats[1]++;
         skip = 1;
       } else if (Q->mode == Convex) {
@@ -543,6 +547,18 @@ static void traverse_quad(OrRaster* r, const QuadSetup* Q) {
             rows[4 + p][xx] = d2;
             rows[6 + p][xx] = d3;
           }
+        if (r->winner) {
+          /* Label mode: a covered pixel whose sampled depth equals the
+           * written buffer's names this quad's batch as a candidate. */
+          for (int yy = 0; yy < 8; ++yy)
+            for (int xx = 0; xx < 8; ++xx) {
+              uint16_t d = rows[yy][xx];
+              if (!((mask >> mask_bit(xx, yy)) & 1) || d == 0 || hz == 1 || out[8 * yy + xx] != d) continue;
+              uint32_t* w = &r->winner[(size_t)blk * 64 + 8 * yy + xx];
+              if (r->curBatch < *w) *w = r->curBatch;
+            }
+          goto next_block;
+        }
         uint16_t mn = 0xFFFF;
         for (int yy = 0; yy < 8; ++yy)
           for (int xx = 0; xx < 8; ++xx) {
@@ -559,6 +575,7 @@ static void traverse_quad(OrRaster* r, const QuadSetup* Q) {
         r->stats[4]++;
         if (r->blockVisits) r->blockVisits[2 * blk + 1]++;
       }
+    next_block:
       for (int k = 0; k < 8; ++k) depth[k] = dDx + depth[k];
       for (int e = 0; e < 4; ++e) offset[e] = Q->nx[e] + offset[e];
     }
@@ -641,7 +658,10 @@ int or_query2d(const OrRaster* r, uint32_t minX, uint32_t maxX, uint32_t minY, u
   return 0;
 }
 
-static int query_impl(const OrRaster* r, const float bmin[4], const float bmax[4], int* needsClipping) {
+/* queryVisibility; with depthTest == 0 it stops before query2D and returns 1
+ * for every box that reaches it (the depth-independent classification). */
+static int query_impl_ex(const OrRaster* r, const float bmin[4], const float bmax[4], int* needsClipping,
+                         int depthTest) {
   float ext[4], cen[4];
   for (int j = 0; j < 4; ++j) {
     ext[j] = bmax[j] - bmin[j];
@@ -734,6 +754,7 @@ static int query_impl(const OrRaster* r, const float bmin[4], const float bmax[4
   bi[1] = -bi[1];
   bi[3] = -bi[3];
   if (bi[0] >= bi[1] || bi[2] >= bi[3]) return 0;
+  if (!depthTest) return 1;
 
   /* Max depth of the corners (:271-273). */
   uint16_t maxZ = 0;
@@ -742,6 +763,10 @@ static int query_impl(const OrRaster* r, const float bmin[4], const float bmax[4
     if (d > maxZ) maxZ = d;
   }
   return or_query2d(r, (uint32_t)bi[0], (uint32_t)bi[1], (uint32_t)bi[2], (uint32_t)bi[3], maxZ);
+}
+
+static int query_impl(const OrRaster* r, const float bmin[4], const float bmax[4], int* needsClipping) {
+  return query_impl_ex(r, bmin, bmax, needsClipping, 1);
 }
 
 int or_query(OrRaster* r, const float bmin[4], const float bmax[4], int* needsClipping) {
@@ -815,6 +840,49 @@ void or_frame(OrRaster* r, uint32_t n, const OrOccluder* occ, uint8_t* status) {
     int clip = 0;
     uint8_t st = 0;
     if (query_impl(r, occ[i].boundsMin, occ[i].boundsMax, &clip)) {
+      or_rasterize(r, occ[i].packets, occ[i].packetCount, occ[i].refMin, occ[i].refMax, clip);
+      st = clip ? 2 : 1;
+    }
+    if (status) status[i] = st;
+  }
+}
+
+void or_labels(OrRaster* r, uint32_t n, const OrOccluder* occ, const uint8_t* status, uint32_t* labels) {
+  const size_t nb = (size_t)r->blocksX * r->blocksY;
+  uint32_t* winner = (uint32_t*)malloc(nb * 64 * sizeof(uint32_t));
+  for (size_t i = 0; i < nb * 64; ++i) winner[i] = 0xFFFFFFFFu;
+  r->winner = winner;
+  uint32_t batch = 0;
+  for (uint32_t i = 0; i < n; ++i) {
+    const uint32_t batches = occ[i].packetCount / 4;
+    if (status[i]) {
+      for (uint32_t b = 0; b < batches; ++b) {
+        r->curBatch = batch + b;
+        or_rasterize(r, occ[i].packets + (size_t)b * 32, 4, occ[i].refMin, occ[i].refMax, status[i] == 2);
+      }
+    }
+    batch += batches;
+  }
+  r->winner = NULL;
+  for (uint32_t b = 0; b < batch; ++b) labels[b] = 0;
+  for (size_t k = 0; k < nb; ++k) {
+    /* Distinct winners of the block, each counted once. */
+    for (int p = 0; p < 64; ++p) {
+      uint32_t w = winner[k * 64 + p];
+      if (w == 0xFFFFFFFFu) continue;
+      int seen = 0;
+      for (int q = 0; q < p && !seen; ++q) seen = winner[k * 64 + q] == w;
+      if (!seen) labels[w]++;
+    }
+  }
+  free(winner);
+}
+
+void or_frame_classified(OrRaster* r, uint32_t n, const OrOccluder* occ, uint8_t* status) {
+  for (uint32_t i = 0; i < n; ++i) {
+    int clip = 0;
+    uint8_t st = 0;
+    if (query_impl_ex(r, occ[i].boundsMin, occ[i].boundsMax, &clip, 0)) {
       or_rasterize(r, occ[i].packets, occ[i].packetCount, occ[i].refMin, occ[i].refMax, clip);
       st = clip ? 2 : 1;
     }

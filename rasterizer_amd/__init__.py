@@ -73,6 +73,20 @@ _SIGNATURES = {
     "rz_device_alloc": ([_P, ctypes.c_size_t, _P, ctypes.POINTER(_P)], ctypes.c_int),
     "rz_device_read": ([_P, _P, ctypes.c_size_t, _P], ctypes.c_int),
     "rz_device_free": ([_P], ctypes.c_int),
+    "rz_views_create": ([_U32, _P, _P, ctypes.POINTER(_P)], ctypes.c_int),
+    "rz_views_destroy": ([_P], ctypes.c_int),
+    "rz_views_set_mvp": ([_P, _U32, _P], ctypes.c_int),
+    "rz_views_set_occluders": ([_P, _U32, _U32, _P], ctypes.c_int),
+    "rz_views_set_queries": ([_P, _U32, _U32, _P], ctypes.c_int),
+    "rz_views_run": ([_P, ctypes.c_int], ctypes.c_int),
+    "rz_views_sync": ([_P, ctypes.POINTER(ctypes.c_float)], ctypes.c_int),
+    "rz_views_read_raw": ([_P, _U32, _P, _P], ctypes.c_int),
+    "rz_views_read_queries": ([_P, _U32, _P, _P], ctypes.c_int),
+    "rz_views_read_status": ([_P, _U32, _P], ctypes.c_int),
+    "rz_views_get_info": ([_P, _P], ctypes.c_int),
+    "rz_views_enable_labels": ([_P, ctypes.c_int], ctypes.c_int),
+    "rz_views_set_batch_order": ([_P, _U32, _U32, _P], ctypes.c_int),
+    "rz_views_read_labels": ([_P, _U32, _P, _P], ctypes.c_int),
 }
 
 
@@ -349,4 +363,100 @@ class Rasterizer:
         return float(ms.value)
 
 
-__all__ = ["Rasterizer", "Occluder", "DeviceBuffer", "RasterizerError", "lib", "LIB_PATH"]
+class Views:
+    """Batched multi-view launches (rz_views_*): per view, clear, classify each
+    occluder with the depth-independent part of queryVisibility and rasterize
+    it, then query the view's boxes; all views in one step.  `owner` is a
+    Rasterizer whose context uploads the occluders (same device)."""
+
+    def __init__(self, owner: "Rasterizer", sizes):
+        self._owner = owner
+        w = np.ascontiguousarray([int(a) for a, _ in sizes], np.uint32)
+        h = np.ascontiguousarray([int(b) for _, b in sizes], np.uint32)
+        self._v = ctypes.c_void_p()
+        _check(lib().rz_views_create(len(sizes), _ptr(w), _ptr(h), ctypes.byref(self._v)), "rz_views_create")
+        self.sizes = [(int(a), int(b)) for a, b in sizes]
+        self._occ = [[] for _ in sizes]
+        self._nq = [0] * len(sizes)
+
+    def close(self) -> None:
+        if self._v and _lib is not None:
+            _lib.rz_views_destroy(self._v)
+        self._v = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_mvp(self, view: int, matrix) -> None:
+        m = np.ascontiguousarray(np.asarray(matrix, dtype=np.float32).reshape(16))
+        _check(lib().rz_views_set_mvp(self._v, view, _ptr(m)), "rz_views_set_mvp")
+
+    def set_occluders(self, view: int, occluders) -> None:
+        self._occ[view] = list(occluders)  # keeps the device copies alive
+        handles = self._owner.handle_array(occluders)
+        _check(lib().rz_views_set_occluders(self._v, view, len(occluders), ctypes.cast(handles, ctypes.c_void_p)),
+               "rz_views_set_occluders")
+
+    def set_queries(self, view: int, bounds) -> None:
+        b = np.ascontiguousarray(np.asarray(bounds, dtype=np.float32).reshape(-1, 8))
+        _check(lib().rz_views_set_queries(self._v, view, b.shape[0], _ptr(b)), "rz_views_set_queries")
+        self._nq[view] = b.shape[0]
+
+    def run(self, graph: bool = True) -> None:
+        _check(lib().rz_views_run(self._v, 1 if graph else 0), "rz_views_run")
+
+    def sync(self) -> float:
+        ms = ctypes.c_float(0.0)
+        _check(lib().rz_views_sync(self._v, ctypes.byref(ms)), "rz_views_sync")
+        return float(ms.value)
+
+    def raw(self, view: int):
+        w, h = self.sizes[view]
+        nb = (w // 8) * (h // 8)
+        depth = np.zeros(nb * 64, np.uint16)
+        hiz = np.zeros(nb + 8, np.uint16)
+        _check(lib().rz_views_read_raw(self._v, view, _ptr(depth), _ptr(hiz)), "rz_views_read_raw")
+        return depth, hiz
+
+    def queries(self, view: int):
+        n = self._nq[view]
+        vis = np.zeros(max(n, 1), np.uint8)
+        clip = np.zeros(max(n, 1), np.uint8)
+        _check(lib().rz_views_read_queries(self._v, view, _ptr(vis), _ptr(clip)), "rz_views_read_queries")
+        return vis[:n].astype(bool), clip[:n].astype(bool)
+
+    def status(self, view: int) -> np.ndarray:
+        n = len(self._occ[view])
+        out = np.zeros(max(n, 1), np.uint8)
+        _check(lib().rz_views_read_status(self._v, view, _ptr(out)), "rz_views_read_status")
+        return out[:n]
+
+    def enable_labels(self, enable: bool = True) -> None:
+        _check(lib().rz_views_enable_labels(self._v, 1 if enable else 0), "rz_views_enable_labels")
+
+    def n_batches(self, view: int) -> int:
+        return sum(int(o.packets.size) // 32 for o in self._occ[view])
+
+    def set_batch_order(self, view: int, order=None) -> None:
+        o = np.ascontiguousarray(np.zeros(0) if order is None else order, np.uint32)
+        _check(lib().rz_views_set_batch_order(self._v, view, o.size, _ptr(o) if o.size else None),
+               "rz_views_set_batch_order")
+
+    def labels(self, view: int):
+        n = self.n_batches(view)
+        lab = np.zeros(max(n, 1), np.uint32)
+        feat = np.zeros((max(n, 1), 16), np.float32)
+        _check(lib().rz_views_read_labels(self._v, view, _ptr(lab), _ptr(feat)), "rz_views_read_labels")
+        return lab[:n], feat[:n]
+
+    def info(self) -> dict:
+        a = np.zeros(8, np.uint32)
+        _check(lib().rz_views_get_info(self._v, _ptr(a)), "rz_views_get_info")
+        keys = ["views", "items", "quads", "units", "queries", "groups", "graph", "dirty"]
+        return dict(zip(keys, (int(x) for x in a)))
+
+
+__all__ = ["Rasterizer", "Occluder", "DeviceBuffer", "Views", "RasterizerError", "lib", "LIB_PATH"]

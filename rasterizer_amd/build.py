@@ -64,6 +64,7 @@ def build(verbose: bool = False) -> Path:
         "dropin.o": ["g++", *HOST_FLAGS, "-c", str(CSRC / "dropin.cpp")],
         "rz_kernels.o": [hipcc, *HIP_FLAGS, "-c", str(CSRC / "rz_kernels.hip")],
         "rz_api.o": [hipcc, *HIP_FLAGS, "-c", str(CSRC / "rz_api.cpp")],
+        "rz_views.o": [hipcc, *HIP_FLAGS, "-c", str(CSRC / "rz_views.cpp")],
     }
     cmds = [cmd + ["-o", str(BUILD / name)] for name, cmd in objs.items()]
     if verbose:

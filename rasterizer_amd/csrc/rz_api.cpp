@@ -11,6 +11,7 @@
 #include <vector>
 
 #include "rz_host.h"
+#include "rz_internal.h"
 #include "rz_kernels.h"
 
 using namespace rzd;
@@ -21,25 +22,6 @@ constexpr int kStagingSlots = 8;
 constexpr int kProfSlots = 256;
 constexpr uint64_t kTimeoutTicks = 200000000ull;  // 2 s of the 100 MHz s_memrealtime clock
 
-struct Tables {
-  bool built = false;
-  int status = RZ_OK;
-  uint64_t cov[4096];
-  uint32_t rcp[4096];
-};
-
-// Both tables depend only on this host (libm, RCPPS); build them once.
-const Tables& host_tables() {
-  static Tables t;
-  static std::once_flag once;
-  std::call_once(once, [] {
-    if (!rzh::build_coverage_table(t.cov)) t.status = RZ_ERR_UNSUPPORTED_HOST;
-    else if (rzh::build_rcp_table(t.rcp) != 0) t.status = RZ_ERR_UNSUPPORTED_HOST;
-    t.built = true;
-  });
-  return t;
-}
-
 template <typename T>
 hipError_t dev_alloc(T** p, size_t count) {
   *p = nullptr;
@@ -49,12 +31,20 @@ hipError_t dev_alloc(T** p, size_t count) {
 
 }  // namespace
 
-struct rz_occluder {
-  rz_context* ctx;
-  uint32_t* d_packets;
-  uint32_t packetCount;
-  float refMin[4], refMax[4], boundsMin[4], boundsMax[4];
-};
+namespace rzi {
+
+// Both tables depend only on this host (libm, RCPPS); build them once.
+const HostTables& host_tables() {
+  static HostTables t;
+  static std::once_flag once;
+  std::call_once(once, [] {
+    if (!rzh::build_coverage_table(t.cov)) t.status = RZ_ERR_UNSUPPORTED_HOST;
+    else if (rzh::build_rcp_table(t.rcp) != 0) t.status = RZ_ERR_UNSUPPORTED_HOST;
+  });
+  return t;
+}
+
+}  // namespace rzi
 
 struct rz_context {
   uint32_t width = 0, height = 0, bX = 0, bY = 0;
@@ -167,7 +157,7 @@ extern "C" int rz_create(uint32_t width, uint32_t height, rz_context** out) {
   if (uint64_t(width / 8) * (height / 8) >= (1u << 24)) return RZ_ERR_TOO_LARGE;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return RZ_ERR_DEVICE;
-  const Tables& tabs = host_tables();
+  const rzi::HostTables& tabs = rzi::host_tables();
   if (tabs.status != RZ_OK) return tabs.status;
 
   rz_context* c = new (std::nothrow) rz_context();
@@ -395,7 +385,7 @@ static int enqueue_draw(rz_context* c, uint32_t n, const rz_occluder* const* occ
     d.quadBase = base;
     d.quadCount = (o->packetCount / 4) * 8;
     d.kind = kinds[i];
-    d.pad = 0;
+    d.view = 0;
     std::memcpy(d.refMin, o->refMin, 16);
     std::memcpy(d.refMax, o->refMax, 16);
     std::memcpy(d.boundsMin, o->boundsMin, 16);

@@ -18,6 +18,10 @@ struct PrepParams {
   uint32_t* done;
   uint32_t* visible;
   Mvp mvp;
+  // Multi-view launches: per-item view table (width, height and matrix come
+  // from views[item.view]) and the work-queue counter that prep resets.
+  const ViewDesc* views;
+  uint32_t* queue;
 };
 
 struct SetupParams {
@@ -29,6 +33,30 @@ struct SetupParams {
   QuadRec* recs;
   uint64_t* bbox;
   uint64_t* groupBox;  // box of each aligned group of 64 quads (union of valid quads)
+  const ViewDesc* views;  // multi-view launches: bX, bY from views[item.view]
+};
+
+// Persistent multi-view raster launch: work units are (view, tile) pairs taken
+// from an atomic queue in view order; no unit waits on another.
+struct ViewsParams {
+  const ItemDesc* items;
+  const ItemState* state;
+  const ViewDesc* views;
+  uint32_t nViews, nUnits;
+  const QuadRec* recs;
+  const uint64_t* groupBox;
+  const uint64_t* cov;
+  uint32_t* queue;
+  // Draw batch items instead of occluder items: view v's items are
+  // [views[v].batchBase, + nBatches) of items/state.
+  uint32_t batchItems;
+};
+
+// queryVisibility over the queries of every view of a multi-view launch.
+struct ViewsQueryParams {
+  const ViewDesc* views;
+  uint32_t nViews, nQueries;
+  const uint32_t* rcp;
 };
 
 struct FrameParams {
@@ -66,9 +94,9 @@ enum FrameStat : int {
   kStatLoadTicks = 10,   // table, depth and hiZ staging at entry
   kStatLoopTicks = 11,   // the whole item loop
   kStatDrawTicks = 12,   // staged-quad drawing (inside raster ticks)
-  kStatSpare0 = 13,
-  kStatSpare1 = 14,
-  kStatSpare2 = 15,
+  kStatBlocksVisited = 13,   // (quad, block) pairs visited by the block loop
+  kStatBlocksRejected = 14,  // of which rejected by hiZ or the edge tests
+  kStatBlocksWritten = 15,   // of which written
   kFrameStatWords = 16,
 };
 
@@ -105,5 +133,41 @@ hipError_t launch_tile_min(const uint16_t* hiz, uint32_t bX, uint32_t bY, uint16
 hipError_t launch_readback(const uint16_t* depth, const uint16_t* hiz, uint32_t width, uint32_t height, uint32_t bX,
                            uint32_t* rgba, hipStream_t stream);
 hipError_t launch_fill_u16(uint16_t* p, uint32_t n, uint16_t v, hipStream_t stream);
+// Multi-view: grid of `groups` persistent work-groups over p.nUnits units of
+// 8x8-block tiles.
+hipError_t launch_views(const ViewsParams& p, uint32_t groups, hipStream_t stream);
+hipError_t views_max_groups(int* maxGroups);
+hipError_t launch_views_query(const ViewsQueryParams& p, hipStream_t stream);
+// hiZ := 1 (whole 8-entry groups) and tile min := 1 for every view.
+hipError_t launch_views_clear(const ViewDesc* views, uint32_t nViews, uint32_t maxEntries, hipStream_t stream);
+
+// ---------------------------------------------------------------------------
+// Draw-order learning (rz_learn.hip).  Batches are numbered over the launch:
+// batch b holds quads 8b .. 8b+7 of the setup records.
+// ---------------------------------------------------------------------------
+struct LearnParams {
+  const ItemDesc* items;
+  const ItemState* state;
+  uint32_t nItems;
+  const ViewDesc* views;
+  uint32_t nViews;
+  const QuadRec* recs;
+  const uint64_t* bbox;
+  uint32_t nQuads;
+  const uint64_t* cov;
+  uint32_t* labels;    // per batch
+  float* features;     // per batch, kFeatures floats
+};
+constexpr int kFeatures = 16;
+
+// Labels of the finished buffers: per-pixel earliest matching batch, then
+// per batch the number of blocks it keeps (see rz_learn.hip).
+hipError_t launch_labels(const LearnParams& p, uint32_t maxBlocks, hipStream_t stream);
+hipError_t launch_features(const LearnParams& p, uint32_t nBatches, hipStream_t stream);
+// Batch items in the order `order` (per view segment, batch numbers): item k
+// draws batch order[k] with its occluder's status and the batch's footprint.
+hipError_t launch_batch_items(const LearnParams& p, const uint32_t* order, uint32_t nBatches, ItemDesc* outItems,
+                              ItemState* outState, hipStream_t stream);
+hipError_t launch_fill_u32(uint32_t* p, uint32_t n, uint32_t v, hipStream_t stream);
 
 }  // namespace rzd

@@ -16,6 +16,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "rz_x86.h"
 
 extern "C" __device__ uint32_t __ockl_wfred_min_u32(uint32_t);
@@ -215,7 +217,17 @@ __global__ void __launch_bounds__(256) rz_prep_kernel(PrepParams p) {
   if (i >= p.nItems) return;
   const ItemDesc& D = p.items[i];
   ItemState& S = p.state[i];
-  const float* P = p.mvp.prebaked;
+  const Mvp* M = &p.mvp;
+  uint32_t width = p.width, height = p.height, tileW = p.tileW, tileH = p.tileH;
+  if (p.views) {
+    const ViewDesc& V = p.views[D.view];
+    M = &V.mvp;
+    width = V.width;
+    height = V.height;
+    tileW = tileH = 8;
+  }
+  if (i == 0 && p.queue) *p.queue = 0u;
+  const float* P = M->prebaked;
   float ext[3];
 #pragma unroll
   for (int j = 0; j < 3; ++j) ext[j] = D.refMax[j] - D.refMin[j];
@@ -262,8 +274,12 @@ __global__ void __launch_bounds__(256) rz_prep_kernel(PrepParams p) {
     status = kStatusNoClip;
   } else if (D.kind == kItemForceClip) {
     status = kStatusClip;
+  } else if (D.kind == kItemClassify) {
+    // Frustum cull, near-plane test and empty rectangle only; no query2D.
+    const QueryPre Q = query_pre(*M, D.boundsMin, D.boundsMax, width, height, p.rcp);
+    status = Q.kind == 1 ? kStatusClip : (Q.kind == 3 ? kStatusNoClip : kStatusSkip);
   } else {
-    const QueryPre Q = query_pre(p.mvp, D.boundsMin, D.boundsMax, p.width, p.height, p.rcp);
+    const QueryPre Q = query_pre(*M, D.boundsMin, D.boundsMax, width, height, p.rcp);
     if (Q.kind == 1) {
       status = kStatusClip;
     } else if (Q.kind == 3) {
@@ -273,8 +289,8 @@ __global__ void __launch_bounds__(256) rz_prep_kernel(PrepParams p) {
       S.qMinY = Q.minY;
       S.qMaxY = Q.maxY;
       S.qMaxZ = Q.maxZ;
-      const uint32_t tx0 = (Q.minX >> 3) / p.tileW, tx1 = (Q.maxX >> 3) / p.tileW;
-      const uint32_t ty0 = (Q.minY >> 3) / p.tileH, ty1 = (Q.maxY >> 3) / p.tileH;
+      const uint32_t tx0 = (Q.minX >> 3) / tileW, tx1 = (Q.maxX >> 3) / tileW;
+      const uint32_t ty0 = (Q.minY >> 3) / tileH, ty1 = (Q.maxY >> 3) / tileH;
       S.expected = (tx1 - tx0 + 1) * (ty1 - ty0 + 1);
     } else {
       status = kStatusSkip;
@@ -515,15 +531,20 @@ __global__ void __launch_bounds__(256) rz_setup_kernel(SetupParams p) {
     const ItemState& S = p.state[item];
     const uint32_t st = S.status;
     if (st != kStatusSkip) {
+      uint32_t bX = p.bX, bY = p.bY;
+      if (p.views) {
+        bX = p.views[D.view].bX;
+        bY = p.views[D.view].bY;
+      }
       const uint32_t q = g - D.quadBase;
       const uint32_t* pk = D.packets + (q >> 3) * 32 + (q & 7);
       const uint32_t I[4] = {pk[0], pk[8], pk[16], pk[24]};
       SetupOut o;
-      setup_quad(S, I, st == kStatusClip, p.bX, p.bY, p.rcp, o);
+      setup_quad(S, I, st == kStatusClip, bX, bY, p.rcp, o);
       if (o.valid) {
         valid = true;
         p.recs[g] = o.rec;
-        box = quad_box(o.rec.fb, o.rec.range & 0xFFFFu, o.rec.range >> 16, p.bX, p.bY);
+        box = quad_box(o.rec.fb, o.rec.range & 0xFFFFu, o.rec.range >> 16, bX, bY);
       }
     }
     if (!valid) p.recs[g].range = 0u;  // an empty rectangle marks the quad as not drawn
@@ -599,6 +620,7 @@ struct TileSmem {
   QuadRec recs[kQuadChunk];        // setup records of the quads being drawn
   uint32_t wcnt[4];
   uint32_t part[2][4];
+  uint32_t blockStats[3];  // observability builds: visited, rejected, written
   uint32_t vis[2];
 };
 
@@ -633,11 +655,21 @@ struct QuadConst {
 // depth samples in lanes 12g .. 12g+11 (offsets first).  The rejections come
 // first and each only waits for the LDS reads it needs: most visited blocks
 // end at the hiZ test.
-template <int TBX, int TBY>
+// Per-wave counters of the block loop (observability builds only): blocks
+// visited, rejected by hiZ or by the edge tests, and written.
+struct BlockCounters {
+  uint32_t visited = 0, rejected = 0, written = 0;
+};
+
+template <int TBX, int TBY, bool STATS>
 __device__ __forceinline__ void raster_block(TileSmem<TBX, TBY>& sm, int lb, float s, int g, uint32_t lane,
-                                             int kLane, const QuadConst& Q) {
+                                             int kLane, const QuadConst& Q, BlockCounters& bc) {
+  if (STATS) bc.visited++;
   const uint32_t hz = sm.hiz[lb];
-  if (hz >= Q.maxZ) return;
+  if (hz >= Q.maxZ) {
+    if (STATS) bc.rejected++;
+    return;
+  }
   float o[4];
   uint32_t idx[4];
   bool outside = false;
@@ -652,7 +684,10 @@ __device__ __forceinline__ void raster_block(TileSmem<TBX, TBY>& sm, int lb, flo
       idx[e] = (Q.sl[e] | uint32_t(min(max(t, 0), 63))) & 4095u;
     }
   }
-  if (outside) return;
+  if (outside) {
+    if (STATS) bc.rejected++;
+    return;
+  }
   const uint64_t A = sm.cov[idx[0]], B = sm.cov[idx[1]], C = sm.cov[idx[2]], D = sm.cov[idx[3]];
   uint64_t mask;
   if (Q.mode == kModeConvex) mask = (A & B) & (C & D);
@@ -661,7 +696,11 @@ __device__ __forceinline__ void raster_block(TileSmem<TBX, TBY>& sm, int lb, flo
   else if (Q.mode == kModeConcaveRight) mask = (A | D) & (B & C);
   else if (Q.mode == kModeConcaveCenter) mask = (A & B) | (C & D);
   else mask = (A & D) & (B | C);
-  if (Q.mode != kModeConvex && mask == 0) return;
+  if (Q.mode != kModeConvex && mask == 0) {
+    if (STATS) bc.rejected++;
+    return;
+  }
+  if (STATS) bc.written++;
 
   const uint32_t old = sm.depth[lb * 64 + int(lane)];
   float v = __shfl(s, 12 * g + 4 + kLane);
@@ -690,10 +729,10 @@ __device__ __forceinline__ void raster_block(TileSmem<TBX, TBY>& sm, int lb, flo
 // once per block row, then the column step once per block column, each add
 // rounded.  Lane 12g + e carries component e (4 offsets, 8 depth samples) for
 // the g-th of up to five rows handled together.
-template <int TBX, int TBY>
+template <int TBX, int TBY, bool STATS>
 __device__ __forceinline__ void raster_quad_wave(const QuadRec& R, TileSmem<TBX, TBY>& sm, int tbx0, int tbx1,
                                                  int tby0, int tby1, int bX, float invBX, uint32_t wave,
-                                                 uint32_t lane) {
+                                                 uint32_t lane, BlockCounters& bc) {
   const int fb = int(R.fb);
   const int rangeX = int(R.range & 0xFFFFu), rangeY = int(R.range >> 16);
   QuadConst Q;
@@ -735,7 +774,7 @@ __device__ __forceinline__ void raster_quad_wave(const QuadRec& R, TileSmem<TBX,
         const int r = div_bx(d, bX, invBX), c = d - r * bX;
         if (r >= rangeY || c >= rangeX) continue;
         const float s = fold_n(fold_n(base, stepY, r), stepX, c);
-        raster_block<TBX, TBY>(sm, (by - tby0) * TBX + (bx - tbx0), s, 0, lane, kLane, Q);
+        raster_block<TBX, TBY, STATS>(sm, (by - tby0) * TBX + (bx - tbx0), s, 0, lane, kLane, Q, bc);
       }
     return;
   }
@@ -761,7 +800,7 @@ __device__ __forceinline__ void raster_quad_wave(const QuadRec& R, TileSmem<TBX,
       for (int gg = 0; gg < 5 && py + gg <= y1; ++gg) {
         const int lbx = bx - tbx0, lby = py + gg - tby0;
         if (block_owner(lbx, lby) != wave) continue;
-        raster_block<TBX, TBY>(sm, lby * TBX + lbx, s, gg, lane, kLane, Q);
+        raster_block<TBX, TBY, STATS>(sm, lby * TBX + lbx, s, gg, lane, kLane, Q, bc);
       }
     }
     line = fold_n(line, stepY, 5);
@@ -859,23 +898,37 @@ __device__ __forceinline__ bool query_overlaps_tile(uint32_t qMinX, uint32_t qMa
 constexpr uint32_t kDoneCountMask = 0xFFFFu;
 constexpr uint32_t kDoneVisibleOne = 0x10000u;
 
-template <int TBX, int TBY, bool STATS>
-__global__ void __launch_bounds__(256) rz_frame_kernel(FrameParams p) {
-  __shared__ TileSmem<TBX, TBY> sm;
-  // Observability counters, kept by thread 0 (compiled out unless STATS).
-  uint64_t ctr[kFrameStatWords] = {};
-  const uint64_t tEntry = STATS ? __builtin_amdgcn_s_memrealtime() : 0;
-  auto stamp = []() -> uint64_t { return STATS ? __builtin_amdgcn_s_memrealtime() : 0; };
-  auto flushStats = [&]() {
-    if (STATS && threadIdx.x == 0) {
-      ctr[kStatTotalTicks] = __builtin_amdgcn_s_memrealtime() - tEntry;
-      for (int k = 0; k < kFrameStatWords; ++k) p.stats[size_t(blockIdx.x) * kFrameStatWords + k] = ctr[k];
-    }
-  };
+// One screen tile of one view: items [itemBegin, itemEnd) of the launch are
+// walked in draw order against the tile's depth and hiZ held in LDS, then the
+// tile is written back.  Shared by the single-view frame kernel (one
+// work-group per tile, gated query items) and the multi-view persistent
+// kernel (work-groups take (view, tile) units from a queue; items are never
+// gated, so no tile waits on another).
+struct TileJob {
+  const ItemDesc* items;
+  const ItemState* state;
+  uint32_t itemBegin, itemEnd;
+  int bX, bY;
+  uint32_t tX, tY;
+  const QuadRec* recs;
+  const uint64_t* groupBox;
+  uint16_t* depth;
+  uint16_t* hiz;
+  uint16_t* tileMin;
+  uint32_t pyrX;
+  uint32_t* done;
+  uint32_t* err;
+  uint64_t timeoutTicks;
+};
 
+template <int TBX, int TBY, bool STATS>
+__device__ __forceinline__ void process_tile(const TileJob& J, TileSmem<TBX, TBY>& sm,
+                                             const uint64_t* __restrict__ cov, bool& covLoaded,
+                                             uint64_t (&ctr)[kFrameStatWords]) {
+  auto stamp = []() -> uint64_t { return STATS ? __builtin_amdgcn_s_memrealtime() : 0; };
   const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
-  const uint32_t tX = blockIdx.x % p.tilesX, tY = blockIdx.x / p.tilesX;
-  const int bX = int(p.bX), bY = int(p.bY);
+  const uint32_t tX = J.tX, tY = J.tY;
+  const int bX = J.bX, bY = J.bY;
   const int tbx0 = int(tX) * TBX, tby0 = int(tY) * TBY;
   const int tbx1 = min(tbx0 + TBX, bX) - 1, tby1 = min(tby0 + TBY, bY) - 1;
   const float invBX = 1.0f / float(bX);
@@ -883,7 +936,7 @@ __global__ void __launch_bounds__(256) rz_frame_kernel(FrameParams p) {
   // Relevance of one item to this tile: bit 0 the query rectangle overlaps
   // it, bit 1 some valid quad's blocks do; status in bits 2-3.
   auto relevance = [&](uint32_t i) -> uint32_t {
-    const ItemState& S = p.state[i];
+    const ItemState& S = J.state[i];
     const uint32_t status = S.status;
     if (status == kStatusSkip) return 0u;
     const bool q = status == kStatusTest && query_overlaps_tile<TBX, TBY>(S.qMinX, S.qMaxX, S.qMinY, S.qMaxY, tX, tY);
@@ -893,38 +946,40 @@ __global__ void __launch_bounds__(256) rz_frame_kernel(FrameParams p) {
   };
 
   int involved = 0;
-  for (uint32_t i = tid; i < p.nItems && !involved; i += 256) involved = relevance(i) != 0u;
-  if (!__syncthreads_or(involved)) {
-    flushStats();
-    return;
-  }
+  for (uint32_t i = J.itemBegin + tid; i < J.itemEnd && !involved; i += 256) involved = relevance(i) != 0u;
+  if (!__syncthreads_or(involved)) return;
 
   const uint64_t tLoad = stamp();
-  for (uint32_t i = tid; i < 4096; i += 256) sm.cov[i] = p.cov[i];
+  if (!covLoaded) {
+    for (uint32_t i = tid; i < 4096; i += 256) sm.cov[i] = cov[i];
+    covLoaded = true;
+  }
   constexpr int NB = TBX * TBY;
   for (int rrow = int(tid); rrow < NB * 8; rrow += 256) {
     const int lb = rrow >> 3, y = rrow & 7;
     const int gx = tbx0 + lb % TBX, gy = tby0 + lb / TBX;
     uint4 v = make_uint4(0, 0, 0, 0);
     if (gx <= tbx1 && gy <= tby1)
-      v = *reinterpret_cast<const uint4*>(p.depth + (size_t(gy) * bX + gx) * 64 + y * 8);
+      v = *reinterpret_cast<const uint4*>(J.depth + (size_t(gy) * bX + gx) * 64 + y * 8);
     *reinterpret_cast<uint4*>(&sm.depth[lb * 64 + y * 8]) = v;
   }
   for (int lb = int(tid); lb < NB; lb += 256) {
     const int gx = tbx0 + lb % TBX, gy = tby0 + lb / TBX;
-    sm.hiz[lb] = (gx <= tbx1 && gy <= tby1) ? p.hiz[size_t(gy) * bX + gx] : uint16_t(0xFFFF);
+    sm.hiz[lb] = (gx <= tbx1 && gy <= tby1) ? J.hiz[size_t(gy) * bX + gx] : uint16_t(0xFFFF);
   }
 
   uint32_t qPar = 0, vPar = 0;
+  BlockCounters bc;
   if (STATS) {
+    if (tid < 3) sm.blockStats[tid] = 0;
     __syncthreads();
-    ctr[kStatLoadTicks] = stamp() - tLoad;
+    ctr[kStatLoadTicks] += stamp() - tLoad;
   }
   const uint64_t tLoop = stamp();
-  for (uint32_t wbase = 0; wbase < p.nItems; wbase += kItemWindow) {
+  for (uint32_t wbase = J.itemBegin; wbase < J.itemEnd; wbase += kItemWindow) {
     // Gather this window's items that touch the tile, in draw order.
     const uint32_t i = wbase + tid;
-    const uint32_t rel = (tid < kItemWindow && i < p.nItems) ? relevance(i) : 0u;
+    const uint32_t rel = (tid < kItemWindow && i < J.itemEnd) ? relevance(i) : 0u;
     const uint64_t m = __ballot(rel != 0u);
     const uint32_t pre = __builtin_amdgcn_mbcnt_hi(uint32_t(m >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(m), 0u));
     if (lane == 0) sm.wcnt[wave] = uint32_t(__popcll(m));
@@ -936,7 +991,7 @@ __global__ void __launch_bounds__(256) rz_frame_kernel(FrameParams p) {
       nRel += sm.wcnt[w];
     }
     if (rel) {
-      const ItemState& S = p.state[i];
+      const ItemState& S = J.state[i];
       ItemLocal& L = sm.items[off + pre];
       L.index = i;
       L.flags = rel;
@@ -946,7 +1001,7 @@ __global__ void __launch_bounds__(256) rz_frame_kernel(FrameParams p) {
       L.qMaxY = S.qMaxY;
       L.qMaxZ = S.qMaxZ;
       L.expected = S.expected;
-      const uint32_t qb = p.items[i].quadBase, qc = p.items[i].quadCount;
+      const uint32_t qb = J.items[i].quadBase, qc = J.items[i].quadCount;
       L.quadBase = qb;
       L.quadCount = qc;
       uint64_t groups = ~0ull;
@@ -955,7 +1010,7 @@ __global__ void __launch_bounds__(256) rz_frame_kernel(FrameParams p) {
         if (g1 - g0 < 64) {
           groups = 0;
           for (uint32_t gi = g0; gi <= g1; ++gi)
-            if (box_touches(p.groupBox[gi], tbx0, tbx1, tby0, tby1)) groups |= 1ull << (gi - g0);
+            if (box_touches(J.groupBox[gi], tbx0, tbx1, tby0, tby1)) groups |= 1ull << (gi - g0);
         }
       }
       L.groups = groups;
@@ -974,7 +1029,7 @@ __global__ void __launch_bounds__(256) rz_frame_kernel(FrameParams p) {
         __syncthreads();
         if (tid == 0) {
           const uint32_t part = sm.part[qPar][0] | sm.part[qPar][1] | sm.part[qPar][2] | sm.part[qPar][3];
-          __hip_atomic_fetch_add(&p.done[item], 1u + (part ? kDoneVisibleOne : 0u), __ATOMIC_RELEASE,
+          __hip_atomic_fetch_add(&J.done[item], 1u + (part ? kDoneVisibleOne : 0u), __ATOMIC_RELEASE,
                                  __HIP_MEMORY_SCOPE_AGENT);
         }
         qPar ^= 1u;
@@ -991,10 +1046,10 @@ __global__ void __launch_bounds__(256) rz_frame_kernel(FrameParams p) {
           const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
           uint32_t d;
           while (true) {
-            d = __hip_atomic_load(&p.done[item], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+            d = __hip_atomic_load(&J.done[item], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
             if ((d & kDoneCountMask) >= L.expected || d >= kDoneVisibleOne) break;
-            if (__builtin_amdgcn_s_memrealtime() - t0 > p.timeoutTicks) {
-              __hip_atomic_fetch_or(p.err, uint32_t(kErrTimeout), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (__builtin_amdgcn_s_memrealtime() - t0 > J.timeoutTicks) {
+              __hip_atomic_fetch_or(J.err, uint32_t(kErrTimeout), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
               break;
             }
             __builtin_amdgcn_s_sleep(1);
@@ -1032,9 +1087,9 @@ __global__ void __launch_bounds__(256) rz_frame_kernel(FrameParams p) {
           const uint32_t wg1 = ((L.quadBase + min(waveFirst + 63u, L.quadCount - 1u)) >> 6) - gBase;
           const bool groupHit = wg1 >= 64 || ((L.groups >> wg0) & 1ull) || ((L.groups >> wg1) & 1ull);
           if (groupHit && base + tid < L.quadCount) {
-            rec = p.recs[L.quadBase + base + tid];
+            rec = J.recs[L.quadBase + base + tid];
             const uint32_t rx = rec.range & 0xFFFFu, ry = rec.range >> 16;
-            hit = rx != 0u && box_touches(quad_box(rec.fb, rx, ry, p.bX, p.bY), tbx0, tbx1, tby0, tby1) &&
+            hit = rx != 0u && box_touches(quad_box(rec.fb, rx, ry, uint32_t(bX), uint32_t(bY)), tbx0, tbx1, tby0, tby1) &&
                   !quad_misses_tile(rec, tbx0, tbx1, tby0, tby1, bX);
           }
         }
@@ -1056,11 +1111,11 @@ __global__ void __launch_bounds__(256) rz_frame_kernel(FrameParams p) {
         if (hit && pos < kQuadChunk) sm.recs[pos] = rec;
         for (uint32_t k0 = 0; k0 < total; k0 += kQuadChunk) {
           const uint32_t n = min(kQuadChunk, total - k0);
-          if (k0 > 0 && hit && pos >= k0 && pos < k0 + n) sm.recs[pos - k0] = p.recs[L.quadBase + base + tid];
+          if (k0 > 0 && hit && pos >= k0 && pos < k0 + n) sm.recs[pos - k0] = J.recs[L.quadBase + base + tid];
           __syncthreads();
           const uint64_t tDraw = stamp();
           for (uint32_t q = 0; q < n; ++q)
-            raster_quad_wave<TBX, TBY>(sm.recs[q], sm, tbx0, tbx1, tby0, tby1, bX, invBX, wave, lane);
+            raster_quad_wave<TBX, TBY, STATS>(sm.recs[q], sm, tbx0, tbx1, tby0, tby1, bX, invBX, wave, lane, bc);
           if (STATS) ctr[kStatQuads] += n;
           __syncthreads();
           if (STATS) ctr[kStatDrawTicks] += stamp() - tDraw;
@@ -1071,18 +1126,31 @@ __global__ void __launch_bounds__(256) rz_frame_kernel(FrameParams p) {
     __syncthreads();
   }
 
-  if (STATS) ctr[kStatLoopTicks] = stamp() - tLoop;
+  if (STATS) {
+    ctr[kStatLoopTicks] += stamp() - tLoop;
+    if (lane == 0) {
+      atomicAdd(&sm.blockStats[0], bc.visited);
+      atomicAdd(&sm.blockStats[1], bc.rejected);
+      atomicAdd(&sm.blockStats[2], bc.written);
+    }
+    __syncthreads();
+    if (tid == 0) {
+      ctr[kStatBlocksVisited] += sm.blockStats[0];
+      ctr[kStatBlocksRejected] += sm.blockStats[1];
+      ctr[kStatBlocksWritten] += sm.blockStats[2];
+    }
+  }
   // Write the tile back.
   for (int rrow = int(tid); rrow < NB * 8; rrow += 256) {
     const int lb = rrow >> 3, y = rrow & 7;
     const int gx = tbx0 + lb % TBX, gy = tby0 + lb / TBX;
     if (gx <= tbx1 && gy <= tby1)
-      *reinterpret_cast<uint4*>(p.depth + (size_t(gy) * bX + gx) * 64 + y * 8) =
+      *reinterpret_cast<uint4*>(J.depth + (size_t(gy) * bX + gx) * 64 + y * 8) =
           *reinterpret_cast<const uint4*>(&sm.depth[lb * 64 + y * 8]);
   }
   for (int lb = int(tid); lb < NB; lb += 256) {
     const int gx = tbx0 + lb % TBX, gy = tby0 + lb / TBX;
-    if (gx <= tbx1 && gy <= tby1) p.hiz[size_t(gy) * bX + gx] = sm.hiz[lb];
+    if (gx <= tbx1 && gy <= tby1) J.hiz[size_t(gy) * bX + gx] = sm.hiz[lb];
   }
   // Min hiZ of each 8x8-block sub-tile for the query kernel (blocks outside
   // the buffer hold 0xFFFF and do not lower it).
@@ -1092,9 +1160,87 @@ __global__ void __launch_bounds__(256) rz_frame_kernel(FrameParams p) {
     const int lb = (sy * 8 + int(lane >> 3)) * TBX + sx * 8 + int(lane & 7u);
     const uint32_t mn = __ockl_wfred_min_u32(uint32_t(sm.hiz[lb]));
     const uint32_t px = tX * kSubX + uint32_t(sx), py = tY * kSubY + uint32_t(sy);
-    if (lane == 0 && px < p.pyrX && py * 8 < p.bY) p.tileMin[py * p.pyrX + px] = uint16_t(mn);
+    if (lane == 0 && px < J.pyrX && py * 8 < uint32_t(bY)) J.tileMin[py * J.pyrX + px] = uint16_t(mn);
   }
-  flushStats();
+}
+
+template <int TBX, int TBY, bool STATS>
+__global__ void __launch_bounds__(256) rz_frame_kernel(FrameParams p) {
+  __shared__ TileSmem<TBX, TBY> sm;
+  // Observability counters, kept by thread 0 (compiled out unless STATS).
+  uint64_t ctr[kFrameStatWords] = {};
+  const uint64_t tEntry = STATS ? __builtin_amdgcn_s_memrealtime() : 0;
+  TileJob J;
+  J.items = p.items;
+  J.state = p.state;
+  J.itemBegin = 0;
+  J.itemEnd = p.nItems;
+  J.bX = int(p.bX);
+  J.bY = int(p.bY);
+  J.tX = blockIdx.x % p.tilesX;
+  J.tY = blockIdx.x / p.tilesX;
+  J.recs = p.recs;
+  J.groupBox = p.groupBox;
+  J.depth = p.depth;
+  J.hiz = p.hiz;
+  J.tileMin = p.tileMin;
+  J.pyrX = p.pyrX;
+  J.done = p.done;
+  J.err = p.err;
+  J.timeoutTicks = p.timeoutTicks;
+  bool covLoaded = false;
+  process_tile<TBX, TBY, STATS>(J, sm, p.cov, covLoaded, ctr);
+  if (STATS && threadIdx.x == 0) {
+    ctr[kStatTotalTicks] = __builtin_amdgcn_s_memrealtime() - tEntry;
+    for (int k = 0; k < kFrameStatWords; ++k) p.stats[size_t(blockIdx.x) * kFrameStatWords + k] = ctr[k];
+  }
+}
+
+// Multi-view persistent raster: each work-group takes (view, tile) units from
+// p.queue until the queue is exhausted.  Units are handed out in view order
+// and no unit waits on another, so every work-group exits after at most
+// nUnits queue reads.
+template <int TBX, int TBY>
+__global__ void __launch_bounds__(256) rz_views_kernel(ViewsParams p) {
+  __shared__ TileSmem<TBX, TBY> sm;
+  __shared__ uint32_t unitSh;
+  uint64_t ctr[kFrameStatWords] = {};
+  bool covLoaded = false;
+  while (true) {
+    if (threadIdx.x == 0) unitSh = __hip_atomic_fetch_add(p.queue, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
+    const uint32_t unit = unitSh;
+    __syncthreads();
+    if (unit >= p.nUnits) break;
+    uint32_t lo = 0, hi = p.nViews - 1;
+    while (lo < hi) {
+      const uint32_t mid = (lo + hi + 1) >> 1;
+      if (p.views[mid].unitBase <= unit) lo = mid;
+      else hi = mid - 1;
+    }
+    const ViewDesc& V = p.views[lo];
+    const uint32_t t = unit - V.unitBase;
+    TileJob J;
+    J.items = p.items;
+    J.state = p.state;
+    J.itemBegin = p.batchItems ? V.batchBase : V.itemBase;
+    J.itemEnd = p.batchItems ? V.batchBase + V.nBatches : V.itemBase + V.nItems;
+    J.bX = int(V.bX);
+    J.bY = int(V.bY);
+    J.tX = t % V.tilesX;
+    J.tY = t / V.tilesX;
+    J.recs = p.recs;
+    J.groupBox = p.groupBox;
+    J.depth = V.depth;
+    J.hiz = V.hiz;
+    J.tileMin = V.tileMin;
+    J.pyrX = V.pyrX;
+    J.done = nullptr;
+    J.err = nullptr;
+    J.timeoutTicks = 0;
+    process_tile<TBX, TBY, false>(J, sm, p.cov, covLoaded, ctr);
+    __syncthreads();
+  }
 }
 
 // Per-item draw status of a frame: 0 not drawn, 1 rasterize<false>,
@@ -1135,6 +1281,47 @@ __global__ void __launch_bounds__(256) rz_query_kernel(QueryParams p) {
   }
 }
 
+// queryVisibility over the queries of every view of a multi-view launch, one
+// wave per box; the view is found from the box index.
+__global__ void __launch_bounds__(256) rz_views_query_kernel(ViewsQueryParams p) {
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t q = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  if (q >= p.nQueries) return;
+  uint32_t lo = 0, hi = p.nViews - 1;
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi + 1) >> 1;
+    if (p.views[mid].queryBase <= q) lo = mid;
+    else hi = mid - 1;
+  }
+  const ViewDesc& V = p.views[lo];
+  const uint32_t k = q - V.queryBase;
+  const float* b = V.queryBounds + 8 * size_t(k);
+  const float bmin[4] = {b[0], b[1], b[2], b[3]};
+  const float bmax[4] = {b[4], b[5], b[6], b[7]};
+  const QueryPre Q = query_pre(V.mvp, bmin, bmax, V.width, V.height, p.rcp);
+  uint32_t vis = 0, clip = 0;
+  if (Q.kind == 1) {
+    vis = 1;
+    clip = 1;
+  } else if (Q.kind == 3) {
+    vis = query2d_wave(Q.minX, Q.maxX, Q.minY, Q.maxY, Q.maxZ, V.depth, V.hiz, V.tileMin, V.bX, V.pyrX, lane) ? 1u
+                                                                                                            : 0u;
+  }
+  if (lane == 0) {
+    V.queryVisible[k] = uint8_t(vis);
+    V.queryClip[k] = uint8_t(clip);
+  }
+}
+
+// clear (Rasterizer.cpp:107-121) of every view: hiZ := 1 in whole 8-entry
+// groups, and the per-tile min hiZ := 1.  blockIdx.y is the view.
+__global__ void rz_views_clear_kernel(const ViewDesc* __restrict__ views) {
+  const ViewDesc& V = views[blockIdx.y];
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < ((V.bX * V.bY + 8u) / 8u) * 8u) V.hiz[i] = 1;
+  if (i < V.pyrX * V.pyrY) V.tileMin[i] = 1;
+}
+
 __global__ void rz_query2d_kernel(uint32_t minX, uint32_t maxX, uint32_t minY, uint32_t maxY, uint32_t maxZ,
                                   QueryParams p, uint8_t* out) {
   const bool v = query2d_wave(minX, maxX, minY, maxY, maxZ, p.depth, p.hiz, p.tileMin, p.bX, p.pyrX, threadIdx.x);
@@ -1173,6 +1360,294 @@ __global__ void __launch_bounds__(256) rz_readback_kernel(const uint16_t* __rest
 }
 
 __global__ void rz_fill_u16_kernel(uint16_t* p, uint32_t n, uint16_t v) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+// ---------------------------------------------------------------------------
+// Draw-order labels and batch features.
+// ---------------------------------------------------------------------------
+// Item whose quad range holds quad g (items are in quad order).
+__device__ __forceinline__ uint32_t item_of_quad(const ItemDesc* __restrict__ items, uint32_t nItems, uint32_t g) {
+  uint32_t lo = 0, hi = nItems - 1;
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi + 1) >> 1;
+    if (items[mid].quadBase <= g) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+// One block of a quad sampled alone, as the block loop samples it (coverage
+// mask, rows 0/1 and 8/9 of the plane, packed averages; no hiZ test, no
+// merge), compared with the finished buffer: a covered pixel of a written
+// block whose sample equals the buffer's non-zero depth names `batch`.
+__device__ __forceinline__ void label_block(const uint64_t* __restrict__ cov, const uint16_t* __restrict__ depth,
+                                            const uint16_t* __restrict__ hiz, uint32_t* __restrict__ winner,
+                                            uint32_t blk, float s, int g, uint32_t lane, int kLane,
+                                            const QuadConst& Q, uint32_t batch) {
+  float o[4];
+  uint32_t idx[4];
+  bool outside = false;
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    o[e] = readlane_f(s, 12 * g + e);
+    const int32_t t = cvtt_x86(o[e]);
+    if (Q.mode == kModeConvex) {
+      outside |= o[e] >= 63.0f;
+      idx[e] = (Q.sl[e] | uint32_t(max(t, 0))) & 4095u;
+    } else {
+      idx[e] = (Q.sl[e] | uint32_t(min(max(t, 0), 63))) & 4095u;
+    }
+  }
+  if (outside) return;
+  const uint64_t A = cov[idx[0]], B = cov[idx[1]], C = cov[idx[2]], D = cov[idx[3]];
+  uint64_t mask;
+  if (Q.mode == kModeConvex) mask = (A & B) & (C & D);
+  else if (Q.mode == kModeTriangle0) mask = A & B & C;
+  else if (Q.mode == kModeTriangle1) mask = A & C & D;
+  else if (Q.mode == kModeConcaveRight) mask = (A | D) & (B & C);
+  else if (Q.mode == kModeConcaveCenter) mask = (A & B) | (C & D);
+  else mask = (A & D) & (B | C);
+  if (mask == 0) return;
+  float v = __shfl(s, 12 * g + 4 + kLane);
+  const uint32_t px = lane & 7u, py = lane >> 3;
+  if (px >= 4u) v = fmaf(Q.dDx, 0.5f, v);
+  const uint32_t lo = pack_depth(v);
+  const uint32_t hi = pack_depth(Q.dDy + v);
+  const uint32_t mid = avg_u16(lo, hi);
+  const uint32_t q1 = avg_u16(lo, mid), q3 = avg_u16(mid, hi);
+  const uint32_t j = py >> 1;
+  const uint32_t a01 = (j & 1u) ? q1 : lo, a23 = (j & 1u) ? q3 : mid;
+  const uint32_t d = (j & 2u) ? a23 : a01;
+  const uint32_t maskBit = 8u * px + ((py & 1u) ? (py >> 1) : 4u + (py >> 1));
+  if (((mask >> maskBit) & 1ull) && d != 0u && hiz[blk] != 1u && uint32_t(depth[size_t(blk) * 64 + lane]) == d)
+    atomicMin(&winner[size_t(blk) * 64 + lane], batch);
+}
+
+// Every block of the quad's rectangle, with the block loop's folds (row
+// steps, then column steps, each add rounded).
+__device__ void label_quad_wave(const QuadRec& R, const uint64_t* __restrict__ cov, const ViewDesc& V,
+                                uint32_t batch, uint32_t lane) {
+  const int fb = int(R.fb), bX = int(V.bX);
+  const int rangeX = int(R.range & 0xFFFFu), rangeY = int(R.range >> 16);
+  QuadConst Q;
+  Q.mode = R.modeZ & 0xFFu;
+  Q.maxZ = R.modeZ >> 16;
+  Q.sl[0] = R.slope01 & 0xFFFFu;
+  Q.sl[1] = R.slope01 >> 16;
+  Q.sl[2] = R.slope23 & 0xFFFFu;
+  Q.sl[3] = R.slope23 >> 16;
+  Q.dDx = R.dDx;
+  Q.dDy = R.dDy;
+  const float invBX = 1.0f / float(bX);
+  const int minY = div_bx(fb, bX, invBX), minX = fb - minY * bX;
+  const int kLane = int(((lane & 7u) & 3u) + 4u * ((lane >> 3) & 1u));
+  const uint32_t g = lane / 12u, e = lane - 12u * g;
+  float base, stepY, stepX;
+  if (e < 4) {
+    base = e == 0 ? R.off[0] : e == 1 ? R.off[1] : e == 2 ? R.off[2] : R.off[3];
+    stepY = e == 0 ? R.ny[0] : e == 1 ? R.ny[1] : e == 2 ? R.ny[2] : R.ny[3];
+    stepX = e == 0 ? R.nx[0] : e == 1 ? R.nx[1] : e == 2 ? R.nx[2] : R.nx[3];
+  } else {
+    const uint32_t k = e - 4;
+    const float spx = -0.4375f + 0.125f * float(k & 3u);
+    const float spy = -0.4375f + 0.125f * float(k >> 2);
+    base = fmaf(Q.dDx, spx, fmaf(Q.dDy, spy, R.dp0));
+    stepY = Q.dDy;
+    stepX = Q.dDx;
+  }
+  if (minX + rangeX > bX) {
+    // Wrapped first-block index: blocks fb + r*bX + c, each folded from the origin.
+    for (int r = 0; r < rangeY; ++r)
+      for (int c = 0; c < rangeX; ++c) {
+        const float s = fold_n(fold_n(base, stepY, r), stepX, c);
+        label_block(cov, V.depth, V.hiz, V.winner, uint32_t(fb + r * bX + c), s, 0, lane, kLane, Q, batch);
+      }
+    return;
+  }
+  float line = base;
+#pragma unroll
+  for (uint32_t i = 0; i < 4; ++i)
+    if (i < g) line = line + stepY;
+  for (int r = 0; r < rangeY; r += 5) {
+    float s = line;
+    for (int c = 0; c < rangeX; ++c, s = stepX + s)
+      for (int gg = 0; gg < 5 && r + gg < rangeY; ++gg)
+        label_block(cov, V.depth, V.hiz, V.winner, uint32_t(fb + (r + gg) * bX + c), s, gg, lane, kLane, Q, batch);
+    line = fold_n(line, stepY, 5);
+  }
+}
+
+// One wave per drawn quad; work-groups loop over the quads so that each
+// stages the coverage table once.
+__global__ void __launch_bounds__(256) rz_label_kernel(LearnParams p) {
+  __shared__ uint64_t cov[4096];
+  for (uint32_t i = threadIdx.x; i < 4096; i += 256) cov[i] = p.cov[i];
+  __syncthreads();
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t waves = gridDim.x * 4u;
+  for (uint32_t g = blockIdx.x * 4u + (threadIdx.x >> 6); g < p.nQuads; g += waves) {
+    const QuadRec R = p.recs[g];
+    if ((R.range & 0xFFFFu) == 0u || (R.range >> 16) == 0u) continue;
+    const uint32_t item = item_of_quad(p.items, p.nItems, g);
+    const ViewDesc& V = p.views[p.items[item].view];
+    label_quad_wave(R, cov, V, g >> 3, lane);
+  }
+}
+
+// Per block (blockIdx.y = view): each distinct batch kept by a pixel of the
+// block gets one count; the winner words are reset for the next step.
+__global__ void __launch_bounds__(256) rz_label_count_kernel(LearnParams p) {
+  const ViewDesc& V = p.views[blockIdx.y];
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t blk = blockIdx.x * 4u + (threadIdx.x >> 6);
+  if (blk >= V.bX * V.bY) return;
+  uint32_t* wp = &V.winner[size_t(blk) * 64 + lane];
+  const uint32_t w = *wp;
+  bool active = w != 0xFFFFFFFFu;
+  while (true) {
+    const uint64_t m = __ballot(active);
+    if (m == 0ull) break;
+    const uint32_t v = uint32_t(__builtin_amdgcn_readlane(int(w), int(__builtin_ctzll(m))));
+    if (active && w == v) active = false;
+    if (lane == 0) atomicAdd(&p.labels[v], 1u);
+  }
+  *wp = 0xFFFFFFFFu;
+}
+
+// 16 features per batch, from its quads' setup records and packets and the
+// labels of the previous step:
+//  0 drawn quads / 8          1 log2(1 + union block-box area)
+//  2 min packed maxZ / 65535  3 max packed maxZ / 65535   4 mean packed maxZ / 65535
+//  5 log2(1 + sum of block-box areas)  6 drawn with rasterize<true>
+//  7, 8 union box centre / buffer size (x, y)   9 log2(1 + |W| of the batch centroid)
+// 10 log2(1 + previous step's label)  11-13 mean, min, max of (unit quad normal . view direction)
+// 14 position of the batch in the view's plain order   15 fraction of Convex quads
+__global__ void __launch_bounds__(256) rz_features_kernel(LearnParams p, uint32_t nBatches) {
+  const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nBatches) return;
+  const uint32_t item = item_of_quad(p.items, p.nItems, b * 8u);
+  const ItemDesc& D = p.items[item];
+  const ItemState& S = p.state[item];
+  const ViewDesc& V = p.views[D.view];
+  float f[kFeatures];
+#pragma unroll
+  for (int k = 0; k < kFeatures; ++k) f[k] = 0.0f;
+  uint32_t n = 0, convex = 0;
+  uint32_t c0 = 0xFFFFu, c1 = 0, r0 = 0xFFFFu, r1 = 0;
+  float zmin = 1.0f, zmax = 0.0f, zsum = 0.0f, areaSum = 0.0f;
+  const float vlen = sqrtf(V.mvp.raw[12] * V.mvp.raw[12] + V.mvp.raw[13] * V.mvp.raw[13] + V.mvp.raw[14] * V.mvp.raw[14]);
+  const float vd[3] = {V.mvp.raw[12] / vlen, V.mvp.raw[13] / vlen, V.mvp.raw[14] / vlen};
+  float ndSum = 0.0f, ndMin = 1.0f, ndMax = -1.0f, cen[3] = {0.0f, 0.0f, 0.0f};
+  float ext[3];
+#pragma unroll
+  for (int j = 0; j < 3; ++j) ext[j] = D.refMax[j] - D.refMin[j];
+  for (uint32_t k = 0; k < 8; ++k) {
+    const uint32_t g = b * 8u + k;
+    const uint32_t q = g - D.quadBase;
+    const uint32_t* pk = D.packets + (q >> 3) * 32 + (q & 7);
+    float P[4][3];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const uint32_t w = pk[8 * j];
+      const float X = float((int32_t(w) >> 21) + 1024), Y = float((w >> 10) & 2047u), Z = float(w & 1023u);
+      P[j][0] = D.refMin[0] + X * (ext[0] / 2047.0f);
+      P[j][1] = D.refMin[1] + Y * (ext[1] / 2047.0f);
+      P[j][2] = D.refMin[2] + Z * (ext[2] / 1023.0f);
+#pragma unroll
+      for (int a = 0; a < 3; ++a) cen[a] += P[j][a] * (1.0f / 32.0f);
+    }
+    const QuadRec& R = p.recs[g];
+    const uint32_t rx = R.range & 0xFFFFu, ry = R.range >> 16;
+    if (rx == 0u || ry == 0u) continue;
+    ++n;
+    const uint64_t bb = p.bbox[g];
+    const uint32_t bc0 = uint32_t(bb & 0xFFFF), bc1 = uint32_t((bb >> 16) & 0xFFFF);
+    const uint32_t br0 = uint32_t((bb >> 32) & 0xFFFF), br1 = uint32_t(bb >> 48);
+    c0 = min(c0, bc0);
+    c1 = max(c1, bc1);
+    r0 = min(r0, br0);
+    r1 = max(r1, br1);
+    areaSum += float((bc1 - bc0 + 1) * (br1 - br0 + 1));
+    const float z = float(R.modeZ >> 16) * (1.0f / 65535.0f);
+    zmin = fminf(zmin, z);
+    zmax = fmaxf(zmax, z);
+    zsum += z;
+    convex += (R.modeZ & 0xFFu) == kModeConvex ? 1u : 0u;
+    float e1[3], e2[3], e3[3], nrm[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      e1[a] = P[1][a] - P[0][a];
+      e2[a] = P[2][a] - P[0][a];
+      e3[a] = P[3][a] - P[0][a];
+    }
+    nrm[0] = (e1[1] * e2[2] - e1[2] * e2[1]) + (e2[1] * e3[2] - e2[2] * e3[1]);
+    nrm[1] = (e1[2] * e2[0] - e1[0] * e2[2]) + (e2[2] * e3[0] - e2[0] * e3[2]);
+    nrm[2] = (e1[0] * e2[1] - e1[1] * e2[0]) + (e2[0] * e3[1] - e2[1] * e3[0]);
+    const float nl = sqrtf(nrm[0] * nrm[0] + nrm[1] * nrm[1] + nrm[2] * nrm[2]);
+    const float nd = nl > 0.0f ? (nrm[0] * vd[0] + nrm[1] * vd[1] + nrm[2] * vd[2]) / nl : 0.0f;
+    ndSum += nd;
+    ndMin = fminf(ndMin, nd);
+    ndMax = fmaxf(ndMax, nd);
+  }
+  f[0] = float(n) * 0.125f;
+  if (n > 0) {
+    f[1] = log2f(1.0f + float((c1 - c0 + 1) * (r1 - r0 + 1)));
+    f[2] = zmin;
+    f[3] = zmax;
+    f[4] = zsum / float(n);
+    f[5] = log2f(1.0f + areaSum);
+    f[7] = (0.5f * float(c0 + c1) + 0.5f) / float(V.bX);
+    f[8] = (0.5f * float(r0 + r1) + 0.5f) / float(V.bY);
+    f[11] = ndSum / float(n);
+    f[12] = ndMin;
+    f[13] = ndMax;
+    f[15] = float(convex) / float(n);
+  } else {
+    f[7] = f[8] = 0.5f;
+  }
+  f[6] = S.status == kStatusClip ? 1.0f : 0.0f;
+  const float w = cen[0] * V.mvp.raw[12] + cen[1] * V.mvp.raw[13] + cen[2] * V.mvp.raw[14] + V.mvp.raw[15];
+  f[9] = log2f(1.0f + fabsf(w));
+  f[10] = log2f(1.0f + float(p.labels[b]));
+  f[14] = V.nBatches > 1 ? float(b - V.batchBase) / float(V.nBatches - 1) : 0.0f;
+#pragma unroll
+  for (int k = 0; k < kFeatures; ++k) p.features[size_t(b) * kFeatures + k] = f[k];
+}
+
+// Item k of a reordered draw: batch order[k] alone, with its occluder's
+// classification and the footprint of the batch's drawn quads.
+__global__ void __launch_bounds__(256) rz_batch_items_kernel(LearnParams p, const uint32_t* __restrict__ order,
+                                                              uint32_t nBatches, ItemDesc* __restrict__ outItems,
+                                                              ItemState* __restrict__ outState) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nBatches) return;
+  const uint32_t b = order[k];
+  const uint32_t item = item_of_quad(p.items, p.nItems, b * 8u);
+  ItemDesc D = p.items[item];
+  ItemState S = p.state[item];
+  D.quadBase = b * 8u;
+  D.quadCount = 8u;
+  uint32_t c0 = 0xFFFFFFFFu, c1 = 0, r0 = 0xFFFFFFFFu, r1 = 0;
+  for (uint32_t j = 0; j < 8; ++j) {
+    const uint64_t bb = p.bbox[b * 8u + j];
+    const uint32_t bc0 = uint32_t(bb & 0xFFFF), bc1 = uint32_t((bb >> 16) & 0xFFFF);
+    if (bc0 > bc1) continue;  // empty box: quad not drawn
+    c0 = min(c0, bc0);
+    c1 = max(c1, bc1);
+    r0 = min(r0, uint32_t((bb >> 32) & 0xFFFF));
+    r1 = max(r1, uint32_t(bb >> 48));
+  }
+  S.fp[0] = c0;
+  S.fp[1] = c1;
+  S.fp[2] = r0;
+  S.fp[3] = r1;
+  outItems[k] = D;
+  outState[k] = S;
+}
+
+__global__ void rz_fill_u32_kernel(uint32_t* p, uint32_t n, uint32_t v) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
 }
@@ -1274,6 +1749,66 @@ hipError_t launch_readback(const uint16_t* depth, const uint16_t* hiz, uint32_t 
 hipError_t launch_fill_u16(uint16_t* p, uint32_t n, uint16_t v, hipStream_t stream) {
   if (n == 0) return hipSuccess;
   hipLaunchKernelGGL(rz_fill_u16_kernel, dim3(groups(n, 256)), dim3(256), 0, stream, p, n, v);
+  return hipGetLastError();
+}
+
+hipError_t launch_views(const ViewsParams& p, uint32_t nGroups, hipStream_t stream) {
+  if (p.nUnits == 0 || nGroups == 0) return hipSuccess;
+  rz_views_kernel<8, 8><<<dim3(nGroups), dim3(256), 0, stream>>>(p);
+  return hipGetLastError();
+}
+
+hipError_t views_max_groups(int* maxGroups) {
+  int dev = 0, cus = 0, n = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  if (e != hipSuccess) return e;
+  e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, reinterpret_cast<const void*>(&rz_views_kernel<8, 8>), 256,
+                                                   0);
+  if (e != hipSuccess) return e;
+  *maxGroups = n * cus;
+  return hipSuccess;
+}
+
+hipError_t launch_views_query(const ViewsQueryParams& p, hipStream_t stream) {
+  if (p.nQueries == 0) return hipSuccess;
+  hipLaunchKernelGGL(rz_views_query_kernel, dim3(groups(p.nQueries, 4)), dim3(256), 0, stream, p);
+  return hipGetLastError();
+}
+
+hipError_t launch_labels(const LearnParams& p, uint32_t maxBlocks, hipStream_t stream) {
+  if (p.nQuads == 0 || p.nItems == 0) return hipSuccess;
+  hipLaunchKernelGGL(rz_label_kernel, dim3(std::min<uint32_t>(groups(p.nQuads, 4), 4096)), dim3(256), 0, stream, p);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(rz_label_count_kernel, dim3(groups(maxBlocks, 4), p.nViews), dim3(256), 0, stream, p);
+  return hipGetLastError();
+}
+
+hipError_t launch_features(const LearnParams& p, uint32_t nBatches, hipStream_t stream) {
+  if (nBatches == 0 || p.nItems == 0) return hipSuccess;
+  hipLaunchKernelGGL(rz_features_kernel, dim3(groups(nBatches, 256)), dim3(256), 0, stream, p, nBatches);
+  return hipGetLastError();
+}
+
+hipError_t launch_batch_items(const LearnParams& p, const uint32_t* order, uint32_t nBatches, ItemDesc* outItems,
+                              ItemState* outState, hipStream_t stream) {
+  if (nBatches == 0 || p.nItems == 0) return hipSuccess;
+  hipLaunchKernelGGL(rz_batch_items_kernel, dim3(groups(nBatches, 256)), dim3(256), 0, stream, p, order, nBatches,
+                     outItems, outState);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill_u32(uint32_t* p, uint32_t n, uint32_t v, hipStream_t stream) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(rz_fill_u32_kernel, dim3(groups(n, 256)), dim3(256), 0, stream, p, n, v);
+  return hipGetLastError();
+}
+
+hipError_t launch_views_clear(const ViewDesc* views, uint32_t nViews, uint32_t maxEntries, hipStream_t stream) {
+  if (nViews == 0 || maxEntries == 0) return hipSuccess;
+  hipLaunchKernelGGL(rz_views_clear_kernel, dim3(groups(maxEntries, 256), nViews), dim3(256), 0, stream, views);
   return hipGetLastError();
 }
 

@@ -10,6 +10,7 @@ enum ItemKind : uint32_t {
   kItemQuery = 0,        // Main.cpp:195-204: queryVisibility, then rasterize<needsClipping> if visible
   kItemForceNoClip = 1,  // rasterize<false>
   kItemForceClip = 2,    // rasterize<true>
+  kItemClassify = 3,     // depth-independent part of queryVisibility only, then rasterize<needsClipping>
 };
 
 // What the prep kernel decided for an item.
@@ -26,7 +27,7 @@ struct ItemDesc {
   uint32_t quadBase;        // index of the item's first quad in the frame's quad arrays
   uint32_t quadCount;       // 8 per batch of 4 packets
   uint32_t kind;            // ItemKind
-  uint32_t pad;
+  uint32_t view;            // index into the view table of a multi-view launch (0 otherwise)
   float refMin[4], refMax[4];
   float boundsMin[4], boundsMax[4];
 };
@@ -60,6 +61,27 @@ struct Mvp {
   float prebaked[16];  // m_modelViewProjection
   float raw[16];       // m_modelViewProjectionRaw
 };
+
+// One view of a multi-view launch (256 bytes): its buffer, its matrix, its
+// slice of the launch's items and queries, and its range of (view, tile)
+// work units.
+struct ViewDesc {
+  uint32_t width, height, bX, bY;
+  uint32_t tilesX, tilesY, pyrX, pyrY;
+  uint32_t itemBase, nItems, unitBase, nUnits;
+  uint32_t queryBase, nQueries;
+  uint32_t batchBase, nBatches;  // the view's 8-quad batches in the launch's batch numbering
+  uint16_t* depth;
+  uint16_t* hiz;
+  uint16_t* tileMin;
+  const float* queryBounds;  // nQueries x 8 floats
+  uint8_t* queryVisible;     // nQueries
+  uint8_t* queryClip;        // nQueries
+  uint32_t* winner;          // per pixel, for draw-order labels (null when labels are off)
+  uint64_t pad;
+  Mvp mvp;
+};
+static_assert(sizeof(ViewDesc) == 256, "ViewDesc layout");
 
 enum ErrorBits : uint32_t {
   kErrTimeout = 1u,  // a frame-kernel wait did not finish in time

@@ -1,0 +1,554 @@
+// Batched multi-view launches (C ABI in rasterizer_amd.h, rz_views_*).
+//
+// A step renders every view of the batch: clear, the depth-independent part
+// of queryVisibility per occluder (frustum cull, near-plane test, empty
+// rectangle; Rasterizer.cpp:123-264), rasterize<needsClipping> of the
+// survivors in each view's draw order, then the view's query boxes.  One
+// prep launch and one setup launch cover every (view, occluder) item and
+// every quad; one persistent raster launch takes (view, tile) units from an
+// atomic queue; one query launch covers every view's boxes.  The sequence is
+// captured once into a hipGraph and replayed per step; matrices live in a
+// pinned host table that the graph copies to the device at the start of each
+// replay, so changing them does not need a new capture.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "rasterizer_amd.h"
+#include "rz_host.h"
+#include "rz_internal.h"
+#include "rz_kernels.h"
+
+using namespace rzd;
+
+namespace {
+
+struct ViewHost {
+  uint32_t width = 0, height = 0, bX = 0, bY = 0, tilesX = 0, tilesY = 0, pyrX = 0, pyrY = 0;
+  uint16_t* d_depth = nullptr;
+  uint16_t* d_hiz = nullptr;
+  uint16_t* d_tileMin = nullptr;
+  std::vector<const rz_occluder*> occ;
+  uint32_t nQueries = 0;
+  float* d_qbounds = nullptr;
+  uint8_t* d_qvis = nullptr;
+  uint8_t* d_qclip = nullptr;
+  uint32_t* d_winner = nullptr;  // per pixel, when labels are on
+  std::vector<uint32_t> order;   // explicit batch order (view-local batch numbers), empty = occluder order
+  Mvp mvp{};
+};
+
+template <typename T>
+hipError_t dev_alloc(T** p, size_t count) {
+  *p = nullptr;
+  if (count == 0) return hipSuccess;
+  return hipMalloc(reinterpret_cast<void**>(p), count * sizeof(T));
+}
+
+}  // namespace
+
+struct rz_views {
+  hipStream_t stream = nullptr;
+  std::vector<ViewHost> views;
+  uint64_t* d_cov = nullptr;
+  uint32_t* d_rcp = nullptr;
+  // plan, rebuilt when an occluder list or a query count changes
+  bool dirty = true;
+  uint32_t nItems = 0, nQuads = 0, nUnits = 0, nQueries = 0, nGroups = 0, maxClear = 0;
+  ItemDesc* d_items = nullptr;
+  ItemState* d_state = nullptr;
+  uint32_t* d_done = nullptr;
+  uint32_t* d_visible = nullptr;
+  QuadRec* d_recs = nullptr;
+  uint64_t* d_bbox = nullptr;
+  uint64_t* d_groupBox = nullptr;
+  ViewDesc* d_views = nullptr;
+  ViewDesc* h_views = nullptr;  // pinned; copied to d_views by every step
+  uint32_t* d_queue = nullptr;
+  // draw-order learning
+  bool labels = false;    // compute features before and labels after the draw
+  bool batchOrder = false;  // draw batch items in d_order
+  uint32_t nBatches = 0, maxBlocks = 0;
+  uint32_t* d_labels = nullptr;
+  float* d_features = nullptr;
+  uint32_t* d_order = nullptr;
+  ItemDesc* d_bitems = nullptr;
+  ItemState* d_bstate = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  hipEvent_t t0 = nullptr, t1 = nullptr;
+  bool ran = false;
+};
+
+#define RZ_HIP(call)                                \
+  do {                                              \
+    if ((call) != hipSuccess) return RZ_ERR_DEVICE; \
+  } while (0)
+
+static void free_plan(rz_views* v) {
+  if (v->exec) (void)hipGraphExecDestroy(v->exec);
+  if (v->graph) (void)hipGraphDestroy(v->graph);
+  v->exec = nullptr;
+  v->graph = nullptr;
+  hipFree(v->d_items);
+  hipFree(v->d_state);
+  hipFree(v->d_done);
+  hipFree(v->d_visible);
+  hipFree(v->d_recs);
+  hipFree(v->d_bbox);
+  hipFree(v->d_groupBox);
+  hipFree(v->d_views);
+  hipFree(v->d_labels);
+  hipFree(v->d_features);
+  hipFree(v->d_order);
+  hipFree(v->d_bitems);
+  hipFree(v->d_bstate);
+  v->d_labels = nullptr;
+  v->d_features = nullptr;
+  v->d_order = nullptr;
+  v->d_bitems = nullptr;
+  v->d_bstate = nullptr;
+  if (v->h_views) (void)hipHostFree(v->h_views);
+  v->d_items = nullptr;
+  v->d_state = nullptr;
+  v->d_done = v->d_visible = nullptr;
+  v->d_recs = nullptr;
+  v->d_bbox = v->d_groupBox = nullptr;
+  v->d_views = nullptr;
+  v->h_views = nullptr;
+}
+
+static void free_views(rz_views* v) {
+  if (!v) return;
+  if (v->stream) (void)hipStreamSynchronize(v->stream);
+  free_plan(v);
+  for (ViewHost& h : v->views) {
+    hipFree(h.d_depth);
+    hipFree(h.d_hiz);
+    hipFree(h.d_tileMin);
+    hipFree(h.d_qbounds);
+    hipFree(h.d_qvis);
+    hipFree(h.d_qclip);
+    hipFree(h.d_winner);
+  }
+  hipFree(v->d_cov);
+  hipFree(v->d_rcp);
+  hipFree(v->d_queue);
+  if (v->t0) (void)hipEventDestroy(v->t0);
+  if (v->t1) (void)hipEventDestroy(v->t1);
+  if (v->stream) (void)hipStreamDestroy(v->stream);
+  delete v;
+}
+
+extern "C" int rz_views_create(uint32_t nViews, const uint32_t* widths, const uint32_t* heights, rz_views** out) {
+  if (!out || !widths || !heights || nViews == 0) return RZ_ERR_INVALID_ARGUMENT;
+  *out = nullptr;
+  for (uint32_t i = 0; i < nViews; ++i) {
+    const uint32_t w = widths[i], h = heights[i];
+    if (w == 0 || h == 0 || (w % 8) || (h % 8) || w > 65528 || h > 65528) return RZ_ERR_INVALID_ARGUMENT;
+    if (uint64_t(w / 8) * (h / 8) >= (1u << 24)) return RZ_ERR_TOO_LARGE;
+  }
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return RZ_ERR_DEVICE;
+  const rzi::HostTables& tabs = rzi::host_tables();
+  if (tabs.status != RZ_OK) return tabs.status;
+  rz_views* v = new (std::nothrow) rz_views();
+  if (!v) return RZ_ERR_OUT_OF_MEMORY;
+  auto fail = [&](int code) {
+    free_views(v);
+    return code;
+  };
+  if (hipStreamCreateWithFlags(&v->stream, hipStreamNonBlocking) != hipSuccess) return fail(RZ_ERR_DEVICE);
+  if (hipEventCreate(&v->t0) != hipSuccess || hipEventCreate(&v->t1) != hipSuccess) return fail(RZ_ERR_DEVICE);
+  if (dev_alloc(&v->d_cov, 4096) != hipSuccess || dev_alloc(&v->d_rcp, 4096) != hipSuccess ||
+      dev_alloc(&v->d_queue, 1) != hipSuccess)
+    return fail(RZ_ERR_OUT_OF_MEMORY);
+  if (hipMemcpy(v->d_cov, tabs.cov, sizeof tabs.cov, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(v->d_rcp, tabs.rcp, sizeof tabs.rcp, hipMemcpyHostToDevice) != hipSuccess)
+    return fail(RZ_ERR_DEVICE);
+  v->views.resize(nViews);
+  const float ident[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
+  for (uint32_t i = 0; i < nViews; ++i) {
+    ViewHost& h = v->views[i];
+    h.width = widths[i];
+    h.height = heights[i];
+    h.bX = h.width / 8;
+    h.bY = h.height / 8;
+    h.tilesX = (h.bX + 7) / 8;
+    h.tilesY = (h.bY + 7) / 8;
+    h.pyrX = h.tilesX;
+    h.pyrY = h.tilesY;
+    const size_t nb = size_t(h.bX) * h.bY;
+    if (dev_alloc(&h.d_depth, nb * 64) != hipSuccess || dev_alloc(&h.d_hiz, nb + 8) != hipSuccess ||
+        dev_alloc(&h.d_tileMin, size_t(h.pyrX) * h.pyrY) != hipSuccess)
+      return fail(RZ_ERR_OUT_OF_MEMORY);
+    // Zero-initialised like the reference's std::vector buffers.
+    if (hipMemset(h.d_depth, 0, nb * 64 * 2) != hipSuccess || hipMemset(h.d_hiz, 0, (nb + 8) * 2) != hipSuccess ||
+        hipMemset(h.d_tileMin, 0, size_t(h.pyrX) * h.pyrY * 2) != hipSuccess)
+      return fail(RZ_ERR_DEVICE);
+    rzh::prebake_mvp(h.width, h.height, ident, h.mvp.prebaked, h.mvp.raw);
+  }
+  *out = v;
+  return RZ_OK;
+}
+
+extern "C" int rz_views_destroy(rz_views* v) {
+  if (!v) return RZ_ERR_INVALID_ARGUMENT;
+  free_views(v);
+  return RZ_OK;
+}
+
+// Waits for the last step so that the pinned view table can be rewritten.
+static int wait_last_step(rz_views* v) {
+  if (v->ran) RZ_HIP(hipEventSynchronize(v->t1));
+  return RZ_OK;
+}
+
+extern "C" int rz_views_set_mvp(rz_views* v, uint32_t view, const float matrix[16]) {
+  if (!v || !matrix || view >= v->views.size()) return RZ_ERR_INVALID_ARGUMENT;
+  ViewHost& h = v->views[view];
+  rzh::prebake_mvp(h.width, h.height, matrix, h.mvp.prebaked, h.mvp.raw);
+  return RZ_OK;
+}
+
+extern "C" int rz_views_set_occluders(rz_views* v, uint32_t view, uint32_t n, const rz_occluder* const* occluders) {
+  if (!v || view >= v->views.size() || (n && !occluders)) return RZ_ERR_INVALID_ARGUMENT;
+  for (uint32_t i = 0; i < n; ++i)
+    if (!occluders[i]) return RZ_ERR_INVALID_ARGUMENT;
+  v->views[view].occ.assign(occluders, occluders + n);
+  v->dirty = true;
+  return RZ_OK;
+}
+
+extern "C" int rz_views_set_queries(rz_views* v, uint32_t view, uint32_t n, const float* bounds) {
+  if (!v || view >= v->views.size() || (n && !bounds)) return RZ_ERR_INVALID_ARGUMENT;
+  int s = wait_last_step(v);
+  if (s != RZ_OK) return s;
+  ViewHost& h = v->views[view];
+  if (n != h.nQueries) {
+    hipFree(h.d_qbounds);
+    hipFree(h.d_qvis);
+    hipFree(h.d_qclip);
+    h.d_qbounds = nullptr;
+    h.d_qvis = h.d_qclip = nullptr;
+    h.nQueries = 0;
+    if (dev_alloc(&h.d_qbounds, size_t(n) * 8) != hipSuccess || dev_alloc(&h.d_qvis, n) != hipSuccess ||
+        dev_alloc(&h.d_qclip, n) != hipSuccess)
+      return RZ_ERR_OUT_OF_MEMORY;
+    h.nQueries = n;
+    v->dirty = true;
+  }
+  if (n) RZ_HIP(hipMemcpy(h.d_qbounds, bounds, size_t(n) * 32, hipMemcpyHostToDevice));
+  return RZ_OK;
+}
+
+// Builds item and view tables for the current occluder lists and captures the
+// step into a graph.
+static int build_plan(rz_views* v) {
+  free_plan(v);
+  const uint32_t nv = uint32_t(v->views.size());
+  std::vector<ItemDesc> items;
+  uint64_t quads = 0;
+  uint32_t units = 0, queries = 0, maxClear = 0;
+  v->maxBlocks = 0;
+  if (hipHostMalloc(reinterpret_cast<void**>(&v->h_views), nv * sizeof(ViewDesc)) != hipSuccess)
+    return RZ_ERR_OUT_OF_MEMORY;
+  for (uint32_t i = 0; i < nv; ++i) {
+    ViewHost& h = v->views[i];
+    ViewDesc& d = v->h_views[i];
+    std::memset(&d, 0, sizeof d);
+    d.width = h.width;
+    d.height = h.height;
+    d.bX = h.bX;
+    d.bY = h.bY;
+    d.tilesX = h.tilesX;
+    d.tilesY = h.tilesY;
+    d.pyrX = h.pyrX;
+    d.pyrY = h.pyrY;
+    d.itemBase = uint32_t(items.size());
+    d.nItems = uint32_t(h.occ.size());
+    d.unitBase = units;
+    d.nUnits = h.tilesX * h.tilesY;
+    d.queryBase = queries;
+    d.nQueries = h.nQueries;
+    d.batchBase = uint32_t(quads / 8);
+    d.depth = h.d_depth;
+    d.hiz = h.d_hiz;
+    d.tileMin = h.d_tileMin;
+    d.queryBounds = h.d_qbounds;
+    d.queryVisible = h.d_qvis;
+    d.queryClip = h.d_qclip;
+    d.mvp = h.mvp;
+    units += d.nUnits;
+    queries += d.nQueries;
+    maxClear = std::max(maxClear, std::max(((h.bX * h.bY + 8) / 8) * 8, h.pyrX * h.pyrY));
+    for (const rz_occluder* o : h.occ) {
+      ItemDesc it{};
+      it.packets = o->d_packets;
+      it.quadBase = uint32_t(quads);
+      it.quadCount = (o->packetCount / 4) * 8;
+      it.kind = kItemClassify;
+      it.view = i;
+      std::memcpy(it.refMin, o->refMin, 16);
+      std::memcpy(it.refMax, o->refMax, 16);
+      std::memcpy(it.boundsMin, o->boundsMin, 16);
+      std::memcpy(it.boundsMax, o->boundsMax, 16);
+      quads += it.quadCount;
+      items.push_back(it);
+    }
+    d.nBatches = uint32_t(quads / 8) - d.batchBase;
+    d.winner = h.d_winner;
+    v->maxBlocks = std::max(v->maxBlocks, h.bX * h.bY);
+    if (!h.order.empty() && h.order.size() != d.nBatches) return RZ_ERR_INVALID_ARGUMENT;
+  }
+  if (quads >= (1ull << 31)) return RZ_ERR_TOO_LARGE;
+  v->nBatches = uint32_t(quads / 8);
+  v->nItems = uint32_t(items.size());
+  v->nQuads = uint32_t(quads);
+  v->nUnits = units;
+  v->nQueries = queries;
+  v->maxClear = maxClear;
+  int maxGroups = 0;
+  RZ_HIP(views_max_groups(&maxGroups));
+  v->nGroups = std::min<uint32_t>(units, uint32_t(std::max(maxGroups, 1)));
+  const size_t ni = std::max<size_t>(items.size(), 1);
+  if (dev_alloc(&v->d_items, ni) != hipSuccess || dev_alloc(&v->d_state, ni) != hipSuccess ||
+      dev_alloc(&v->d_done, ni) != hipSuccess || dev_alloc(&v->d_visible, ni) != hipSuccess ||
+      dev_alloc(&v->d_recs, std::max<size_t>(quads, 1)) != hipSuccess ||
+      dev_alloc(&v->d_bbox, std::max<size_t>(quads, 1)) != hipSuccess ||
+      dev_alloc(&v->d_groupBox, std::max<size_t>((quads + 63) / 64, 1)) != hipSuccess ||
+      dev_alloc(&v->d_views, nv) != hipSuccess)
+    return RZ_ERR_OUT_OF_MEMORY;
+  if (!items.empty()) RZ_HIP(hipMemcpy(v->d_items, items.data(), items.size() * sizeof(ItemDesc), hipMemcpyHostToDevice));
+  const size_t nbt = std::max<size_t>(v->nBatches, 1);
+  if (dev_alloc(&v->d_labels, nbt) != hipSuccess || dev_alloc(&v->d_features, nbt * kFeatures) != hipSuccess)
+    return RZ_ERR_OUT_OF_MEMORY;
+  RZ_HIP(hipMemset(v->d_labels, 0, nbt * sizeof(uint32_t)));
+  RZ_HIP(hipMemset(v->d_features, 0, nbt * kFeatures * sizeof(float)));
+  // An explicit batch order in any view switches the step to batch items;
+  // views without one keep their batches in occluder order.
+  v->batchOrder = false;
+  for (const ViewHost& h : v->views) v->batchOrder |= !h.order.empty();
+  if (v->batchOrder) {
+    std::vector<uint32_t> order(v->nBatches);
+    for (uint32_t i = 0; i < nv; ++i) {
+      const ViewDesc& d = v->h_views[i];
+      const ViewHost& h = v->views[i];
+      for (uint32_t k = 0; k < d.nBatches; ++k) {
+        const uint32_t b = h.order.empty() ? k : h.order[k];
+        if (b >= d.nBatches) return RZ_ERR_INVALID_ARGUMENT;
+        order[d.batchBase + k] = d.batchBase + b;
+      }
+    }
+    if (dev_alloc(&v->d_order, nbt) != hipSuccess || dev_alloc(&v->d_bitems, nbt) != hipSuccess ||
+        dev_alloc(&v->d_bstate, nbt) != hipSuccess)
+      return RZ_ERR_OUT_OF_MEMORY;
+    if (v->nBatches) RZ_HIP(hipMemcpy(v->d_order, order.data(), order.size() * 4, hipMemcpyHostToDevice));
+  }
+  v->dirty = false;
+  return RZ_OK;
+}
+
+static LearnParams learn_params(rz_views* v) {
+  LearnParams lp{};
+  lp.items = v->d_items;
+  lp.state = v->d_state;
+  lp.nItems = v->nItems;
+  lp.views = v->d_views;
+  lp.nViews = uint32_t(v->views.size());
+  lp.recs = v->d_recs;
+  lp.bbox = v->d_bbox;
+  lp.nQuads = v->nQuads;
+  lp.cov = v->d_cov;
+  lp.labels = v->d_labels;
+  lp.features = v->d_features;
+  return lp;
+}
+
+// The launches of one step, on v->stream.
+static int enqueue_step(rz_views* v) {
+  const uint32_t nv = uint32_t(v->views.size());
+  RZ_HIP(hipMemcpyAsync(v->d_views, v->h_views, nv * sizeof(ViewDesc), hipMemcpyHostToDevice, v->stream));
+  RZ_HIP(launch_views_clear(v->d_views, nv, v->maxClear, v->stream));
+  PrepParams pp{};
+  pp.items = v->d_items;
+  pp.state = v->d_state;
+  pp.nItems = v->nItems;
+  pp.tileW = pp.tileH = 8;
+  pp.rcp = v->d_rcp;
+  pp.done = v->d_done;
+  pp.visible = v->d_visible;
+  pp.views = v->d_views;
+  pp.queue = v->d_queue;
+  if (v->nItems == 0) {
+    // Nothing to draw: the prep launch that resets the queue is skipped, and
+    // so is the raster launch that reads it.
+  } else {
+    RZ_HIP(launch_prep(pp, v->stream));
+    SetupParams sp{};
+    sp.items = v->d_items;
+    sp.state = v->d_state;
+    sp.nItems = v->nItems;
+    sp.nQuads = v->nQuads;
+    sp.rcp = v->d_rcp;
+    sp.recs = v->d_recs;
+    sp.bbox = v->d_bbox;
+    sp.groupBox = v->d_groupBox;
+    sp.views = v->d_views;
+    RZ_HIP(launch_setup(sp, v->stream));
+    const LearnParams lp = learn_params(v);
+    if (v->labels) RZ_HIP(launch_features(lp, v->nBatches, v->stream));
+    if (v->batchOrder) RZ_HIP(launch_batch_items(lp, v->d_order, v->nBatches, v->d_bitems, v->d_bstate, v->stream));
+    ViewsParams rp{};
+    rp.items = v->batchOrder ? v->d_bitems : v->d_items;
+    rp.state = v->batchOrder ? v->d_bstate : v->d_state;
+    rp.batchItems = v->batchOrder ? 1u : 0u;
+    rp.views = v->d_views;
+    rp.nViews = nv;
+    rp.nUnits = v->nUnits;
+    rp.recs = v->d_recs;
+    rp.groupBox = v->d_groupBox;
+    rp.cov = v->d_cov;
+    rp.queue = v->d_queue;
+    RZ_HIP(launch_views(rp, v->nGroups, v->stream));
+    if (v->labels) {
+      RZ_HIP(launch_fill_u32(v->d_labels, v->nBatches, 0u, v->stream));
+      RZ_HIP(launch_labels(lp, v->maxBlocks, v->stream));
+    }
+  }
+  ViewsQueryParams qp{};
+  qp.views = v->d_views;
+  qp.nViews = nv;
+  qp.nQueries = v->nQueries;
+  qp.rcp = v->d_rcp;
+  RZ_HIP(launch_views_query(qp, v->stream));
+  return RZ_OK;
+}
+
+extern "C" int rz_views_run(rz_views* v, int useGraph) {
+  if (!v) return RZ_ERR_INVALID_ARGUMENT;
+  int s = wait_last_step(v);
+  if (s != RZ_OK) return s;
+  if (v->dirty) {
+    s = build_plan(v);
+    if (s != RZ_OK) return s;
+  }
+  for (uint32_t i = 0; i < v->views.size(); ++i) v->h_views[i].mvp = v->views[i].mvp;
+  if (useGraph && !v->exec) {
+    RZ_HIP(hipStreamBeginCapture(v->stream, hipStreamCaptureModeThreadLocal));
+    s = enqueue_step(v);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(v->stream, &g);
+    if (s != RZ_OK || e != hipSuccess) {
+      if (g) (void)hipGraphDestroy(g);
+      return s != RZ_OK ? s : RZ_ERR_DEVICE;
+    }
+    v->graph = g;
+    RZ_HIP(hipGraphInstantiate(&v->exec, v->graph, nullptr, nullptr, 0));
+  }
+  RZ_HIP(hipEventRecord(v->t0, v->stream));
+  if (useGraph) {
+    RZ_HIP(hipGraphLaunch(v->exec, v->stream));
+  } else {
+    s = enqueue_step(v);
+    if (s != RZ_OK) return s;
+  }
+  RZ_HIP(hipEventRecord(v->t1, v->stream));
+  v->ran = true;
+  return RZ_OK;
+}
+
+extern "C" int rz_views_sync(rz_views* v, float* ms) {
+  if (!v) return RZ_ERR_INVALID_ARGUMENT;
+  RZ_HIP(hipStreamSynchronize(v->stream));
+  if (ms) {
+    *ms = 0.0f;
+    if (v->ran) RZ_HIP(hipEventElapsedTime(ms, v->t0, v->t1));
+  }
+  return RZ_OK;
+}
+
+extern "C" int rz_views_read_raw(rz_views* v, uint32_t view, uint16_t* depth, uint16_t* hiz) {
+  if (!v || view >= v->views.size()) return RZ_ERR_INVALID_ARGUMENT;
+  RZ_HIP(hipStreamSynchronize(v->stream));
+  const ViewHost& h = v->views[view];
+  const size_t nb = size_t(h.bX) * h.bY;
+  if (depth) RZ_HIP(hipMemcpy(depth, h.d_depth, nb * 64 * 2, hipMemcpyDeviceToHost));
+  if (hiz) RZ_HIP(hipMemcpy(hiz, h.d_hiz, (nb + 8) * 2, hipMemcpyDeviceToHost));
+  return RZ_OK;
+}
+
+extern "C" int rz_views_read_queries(rz_views* v, uint32_t view, uint8_t* visible, uint8_t* needsClipping) {
+  if (!v || view >= v->views.size()) return RZ_ERR_INVALID_ARGUMENT;
+  RZ_HIP(hipStreamSynchronize(v->stream));
+  const ViewHost& h = v->views[view];
+  if (h.nQueries == 0) return RZ_OK;
+  if (visible) RZ_HIP(hipMemcpy(visible, h.d_qvis, h.nQueries, hipMemcpyDeviceToHost));
+  if (needsClipping) RZ_HIP(hipMemcpy(needsClipping, h.d_qclip, h.nQueries, hipMemcpyDeviceToHost));
+  return RZ_OK;
+}
+
+extern "C" int rz_views_read_status(rz_views* v, uint32_t view, uint8_t* status) {
+  if (!v || view >= v->views.size() || !status) return RZ_ERR_INVALID_ARGUMENT;
+  if (v->dirty || !v->h_views) return RZ_ERR_INVALID_ARGUMENT;
+  RZ_HIP(hipStreamSynchronize(v->stream));
+  const ViewDesc& d = v->h_views[view];
+  std::vector<ItemState> st(d.nItems);
+  if (d.nItems)
+    RZ_HIP(hipMemcpy(st.data(), v->d_state + d.itemBase, d.nItems * sizeof(ItemState), hipMemcpyDeviceToHost));
+  for (uint32_t i = 0; i < d.nItems; ++i) status[i] = uint8_t(st[i].status);
+  return RZ_OK;
+}
+
+extern "C" int rz_views_enable_labels(rz_views* v, int enable) {
+  if (!v) return RZ_ERR_INVALID_ARGUMENT;
+  int s = wait_last_step(v);
+  if (s != RZ_OK) return s;
+  if (enable && !v->labels) {
+    for (ViewHost& h : v->views) {
+      const size_t n = size_t(h.bX) * h.bY * 64;
+      if (!h.d_winner) {
+        if (dev_alloc(&h.d_winner, n) != hipSuccess) return RZ_ERR_OUT_OF_MEMORY;
+        RZ_HIP(hipMemset(h.d_winner, 0xFF, n * sizeof(uint32_t)));
+      }
+    }
+  }
+  v->labels = enable != 0;
+  v->dirty = true;
+  return RZ_OK;
+}
+
+extern "C" int rz_views_set_batch_order(rz_views* v, uint32_t view, uint32_t n, const uint32_t* order) {
+  if (!v || view >= v->views.size() || (n && !order)) return RZ_ERR_INVALID_ARGUMENT;
+  v->views[view].order.assign(order, order + n);
+  v->dirty = true;
+  return RZ_OK;
+}
+
+extern "C" int rz_views_read_labels(rz_views* v, uint32_t view, uint32_t* labels, float* features) {
+  if (!v || view >= v->views.size() || v->dirty || !v->h_views) return RZ_ERR_INVALID_ARGUMENT;
+  RZ_HIP(hipStreamSynchronize(v->stream));
+  const ViewDesc& d = v->h_views[view];
+  if (labels && d.nBatches)
+    RZ_HIP(hipMemcpy(labels, v->d_labels + d.batchBase, d.nBatches * sizeof(uint32_t), hipMemcpyDeviceToHost));
+  if (features && d.nBatches)
+    RZ_HIP(hipMemcpy(features, v->d_features + size_t(d.batchBase) * kFeatures,
+                     size_t(d.nBatches) * kFeatures * sizeof(float), hipMemcpyDeviceToHost));
+  return RZ_OK;
+}
+
+extern "C" int rz_views_get_info(rz_views* v, uint32_t info[8]) {
+  if (!v || !info) return RZ_ERR_INVALID_ARGUMENT;
+  info[0] = uint32_t(v->views.size());
+  info[1] = v->nItems;
+  info[2] = v->nQuads;
+  info[3] = v->nUnits;
+  info[4] = v->nQueries;
+  info[5] = v->nGroups;
+  info[6] = v->exec ? 1u : 0u;
+  info[7] = v->dirty ? 1u : 0u;
+  return RZ_OK;
+}

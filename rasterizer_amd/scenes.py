@@ -414,6 +414,52 @@ def config2(seed: int = 1, frames: int = 256, n_queries: int = 65536, grid: int 
     return scene
 
 
+CONFIG3_SIZES = [(2048, 2048)] * 16 + [(1024, 1024)] * 24 + [(512, 512)] * 16 + [(1920, 1080)] * 8
+
+
+def config3(seed: int = 3, n_boxes: int = 40000, half_cube: float = 171.0, n_queries: int = 16384,
+            sizes=None):
+    """Batched multi-view: config 1's box distribution (half-extents
+    log-U[0.5,5]) over a cube of the same box density but 40,000 boxes
+    (centres U[-171,171]^3, 240,000 quads), so that per-view subsets of up to
+    200k quads exist.  Views (default 64: 16 at 2048^2, 24 at 1024^2, 16 at
+    512^2, 8 at 1920x1080) each get an eye inside the cube, a direction, an
+    occluder subset whose real quad count is log-U[1k,200k] in front-to-back
+    order for that eye, and 16,384 query boxes of config 1's query
+    distribution around the eye."""
+    rng = SplitMix64(seed)
+    c, h = random_boxes(rng, n_boxes, -half_cube, half_cube, 0.5, 5.0)
+    quads = np.concatenate([box_quads(c[i], h[i]) for i in range(n_boxes)], axis=0)
+    scene = bake_scene(quads)
+    sizes = list(CONFIG3_SIZES if sizes is None else sizes)
+    pick = np.random.default_rng(seed)
+    real = np.array([o.quad_count for o in scene.occluders])
+    views = []
+    for w, hh in sizes:
+        eye = pick.uniform(-0.8 * half_cube, 0.8 * half_cube, 3).astype(np.float32)
+        d = pick.normal(size=3)
+        d /= np.linalg.norm(d)
+        up = (0.0, 1.0, 0.0) if abs(d[1]) < 0.95 else (1.0, 0.0, 0.0)
+        budget = math.exp(pick.uniform(math.log(1000.0), math.log(200000.0)))
+        order = pick.permutation(len(scene.occluders))
+        k = int(np.searchsorted(np.cumsum(real[order]), budget)) + 1
+        sub = [scene.occluders[i] for i in order[:k]]
+        sub = [sub[i] for i in front_to_back(sub, eye)]
+        qc = eye + pick.uniform(-50.0, 50.0, (n_queries, 3))
+        qh = np.exp(pick.uniform(math.log(0.1), math.log(2.0), (n_queries, 3)))
+        views.append(dict(size=(w, hh), eye=eye, dir=d.astype(np.float32), up=up, occluders=sub,
+                          quads=int(sum(o.quad_count for o in sub)), queries=aabb_bounds(qc, qh)))
+    scene.meta.update(views=views, seed=seed)
+    return scene
+
+
+def view_mvp(view: dict, step: int = 0, speed: float = 0.5) -> np.ndarray:
+    """Matrix of a config-3 view at a step: the eye moves along its direction."""
+    w, h = view["size"]
+    eye = view["eye"] + np.float32(speed * step) * view["dir"]
+    return view_projection(eye, view["dir"], view["up"], math.pi / 3, w / h)
+
+
 def front_to_back(occluders, eye) -> np.ndarray:
     """Draw order of Main.cpp:184-190: ascending squared distance from the eye
     to each occluder's centre (float32, DPPS order), ties by index."""

@@ -1,0 +1,214 @@
+"""GPU draw-order learning pieces against the oracle: per-batch labels (bit
+exact), draws in an explicit batch order (bit exact against the oracle drawing
+the same batches in the same order), and the 16 batch features (against a
+numpy restatement of their definitions on the oracle's per-quad setup)."""
+
+from __future__ import annotations
+
+from types import SimpleNamespace
+
+import numpy as np
+import pytest
+
+from rasterizer_amd import Rasterizer, Views, scenes
+
+pytestmark = pytest.mark.gpu
+
+SIZES = [(320, 192), (256, 256), (640, 360), (128, 64)]
+
+
+@pytest.fixture(scope="module")
+def scene():
+    return scenes.config1(seed=4, n_boxes=512)
+
+
+def _setup(scene, seed):
+    rng = np.random.default_rng(seed)
+    out = []
+    for w, h in SIZES:
+        eye = rng.uniform(-40, 40, 3).astype(np.float32)
+        d = rng.normal(size=3)
+        d /= np.linalg.norm(d)
+        mvp = scenes.view_projection(eye, d, (0, 1, 0) if abs(d[1]) < 0.95 else (1, 0, 0), np.pi / 3, w / h)
+        occ = [scene.occluders[i] for i in scenes.front_to_back(scene.occluders, eye)]
+        out.append((mvp, occ))
+    return out
+
+
+def _batches(occ):
+    """(occluder index, batch index within it) of each batch, in draw order."""
+    return [(i, b) for i, o in enumerate(occ) for b in range(o.packets.size // 32)]
+
+
+def _oracle_batch_draw(oracle_mod, w, h, mvp, occ, status, order):
+    o = oracle_mod.OracleRasterizer(w, h)
+    o.clear()
+    o.setModelViewProjection(mvp)
+    bl = _batches(occ)
+    for k in order:
+        i, b = bl[k]
+        if status[i]:
+            src = occ[i]
+            part = SimpleNamespace(packets=src.packets[32 * b:32 * b + 32], m_refMin=src.m_refMin,
+                                   m_refMax=src.m_refMax)
+            o.rasterize(part, status[i] == 2)
+    return o
+
+
+def test_labels_match_oracle(oracle_mod, scene):
+    owner = Rasterizer(64, 64)
+    V = Views(owner, SIZES)
+    setup = _setup(scene, 1)
+    for k, (mvp, occ) in enumerate(setup):
+        V.set_mvp(k, mvp)
+        V.set_occluders(k, occ)
+    V.enable_labels(True)
+    V.run(graph=True)
+    V.sync()
+    for k, (w, h) in enumerate(SIZES):
+        mvp, occ = setup[k]
+        o = oracle_mod.OracleRasterizer(w, h)
+        o.clear()
+        o.setModelViewProjection(mvp)
+        st = o.draw_frame(occ, gated=False)
+        want = o.labels(occ, st)
+        got, _ = V.labels(k)
+        assert got.shape == want.shape
+        assert np.array_equal(got, want), f"view {k}: {int((got != want).sum())} labels differ"
+        # Every written block with a non-zero pixel has at least one winner.
+        d, hz = o.raw()
+        nb = (w // 8) * (h // 8)
+        live = (hz[:nb] != 1) & (d.reshape(nb, 64).max(axis=1) > 0)
+        assert want.sum() >= live.sum()
+        print(f"view {k}: {want.size} batches, {int((want > 0).sum())} labelled, {int(live.sum())} live blocks",
+              flush=True)
+
+
+def test_batch_order_matches_oracle(oracle_mod, scene, report):
+    owner = Rasterizer(64, 64)
+    V = Views(owner, SIZES)
+    setup = _setup(scene, 2)
+    rng = np.random.default_rng(9)
+    orders = []
+    for k, (mvp, occ) in enumerate(setup):
+        V.set_mvp(k, mvp)
+        V.set_occluders(k, occ)
+        n = V.n_batches(k)
+        order = np.arange(n) if k == 0 else rng.permutation(n)
+        orders.append(order)
+        V.set_batch_order(k, order)
+    V.run(graph=True)
+    V.sync()
+    for k, (w, h) in enumerate(SIZES):
+        mvp, occ = setup[k]
+        o0 = oracle_mod.OracleRasterizer(w, h)
+        o0.clear()
+        o0.setModelViewProjection(mvp)
+        st = o0.draw_frame(occ, gated=False)
+        o = _oracle_batch_draw(oracle_mod, w, h, mvp, occ, st, orders[k])
+        dg, hg = V.raw(k)
+        do, ho = o.raw()
+        assert np.array_equal(hg, ho), f"view {k}: hiZ {report(hg, ho)}"
+        assert np.array_equal(dg, do), f"view {k}: depth {report(dg, do)}"
+        if k == 0:
+            # Batch items in occluder order draw exactly the occluder-order frame.
+            d0, h0 = o0.raw()
+            assert np.array_equal(dg, d0) and np.array_equal(hg, h0)
+        else:
+            d0, _ = o0.raw()
+            diff = np.abs(dg.astype(int) - d0.astype(int))
+            print(f"view {k}: shuffled batch order vs occluder order: {int((diff > 0).sum())} px differ, "
+                  f"max {int(diff.max())} LSB", flush=True)
+
+
+def _features_ref(oracle_mod, w, h, mvp, occ, st, prev_labels):
+    """The feature definitions of rz_features_kernel restated in numpy on the
+    oracle's per-quad setup (float64; compared with a tolerance)."""
+    o = oracle_mod.OracleRasterizer(w, h)
+    o.setModelViewProjection(mvp)
+    pre, raw = o.mvp_state()
+    bX, bY = w // 8, h // 8
+    vd = raw[12:15].astype(np.float64)
+    vd /= np.linalg.norm(vd)
+    rows = []
+    nbt = sum(x.packets.size // 32 for x in occ)
+    k = 0
+    for i, oc in enumerate(occ):
+        sd = o.setup_dump(oc, st[i] == 2) if st[i] else None
+        pk = oc.packets.reshape(-1, 4, 8)
+        ext = (oc.m_refMax - oc.m_refMin)[:3].astype(np.float64)
+        for b in range(pk.shape[0]):
+            f = np.zeros(16)
+            boxes, zs, nds, convex = [], [], [], 0
+            cen = np.zeros(3)
+            for lane in range(8):
+                wds = pk[b, :, lane]
+                X = (wds.view(np.int32) >> 21).astype(np.float64) + 1024
+                Y = ((wds >> 10) & 2047).astype(np.float64)
+                Z = (wds & 1023).astype(np.float64)
+                P = oc.m_refMin[:3].astype(np.float64) + np.stack([X / 2047, Y / 2047, Z / 1023], 1) * ext
+                cen += P.sum(0) / 32
+                if sd is None or not sd[8 * b + lane, 0]:
+                    continue
+                q = sd[8 * b + lane]
+                fb, rx, ry = int(q[22]), int(q[4]), int(q[5])
+                col, row = fb % bX, fb // bX
+                if col + rx <= bX:
+                    box = (col, col + rx - 1, row, min(row + ry - 1, bY - 1))
+                else:
+                    box = (0, bX - 1, row, min(row + ry, bY - 1))
+                boxes.append(box)
+                zs.append(q[6] / 65535)
+                convex += int(q[1]) == 6
+                e1, e2, e3 = P[1] - P[0], P[2] - P[0], P[3] - P[0]
+                n = np.cross(e1, e2) + np.cross(e2, e3)
+                nl = np.linalg.norm(n)
+                nds.append(n @ vd / nl if nl > 0 else 0.0)
+            n = len(boxes)
+            f[0] = n / 8
+            if n:
+                c0 = min(b_[0] for b_ in boxes); c1 = max(b_[1] for b_ in boxes)
+                r0 = min(b_[2] for b_ in boxes); r1 = max(b_[3] for b_ in boxes)
+                f[1] = np.log2(1 + (c1 - c0 + 1) * (r1 - r0 + 1))
+                f[2], f[3], f[4] = min(zs), max(zs), np.mean(zs)
+                f[5] = np.log2(1 + sum((b_[1] - b_[0] + 1) * (b_[3] - b_[2] + 1) for b_ in boxes))
+                f[7] = (0.5 * (c0 + c1) + 0.5) / bX
+                f[8] = (0.5 * (r0 + r1) + 0.5) / bY
+                f[11], f[12], f[13] = np.mean(nds), min(nds), max(nds)
+                f[15] = convex / n
+            else:
+                f[7] = f[8] = 0.5
+            f[6] = 1.0 if st[i] == 2 else 0.0
+            wv = cen @ raw[12:15].astype(np.float64) + raw[15]
+            f[9] = np.log2(1 + abs(wv))
+            f[10] = np.log2(1 + prev_labels[k])
+            f[14] = k / (nbt - 1) if nbt > 1 else 0.0
+            rows.append(f)
+            k += 1
+    return np.array(rows)
+
+
+def test_features_match_definition(oracle_mod, scene):
+    owner = Rasterizer(64, 64)
+    V = Views(owner, SIZES[:2])
+    setup = _setup(scene, 3)
+    for k in range(2):
+        V.set_mvp(k, setup[k][0])
+        V.set_occluders(k, setup[k][1])
+    V.enable_labels(True)
+    V.run(graph=True)
+    V.sync()
+    prev = [V.labels(k)[0] for k in range(2)]
+    V.run(graph=True)  # the second step's features see the first step's labels
+    V.sync()
+    for k, (w, h) in enumerate(SIZES[:2]):
+        mvp, occ = setup[k]
+        o = oracle_mod.OracleRasterizer(w, h)
+        o.clear()
+        o.setModelViewProjection(mvp)
+        st = o.draw_frame(occ, gated=False)
+        want = _features_ref(oracle_mod, w, h, mvp, occ, st, prev[k])
+        _, got = V.labels(k)
+        assert got.shape == want.shape
+        err = np.abs(got - want)
+        assert np.all(err <= 1e-4 + 1e-4 * np.abs(want)), (np.unravel_index(np.argmax(err), err.shape), err.max())

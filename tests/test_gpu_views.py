@@ -1,0 +1,120 @@
+"""GPU parity of batched multi-view launches (rz_views_*): every view equals
+the oracle's classify-then-draw sequence on one buffer of its size (clear,
+setModelViewProjection, per occluder the depth-independent part of
+queryVisibility and rasterize<needsClipping>, then the view's queries), bit
+for bit, for graph replays and direct launches alike."""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from rasterizer_amd import Rasterizer, Views, scenes
+
+pytestmark = pytest.mark.gpu
+
+SIZES = [(64, 64), (320, 192), (256, 256), (1280, 720), (200, 104), (16, 8), (512, 512), (8, 8), (1920, 1080)]
+
+
+@pytest.fixture(scope="module")
+def scene():
+    return scenes.config1(seed=2, n_boxes=768)
+
+
+def _views_setup(scene, step: int):
+    """Per view: eye, MVP, occluder subset in front-to-back order, queries."""
+    rng = np.random.default_rng(100 + step)
+    out = []
+    for k, (w, h) in enumerate(SIZES):
+        eye = rng.uniform(-60, 60, 3).astype(np.float32)
+        d = rng.normal(size=3)
+        d /= np.linalg.norm(d)
+        up = (0, 1, 0) if abs(d[1]) < 0.95 else (1, 0, 0)
+        mvp = scenes.view_projection(eye, d, up, np.pi / 3, w / h)
+        sel = np.nonzero(rng.random(len(scene.occluders)) < rng.uniform(0.2, 1.0))[0]
+        occ = [scene.occluders[i] for i in sel]
+        occ = [occ[i] for i in scenes.front_to_back(occ, eye)]
+        q = scene.meta["queries"][rng.choice(len(scene.meta["queries"]), 300 + 100 * k, replace=False)]
+        out.append((mvp, occ, q))
+    return out
+
+
+def _check_view(oracle_mod, V, view, w, h, mvp, occ, q, report, what):
+    o = oracle_mod.OracleRasterizer(w, h)
+    o.clear()
+    o.setModelViewProjection(mvp)
+    st_o = o.draw_frame(occ, gated=False)
+    st_g = V.status(view)
+    assert np.array_equal(st_g, st_o), f"{what}: statuses differ at {np.nonzero(st_g != st_o)[0][:10]}"
+    dg, hg = V.raw(view)
+    do, ho = o.raw()
+    assert np.array_equal(hg, ho), f"{what}: hiZ {report(hg, ho)}"
+    assert np.array_equal(dg, do), f"{what}: depth {report(dg, do)}"
+    vg, cg = V.queries(view)
+    vo, co = o.query_batch(q)
+    assert np.array_equal(vg, vo), f"{what}: {int((vg != vo).sum())} query bits differ"
+    assert np.array_equal(cg, co), f"{what}: {int((cg != co).sum())} clip bits differ"
+    return int((st_o > 0).sum())
+
+
+def test_views_match_oracle(oracle_mod, scene, report):
+    owner = Rasterizer(64, 64)
+    V = Views(owner, SIZES)
+    drawn = 0
+    for step, graph in enumerate([True, False, True]):
+        setup = _views_setup(scene, step)
+        for k, (mvp, occ, q) in enumerate(setup):
+            V.set_mvp(k, mvp)
+            V.set_occluders(k, occ)
+            V.set_queries(k, q)
+        print(f"step {step} graph={graph}: {V.info()}", flush=True)
+        V.run(graph=graph)
+        V.sync()
+        for k, (w, h) in enumerate(SIZES):
+            mvp, occ, q = setup[k]
+            drawn += _check_view(oracle_mod, V, k, w, h, mvp, occ, q, report, f"step {step} view {k} {w}x{h}")
+    assert drawn > 100
+
+
+def test_views_graph_replays_match_direct(oracle_mod, scene, report):
+    """Replays of one captured graph with new matrices each time, with direct
+    launches and host synchronisation between them: every replay equals the
+    direct launch of the same step and the oracle."""
+    owner = Rasterizer(64, 64)
+    V = Views(owner, SIZES[:6])
+    setup = _views_setup(scene, 7)
+    for k in range(6):
+        V.set_occluders(k, setup[k][1])
+        V.set_queries(k, setup[k][2])
+    rng = np.random.default_rng(5)
+    for rep in range(4):
+        mvps = []
+        for k, (w, h) in enumerate(SIZES[:6]):
+            eye = rng.uniform(-40, 40, 3).astype(np.float32)
+            mvps.append(scenes.view_projection(eye, (0, 0, 1), (0, 1, 0), np.pi / 3, w / h))
+            V.set_mvp(k, mvps[k])
+        V.run(graph=True)
+        V.sync()
+        assert V.info()["graph"] == 1
+        replay = [V.raw(k) + V.queries(k) for k in range(6)]
+        V.run(graph=False)
+        V.sync()
+        for k in range(6):
+            direct = V.raw(k) + V.queries(k)
+            for a, b in zip(replay[k], direct):
+                assert np.array_equal(a, b), f"replay {rep} view {k} differs from the direct launch"
+        w, h = SIZES[rep % 6]
+        _check_view(oracle_mod, V, rep % 6, w, h, mvps[rep % 6], setup[rep % 6][1], setup[rep % 6][2], report,
+                    f"replay {rep}")
+
+
+def test_views_empty_lists():
+    owner = Rasterizer(64, 64)
+    V = Views(owner, [(64, 64), (32, 16)])
+    V.set_queries(1, np.array([[-1, -1, 5, 1, 1, 1, 6, 1]], np.float32))
+    V.run(graph=True)
+    V.sync()
+    d, h = V.raw(0)
+    assert np.all(h[: 64] == 1) and np.all(d == 0)
+    vis, clip = V.queries(1)
+    assert vis.shape == (1,)
