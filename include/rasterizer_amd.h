@@ -234,6 +234,32 @@ int rz_views_set_batch_order(rz_views* views, uint32_t view, uint32_t n, const u
 /* Labels (u32 per batch) and features (16 floats per batch) of the last step. */
 int rz_views_read_labels(rz_views* views, uint32_t view, uint32_t* labels, float* features);
 
+/* The batch-scoring MLP (no counterpart in the reference): 16 features ->
+ * 64 -> 64 -> 1 with GELU, BF16 operands on MFMA with FP32 accumulation.
+ * params: 5313 floats (W1[16][64], b1[64], W2[64][64], b2[64], W3[64], b3);
+ * norm: 16 feature means then 16 inverse deviations. */
+typedef struct rz_mlp rz_mlp;
+int rz_mlp_create(uint32_t seed, rz_mlp** out);
+int rz_mlp_destroy(rz_mlp* mlp);
+int rz_mlp_set_params(rz_mlp* mlp, const float* params, const float* norm);
+int rz_mlp_get_params(rz_mlp* mlp, float* params, float* norm);
+/* Scores of n samples (16 floats each); host arrays. */
+int rz_mlp_forward(rz_mlp* mlp, uint32_t n, const float* x, float* scores);
+/* Mean pairwise ranking loss softplus(s_j - s_i) over pairs (i, j) (sample i
+ * should score higher) and its gradient in the parameters, without update. */
+int rz_mlp_gradients(rz_mlp* mlp, uint32_t n, const float* x, uint32_t nPairs, const uint32_t* pairs, float* grad,
+                     float* loss);
+/* One Adam step (beta 0.9 / 0.999, eps 1e-8) on that loss. */
+int rz_mlp_train_step(rz_mlp* mlp, uint32_t n, const float* x, uint32_t nPairs, const uint32_t* pairs, float lr,
+                      float* loss);
+/* Learned draw order for a multi-view batch: every step scores each view's
+ * batches with the model (from this step's features and the previous step's
+ * labels) and draws them by descending score, ties in occluder order.  NULL
+ * restores occluder order.  The model must outlive its use. */
+int rz_views_set_model(rz_views* views, rz_mlp* model);
+/* The batch order of the last step (view-local batch numbers). */
+int rz_views_read_order(rz_views* views, uint32_t view, uint32_t* order);
+
 #ifdef __cplusplus
 }
 #endif

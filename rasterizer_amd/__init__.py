@@ -87,6 +87,15 @@ _SIGNATURES = {
     "rz_views_enable_labels": ([_P, ctypes.c_int], ctypes.c_int),
     "rz_views_set_batch_order": ([_P, _U32, _U32, _P], ctypes.c_int),
     "rz_views_read_labels": ([_P, _U32, _P, _P], ctypes.c_int),
+    "rz_mlp_create": ([_U32, ctypes.POINTER(_P)], ctypes.c_int),
+    "rz_mlp_destroy": ([_P], ctypes.c_int),
+    "rz_mlp_set_params": ([_P, _P, _P], ctypes.c_int),
+    "rz_mlp_get_params": ([_P, _P, _P], ctypes.c_int),
+    "rz_mlp_forward": ([_P, _U32, _P, _P], ctypes.c_int),
+    "rz_mlp_gradients": ([_P, _U32, _P, _U32, _P, _P, _P], ctypes.c_int),
+    "rz_mlp_train_step": ([_P, _U32, _P, _U32, _P, ctypes.c_float, _P], ctypes.c_int),
+    "rz_views_set_model": ([_P, _P], ctypes.c_int),
+    "rz_views_read_order": ([_P, _U32, _P], ctypes.c_int),
 }
 
 
@@ -452,6 +461,16 @@ class Views:
         _check(lib().rz_views_read_labels(self._v, view, _ptr(lab), _ptr(feat)), "rz_views_read_labels")
         return lab[:n], feat[:n]
 
+    def set_model(self, mlp: "Mlp | None") -> None:
+        self._model = mlp
+        _check(lib().rz_views_set_model(self._v, mlp._m if mlp is not None else None), "rz_views_set_model")
+
+    def order(self, view: int) -> np.ndarray:
+        n = self.n_batches(view)
+        out = np.zeros(max(n, 1), np.uint32)
+        _check(lib().rz_views_read_order(self._v, view, _ptr(out)), "rz_views_read_order")
+        return out[:n]
+
     def info(self) -> dict:
         a = np.zeros(8, np.uint32)
         _check(lib().rz_views_get_info(self._v, _ptr(a)), "rz_views_get_info")
@@ -459,4 +478,60 @@ class Views:
         return dict(zip(keys, (int(x) for x in a)))
 
 
-__all__ = ["Rasterizer", "Occluder", "DeviceBuffer", "Views", "RasterizerError", "lib", "LIB_PATH"]
+class Mlp:
+    """The batch-scoring MLP (rz_mlp_*): 16 -> 64 -> 64 -> 1, GELU, BF16 MFMA."""
+
+    N_PARAMS = 5313
+
+    def __init__(self, seed: int = 1):
+        self._m = ctypes.c_void_p()
+        _check(lib().rz_mlp_create(seed, ctypes.byref(self._m)), "rz_mlp_create")
+
+    def close(self) -> None:
+        if self._m and _lib is not None:
+            _lib.rz_mlp_destroy(self._m)
+        self._m = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def params(self):
+        w = np.zeros(self.N_PARAMS, np.float32)
+        n = np.zeros(32, np.float32)
+        _check(lib().rz_mlp_get_params(self._m, _ptr(w), _ptr(n)), "rz_mlp_get_params")
+        return w, n
+
+    def set_params(self, params=None, norm=None) -> None:
+        w = None if params is None else np.ascontiguousarray(params, np.float32).reshape(self.N_PARAMS)
+        n = None if norm is None else np.ascontiguousarray(norm, np.float32).reshape(32)
+        _check(lib().rz_mlp_set_params(self._m, _ptr(w) if w is not None else None,
+                                       _ptr(n) if n is not None else None), "rz_mlp_set_params")
+
+    def forward(self, x) -> np.ndarray:
+        x = np.ascontiguousarray(x, np.float32).reshape(-1, 16)
+        out = np.zeros(max(x.shape[0], 1), np.float32)
+        _check(lib().rz_mlp_forward(self._m, x.shape[0], _ptr(x), _ptr(out)), "rz_mlp_forward")
+        return out[: x.shape[0]]
+
+    def gradients(self, x, pairs):
+        x = np.ascontiguousarray(x, np.float32).reshape(-1, 16)
+        pr = np.ascontiguousarray(pairs, np.uint32).reshape(-1, 2)
+        g = np.zeros(self.N_PARAMS, np.float32)
+        loss = np.zeros(1, np.float32)
+        _check(lib().rz_mlp_gradients(self._m, x.shape[0], _ptr(x), pr.shape[0], _ptr(pr), _ptr(g), _ptr(loss)),
+               "rz_mlp_gradients")
+        return g, float(loss[0])
+
+    def train_step(self, x, pairs, lr: float) -> float:
+        x = np.ascontiguousarray(x, np.float32).reshape(-1, 16)
+        pr = np.ascontiguousarray(pairs, np.uint32).reshape(-1, 2)
+        loss = np.zeros(1, np.float32)
+        _check(lib().rz_mlp_train_step(self._m, x.shape[0], _ptr(x), pr.shape[0], _ptr(pr), ctypes.c_float(lr),
+                                       _ptr(loss)), "rz_mlp_train_step")
+        return float(loss[0])
+
+
+__all__ = ["Rasterizer", "Occluder", "DeviceBuffer", "Views", "Mlp", "RasterizerError", "lib", "LIB_PATH"]

@@ -65,6 +65,8 @@ def build(verbose: bool = False) -> Path:
         "rz_kernels.o": [hipcc, *HIP_FLAGS, "-c", str(CSRC / "rz_kernels.hip")],
         "rz_api.o": [hipcc, *HIP_FLAGS, "-c", str(CSRC / "rz_api.cpp")],
         "rz_views.o": [hipcc, *HIP_FLAGS, "-c", str(CSRC / "rz_views.cpp")],
+        "rz_learn.o": [hipcc, *HIP_FLAGS, "-c", str(CSRC / "rz_learn.cpp")],
+        "rz_mlp.o": [hipcc, *HIP_FLAGS, "-c", str(CSRC / "rz_mlp.hip")],
     }
     cmds = [cmd + ["-o", str(BUILD / name)] for name, cmd in objs.items()]
     if verbose:

@@ -1,6 +1,7 @@
 // Definitions shared by the library's C ABI translation units (not installed).
 #pragma once
 
+#include <hip/hip_runtime.h>
 #include <stdint.h>
 
 #include "rasterizer_amd.h"
@@ -12,6 +13,24 @@ struct rz_occluder {
   uint32_t* d_packets;
   uint32_t packetCount;
   float refMin[4], refMax[4], boundsMin[4], boundsMax[4];
+};
+
+// The batch-scoring MLP's device state (rz_learn.cpp).
+struct rz_mlp {
+  hipStream_t stream = nullptr;
+  float* d_w = nullptr;     // kMlpParams
+  float* d_norm = nullptr;  // 32
+  float* d_grad = nullptr;
+  float* d_m = nullptr;
+  float* d_v = nullptr;
+  float* d_x = nullptr;
+  float* d_scores = nullptr;
+  float* d_ds = nullptr;
+  uint32_t capX = 0;
+  uint32_t* d_pairs = nullptr;
+  uint32_t capPairs = 0;
+  float* d_loss = nullptr;
+  uint32_t step = 0;
 };
 
 namespace rzi {

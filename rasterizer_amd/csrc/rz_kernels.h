@@ -170,4 +170,33 @@ hipError_t launch_batch_items(const LearnParams& p, const uint32_t* order, uint3
                               ItemState* outState, hipStream_t stream);
 hipError_t launch_fill_u32(uint32_t* p, uint32_t n, uint32_t v, hipStream_t stream);
 
+// ---------------------------------------------------------------------------
+// Batch-scoring MLP 16 -> 64 -> 64 -> 1, GELU, BF16 operands on MFMA with FP32
+// accumulation (rz_mlp.hip).  Parameters, fp32, in one array:
+//   W1 [16][64] at 0, b1 [64] at 1024, W2 [64][64] at 1088, b2 [64] at 5184,
+//   W3 [64] at 5248, b3 at 5312.  Inputs are normalised per feature by
+//   (x - mean) * invStd (norm = 16 means then 16 inverse deviations).
+// ---------------------------------------------------------------------------
+constexpr int kMlpIn = 16, kMlpHidden = 64, kMlpParams = 5313;
+constexpr int kMlpW1 = 0, kMlpB1 = 1024, kMlpW2 = 1088, kMlpB2 = 5184, kMlpW3 = 5248, kMlpB3 = 5312;
+
+struct MlpParams {
+  const float* w;
+  const float* norm;
+  const float* x;       // n x 16
+  uint32_t n;
+  float* scores;        // forward output
+  const float* dscore;  // backward input: dLoss/dscore per sample
+  float* grad;          // backward output, accumulated (zero it first)
+};
+hipError_t launch_mlp_forward(const MlpParams& p, hipStream_t stream);
+hipError_t launch_mlp_backward(const MlpParams& p, hipStream_t stream);
+// Pairwise ranking loss over pairs (i, j) whose sample i should rank above j:
+// mean softplus(s_j - s_i); adds its gradient to dscore and its value to *loss.
+hipError_t launch_mlp_pair_loss(const float* scores, const uint32_t* pairs, uint32_t nPairs, float* dscore,
+                                float* loss, hipStream_t stream);
+hipError_t launch_mlp_adam(float* w, const float* grad, float* m, float* v, uint32_t n, float lr, float beta1,
+                           float beta2, float eps, float corr1, float corr2, hipStream_t stream);
+hipError_t launch_fill_f32(float* p, uint32_t n, float v, hipStream_t stream);
+
 }  // namespace rzd

@@ -11,6 +11,7 @@
 // pinned host table that the graph copies to the device at the start of each
 // replay, so changing them does not need a new capture.
 #include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
 
 #include <algorithm>
 #include <cstring>
@@ -77,6 +78,14 @@ struct rz_views {
   uint32_t* d_order = nullptr;
   ItemDesc* d_bitems = nullptr;
   ItemState* d_bstate = nullptr;
+  // learned order: scores of the batches by the model, sorted per view
+  rz_mlp* model = nullptr;
+  float* d_scores = nullptr;
+  float* d_sortedScores = nullptr;
+  uint32_t* d_iota = nullptr;
+  uint32_t* d_segments = nullptr;  // nViews + 1 batch offsets
+  void* d_sortTemp = nullptr;
+  size_t sortTempBytes = 0;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   hipEvent_t t0 = nullptr, t1 = nullptr;
@@ -106,6 +115,15 @@ static void free_plan(rz_views* v) {
   hipFree(v->d_order);
   hipFree(v->d_bitems);
   hipFree(v->d_bstate);
+  hipFree(v->d_scores);
+  hipFree(v->d_sortedScores);
+  hipFree(v->d_iota);
+  hipFree(v->d_segments);
+  hipFree(v->d_sortTemp);
+  v->d_scores = v->d_sortedScores = nullptr;
+  v->d_iota = v->d_segments = nullptr;
+  v->d_sortTemp = nullptr;
+  v->sortTempBytes = 0;
   v->d_labels = nullptr;
   v->d_features = nullptr;
   v->d_order = nullptr;
@@ -330,8 +348,25 @@ static int build_plan(rz_views* v) {
   RZ_HIP(hipMemset(v->d_features, 0, nbt * kFeatures * sizeof(float)));
   // An explicit batch order in any view switches the step to batch items;
   // views without one keep their batches in occluder order.
-  v->batchOrder = false;
+  v->batchOrder = v->model != nullptr;
   for (const ViewHost& h : v->views) v->batchOrder |= !h.order.empty();
+  if (v->model) {
+    std::vector<uint32_t> iota(v->nBatches), seg(nv + 1);
+    for (uint32_t b = 0; b < v->nBatches; ++b) iota[b] = b;
+    for (uint32_t i = 0; i < nv; ++i) seg[i] = v->h_views[i].batchBase;
+    seg[nv] = v->nBatches;
+    if (dev_alloc(&v->d_scores, nbt) != hipSuccess || dev_alloc(&v->d_sortedScores, nbt) != hipSuccess ||
+        dev_alloc(&v->d_iota, nbt) != hipSuccess || dev_alloc(&v->d_segments, nv + 1) != hipSuccess)
+      return RZ_ERR_OUT_OF_MEMORY;
+    if (v->nBatches) RZ_HIP(hipMemcpy(v->d_iota, iota.data(), iota.size() * 4, hipMemcpyHostToDevice));
+    RZ_HIP(hipMemcpy(v->d_segments, seg.data(), seg.size() * 4, hipMemcpyHostToDevice));
+    size_t bytes = 0;
+    RZ_HIP(hipcub::DeviceSegmentedRadixSort::SortPairsDescending(
+        nullptr, bytes, v->d_scores, v->d_sortedScores, v->d_iota, v->d_iota, int(v->nBatches), int(nv),
+        v->d_segments, v->d_segments + 1, 0, 32, v->stream));
+    if (hipMalloc(&v->d_sortTemp, std::max<size_t>(bytes, 16)) != hipSuccess) return RZ_ERR_OUT_OF_MEMORY;
+    v->sortTempBytes = bytes;
+  }
   if (v->batchOrder) {
     std::vector<uint32_t> order(v->nBatches);
     for (uint32_t i = 0; i < nv; ++i) {
@@ -400,7 +435,22 @@ static int enqueue_step(rz_views* v) {
     sp.views = v->d_views;
     RZ_HIP(launch_setup(sp, v->stream));
     const LearnParams lp = learn_params(v);
-    if (v->labels) RZ_HIP(launch_features(lp, v->nBatches, v->stream));
+    if (v->labels || v->model) RZ_HIP(launch_features(lp, v->nBatches, v->stream));
+    if (v->model && v->nBatches) {
+      // Score every batch, then order each view's batches by descending
+      // score; the radix sort is stable, so ties keep occluder order.
+      MlpParams mp{};
+      mp.w = v->model->d_w;
+      mp.norm = v->model->d_norm;
+      mp.x = v->d_features;
+      mp.n = v->nBatches;
+      mp.scores = v->d_scores;
+      RZ_HIP(launch_mlp_forward(mp, v->stream));
+      size_t bytes = v->sortTempBytes;
+      RZ_HIP(hipcub::DeviceSegmentedRadixSort::SortPairsDescending(
+          v->d_sortTemp, bytes, v->d_scores, v->d_sortedScores, v->d_iota, v->d_order, int(v->nBatches), int(nv),
+          v->d_segments, v->d_segments + 1, 0, 32, v->stream));
+    }
     if (v->batchOrder) RZ_HIP(launch_batch_items(lp, v->d_order, v->nBatches, v->d_bitems, v->d_bstate, v->stream));
     ViewsParams rp{};
     rp.items = v->batchOrder ? v->d_bitems : v->d_items;
@@ -414,7 +464,7 @@ static int enqueue_step(rz_views* v) {
     rp.cov = v->d_cov;
     rp.queue = v->d_queue;
     RZ_HIP(launch_views(rp, v->nGroups, v->stream));
-    if (v->labels) {
+    if (v->labels || v->model) {
       RZ_HIP(launch_fill_u32(v->d_labels, v->nBatches, 0u, v->stream));
       RZ_HIP(launch_labels(lp, v->maxBlocks, v->stream));
     }
@@ -518,6 +568,29 @@ extern "C" int rz_views_enable_labels(rz_views* v, int enable) {
   }
   v->labels = enable != 0;
   v->dirty = true;
+  return RZ_OK;
+}
+
+extern "C" int rz_views_set_model(rz_views* v, rz_mlp* model) {
+  if (!v) return RZ_ERR_INVALID_ARGUMENT;
+  int s = wait_last_step(v);
+  if (s != RZ_OK) return s;
+  if (model && !v->labels) {
+    s = rz_views_enable_labels(v, 1);  // features read the previous step's labels
+    if (s != RZ_OK) return s;
+  }
+  v->model = model;
+  v->dirty = true;
+  return RZ_OK;
+}
+
+extern "C" int rz_views_read_order(rz_views* v, uint32_t view, uint32_t* order) {
+  if (!v || !order || view >= v->views.size() || v->dirty || !v->h_views || !v->batchOrder)
+    return RZ_ERR_INVALID_ARGUMENT;
+  RZ_HIP(hipStreamSynchronize(v->stream));
+  const ViewDesc& d = v->h_views[view];
+  if (d.nBatches) RZ_HIP(hipMemcpy(order, v->d_order + d.batchBase, d.nBatches * 4, hipMemcpyDeviceToHost));
+  for (uint32_t k = 0; k < d.nBatches; ++k) order[k] -= d.batchBase;
   return RZ_OK;
 }
 

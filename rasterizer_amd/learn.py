@@ -1,0 +1,163 @@
+"""Training of the batch-scoring MLP that orders occluder batches.
+
+Data: frames of a scene through the batched multi-view path with labels on
+(rz_views_*).  Each frame is drawn twice, the second time after a small
+camera move, so that the second step's features carry a previous frame's
+labels as the reference scene's camera path would; the second step's labels
+(blocks whose final depth the batch keeps) are the targets.
+
+Training: pairs of batches of one frame with different labels; the network
+should score the batch with the larger label higher (mean softplus ranking
+loss); Adam steps run on the GPU through the library's own forward and
+backward kernels (rz_mlp_*).
+"""
+
+from __future__ import annotations
+
+import math
+import time
+from pathlib import Path
+
+import numpy as np
+
+from . import Mlp, Rasterizer, Views, scenes
+
+WEIGHTS = Path(__file__).resolve().parent / "order_mlp.npz"
+
+
+def random_camera(rng: np.random.Generator, half: float = 50.0):
+    eye = rng.uniform(-half, half, 3).astype(np.float32)
+    d = rng.normal(size=3)
+    d /= np.linalg.norm(d)
+    up = (0.0, 1.0, 0.0) if abs(d[1]) < 0.95 else (1.0, 0.0, 0.0)
+    return eye, d.astype(np.float32), up
+
+
+def generate(scene, frames: int, seed: int, width: int = 1280, height: int = 720, views_per_launch: int = 64,
+             log=print):
+    """Features (list per frame of (nBatches, 16)) and labels (per frame)."""
+    rng = np.random.default_rng(seed)
+    owner = Rasterizer(64, 64)
+    V = Views(owner, [(width, height)] * views_per_launch)
+    V.enable_labels(True)
+    feats, labels = [], []
+    t0 = time.perf_counter()
+    done = 0
+    while done < frames:
+        k = min(views_per_launch, frames - done)
+        cams = [random_camera(rng) for _ in range(views_per_launch)]
+        for v in range(views_per_launch):
+            eye, d, up = cams[v]
+            occ = [scene.occluders[i] for i in scenes.front_to_back(scene.occluders, eye)]
+            V.set_occluders(v, occ)
+            V.set_mvp(v, scenes.view_projection(eye - 0.5 * d, d, up, math.pi / 3, width / height))
+        V.run(graph=True)  # previous frame: the eye half a unit back
+        for v in range(views_per_launch):
+            eye, d, up = cams[v]
+            V.set_mvp(v, scenes.view_projection(eye, d, up, math.pi / 3, width / height))
+        V.run(graph=True)
+        V.sync()
+        for v in range(k):
+            lab, f = V.labels(v)
+            feats.append(f.copy())
+            labels.append(lab.copy())
+        done += k
+    log(f"labelled {frames} frames in {time.perf_counter() - t0:.1f} s")
+    return feats, labels
+
+
+def frame_pairs(labels: np.ndarray, rng: np.random.Generator, count: int) -> np.ndarray:
+    """Pairs (i, j) of one frame with labels[i] > labels[j]."""
+    n = labels.size
+    i = rng.integers(0, n, count)
+    j = rng.integers(0, n, count)
+    keep = labels[i] != labels[j]
+    i, j = i[keep], j[keep]
+    hi = np.where(labels[i] > labels[j], i, j)
+    lo = np.where(labels[i] > labels[j], j, i)
+    return np.stack([hi, lo], 1)
+
+
+def normalisation(feats) -> np.ndarray:
+    x = np.concatenate(feats)
+    mean = x.mean(0)
+    std = x.std(0)
+    inv = np.where(std > 1e-6, 1.0 / np.maximum(std, 1e-6), 1.0)
+    return np.concatenate([mean, inv]).astype(np.float32)
+
+
+def train(feats, labels, steps: int = 400, frames_per_step: int = 32, pairs_per_frame: int = 256, lr: float = 2e-3,
+          seed: int = 1, log=print):
+    rng = np.random.default_rng(seed)
+    m = Mlp(seed=seed)
+    norm = normalisation(feats)
+    m.set_params(norm=norm)
+    losses = []
+    t0 = time.perf_counter()
+    nf = len(feats)
+    for step in range(steps):
+        pick = rng.choice(nf, min(frames_per_step, nf), replace=False)
+        xs, ps, base = [], [], 0
+        for f in pick:
+            xs.append(feats[f])
+            ps.append(frame_pairs(labels[f], rng, pairs_per_frame) + base)
+            base += feats[f].shape[0]
+        x = np.concatenate(xs)
+        pr = np.concatenate(ps).astype(np.uint32)
+        if pr.size == 0:
+            continue
+        losses.append(m.train_step(x, pr, lr * (0.5 if step > steps * 3 // 4 else 1.0)))
+    log(f"trained {steps} steps in {time.perf_counter() - t0:.1f} s: loss {losses[0]:.4f} -> "
+        f"{np.mean(losses[-20:]):.4f}")
+    return m, norm, losses
+
+
+def pair_agreement(m: Mlp, feats, labels, seed: int = 5, pairs_per_frame: int = 512) -> float:
+    """Fraction of held-out within-frame pairs with different labels that the
+    scores rank the same way."""
+    rng = np.random.default_rng(seed)
+    ok = tot = 0
+    for f, lab in zip(feats, labels):
+        pr = frame_pairs(lab, rng, pairs_per_frame)
+        if pr.size == 0:
+            continue
+        s = m.forward(f)
+        ok += int(np.sum(s[pr[:, 0]] > s[pr[:, 1]]))
+        tot += pr.shape[0]
+    return ok / max(tot, 1)
+
+
+def save(m: Mlp, path: Path = WEIGHTS, **meta) -> None:
+    w, norm = m.params()
+    np.savez(path, params=w, norm=norm, **{k: np.asarray(v) for k, v in meta.items()})
+
+
+def load(path: Path = WEIGHTS) -> Mlp:
+    d = np.load(path, allow_pickle=False)
+    m = Mlp(seed=0)
+    m.set_params(d["params"], d["norm"])
+    return m
+
+
+def main() -> int:
+    import argparse
+
+    ap = argparse.ArgumentParser(description="train the batch-ordering MLP on the 1,024-box scene")
+    ap.add_argument("--frames", type=int, default=10000)
+    ap.add_argument("--steps", type=int, default=400)
+    ap.add_argument("--out", default=str(WEIGHTS))
+    args = ap.parse_args()
+    scene = scenes.config1(seed=1)
+    n_test = max(args.frames // 10, 64)
+    feats, labels = generate(scene, args.frames + n_test, seed=10)
+    tr_f, tr_l = feats[:args.frames], labels[:args.frames]
+    te_f, te_l = feats[args.frames:], labels[args.frames:]
+    m, norm, losses = train(tr_f, tr_l, steps=args.steps)
+    agree = pair_agreement(m, te_f, te_l)
+    print(f"held-out pair agreement {agree:.3f} over {len(te_f)} frames")
+    save(m, Path(args.out), agreement=agree, frames=args.frames, steps=args.steps, final_loss=np.mean(losses[-20:]))
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())
