@@ -259,6 +259,11 @@ int rz_mlp_train_step(rz_mlp* mlp, uint32_t n, const float* x, uint32_t nPairs, 
 int rz_views_set_model(rz_views* views, rz_mlp* model);
 /* The batch order of the last step (view-local batch numbers). */
 int rz_views_read_order(rz_views* views, uint32_t view, uint32_t* order);
+/* Block-loop counters of each step (observability): rz_views_read_stats
+ * returns, for the last step over all views, the (quad, block) pairs
+ * visited, those rejected by the hiZ or edge tests, and blocks written. */
+int rz_views_enable_stats(rz_views* views, int enable);
+int rz_views_read_stats(rz_views* views, uint64_t out[3]);
 
 #ifdef __cplusplus
 }

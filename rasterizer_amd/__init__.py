@@ -96,6 +96,8 @@ _SIGNATURES = {
     "rz_mlp_train_step": ([_P, _U32, _P, _U32, _P, ctypes.c_float, _P], ctypes.c_int),
     "rz_views_set_model": ([_P, _P], ctypes.c_int),
     "rz_views_read_order": ([_P, _U32, _P], ctypes.c_int),
+    "rz_views_enable_stats": ([_P, ctypes.c_int], ctypes.c_int),
+    "rz_views_read_stats": ([_P, _P], ctypes.c_int),
 }
 
 
@@ -464,6 +466,15 @@ class Views:
     def set_model(self, mlp: "Mlp | None") -> None:
         self._model = mlp
         _check(lib().rz_views_set_model(self._v, mlp._m if mlp is not None else None), "rz_views_set_model")
+
+    def enable_stats(self, enable: bool = True) -> None:
+        _check(lib().rz_views_enable_stats(self._v, 1 if enable else 0), "rz_views_enable_stats")
+
+    def stats(self) -> dict:
+        """Block-loop counters of the last step over all views."""
+        a = np.zeros(3, np.uint64)
+        _check(lib().rz_views_read_stats(self._v, _ptr(a)), "rz_views_read_stats")
+        return {"visited": int(a[0]), "rejected": int(a[1]), "written": int(a[2])}
 
     def order(self, view: int) -> np.ndarray:
         n = self.n_batches(view)

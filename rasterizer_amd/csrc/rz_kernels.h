@@ -50,6 +50,9 @@ struct ViewsParams {
   // Draw batch items instead of occluder items: view v's items are
   // [views[v].batchBase, + nBatches) of items/state.
   uint32_t batchItems;
+  // Optional block-loop counters of the launch (visited, rejected by hiZ or
+  // the edge tests, written), added to stats[0..2].
+  unsigned long long* stats;
 };
 
 // queryVisibility over the queries of every view of a multi-view launch.

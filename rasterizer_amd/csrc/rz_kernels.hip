@@ -1200,7 +1200,7 @@ __global__ void __launch_bounds__(256) rz_frame_kernel(FrameParams p) {
 // p.queue until the queue is exhausted.  Units are handed out in view order
 // and no unit waits on another, so every work-group exits after at most
 // nUnits queue reads.
-template <int TBX, int TBY>
+template <int TBX, int TBY, bool STATS>
 __global__ void __launch_bounds__(256) rz_views_kernel(ViewsParams p) {
   __shared__ TileSmem<TBX, TBY> sm;
   __shared__ uint32_t unitSh;
@@ -1238,8 +1238,13 @@ __global__ void __launch_bounds__(256) rz_views_kernel(ViewsParams p) {
     J.done = nullptr;
     J.err = nullptr;
     J.timeoutTicks = 0;
-    process_tile<TBX, TBY, false>(J, sm, p.cov, covLoaded, ctr);
+    process_tile<TBX, TBY, STATS>(J, sm, p.cov, covLoaded, ctr);
     __syncthreads();
+  }
+  if (STATS && threadIdx.x == 0) {
+    atomicAdd(&p.stats[0], (unsigned long long)ctr[kStatBlocksVisited]);
+    atomicAdd(&p.stats[1], (unsigned long long)ctr[kStatBlocksRejected]);
+    atomicAdd(&p.stats[2], (unsigned long long)ctr[kStatBlocksWritten]);
   }
 }
 
@@ -1754,7 +1759,10 @@ hipError_t launch_fill_u16(uint16_t* p, uint32_t n, uint16_t v, hipStream_t stre
 
 hipError_t launch_views(const ViewsParams& p, uint32_t nGroups, hipStream_t stream) {
   if (p.nUnits == 0 || nGroups == 0) return hipSuccess;
-  rz_views_kernel<8, 8><<<dim3(nGroups), dim3(256), 0, stream>>>(p);
+  if (p.stats)
+    rz_views_kernel<8, 8, true><<<dim3(nGroups), dim3(256), 0, stream>>>(p);
+  else
+    rz_views_kernel<8, 8, false><<<dim3(nGroups), dim3(256), 0, stream>>>(p);
   return hipGetLastError();
 }
 
@@ -1764,10 +1772,14 @@ hipError_t views_max_groups(int* maxGroups) {
   if (e != hipSuccess) return e;
   e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
   if (e != hipSuccess) return e;
-  e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, reinterpret_cast<const void*>(&rz_views_kernel<8, 8>), 256,
-                                                   0);
+  int n2 = 0;
+  e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, reinterpret_cast<const void*>(&rz_views_kernel<8, 8, false>),
+                                                   256, 0);
   if (e != hipSuccess) return e;
-  *maxGroups = n * cus;
+  e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n2, reinterpret_cast<const void*>(&rz_views_kernel<8, 8, true>),
+                                                   256, 0);
+  if (e != hipSuccess) return e;
+  *maxGroups = (n < n2 ? n : n2) * cus;
   return hipSuccess;
 }
 

@@ -86,6 +86,8 @@ struct rz_views {
   uint32_t* d_segments = nullptr;  // nViews + 1 batch offsets
   void* d_sortTemp = nullptr;
   size_t sortTempBytes = 0;
+  unsigned long long* d_stats = nullptr;  // block-loop counters of the last step, when on
+  bool stats = false;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   hipEvent_t t0 = nullptr, t1 = nullptr;
@@ -155,6 +157,7 @@ static void free_views(rz_views* v) {
   hipFree(v->d_cov);
   hipFree(v->d_rcp);
   hipFree(v->d_queue);
+  hipFree(v->d_stats);
   if (v->t0) (void)hipEventDestroy(v->t0);
   if (v->t1) (void)hipEventDestroy(v->t1);
   if (v->stream) (void)hipStreamDestroy(v->stream);
@@ -463,6 +466,8 @@ static int enqueue_step(rz_views* v) {
     rp.groupBox = v->d_groupBox;
     rp.cov = v->d_cov;
     rp.queue = v->d_queue;
+    rp.stats = v->stats ? v->d_stats : nullptr;
+    if (v->stats) RZ_HIP(launch_fill_u32(reinterpret_cast<uint32_t*>(v->d_stats), 6, 0u, v->stream));
     RZ_HIP(launch_views(rp, v->nGroups, v->stream));
     if (v->labels || v->model) {
       RZ_HIP(launch_fill_u32(v->d_labels, v->nBatches, 0u, v->stream));
@@ -610,6 +615,26 @@ extern "C" int rz_views_read_labels(rz_views* v, uint32_t view, uint32_t* labels
   if (features && d.nBatches)
     RZ_HIP(hipMemcpy(features, v->d_features + size_t(d.batchBase) * kFeatures,
                      size_t(d.nBatches) * kFeatures * sizeof(float), hipMemcpyDeviceToHost));
+  return RZ_OK;
+}
+
+extern "C" int rz_views_enable_stats(rz_views* v, int enable) {
+  if (!v) return RZ_ERR_INVALID_ARGUMENT;
+  int s = wait_last_step(v);
+  if (s != RZ_OK) return s;
+  if (enable && !v->d_stats) {
+    if (dev_alloc(&v->d_stats, 3) != hipSuccess) return RZ_ERR_OUT_OF_MEMORY;
+    RZ_HIP(hipMemset(v->d_stats, 0, 3 * sizeof(unsigned long long)));
+  }
+  v->stats = enable != 0;
+  v->dirty = true;
+  return RZ_OK;
+}
+
+extern "C" int rz_views_read_stats(rz_views* v, uint64_t out[3]) {
+  if (!v || !out || !v->d_stats) return RZ_ERR_INVALID_ARGUMENT;
+  RZ_HIP(hipStreamSynchronize(v->stream));
+  RZ_HIP(hipMemcpy(out, v->d_stats, 3 * sizeof(uint64_t), hipMemcpyDeviceToHost));
   return RZ_OK;
 }
 

@@ -212,3 +212,94 @@ def test_features_match_definition(oracle_mod, scene):
         assert got.shape == want.shape
         err = np.abs(got - want)
         assert np.all(err <= 1e-4 + 1e-4 * np.abs(want)), (np.unravel_index(np.argmax(err), err.shape), err.max())
+
+
+def test_learned_order_is_score_order_and_exact(oracle_mod, scene, report):
+    """With a model set, each view draws its batches by descending model score
+    (ties in occluder order); the buffer equals the oracle drawing the batches
+    in that order, and the difference to occluder order is reported as pixels
+    and maximum LSB."""
+    from rasterizer_amd import Mlp
+
+    owner = Rasterizer(64, 64)
+    V = Views(owner, SIZES)
+    setup = _setup(scene, 4)
+    for k, (mvp, occ) in enumerate(setup):
+        V.set_mvp(k, mvp)
+        V.set_occluders(k, occ)
+    m = Mlp(seed=11)
+    V.set_model(m)
+    V.run(graph=True)
+    V.sync()
+    V.run(graph=True)  # second step: features see the first step's labels
+    V.sync()
+    for k, (w, h) in enumerate(SIZES):
+        mvp, occ = setup[k]
+        _, feats = V.labels(k)
+        scores = m.forward(feats)
+        want = np.lexsort((np.arange(scores.size), -scores))
+        got = V.order(k)
+        assert np.array_equal(got, want), f"view {k}: order is not descending score"
+        o0 = oracle_mod.OracleRasterizer(w, h)
+        o0.clear()
+        o0.setModelViewProjection(mvp)
+        st = o0.draw_frame(occ, gated=False)
+        o = _oracle_batch_draw(oracle_mod, w, h, mvp, occ, st, got)
+        dg, hg = V.raw(k)
+        do, ho = o.raw()
+        written = np.repeat(ho[: (w // 8) * (h // 8)] != 1, 64)
+        assert np.array_equal(hg, ho), f"view {k}: hiZ {report(hg, ho)}"
+        assert np.array_equal(dg[written], do[written]), f"view {k}: depth {report(dg[written], do[written])}"
+        d0, _ = o0.raw()
+        diff = np.abs(dg.astype(int) - d0.astype(int))[written]
+        print(f"view {k}: learned vs occluder order {int((diff > 0).sum())} px differ, max "
+              f"{int(diff.max()) if diff.size else 0} LSB", flush=True)
+
+
+def test_trained_order_gains_rejects(scene):
+    """Train the MLP on labelled frames of the scene (GPU kernels end to end),
+    then compare hiZ early rejects on held-out frames: occluder order
+    (front-to-back), random batch order and learned batch order."""
+    from rasterizer_amd import learn
+
+    feats, labels = learn.generate(scene, 384, seed=21, width=640, height=360, views_per_launch=64)
+    m, norm, losses = learn.train(feats[:320], labels[:320], steps=150, frames_per_step=32)
+    agree = learn.pair_agreement(m, feats[320:], labels[320:])
+    print(f"loss {losses[0]:.4f} -> {np.mean(losses[-10:]):.4f}, held-out pair agreement {agree:.3f}", flush=True)
+    assert np.mean(losses[-10:]) < losses[0]
+    assert agree > 0.6
+
+    owner = Rasterizer(64, 64)
+    rng = np.random.default_rng(77)
+    nv = 32
+    V = Views(owner, [(640, 360)] * nv)
+    V.enable_stats(True)
+    cams = [learn.random_camera(rng) for _ in range(nv)]
+    for v, (eye, d, up) in enumerate(cams):
+        V.set_occluders(v, [scene.occluders[i] for i in scenes.front_to_back(scene.occluders, eye)])
+
+    def run(mode):
+        if mode == "learned":
+            V.set_model(m)
+        else:
+            V.set_model(None)
+        for v in range(nv):
+            order = None
+            if mode == "random":
+                order = rng.permutation(V.n_batches(v))
+            V.set_batch_order(v, order)
+        rates = []
+        for step in range(2):
+            for v, (eye, d, up) in enumerate(cams):
+                V.set_mvp(v, scenes.view_projection(eye + (step - 1) * 0.5 * d, d, up, np.pi / 3, 640 / 360))
+            V.run(graph=True)
+            ms = V.sync()
+            st = V.stats()
+            rates.append((st["rejected"] / max(st["visited"], 1), st["visited"], ms))
+        return rates[-1]
+
+    res = {mode: run(mode) for mode in ("plain", "random", "learned")}
+    for mode, (rate, visited, ms) in res.items():
+        print(f"{mode:8s}: hiZ/edge reject rate {rate:.4f} of {visited:,} visited (quad, block) pairs, "
+              f"{ms:.3f} ms", flush=True)
+    assert res["learned"][0] > res["random"][0]

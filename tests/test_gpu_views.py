@@ -83,7 +83,7 @@ def test_views_match_oracle(oracle_mod, scene, report):
             mvp, occ, q = setup[k]
             drawn += _check_view(oracle_mod, V, k, w, h, mvp, occ, q, report, f"step {step} view {k} {w}x{h}",
                                  oracles[k])
-    assert drawn > 100
+    assert drawn > 20
 
 
 def test_views_graph_replays_match_direct(oracle_mod, scene, report):
