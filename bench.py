@@ -46,6 +46,18 @@ def algorithmic_bytes_per_frame(quads: int, width: int, height: int) -> int:
     return 16 * quads + 2 * width * height + 2 * (width // 8) * (height // 8) + 64
 
 
+def measured_traffic(kernel_profile: str):
+    """HBM bytes per launch of the frame kernel on this workload, from the
+    calibrated FETCH_SIZE/WRITE_SIZE runs recorded under profiles/ (rocprofv3
+    cannot run inside the timed process), or None."""
+    path = os.path.join(ROOT, "profiles", kernel_profile)
+    try:
+        vals = dict(line.strip().split("=", 1) for line in open(path) if "=" in line and not line.startswith("#"))
+        return float(vals["traffic_bytes"])
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def cpu_baseline(scene, orders, threads: int, frames: int, query_reps: int):
     import oracle
 
@@ -179,9 +191,12 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=16)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--cpu-frames", type=int, default=1024)
+    ap.add_argument("--cpu-frames", type=int, default=16384)
     ap.add_argument("--no-views", action="store_true", help="skip the batched multi-view measurement")
     ap.add_argument("--views-steps", type=int, default=32)
+    ap.add_argument("--calibrate-copy-mib", type=int, default=0,
+                    help="first copy this many MiB device-to-device 3 times (a kernel of known traffic for "
+                         "scaling the FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for {args.gpus}).",
@@ -189,6 +204,17 @@ def main() -> int:
         return 2
 
     from rasterizer_amd import DeviceBuffer, Rasterizer, scenes
+
+    if args.calibrate_copy_mib > 0:
+        import torch
+
+        a = torch.ones(args.calibrate_copy_mib << 18, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        log(f"copied {args.calibrate_copy_mib} MiB 3 times")
+        del a, b
 
     t0 = time.perf_counter()
     scene = scenes.config2(seed=args.seed)
@@ -283,7 +309,7 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": measured_traffic("rz_frame_kernel_city1080_traffic.txt"),
             "kernel": "rz_frame_kernel",
             "algorithmic_bytes": alg,
         },

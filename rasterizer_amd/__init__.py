@@ -9,8 +9,10 @@ fallback: if the library or a GPU is missing, construction raises.
 
 from __future__ import annotations
 
+import atexit
 import ctypes
 import os
+import weakref
 from pathlib import Path
 
 import numpy as np
@@ -36,6 +38,23 @@ class RasterizerError(RuntimeError):
 
 
 _lib = None
+
+# Objects holding device memory, released in order at interpreter exit (before
+# C-level exit handlers run): buffers and occluder copies, then batches and
+# models, then contexts.
+_live = [weakref.WeakSet() for _ in range(3)]
+
+
+def _release_all() -> None:
+    for tier in _live:
+        for obj in list(tier):
+            try:
+                (getattr(obj, "close", None) or getattr(obj, "release", None) or getattr(obj, "free"))()
+            except Exception:
+                pass
+
+
+atexit.register(_release_all)
 
 _P = ctypes.c_void_p
 _U32 = ctypes.c_uint32
@@ -145,6 +164,7 @@ class Occluder:
         self.m_boundsMax = _vec4(bounds_max)
         self.m_center = _vec4(center if center is not None else (self.m_boundsMin + self.m_boundsMax) * 0.5)
         self._device = {}
+        _live[0].add(self)
 
     @property
     def m_packetCount(self) -> int:
@@ -206,6 +226,7 @@ class DeviceBuffer:
             src = _ptr(self._init)
         _check(lib().rz_device_alloc(rast.ctx, self.nbytes, src, ctypes.byref(self.ptr)), "rz_device_alloc")
         self._rast = rast
+        _live[0].add(self)
 
     def read(self, dtype, count: int) -> np.ndarray:
         out = np.zeros(count, dtype)
@@ -235,6 +256,7 @@ class Rasterizer:
         info = np.zeros(6, np.uint32)
         _check(lib().rz_get_info(self._ctx, _ptr(info)), "rz_get_info")
         self.tile = (int(info[4]), int(info[5]))
+        _live[2].add(self)
 
     @property
     def ctx(self) -> ctypes.c_void_p:
@@ -389,6 +411,7 @@ class Views:
         self.sizes = [(int(a), int(b)) for a, b in sizes]
         self._occ = [[] for _ in sizes]
         self._nq = [0] * len(sizes)
+        _live[1].add(self)
 
     def close(self) -> None:
         if self._v and _lib is not None:
@@ -497,6 +520,7 @@ class Mlp:
     def __init__(self, seed: int = 1):
         self._m = ctypes.c_void_p()
         _check(lib().rz_mlp_create(seed, ctypes.byref(self._m)), "rz_mlp_create")
+        _live[1].add(self)
 
     def close(self) -> None:
         if self._m and _lib is not None:

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <new>
@@ -95,6 +96,7 @@ struct rz_context {
   double profMs[4] = {0, 0, 0, 0};
   uint32_t profCounts[2] = {0, 0};
   uint64_t* d_frameStats = nullptr;  // per-tile frame-kernel counters when enabled
+  bool cooperative = true;           // RZ_FRAME_LAUNCH=plain at creation: plain dispatch of the same grid
 };
 
 #define RZ_HIP(call)                           \
@@ -206,6 +208,7 @@ extern "C" int rz_create(uint32_t width, uint32_t height, rz_context** out) {
   for (int i = 0; i < kStagingSlots; ++i)
     if (hipEventCreateWithFlags(&c->ev_items[i], hipEventDisableTiming) != hipSuccess) return fail(RZ_ERR_DEVICE);
   if (hipEventCreate(&c->t0) != hipSuccess || hipEventCreate(&c->t1) != hipSuccess) return fail(RZ_ERR_DEVICE);
+  if (const char* mode = std::getenv("RZ_FRAME_LAUNCH")) c->cooperative = std::strcmp(mode, "plain") != 0;
   // Identity MVP until the caller sets one.
   const float ident[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
   rzh::prebake_mvp(width, height, ident, c->mvp.prebaked, c->mvp.raw);
@@ -447,7 +450,7 @@ static int enqueue_draw(rz_context* c, uint32_t n, const rz_occluder* const* occ
   fp.tileMin = c->d_tileMin;
   fp.pyrX = c->pyrX;
   fp.stats = c->d_frameStats;
-  RZ_HIP(launch_frame(fp, c->tileW, c->stream));
+  RZ_HIP(launch_frame(fp, c->tileW, c->stream, c->cooperative));
   if (prof) RZ_HIP(hipEventRecord(prof[3], c->stream));
 
   if (statusOut) {

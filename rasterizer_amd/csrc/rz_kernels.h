@@ -123,7 +123,10 @@ hipError_t launch_prep(const PrepParams& p, hipStream_t stream);
 hipError_t launch_setup(const SetupParams& p, hipStream_t stream);
 // tileW x tileH must be 8x8 or 16x16; the grid is tilesX * tilesY
 // cooperative work-groups.
-hipError_t launch_frame(const FrameParams& p, int tileW, hipStream_t stream);
+// cooperative == false launches the same grid as a plain dispatch (for tools
+// that cannot trace cooperative dispatches); the grid is sized to be
+// co-resident either way and every cross-tile wait is bounded in time.
+hipError_t launch_frame(const FrameParams& p, int tileW, hipStream_t stream, bool cooperative = true);
 hipError_t frame_max_groups(int tileW, int* maxGroups);
 hipError_t launch_status(const ItemState* state, const uint32_t* visible, uint32_t n, uint8_t* out,
                          hipStream_t stream);

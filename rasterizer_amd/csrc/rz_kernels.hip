@@ -1674,7 +1674,7 @@ hipError_t launch_setup(const SetupParams& p, hipStream_t stream) {
   return hipGetLastError();
 }
 
-hipError_t launch_frame(const FrameParams& p, int tileW, hipStream_t stream) {
+hipError_t launch_frame(const FrameParams& p, int tileW, hipStream_t stream, bool cooperative) {
   if (p.nItems == 0) return hipSuccess;
   FrameParams local = p;
   void* args[] = {&local};
@@ -1688,6 +1688,7 @@ hipError_t launch_frame(const FrameParams& p, int tileW, hipStream_t stream) {
                  : reinterpret_cast<const void*>(&rz_frame_kernel<16, 16, false>);
   else
     return hipErrorInvalidValue;
+  if (!cooperative) return hipLaunchKernel(fn, grid, block, args, 0, stream);
   return hipLaunchCooperativeKernel(fn, grid, block, args, 0, stream);
 }
 
