@@ -618,6 +618,11 @@ struct TileSmem {
   uint16_t hiz[TBX * TBY];
   ItemLocal items[kItemWindow];    // this window's items that touch the tile
   QuadRec recs[kQuadChunk];        // setup records of the quads being drawn
+  // Block-major draw (8x8-block tiles): per staged quad its rectangle in the
+  // tile and origin, and per tile row the folded edge offsets and depth
+  // samples at the rectangle's first column in the tile.
+  uint32_t qinfo[kQuadChunk][4];
+  float entry[(TBX == 8 && TBY == 8) ? kQuadChunk * TBY * 12 : 1];
   uint32_t wcnt[4];
   uint32_t part[2][4];
   uint32_t blockStats[3];  // observability builds: visited, rejected, written
@@ -804,6 +809,202 @@ __device__ __forceinline__ void raster_quad_wave(const QuadRec& R, TileSmem<TBX,
       }
     }
     line = fold_n(line, stepY, 5);
+  }
+}
+
+// One block of one quad with the block's hiZ and depth in registers (the
+// block-major draw).  Same arithmetic as raster_block with g = 0: lanes 0-3
+// of `s` hold the folded edge offsets, lanes 4-11 the depth samples.
+template <bool STATS>
+__device__ __forceinline__ void raster_block_reg(const uint64_t* __restrict__ cov, float s, uint32_t lane, int kLane,
+                                                 const QuadConst& Q, uint32_t& hz, uint32_t& dval, bool& dirty,
+                                                 BlockCounters& bc) {
+  if (STATS) bc.visited++;
+  if (hz >= Q.maxZ) {
+    if (STATS) bc.rejected++;
+    return;
+  }
+  float o[4];
+  uint32_t idx[4];
+  bool outside = false;
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    o[e] = readlane_f(s, e);
+    const int32_t t = cvtt_x86(o[e]);
+    if (Q.mode == kModeConvex) {
+      outside |= o[e] >= 63.0f;
+      idx[e] = (Q.sl[e] | uint32_t(max(t, 0))) & 4095u;
+    } else {
+      idx[e] = (Q.sl[e] | uint32_t(min(max(t, 0), 63))) & 4095u;
+    }
+  }
+  if (outside) {
+    if (STATS) bc.rejected++;
+    return;
+  }
+  const uint64_t A = cov[idx[0]], B = cov[idx[1]], C = cov[idx[2]], D = cov[idx[3]];
+  uint64_t mask;
+  if (Q.mode == kModeConvex) mask = (A & B) & (C & D);
+  else if (Q.mode == kModeTriangle0) mask = A & B & C;
+  else if (Q.mode == kModeTriangle1) mask = A & C & D;
+  else if (Q.mode == kModeConcaveRight) mask = (A | D) & (B & C);
+  else if (Q.mode == kModeConcaveCenter) mask = (A & B) | (C & D);
+  else mask = (A & D) & (B | C);
+  if (Q.mode != kModeConvex && mask == 0) {
+    if (STATS) bc.rejected++;
+    return;
+  }
+  if (STATS) bc.written++;
+  float v = __shfl(s, 4 + kLane);
+  const uint32_t px = lane & 7u, py = lane >> 3;
+  if (px >= 4u) v = fmaf(Q.dDx, 0.5f, v);
+  const uint32_t lo = pack_depth(v);
+  const uint32_t hi = pack_depth(Q.dDy + v);
+  const uint32_t mid = avg_u16(lo, hi);
+  const uint32_t q1 = avg_u16(lo, mid), q3 = avg_u16(mid, hi);
+  const uint32_t j = py >> 1;
+  const uint32_t a01 = (j & 1u) ? q1 : lo, a23 = (j & 1u) ? q3 : mid;
+  uint32_t d = (j & 2u) ? a23 : a01;
+  const uint32_t maskBit = 8u * px + ((py & 1u) ? (py >> 1) : 4u + (py >> 1));
+  d = ((mask >> maskBit) & 1ull) ? d : 0u;
+  if (hz != 1u) d = max(d, dval);
+  dval = d;
+  hz = __ockl_wfred_min_u32(d);
+  dirty = true;
+}
+
+__device__ __forceinline__ QuadConst quad_const(const QuadRec& R) {
+  QuadConst Q;
+  Q.mode = R.modeZ & 0xFFu;
+  Q.maxZ = R.modeZ >> 16;
+  Q.sl[0] = R.slope01 & 0xFFFFu;
+  Q.sl[1] = R.slope01 >> 16;
+  Q.sl[2] = R.slope23 & 0xFFFFu;
+  Q.sl[3] = R.slope23 >> 16;
+  Q.dDx = R.dDx;
+  Q.dDy = R.dDy;
+  return Q;
+}
+
+// Component e of a quad's fold at its rectangle origin and its row and column
+// steps: e < 4 edge offset e, else depth sample e - 4 (rows 0/1 of a block).
+__device__ __forceinline__ void quad_component(const QuadRec& R, uint32_t e, float& base, float& stepY,
+                                               float& stepX) {
+  if (e < 4) {
+    base = e == 0 ? R.off[0] : e == 1 ? R.off[1] : e == 2 ? R.off[2] : R.off[3];
+    stepY = e == 0 ? R.ny[0] : e == 1 ? R.ny[1] : e == 2 ? R.ny[2] : R.ny[3];
+    stepX = e == 0 ? R.nx[0] : e == 1 ? R.nx[1] : e == 2 ? R.nx[2] : R.nx[3];
+  } else {
+    const uint32_t k = e - 4;
+    const float spx = -0.4375f + 0.125f * float(k & 3u);
+    const float spy = -0.4375f + 0.125f * float(k >> 2);
+    base = fmaf(R.dDx, spx, fmaf(R.dDy, spy, R.dp0));
+    stepY = R.dDy;
+    stepX = R.dDx;
+  }
+}
+
+// Block-major draw of a staged chunk of n quads in an 8x8-block tile.
+// Phase A: each quad's rectangle in the tile, and for each of its rows in the
+// tile the twelve folded components at the rectangle's first column (row
+// steps from the origin, then column steps, each add rounded, as the block
+// loop folds them); quads are split over the waves.  Phase B: each wave walks
+// its blocks; for a block, lane q tests staged quad q against the block
+// (inside its rectangle, and not below the block's hiZ unless the block is
+// fast-cleared, since hiZ only grows once written), and the survivors are
+// drawn in submission order with the block's hiZ and depth in registers,
+// each finishing its fold with the column steps left.
+template <int TBX, int TBY, bool STATS>
+__device__ void draw_chunk_block_major(TileSmem<TBX, TBY>& sm, uint32_t n, int tbx0, int tbx1, int tby0, int tby1,
+                                       int bX, float invBX, uint32_t wave, uint32_t lane, BlockCounters& bc) {
+  const uint32_t g = lane / 12u, e = lane - 12u * g;
+  for (uint32_t q = wave; q < n; q += 4) {
+    const QuadRec& R = sm.recs[q];
+    const int fb = int(R.fb);
+    const int rangeX = int(R.range & 0xFFFFu), rangeY = int(R.range >> 16);
+    const int minY = div_bx(fb, bX, invBX), minX = fb - minY * bX;
+    const bool wrap = minX + rangeX > bX;
+    int x0 = 0, x1 = TBX - 1, y0 = 0, y1 = TBY - 1;
+    if (!wrap) {
+      x0 = max(tbx0, minX) - tbx0;
+      x1 = min(tbx1, minX + rangeX - 1) - tbx0;
+      y0 = max(tby0, minY) - tby0;
+      y1 = min(tby1, minY + rangeY - 1) - tby0;
+    }
+    const bool empty = x0 > x1 || y0 > y1;
+    if (lane == 0) {
+      sm.qinfo[q][0] = empty ? 0xFFu : uint32_t(x0) | uint32_t(x1) << 8 | uint32_t(y0) << 16 | uint32_t(y1) << 24;
+      sm.qinfo[q][1] = uint32_t(minX);
+      sm.qinfo[q][2] = uint32_t(minY);
+      sm.qinfo[q][3] = wrap ? 1u : 0u;
+    }
+    if (wrap || empty) continue;
+    float base, stepY, stepX;
+    quad_component(R, e, base, stepY, stepX);
+    const int cLo = x0 + tbx0 - minX;
+    float line = fold_n(base, stepY, y0 + tby0 - minY);
+#pragma unroll
+    for (uint32_t i = 0; i < 4; ++i)
+      if (i < g) line = line + stepY;
+    for (int r0 = y0; r0 <= y1; r0 += 5) {
+      const float sv = fold_n(line, stepX, cLo);
+      const int row = r0 + int(g);
+      if (g < 5u && row <= y1) sm.entry[(q * TBY + uint32_t(row)) * 12u + e] = sv;
+      line = fold_n(line, stepY, 5);
+    }
+  }
+  __syncthreads();
+
+  const int kLane = int(((lane & 7u) & 3u) + 4u * ((lane >> 3) & 1u));
+  const bool valid = lane < n;
+  const uint32_t rect = valid ? sm.qinfo[lane][0] : 0xFFu;
+  const uint32_t maxZq = valid ? (sm.recs[lane].modeZ >> 16) : 0u;
+  const int qx0 = int(rect & 0xFFu), qx1 = int((rect >> 8) & 0xFFu);
+  const int qy0 = int((rect >> 16) & 0xFFu), qy1 = int(rect >> 24);
+  const uint32_t ce = lane < 12u ? lane : 0u;
+  for (int lb = 0; lb < TBX * TBY; ++lb) {
+    const int lbx = lb % TBX, lby = lb / TBX;
+    if (block_owner(lbx, lby) != wave) continue;
+    uint32_t hz = sm.hiz[lb];
+    const bool inRect = valid && lbx >= qx0 && lbx <= qx1 && lby >= qy0 && lby <= qy1;
+    const bool cand = inRect && !(hz != 1u && hz >= maxZq);
+    uint64_t m = __ballot(cand);
+    if (STATS) {
+      const uint32_t pre = uint32_t(__popcll(__ballot(inRect) & ~m));
+      bc.visited += pre;
+      bc.rejected += pre;
+    }
+    if (m == 0ull) continue;
+    uint32_t dval = sm.depth[lb * 64 + int(lane)];
+    bool dirty = false;
+    while (m) {
+      const uint32_t q = uint32_t(__builtin_ctzll(m));
+      m &= m - 1ull;
+      const QuadRec& R = sm.recs[q];
+      const QuadConst Q = quad_const(R);
+      float sv;
+      if (sm.qinfo[q][3] & 1u) {
+        // Wrapped first-block index: this block's (row, column) in the quad's
+        // block numbering, folded from the origin.
+        const int d = (tby0 + lby) * bX + tbx0 + lbx - int(R.fb);
+        const int rangeX = int(R.range & 0xFFFFu), rangeY = int(R.range >> 16);
+        if (d < 0) continue;
+        const int r = div_bx(d, bX, invBX), c = d - r * bX;
+        if (r >= rangeY || c >= rangeX) continue;
+        float base, stepY, stepX;
+        quad_component(R, ce, base, stepY, stepX);
+        sv = fold_n(fold_n(base, stepY, r), stepX, c);
+      } else {
+        const int x0 = int(sm.qinfo[q][0] & 0xFFu);
+        const float stepX = ce < 4u ? (ce == 0 ? R.nx[0] : ce == 1 ? R.nx[1] : ce == 2 ? R.nx[2] : R.nx[3]) : R.dDx;
+        sv = fold_n(sm.entry[(q * TBY + uint32_t(lby)) * 12u + ce], stepX, lbx - x0);
+      }
+      raster_block_reg<STATS>(sm.cov, sv, lane, kLane, Q, hz, dval, dirty, bc);
+    }
+    if (dirty) {
+      sm.depth[lb * 64 + int(lane)] = uint16_t(dval);
+      if (lane == 0) sm.hiz[lb] = uint16_t(hz);
+    }
   }
 }
 
@@ -1114,8 +1315,12 @@ __device__ __forceinline__ void process_tile(const TileJob& J, TileSmem<TBX, TBY
           if (k0 > 0 && hit && pos >= k0 && pos < k0 + n) sm.recs[pos - k0] = J.recs[L.quadBase + base + tid];
           __syncthreads();
           const uint64_t tDraw = stamp();
-          for (uint32_t q = 0; q < n; ++q)
-            raster_quad_wave<TBX, TBY, STATS>(sm.recs[q], sm, tbx0, tbx1, tby0, tby1, bX, invBX, wave, lane, bc);
+          if constexpr (TBX == 8 && TBY == 8) {
+            draw_chunk_block_major<TBX, TBY, STATS>(sm, n, tbx0, tbx1, tby0, tby1, bX, invBX, wave, lane, bc);
+          } else {
+            for (uint32_t q = 0; q < n; ++q)
+              raster_quad_wave<TBX, TBY, STATS>(sm.recs[q], sm, tbx0, tbx1, tby0, tby1, bX, invBX, wave, lane, bc);
+          }
           if (STATS) ctr[kStatQuads] += n;
           __syncthreads();
           if (STATS) ctr[kStatDrawTicks] += stamp() - tDraw;
