@@ -97,6 +97,7 @@ struct rz_context {
   uint32_t profCounts[2] = {0, 0};
   uint64_t* d_frameStats = nullptr;  // per-tile frame-kernel counters when enabled
   bool cooperative = true;           // RZ_FRAME_LAUNCH=plain at creation: plain dispatch of the same grid
+  int waves = 8;                     // wave64s per frame-kernel tile
 };
 
 #define RZ_HIP(call)                           \
@@ -194,7 +195,7 @@ extern "C" int rz_create(uint32_t width, uint32_t height, rz_context** out) {
   bool placed = false;
   for (int tw : {8, 16}) {
     int maxGroups = 0;
-    if (frame_max_groups(tw, &maxGroups) != hipSuccess) return fail(RZ_ERR_DEVICE);
+    if (frame_max_groups(tw, c->waves, &maxGroups) != hipSuccess) return fail(RZ_ERR_DEVICE);
     const int tx = int((c->bX + tw - 1) / tw), ty = int((c->bY + tw - 1) / tw);
     if (tx * ty <= maxGroups) {
       c->tileW = tw;
@@ -450,7 +451,7 @@ static int enqueue_draw(rz_context* c, uint32_t n, const rz_occluder* const* occ
   fp.tileMin = c->d_tileMin;
   fp.pyrX = c->pyrX;
   fp.stats = c->d_frameStats;
-  RZ_HIP(launch_frame(fp, c->tileW, c->stream, c->cooperative));
+  RZ_HIP(launch_frame(fp, c->tileW, c->stream, c->cooperative, c->waves));
   if (prof) RZ_HIP(hipEventRecord(prof[3], c->stream));
 
   if (statusOut) {

@@ -50,6 +50,7 @@ struct ViewsParams {
   // Draw batch items instead of occluder items: view v's items are
   // [views[v].batchBase, + nBatches) of items/state.
   uint32_t batchItems;
+  uint32_t waves;  // wave64s per work-group: 4 or 8
   // Optional block-loop counters of the launch (visited, rejected by hiZ or
   // the edge tests, written), added to stats[0..2].
   unsigned long long* stats;
@@ -126,8 +127,9 @@ hipError_t launch_setup(const SetupParams& p, hipStream_t stream);
 // cooperative == false launches the same grid as a plain dispatch (for tools
 // that cannot trace cooperative dispatches); the grid is sized to be
 // co-resident either way and every cross-tile wait is bounded in time.
-hipError_t launch_frame(const FrameParams& p, int tileW, hipStream_t stream, bool cooperative = true);
-hipError_t frame_max_groups(int tileW, int* maxGroups);
+// waves: 4 or 8 wave64s per tile (block ownership interleaves over them).
+hipError_t launch_frame(const FrameParams& p, int tileW, hipStream_t stream, bool cooperative = true, int waves = 8);
+hipError_t frame_max_groups(int tileW, int waves, int* maxGroups);
 hipError_t launch_status(const ItemState* state, const uint32_t* visible, uint32_t n, uint8_t* out,
                          hipStream_t stream);
 hipError_t launch_query(const QueryParams& p, hipStream_t stream);
@@ -142,7 +144,7 @@ hipError_t launch_fill_u16(uint16_t* p, uint32_t n, uint16_t v, hipStream_t stre
 // Multi-view: grid of `groups` persistent work-groups over p.nUnits units of
 // 8x8-block tiles.
 hipError_t launch_views(const ViewsParams& p, uint32_t groups, hipStream_t stream);
-hipError_t views_max_groups(int* maxGroups);
+hipError_t views_max_groups(int waves, int* maxGroups);
 hipError_t launch_views_query(const ViewsQueryParams& p, hipStream_t stream);
 // hiZ := 1 (whole 8-entry groups) and tile min := 1 for every view.
 hipError_t launch_views_clear(const ViewDesc* views, uint32_t nViews, uint32_t maxEntries, hipStream_t stream);

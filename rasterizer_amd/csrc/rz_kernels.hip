@@ -609,7 +609,7 @@ struct ItemLocal {
 
 constexpr uint32_t kItemWindow = 128;
 constexpr uint32_t kQuadChunk = 64;   // records staged in LDS at a time
-constexpr uint32_t kQuadScan = 256;   // quads tested for overlap per pass (one per thread)
+// A scan pass tests one quad per thread: 64 * NW quads.
 
 template <int TBX, int TBY>
 struct TileSmem {
@@ -623,8 +623,8 @@ struct TileSmem {
   // samples at the rectangle's first column in the tile.
   uint32_t qinfo[kQuadChunk][4];
   float entry[(TBX == 8 && TBY == 8) ? kQuadChunk * TBY * 12 : 1];
-  uint32_t wcnt[4];
-  uint32_t part[2][4];
+  uint32_t wcnt[8];
+  uint32_t part[2][8];
   uint32_t blockStats[3];  // observability builds: visited, rejected, written
   uint32_t vis[2];
 };
@@ -632,7 +632,8 @@ struct TileSmem {
 // Wave that owns tile-local block (lbx, lby) while a chunk of quads is drawn:
 // diagonals, so that any horizontal or vertical run of blocks is shared by all
 // four waves.
-__device__ __forceinline__ uint32_t block_owner(int lbx, int lby) { return uint32_t(lbx + lby) & 3u; }
+template <int NW>
+__device__ __forceinline__ uint32_t block_owner(int lbx, int lby) { return uint32_t(lbx + lby) & uint32_t(NW - 1); }
 
 // Adds `step` to `s` n times (uniform n), each add rounded: the reference's
 // per-row and per-block folds (Rasterizer.cpp:1130-1131, 1145-1146).  IEEE
@@ -734,7 +735,7 @@ __device__ __forceinline__ void raster_block(TileSmem<TBX, TBY>& sm, int lb, flo
 // once per block row, then the column step once per block column, each add
 // rounded.  Lane 12g + e carries component e (4 offsets, 8 depth samples) for
 // the g-th of up to five rows handled together.
-template <int TBX, int TBY, bool STATS>
+template <int TBX, int TBY, int NW, bool STATS>
 __device__ __forceinline__ void raster_quad_wave(const QuadRec& R, TileSmem<TBX, TBY>& sm, int tbx0, int tbx1,
                                                  int tby0, int tby1, int bX, float invBX, uint32_t wave,
                                                  uint32_t lane, BlockCounters& bc) {
@@ -773,7 +774,7 @@ __device__ __forceinline__ void raster_quad_wave(const QuadRec& R, TileSmem<TBX,
     // each owned block finds its (r, c) and folds from the origin.
     for (int by = tby0; by <= tby1; ++by)
       for (int bx = tbx0; bx <= tbx1; ++bx) {
-        if (block_owner(bx - tbx0, by - tby0) != wave) continue;
+        if (block_owner<NW>(bx - tbx0, by - tby0) != wave) continue;
         const int d = by * bX + bx - fb;
         if (d < 0) continue;
         const int r = div_bx(d, bX, invBX), c = d - r * bX;
@@ -790,7 +791,7 @@ __device__ __forceinline__ void raster_quad_wave(const QuadRec& R, TileSmem<TBX,
   if ((x1 - x0) + (y1 - y0) < 3) {
     bool mine = false;
     for (int by = y0; by <= y1; ++by)
-      for (int bx = x0; bx <= x1; ++bx) mine |= block_owner(bx - tbx0, by - tby0) == wave;
+      for (int bx = x0; bx <= x1; ++bx) mine |= block_owner<NW>(bx - tbx0, by - tby0) == wave;
     if (!mine) return;
   }
   const int cLo = x0 - minX;
@@ -804,7 +805,7 @@ __device__ __forceinline__ void raster_quad_wave(const QuadRec& R, TileSmem<TBX,
     for (int bx = x0; bx <= x1; ++bx, s = stepX + s) {
       for (int gg = 0; gg < 5 && py + gg <= y1; ++gg) {
         const int lbx = bx - tbx0, lby = py + gg - tby0;
-        if (block_owner(lbx, lby) != wave) continue;
+        if (block_owner<NW>(lbx, lby) != wave) continue;
         raster_block<TBX, TBY, STATS>(sm, lby * TBX + lbx, s, gg, lane, kLane, Q, bc);
       }
     }
@@ -914,11 +915,11 @@ __device__ __forceinline__ void quad_component(const QuadRec& R, uint32_t e, flo
 // fast-cleared, since hiZ only grows once written), and the survivors are
 // drawn in submission order with the block's hiZ and depth in registers,
 // each finishing its fold with the column steps left.
-template <int TBX, int TBY, bool STATS>
+template <int TBX, int TBY, int NW, bool STATS>
 __device__ void draw_chunk_block_major(TileSmem<TBX, TBY>& sm, uint32_t n, int tbx0, int tbx1, int tby0, int tby1,
                                        int bX, float invBX, uint32_t wave, uint32_t lane, BlockCounters& bc) {
   const uint32_t g = lane / 12u, e = lane - 12u * g;
-  for (uint32_t q = wave; q < n; q += 4) {
+  for (uint32_t q = wave; q < n; q += NW) {
     const QuadRec& R = sm.recs[q];
     const int fb = int(R.fb);
     const int rangeX = int(R.range & 0xFFFFu), rangeY = int(R.range >> 16);
@@ -964,7 +965,7 @@ __device__ void draw_chunk_block_major(TileSmem<TBX, TBY>& sm, uint32_t n, int t
   const uint32_t ce = lane < 12u ? lane : 0u;
   for (int lb = 0; lb < TBX * TBY; ++lb) {
     const int lbx = lb % TBX, lby = lb / TBX;
-    if (block_owner(lbx, lby) != wave) continue;
+    if (block_owner<NW>(lbx, lby) != wave) continue;
     uint32_t hz = sm.hiz[lb];
     const bool inRect = valid && lbx >= qx0 && lbx <= qx1 && lby >= qy0 && lby <= qy1;
     const bool cand = inRect && !(hz != 1u && hz >= maxZq);
@@ -1009,13 +1010,13 @@ __device__ void draw_chunk_block_major(TileSmem<TBX, TBY>& sm, uint32_t n, int t
 }
 
 // query2D restricted to this wave's rows of the tile (Rasterizer.cpp:283-349).
-template <int TBX, int TBY>
+template <int TBX, int TBY, int NW>
 __device__ __forceinline__ bool query_wave(const ItemLocal& S, TileSmem<TBX, TBY>& sm, int tbx0, int tbx1,
                                            int tby0, int tby1, uint32_t wave, uint32_t lane) {
   const int qbx0 = int(S.qMinX >> 3), qbx1 = int(S.qMaxX >> 3), qby0 = int(S.qMinY >> 3), qby1 = int(S.qMaxY >> 3);
   const uint32_t maxZ = S.qMaxZ;
   const int px = int(lane & 7u), py = int(lane >> 3);
-  for (int lr = int(wave); lr < TBY; lr += 4) {
+  for (int lr = int(wave); lr < TBY; lr += NW) {
     const int by = tby0 + lr;
     if (by > tby1) break;
     if (by < qby0 || by > qby1) continue;
@@ -1122,7 +1123,7 @@ struct TileJob {
   uint64_t timeoutTicks;
 };
 
-template <int TBX, int TBY, bool STATS>
+template <int TBX, int TBY, int NW, bool STATS>
 __device__ __forceinline__ void process_tile(const TileJob& J, TileSmem<TBX, TBY>& sm,
                                              const uint64_t* __restrict__ cov, bool& covLoaded,
                                              uint64_t (&ctr)[kFrameStatWords]) {
@@ -1147,16 +1148,17 @@ __device__ __forceinline__ void process_tile(const TileJob& J, TileSmem<TBX, TBY
   };
 
   int involved = 0;
-  for (uint32_t i = J.itemBegin + tid; i < J.itemEnd && !involved; i += 256) involved = relevance(i) != 0u;
+  constexpr uint32_t NT = 64u * NW;
+  for (uint32_t i = J.itemBegin + tid; i < J.itemEnd && !involved; i += NT) involved = relevance(i) != 0u;
   if (!__syncthreads_or(involved)) return;
 
   const uint64_t tLoad = stamp();
   if (!covLoaded) {
-    for (uint32_t i = tid; i < 4096; i += 256) sm.cov[i] = cov[i];
+    for (uint32_t i = tid; i < 4096; i += NT) sm.cov[i] = cov[i];
     covLoaded = true;
   }
   constexpr int NB = TBX * TBY;
-  for (int rrow = int(tid); rrow < NB * 8; rrow += 256) {
+  for (int rrow = int(tid); rrow < NB * 8; rrow += int(NT)) {
     const int lb = rrow >> 3, y = rrow & 7;
     const int gx = tbx0 + lb % TBX, gy = tby0 + lb / TBX;
     uint4 v = make_uint4(0, 0, 0, 0);
@@ -1164,7 +1166,7 @@ __device__ __forceinline__ void process_tile(const TileJob& J, TileSmem<TBX, TBY
       v = *reinterpret_cast<const uint4*>(J.depth + (size_t(gy) * bX + gx) * 64 + y * 8);
     *reinterpret_cast<uint4*>(&sm.depth[lb * 64 + y * 8]) = v;
   }
-  for (int lb = int(tid); lb < NB; lb += 256) {
+  for (int lb = int(tid); lb < NB; lb += int(NT)) {
     const int gx = tbx0 + lb % TBX, gy = tby0 + lb / TBX;
     sm.hiz[lb] = (gx <= tbx1 && gy <= tby1) ? J.hiz[size_t(gy) * bX + gx] : uint16_t(0xFFFF);
   }
@@ -1174,7 +1176,7 @@ __device__ __forceinline__ void process_tile(const TileJob& J, TileSmem<TBX, TBY
   if (STATS) {
     if (tid < 3) sm.blockStats[tid] = 0;
     __syncthreads();
-    ctr[kStatLoadTicks] += stamp() - tLoad;
+    if (tid == 0) ctr[kStatLoadTicks] += stamp() - tLoad;
   }
   const uint64_t tLoop = stamp();
   for (uint32_t wbase = J.itemBegin; wbase < J.itemEnd; wbase += kItemWindow) {
@@ -1187,7 +1189,7 @@ __device__ __forceinline__ void process_tile(const TileJob& J, TileSmem<TBX, TBY
     __syncthreads();
     uint32_t off = 0, nRel = 0;
 #pragma unroll
-    for (uint32_t w = 0; w < 4; ++w) {
+    for (uint32_t w = 0; w < NW; ++w) {
       off += w < wave ? sm.wcnt[w] : 0u;
       nRel += sm.wcnt[w];
     }
@@ -1221,15 +1223,17 @@ __device__ __forceinline__ void process_tile(const TileJob& J, TileSmem<TBX, TBY
     for (uint32_t k = 0; k < nRel; ++k) {
       const ItemLocal& L = sm.items[k];
       const uint32_t item = L.index, flags = L.flags, status = flags >> 2;
-      if (STATS) ctr[kStatItems]++;
+      if (STATS && tid == 0) ctr[kStatItems]++;
       if (flags & 1u) {
         // This tile's share of query2D for the item, at its exact turn.
         const uint64_t tq = stamp();
-        const bool found = query_wave<TBX, TBY>(L, sm, tbx0, tbx1, tby0, tby1, wave, lane);
+        const bool found = query_wave<TBX, TBY, NW>(L, sm, tbx0, tbx1, tby0, tby1, wave, lane);
         if (lane == 0) sm.part[qPar][wave] = found ? 1u : 0u;
         __syncthreads();
         if (tid == 0) {
-          const uint32_t part = sm.part[qPar][0] | sm.part[qPar][1] | sm.part[qPar][2] | sm.part[qPar][3];
+          uint32_t part = 0;
+#pragma unroll
+          for (int w = 0; w < NW; ++w) part |= sm.part[qPar][w];
           __hip_atomic_fetch_add(&J.done[item], 1u + (part ? kDoneVisibleOne : 0u), __ATOMIC_RELEASE,
                                  __HIP_MEMORY_SCOPE_AGENT);
         }
@@ -1272,14 +1276,14 @@ __device__ __forceinline__ void process_tile(const TileJob& J, TileSmem<TBX, TBY
       // in registers until it is staged into LDS and drawn, 64 at a time.
       const uint64_t tr = stamp();
       const uint32_t gBase = L.quadBase >> 6;
-      for (uint32_t base = 0; base < L.quadCount; base += kQuadScan) {
+      for (uint32_t base = 0; base < L.quadCount; base += NT) {
         // groups overlapping this pass, relative to gBase
         const uint32_t pg0 = ((L.quadBase + base) >> 6) - gBase;
-        const uint32_t pg1 = ((L.quadBase + min(base + kQuadScan, L.quadCount) - 1) >> 6) - gBase;
+        const uint32_t pg1 = ((L.quadBase + min(base + NT, L.quadCount) - 1) >> 6) - gBase;
         const uint64_t passMask = (pg1 - pg0 >= 63) ? ~0ull : (((2ull << (pg1 - pg0)) - 1) << pg0);
         if (pg0 < 64 && (L.groups & passMask) == 0) continue;
         const uint64_t tScan = stamp();
-        if (STATS) ctr[kStatPasses]++;
+        if (STATS && tid == 0) ctr[kStatPasses]++;
         const uint32_t waveFirst = base + wave * 64u;
         bool hit = false;
         QuadRec rec;
@@ -1301,12 +1305,12 @@ __device__ __forceinline__ void process_tile(const TileJob& J, TileSmem<TBX, TBY
         __syncthreads();
         uint32_t hoff = 0, total = 0;
 #pragma unroll
-        for (uint32_t w = 0; w < 4; ++w) {
+        for (uint32_t w = 0; w < NW; ++w) {
           hoff += w < wave ? sm.wcnt[w] : 0u;
           total += sm.wcnt[w];
         }
         const uint32_t pos = hoff + hpre;
-        if (STATS) ctr[kStatScanTicks] += stamp() - tScan;
+        if (STATS && tid == 0) ctr[kStatScanTicks] += stamp() - tScan;
         // The first 64 hits are staged from registers; later batches (rare)
         // reload their records.
         if (hit && pos < kQuadChunk) sm.recs[pos] = rec;
@@ -1316,23 +1320,23 @@ __device__ __forceinline__ void process_tile(const TileJob& J, TileSmem<TBX, TBY
           __syncthreads();
           const uint64_t tDraw = stamp();
           if constexpr (TBX == 8 && TBY == 8) {
-            draw_chunk_block_major<TBX, TBY, STATS>(sm, n, tbx0, tbx1, tby0, tby1, bX, invBX, wave, lane, bc);
+            draw_chunk_block_major<TBX, TBY, NW, STATS>(sm, n, tbx0, tbx1, tby0, tby1, bX, invBX, wave, lane, bc);
           } else {
             for (uint32_t q = 0; q < n; ++q)
-              raster_quad_wave<TBX, TBY, STATS>(sm.recs[q], sm, tbx0, tbx1, tby0, tby1, bX, invBX, wave, lane, bc);
+              raster_quad_wave<TBX, TBY, NW, STATS>(sm.recs[q], sm, tbx0, tbx1, tby0, tby1, bX, invBX, wave, lane, bc);
           }
-          if (STATS) ctr[kStatQuads] += n;
+          if (STATS && tid == 0) ctr[kStatQuads] += n;
           __syncthreads();
-          if (STATS) ctr[kStatDrawTicks] += stamp() - tDraw;
+          if (STATS && tid == 0) ctr[kStatDrawTicks] += stamp() - tDraw;
         }
       }
-      if (STATS) ctr[kStatRasterTicks] += stamp() - tr;
+      if (STATS && tid == 0) ctr[kStatRasterTicks] += stamp() - tr;
     }
     __syncthreads();
   }
 
   if (STATS) {
-    ctr[kStatLoopTicks] += stamp() - tLoop;
+    if (tid == 0) ctr[kStatLoopTicks] += stamp() - tLoop;
     if (lane == 0) {
       atomicAdd(&sm.blockStats[0], bc.visited);
       atomicAdd(&sm.blockStats[1], bc.rejected);
@@ -1346,14 +1350,14 @@ __device__ __forceinline__ void process_tile(const TileJob& J, TileSmem<TBX, TBY
     }
   }
   // Write the tile back.
-  for (int rrow = int(tid); rrow < NB * 8; rrow += 256) {
+  for (int rrow = int(tid); rrow < NB * 8; rrow += int(NT)) {
     const int lb = rrow >> 3, y = rrow & 7;
     const int gx = tbx0 + lb % TBX, gy = tby0 + lb / TBX;
     if (gx <= tbx1 && gy <= tby1)
       *reinterpret_cast<uint4*>(J.depth + (size_t(gy) * bX + gx) * 64 + y * 8) =
           *reinterpret_cast<const uint4*>(&sm.depth[lb * 64 + y * 8]);
   }
-  for (int lb = int(tid); lb < NB; lb += 256) {
+  for (int lb = int(tid); lb < NB; lb += int(NT)) {
     const int gx = tbx0 + lb % TBX, gy = tby0 + lb / TBX;
     if (gx <= tbx1 && gy <= tby1) J.hiz[size_t(gy) * bX + gx] = sm.hiz[lb];
   }
@@ -1369,11 +1373,12 @@ __device__ __forceinline__ void process_tile(const TileJob& J, TileSmem<TBX, TBY
   }
 }
 
-template <int TBX, int TBY, bool STATS>
-__global__ void __launch_bounds__(256) rz_frame_kernel(FrameParams p) {
+template <int TBX, int TBY, int NW, bool STATS>
+__global__ void __launch_bounds__(64 * NW) rz_frame_kernel(FrameParams p) {
   __shared__ TileSmem<TBX, TBY> sm;
-  // Observability counters, kept by thread 0 (compiled out unless STATS).
-  uint64_t ctr[kFrameStatWords] = {};
+  // Observability counters, kept in LDS by thread 0 (unused unless STATS).
+  __shared__ uint64_t ctr[kFrameStatWords];
+  if (STATS && threadIdx.x < kFrameStatWords) ctr[threadIdx.x] = 0;
   const uint64_t tEntry = STATS ? __builtin_amdgcn_s_memrealtime() : 0;
   TileJob J;
   J.items = p.items;
@@ -1394,7 +1399,7 @@ __global__ void __launch_bounds__(256) rz_frame_kernel(FrameParams p) {
   J.err = p.err;
   J.timeoutTicks = p.timeoutTicks;
   bool covLoaded = false;
-  process_tile<TBX, TBY, STATS>(J, sm, p.cov, covLoaded, ctr);
+  process_tile<TBX, TBY, NW, STATS>(J, sm, p.cov, covLoaded, ctr);
   if (STATS && threadIdx.x == 0) {
     ctr[kStatTotalTicks] = __builtin_amdgcn_s_memrealtime() - tEntry;
     for (int k = 0; k < kFrameStatWords; ++k) p.stats[size_t(blockIdx.x) * kFrameStatWords + k] = ctr[k];
@@ -1405,11 +1410,12 @@ __global__ void __launch_bounds__(256) rz_frame_kernel(FrameParams p) {
 // p.queue until the queue is exhausted.  Units are handed out in view order
 // and no unit waits on another, so every work-group exits after at most
 // nUnits queue reads.
-template <int TBX, int TBY, bool STATS>
-__global__ void __launch_bounds__(256) rz_views_kernel(ViewsParams p) {
+template <int TBX, int TBY, int NW, bool STATS>
+__global__ void __launch_bounds__(64 * NW) rz_views_kernel(ViewsParams p) {
   __shared__ TileSmem<TBX, TBY> sm;
   __shared__ uint32_t unitSh;
-  uint64_t ctr[kFrameStatWords] = {};
+  __shared__ uint64_t ctr[kFrameStatWords];
+  if (STATS && threadIdx.x < kFrameStatWords) ctr[threadIdx.x] = 0;
   bool covLoaded = false;
   while (true) {
     if (threadIdx.x == 0) unitSh = __hip_atomic_fetch_add(p.queue, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -1443,7 +1449,7 @@ __global__ void __launch_bounds__(256) rz_views_kernel(ViewsParams p) {
     J.done = nullptr;
     J.err = nullptr;
     J.timeoutTicks = 0;
-    process_tile<TBX, TBY, STATS>(J, sm, p.cov, covLoaded, ctr);
+    process_tile<TBX, TBY, NW, STATS>(J, sm, p.cov, covLoaded, ctr);
     __syncthreads();
   }
   if (STATS && threadIdx.x == 0) {
@@ -1879,44 +1885,41 @@ hipError_t launch_setup(const SetupParams& p, hipStream_t stream) {
   return hipGetLastError();
 }
 
-hipError_t launch_frame(const FrameParams& p, int tileW, hipStream_t stream, bool cooperative) {
+template <int T, int NW>
+static const void* frame_fn(bool stats) {
+  return stats ? reinterpret_cast<const void*>(&rz_frame_kernel<T, T, NW, true>)
+               : reinterpret_cast<const void*>(&rz_frame_kernel<T, T, NW, false>);
+}
+
+static const void* frame_kernel_fn(int tileW, int waves, bool stats) {
+  if (tileW == 8) return waves == 8 ? frame_fn<8, 8>(stats) : waves == 4 ? frame_fn<8, 4>(stats) : nullptr;
+  if (tileW == 16) return waves == 8 ? frame_fn<16, 8>(stats) : waves == 4 ? frame_fn<16, 4>(stats) : nullptr;
+  return nullptr;
+}
+
+hipError_t launch_frame(const FrameParams& p, int tileW, hipStream_t stream, bool cooperative, int waves) {
   if (p.nItems == 0) return hipSuccess;
   FrameParams local = p;
   void* args[] = {&local};
-  const dim3 grid(p.tilesX * p.tilesY), block(256);
-  const void* fn = nullptr;
-  if (tileW == 8)
-    fn = p.stats ? reinterpret_cast<const void*>(&rz_frame_kernel<8, 8, true>)
-                 : reinterpret_cast<const void*>(&rz_frame_kernel<8, 8, false>);
-  else if (tileW == 16)
-    fn = p.stats ? reinterpret_cast<const void*>(&rz_frame_kernel<16, 16, true>)
-                 : reinterpret_cast<const void*>(&rz_frame_kernel<16, 16, false>);
-  else
-    return hipErrorInvalidValue;
+  const dim3 grid(p.tilesX * p.tilesY), block(64 * waves);
+  const void* fn = frame_kernel_fn(tileW, waves, p.stats != nullptr);
+  if (!fn) return hipErrorInvalidValue;
   if (!cooperative) return hipLaunchKernel(fn, grid, block, args, 0, stream);
   return hipLaunchCooperativeKernel(fn, grid, block, args, 0, stream);
 }
 
-hipError_t frame_max_groups(int tileW, int* maxGroups) {
+hipError_t frame_max_groups(int tileW, int waves, int* maxGroups) {
   int dev = 0, cus = 0;
   hipError_t e = hipGetDevice(&dev);
   if (e != hipSuccess) return e;
   e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
   if (e != hipSuccess) return e;
-  const void* variants[2];
-  if (tileW == 8) {
-    variants[0] = reinterpret_cast<const void*>(&rz_frame_kernel<8, 8, false>);
-    variants[1] = reinterpret_cast<const void*>(&rz_frame_kernel<8, 8, true>);
-  } else if (tileW == 16) {
-    variants[0] = reinterpret_cast<const void*>(&rz_frame_kernel<16, 16, false>);
-    variants[1] = reinterpret_cast<const void*>(&rz_frame_kernel<16, 16, true>);
-  } else {
-    return hipErrorInvalidValue;
-  }
   int perCU = 1 << 30;
-  for (const void* fn : variants) {
+  for (bool stats : {false, true}) {
+    const void* fn = frame_kernel_fn(tileW, waves, stats);
+    if (!fn) return hipErrorInvalidValue;
     int n = 0;
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fn, 256, 0);
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fn, 64 * waves, 0);
     if (e != hipSuccess) return e;
     perCU = n < perCU ? n : perCU;
   }
@@ -1963,29 +1966,35 @@ hipError_t launch_fill_u16(uint16_t* p, uint32_t n, uint16_t v, hipStream_t stre
   return hipGetLastError();
 }
 
-hipError_t launch_views(const ViewsParams& p, uint32_t nGroups, hipStream_t stream) {
-  if (p.nUnits == 0 || nGroups == 0) return hipSuccess;
-  if (p.stats)
-    rz_views_kernel<8, 8, true><<<dim3(nGroups), dim3(256), 0, stream>>>(p);
-  else
-    rz_views_kernel<8, 8, false><<<dim3(nGroups), dim3(256), 0, stream>>>(p);
-  return hipGetLastError();
+template <int NW>
+static const void* views_fn(bool stats) {
+  return stats ? reinterpret_cast<const void*>(&rz_views_kernel<8, 8, NW, true>)
+               : reinterpret_cast<const void*>(&rz_views_kernel<8, 8, NW, false>);
 }
 
-hipError_t views_max_groups(int* maxGroups) {
-  int dev = 0, cus = 0, n = 0;
+hipError_t launch_views(const ViewsParams& p, uint32_t nGroups, hipStream_t stream) {
+  if (p.nUnits == 0 || nGroups == 0) return hipSuccess;
+  const void* fn = p.waves == 8 ? views_fn<8>(p.stats != nullptr) : views_fn<4>(p.stats != nullptr);
+  ViewsParams local = p;
+  void* args[] = {&local};
+  return hipLaunchKernel(fn, dim3(nGroups), dim3(64 * (p.waves == 8 ? 8 : 4)), args, 0, stream);
+}
+
+hipError_t views_max_groups(int waves, int* maxGroups) {
+  int dev = 0, cus = 0;
   hipError_t e = hipGetDevice(&dev);
   if (e != hipSuccess) return e;
   e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
   if (e != hipSuccess) return e;
-  int n2 = 0;
-  e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, reinterpret_cast<const void*>(&rz_views_kernel<8, 8, false>),
-                                                   256, 0);
-  if (e != hipSuccess) return e;
-  e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n2, reinterpret_cast<const void*>(&rz_views_kernel<8, 8, true>),
-                                                   256, 0);
-  if (e != hipSuccess) return e;
-  *maxGroups = (n < n2 ? n : n2) * cus;
+  int perCU = 1 << 30;
+  for (bool stats : {false, true}) {
+    int n = 0;
+    const void* fn = waves == 8 ? views_fn<8>(stats) : views_fn<4>(stats);
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fn, 64 * (waves == 8 ? 8 : 4), 0);
+    if (e != hipSuccess) return e;
+    perCU = n < perCU ? n : perCU;
+  }
+  *maxGroups = perCU * cus;
   return hipSuccess;
 }
 

@@ -86,6 +86,7 @@ struct rz_views {
   uint32_t* d_segments = nullptr;  // nViews + 1 batch offsets
   void* d_sortTemp = nullptr;
   size_t sortTempBytes = 0;
+  int waves = 4;  // wave64s per raster work-group
   unsigned long long* d_stats = nullptr;  // block-loop counters of the last step, when on
   bool stats = false;
   hipGraph_t graph = nullptr;
@@ -333,7 +334,7 @@ static int build_plan(rz_views* v) {
   v->nQueries = queries;
   v->maxClear = maxClear;
   int maxGroups = 0;
-  RZ_HIP(views_max_groups(&maxGroups));
+  RZ_HIP(views_max_groups(v->waves, &maxGroups));
   v->nGroups = std::min<uint32_t>(units, uint32_t(std::max(maxGroups, 1)));
   const size_t ni = std::max<size_t>(items.size(), 1);
   if (dev_alloc(&v->d_items, ni) != hipSuccess || dev_alloc(&v->d_state, ni) != hipSuccess ||
@@ -466,6 +467,7 @@ static int enqueue_step(rz_views* v) {
     rp.groupBox = v->d_groupBox;
     rp.cov = v->d_cov;
     rp.queue = v->d_queue;
+    rp.waves = uint32_t(v->waves);
     rp.stats = v->stats ? v->d_stats : nullptr;
     if (v->stats) RZ_HIP(launch_fill_u32(reinterpret_cast<uint32_t*>(v->d_stats), 6, 0u, v->stream));
     RZ_HIP(launch_views(rp, v->nGroups, v->stream));
