@@ -126,20 +126,46 @@ def views_bench(steps: int, warmup: int, seed: int = 3) -> dict:
             V.set_mvp(k, mvps[s % n_mvp][k])
         V.run(graph=True)
 
-    for s in range(warmup):
-        batched_step(s)
-    V.sync()
-    t0 = time.perf_counter()
-    for s in range(steps):
-        batched_step(s)
-    V.sync()
-    t_b = time.perf_counter() - t0
-    gpu_ms = []
-    for s in range(min(steps, 8)):
-        batched_step(s)
-        gpu_ms.append(V.sync())
-    info = V.info()
-    log(f"batched: {t_b * 1e3 / steps:.3f} ms/step (GPU {np.median(gpu_ms):.3f} ms), {info}")
+    def time_batched(what: str):
+        for s in range(warmup):
+            batched_step(s)
+        V.sync()
+        t0 = time.perf_counter()
+        for s in range(steps):
+            batched_step(s)
+        V.sync()
+        t = time.perf_counter() - t0
+        gpu_ms = []
+        for s in range(min(steps, 8)):
+            batched_step(s)
+            gpu_ms.append(V.sync())
+        log(f"{what}: {t * 1e3 / steps:.3f} ms/step (GPU {np.median(gpu_ms):.3f} ms), {V.info()}")
+        return t, float(np.median(gpu_ms))
+
+    def block_counters():
+        """hiZ/edge rejects of (quad, block) pairs over one step (separate, untimed)."""
+        V.enable_stats(True)
+        batched_step(1)  # the learned order reads the previous step's labels
+        batched_step(2)
+        V.sync()
+        st = V.stats()
+        V.enable_stats(False)
+        return {**st, "reject_rate": st["rejected"] / max(st["visited"], 1)}
+
+    # Occluder order: each view's subset front to back for its eye.
+    t_b, gpu_b = time_batched("batched, occluder order")
+    cnt_b = block_counters()
+    # Learned order: the trained MLP scores every batch each step (features
+    # of this step's setup, labels of the previous step); occluders are drawn
+    # by their best batch's score.
+    from rasterizer_amd import learn
+
+    model = learn.load()
+    V.set_model(model, by_occluder=True)
+    t_l, gpu_l = time_batched("batched, learned order")
+    cnt_l = block_counters()
+    log(f"reject rate: occluder order {cnt_b['reject_rate']:.4f}, learned {cnt_l['reject_rate']:.4f}")
+    V.set_model(None)
 
     # Plain: one context per view, the single-view frame path.
     ctxs = [Rasterizer(*v["size"]) for v in views]
@@ -167,6 +193,7 @@ def views_bench(steps: int, warmup: int, seed: int = 3) -> dict:
         R.sync()
     t_p = time.perf_counter() - t0
     log(f"plain per-view: {t_p * 1e3 / plain_steps:.3f} ms/step")
+    plain_qps = plain_steps * total_quads / t_p
     return {
         "workload": (f"{nv} views ({pixels:,} px: 16x2048^2, 24x1024^2, 16x512^2, 8x1920x1080), "
                      f"{total_quads:,} quads submitted per step from a 240,000-quad box scene, "
@@ -174,13 +201,23 @@ def views_bench(steps: int, warmup: int, seed: int = 3) -> dict:
         "batched_quads_per_s": steps * total_quads / t_b,
         "batched_queries_per_s": steps * total_q / t_b,
         "batched_ms_per_step": t_b * 1e3 / steps,
-        "batched_gpu_ms_per_step": float(np.median(gpu_ms)),
-        "plain_quads_per_s": plain_steps * total_quads / t_p,
+        "batched_gpu_ms_per_step": gpu_b,
+        "batched_block_counters": cnt_b,
+        "plain_quads_per_s": plain_qps,
         "plain_ms_per_step": t_p * 1e3 / plain_steps,
-        "speedup_vs_plain": (steps * total_quads / t_b) / (plain_steps * total_quads / t_p),
+        "speedup_vs_plain": (steps * total_quads / t_b) / plain_qps,
         "steps": steps,
         "plain_steps": plain_steps,
-        "learned_order": None,
+        "learned_order": {
+            "granularity": "occluder (best batch score)",
+            "weights": "rasterizer_amd/order_mlp.npz",
+            "quads_per_s": steps * total_quads / t_l,
+            "queries_per_s": steps * total_q / t_l,
+            "ms_per_step": t_l * 1e3 / steps,
+            "gpu_ms_per_step": gpu_l,
+            "block_counters": cnt_l,
+            "speedup_vs_plain": (steps * total_quads / t_l) / plain_qps,
+        },
     }
 
 

@@ -222,10 +222,12 @@ int rz_views_get_info(rz_views* views, uint32_t info[8]);
 /* Draw-order learning (no counterpart in the reference).  With labels on,
  * every step also computes, per 8-quad batch (the k-th batch of a view is the
  * k-th in its occluder order): 16 features before the draw and, after it,
- * the label = number of blocks in which the batch keeps some pixel of the
- * finished buffer (each quad sampled alone as the block loop samples it,
- * without hiZ test or merge; a pixel keeps the earliest batch whose sample
- * equals its non-zero depth in a written block). */
+ * the label = number of written blocks in which the batch wins some pixel.
+ * The draw tracks each pixel's winner: the first write of a cleared block
+ * sets it for every pixel (none where the written depth is 0), a later write
+ * only where it raises the depth; so a pixel's winner is the first batch in
+ * submission order whose write produced its final depth.  Counted by the
+ * draw itself, at no extra raster pass. */
 int rz_views_enable_labels(rz_views* views, int enable);
 /* Draws the view's batches in the given order (a permutation of its batch
  * numbers 0..n-1), each with its occluder's classification; n = 0 restores
@@ -254,10 +256,13 @@ int rz_mlp_train_step(rz_mlp* mlp, uint32_t n, const float* x, uint32_t nPairs, 
                       float* loss);
 /* Learned draw order for a multi-view batch: every step scores each view's
  * batches with the model (from this step's features and the previous step's
- * labels) and draws them by descending score, ties in occluder order.  NULL
- * restores occluder order.  The model must outlive its use. */
-int rz_views_set_model(rz_views* views, rz_mlp* model);
-/* The batch order of the last step (view-local batch numbers). */
+ * labels).  byOccluder == 0 draws the batches by descending score; != 0 draws
+ * whole occluders by their best batch score (no batch items, so it scales to
+ * large occluder sets).  Ties keep occluder order.  NULL restores occluder
+ * order.  The model must outlive its use. */
+int rz_views_set_model(rz_views* views, rz_mlp* model, int byOccluder);
+/* The order of the last learned step: view-local batch numbers, or occluder
+ * numbers when ordering by occluder. */
 int rz_views_read_order(rz_views* views, uint32_t view, uint32_t* order);
 /* Block-loop counters of each step (observability): rz_views_read_stats
  * returns, for the last step over all views, the (quad, block) pairs

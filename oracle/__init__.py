@@ -66,7 +66,7 @@ def lib() -> ctypes.CDLL:
             "or_block_stats": ([P, P], None),
             "or_frame": ([P, U, P, P], None),
             "or_frame_classified": ([P, U, P, P], None),
-            "or_labels": ([P, U, P, P, P], None),
+            "or_frame_labels": ([P, U, P, P, P, P], None),
             "or_setup_dump": ([P, P, U, P, P, ctypes.c_int, P], ctypes.c_int),
             "or_bake": ([P, U, P, P, P, P, P, P], ctypes.c_int),
             "or_x86_rcp": ([ctypes.c_float], ctypes.c_float),
@@ -203,15 +203,23 @@ class OracleRasterizer:
                                 1 if clip else 0, _ptr(out))
         return out[:n]
 
-    def labels(self, occluders, status) -> np.ndarray:
-        """Per-batch draw-order labels of the current buffer (see oracle.h)."""
+    def frame_labels(self, occluders, order=None):
+        """Labelled classify-then-draw (oracle.h or_frame_labels): returns
+        (per-occluder status, per-batch label counts)."""
         arr, keep = occluder_structs(occluders)
-        st = np.ascontiguousarray(status, np.uint8)
         nb = sum(int(np.asarray(o.packets).size) // 32 for o in occluders)
+        status = np.zeros(max(len(occluders), 1), np.uint8)
         out = np.zeros(max(nb, 1), np.uint32)
-        lib().or_labels(self._r, len(occluders), ctypes.cast(arr, ctypes.c_void_p), _ptr(st), _ptr(out))
+        ordp = None
+        if order is not None:
+            order = np.ascontiguousarray(order, np.uint32)
+            if sorted(order.tolist()) != list(range(nb)):
+                raise ValueError("order must be a permutation of the batch numbers")
+            ordp = _ptr(order)
+        lib().or_frame_labels(self._r, len(occluders), ctypes.cast(arr, ctypes.c_void_p), ordp, _ptr(status),
+                              _ptr(out))
         del keep
-        return out[:nb]
+        return status[: len(occluders)], out[:nb]
 
     def draw_frame(self, occluders, gated: bool = True) -> np.ndarray:
         """Main.cpp:192-206 (gated) or the batched path's classify-then-draw

@@ -99,8 +99,8 @@ struct OrRaster {
   uint16_t* hiz;      /* blocksX*blocksY + 8 */
   uint64_t stats[7];
   uint32_t* blockVisits; /* optional per-block counters: [2*b] visited, [2*b+1] written */
-  /* Label mode (or_labels): quads are compared with the buffer instead of
-   * drawn; winner[pixel] keeps the smallest batch index that matches. */
+  /* Write tracking (or_frame_labels): when set, winner[pixel] is the batch
+   * whose write produced the pixel's current depth (first such write). */
   uint32_t* winner;
   uint32_t curBatch;
 };
@@ -488,7 +488,7 @@ static void traverse_quad(OrRaster* r, const QuadSetup* Q) {
       if (r->blockVisits) r->blockVisits[2 * blk]++;
       uint64_t mask = 0;
       int skip = 0;
-      if (!r->winner && hz >= Q->maxZ) {
+      if (hz >= Q->maxZ) {
         r->stats[1]++;
         skip = 1;
       } else if (Q->mode == Convex) {
@@ -547,26 +547,19 @@ static void traverse_quad(OrRaster* r, const QuadSetup* Q) {
             rows[4 + p][xx] = d2;
             rows[6 + p][xx] = d3;
           }
-        if (r->winner) {
-          /* Label mode: a covered pixel whose sampled depth equals the
-           * written buffer's names this quad's batch as a candidate. */
-          for (int yy = 0; yy < 8; ++yy)
-            for (int xx = 0; xx < 8; ++xx) {
-              uint16_t d = rows[yy][xx];
-              if (!((mask >> mask_bit(xx, yy)) & 1) || d == 0 || hz == 1 || out[8 * yy + xx] != d) continue;
-              uint32_t* w = &r->winner[(size_t)blk * 64 + 8 * yy + xx];
-              if (r->curBatch < *w) *w = r->curBatch;
-            }
-          goto next_block;
-        }
         uint16_t mn = 0xFFFF;
         for (int yy = 0; yy < 8; ++yy)
           for (int xx = 0; xx < 8; ++xx) {
             uint16_t d = rows[yy][xx];
             if (!((mask >> mask_bit(xx, yy)) & 1)) d = 0;
+            uint32_t* w = r->winner ? &r->winner[(size_t)blk * 64 + 8 * yy + xx] : NULL;
             if (hz != 1) {
               uint16_t old = out[8 * yy + xx];
+              if (w && d > old) *w = r->curBatch;
               if (old > d) d = old;
+            } else if (w) {
+              /* First write of a cleared block: every pixel is replaced. */
+              *w = d ? r->curBatch : 0xFFFFFFFFu;
             }
             out[8 * yy + xx] = d;
             if (d < mn) mn = d;
@@ -575,7 +568,6 @@ static void traverse_quad(OrRaster* r, const QuadSetup* Q) {
         r->stats[4]++;
         if (r->blockVisits) r->blockVisits[2 * blk + 1]++;
       }
-    next_block:
       for (int k = 0; k < 8; ++k) depth[k] = dDx + depth[k];
       for (int e = 0; e < 4; ++e) offset[e] = Q->nx[e] + offset[e];
     }
@@ -847,35 +839,49 @@ void or_frame(OrRaster* r, uint32_t n, const OrOccluder* occ, uint8_t* status) {
   }
 }
 
-void or_labels(OrRaster* r, uint32_t n, const OrOccluder* occ, const uint8_t* status, uint32_t* labels) {
-  const size_t nb = (size_t)r->blocksX * r->blocksY;
-  uint32_t* winner = (uint32_t*)malloc(nb * 64 * sizeof(uint32_t));
-  for (size_t i = 0; i < nb * 64; ++i) winner[i] = 0xFFFFFFFFu;
-  r->winner = winner;
-  uint32_t batch = 0;
+void or_frame_labels(OrRaster* r, uint32_t n, const OrOccluder* occ, const uint32_t* order, uint8_t* status,
+                     uint32_t* labels) {
+  uint32_t nb = 0;
   for (uint32_t i = 0; i < n; ++i) {
-    const uint32_t batches = occ[i].packetCount / 4;
-    if (status[i]) {
-      for (uint32_t b = 0; b < batches; ++b) {
-        r->curBatch = batch + b;
-        or_rasterize(r, occ[i].packets + (size_t)b * 32, 4, occ[i].refMin, occ[i].refMax, status[i] == 2);
-      }
+    int clip = 0;
+    status[i] = query_impl_ex(r, occ[i].boundsMin, occ[i].boundsMax, &clip, 0) ? (clip ? 2 : 1) : 0;
+    nb += occ[i].packetCount / 4;
+  }
+  /* batch k -> (occluder, first packet) */
+  uint32_t* owner = (uint32_t*)malloc((nb ? nb : 1) * sizeof(uint32_t));
+  uint32_t* first = (uint32_t*)malloc((nb ? nb : 1) * sizeof(uint32_t));
+  for (uint32_t i = 0, k = 0; i < n; ++i)
+    for (uint32_t b = 0; b < occ[i].packetCount / 4; ++b, ++k) {
+      owner[k] = i;
+      first[k] = 4 * b;
     }
-    batch += batches;
+  const size_t blocks = (size_t)r->blocksX * r->blocksY;
+  uint32_t* winner = (uint32_t*)malloc(blocks * 64 * sizeof(uint32_t));
+  for (size_t i = 0; i < blocks * 64; ++i) winner[i] = 0xFFFFFFFFu;
+  r->winner = winner;
+  for (uint32_t t = 0; t < nb; ++t) {
+    const uint32_t k = order ? order[t] : t;
+    const OrOccluder* o = &occ[owner[k]];
+    if (!status[owner[k]]) continue;
+    r->curBatch = k;
+    or_rasterize(r, o->packets + (size_t)first[k] * 8, 4, o->refMin, o->refMax, status[owner[k]] == 2);
   }
   r->winner = NULL;
-  for (uint32_t b = 0; b < batch; ++b) labels[b] = 0;
-  for (size_t k = 0; k < nb; ++k) {
+  for (uint32_t k = 0; k < nb; ++k) labels[k] = 0;
+  for (size_t b = 0; b < blocks; ++b) {
+    if (r->hiz[b] == 1) continue; /* not written this frame: winners are stale */
     /* Distinct winners of the block, each counted once. */
     for (int p = 0; p < 64; ++p) {
-      uint32_t w = winner[k * 64 + p];
+      uint32_t w = winner[b * 64 + p];
       if (w == 0xFFFFFFFFu) continue;
       int seen = 0;
-      for (int q = 0; q < p && !seen; ++q) seen = winner[k * 64 + q] == w;
+      for (int q = 0; q < p && !seen; ++q) seen = winner[b * 64 + q] == w;
       if (!seen) labels[w]++;
     }
   }
   free(winner);
+  free(owner);
+  free(first);
 }
 
 void or_frame_classified(OrRaster* r, uint32_t n, const OrOccluder* occ, uint8_t* status) {

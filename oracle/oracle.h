@@ -97,15 +97,17 @@ void or_frame(OrRaster* r, uint32_t n, const OrOccluder* occ, uint8_t* status);
  * frustum cull, near-plane test, empty screen rectangle) and rasterized with
  * the resulting needsClipping, with no occlusion test between draws. */
 void or_frame_classified(OrRaster* r, uint32_t n, const OrOccluder* occ, uint8_t* status);
-/* Draw-order labels of the current buffer (after drawing occ with the given
- * statuses, 0 not drawn, 1 rasterize<false>, 2 rasterize<true>).  Batch k
- * is the k-th 8-quad batch in draw order.  Each quad of a drawn batch is
- * sampled alone (rows, coverage mask and packed depth as the block loop
- * computes them, without the hiZ test and without merging); a covered pixel
- * of a written block (hiZ != 1) whose sample equals the buffer's non-zero
- * depth names the batch, and each pixel keeps the earliest batch named.
- * labels[k] = number of blocks in which batch k is kept for some pixel. */
-void or_labels(OrRaster* r, uint32_t n, const OrOccluder* occ, const uint8_t* status, uint32_t* labels);
+/* Labelled classify-then-draw of one frame (caller has cleared and set the
+ * matrix): status[i] as or_frame_classified (0 not drawn, 1 rasterize<false>,
+ * 2 rasterize<true>); then the 8-quad batches of the drawn occluders in
+ * `order` (batch numbers counted in occluder order; NULL: occluder order).
+ * Each pixel tracks the batch whose write produced its depth: the first write
+ * of a cleared block (hiZ == 1) sets every pixel's winner (none where the
+ * written depth is 0), a later write only where it raises the depth.  So the
+ * winner is the first batch in submission order that wrote the final depth.
+ * labels[k] = number of written blocks in which batch k wins some pixel. */
+void or_frame_labels(OrRaster* r, uint32_t n, const OrOccluder* occ, const uint32_t* order, uint8_t* status,
+                     uint32_t* labels);
 
 /* Occluder::bake (Occluder.cpp:7-181).  vertices: n*4 floats, n a multiple
  * of 32 (8 quads).  packetsOut: n words.  Returns 0, or -1 on bad n. */

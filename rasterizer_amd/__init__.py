@@ -113,7 +113,7 @@ _SIGNATURES = {
     "rz_mlp_forward": ([_P, _U32, _P, _P], ctypes.c_int),
     "rz_mlp_gradients": ([_P, _U32, _P, _U32, _P, _P, _P], ctypes.c_int),
     "rz_mlp_train_step": ([_P, _U32, _P, _U32, _P, ctypes.c_float, _P], ctypes.c_int),
-    "rz_views_set_model": ([_P, _P], ctypes.c_int),
+    "rz_views_set_model": ([_P, _P, ctypes.c_int], ctypes.c_int),
     "rz_views_read_order": ([_P, _U32, _P], ctypes.c_int),
     "rz_views_enable_stats": ([_P, ctypes.c_int], ctypes.c_int),
     "rz_views_read_stats": ([_P, _P], ctypes.c_int),
@@ -486,9 +486,13 @@ class Views:
         _check(lib().rz_views_read_labels(self._v, view, _ptr(lab), _ptr(feat)), "rz_views_read_labels")
         return lab[:n], feat[:n]
 
-    def set_model(self, mlp: "Mlp | None") -> None:
+    def set_model(self, mlp: "Mlp | None", by_occluder: bool = False) -> None:
+        """Learned order: batches by descending score, or with by_occluder
+        whole occluders by their best batch's score."""
         self._model = mlp
-        _check(lib().rz_views_set_model(self._v, mlp._m if mlp is not None else None), "rz_views_set_model")
+        self._by_occluder = by_occluder and mlp is not None
+        _check(lib().rz_views_set_model(self._v, mlp._m if mlp is not None else None, 1 if by_occluder else 0),
+               "rz_views_set_model")
 
     def enable_stats(self, enable: bool = True) -> None:
         _check(lib().rz_views_enable_stats(self._v, 1 if enable else 0), "rz_views_enable_stats")
@@ -500,7 +504,7 @@ class Views:
         return {"visited": int(a[0]), "rejected": int(a[1]), "written": int(a[2])}
 
     def order(self, view: int) -> np.ndarray:
-        n = self.n_batches(view)
+        n = len(self._occ[view]) if getattr(self, "_by_occluder", False) else self.n_batches(view)
         out = np.zeros(max(n, 1), np.uint32)
         _check(lib().rz_views_read_order(self._v, view, _ptr(out)), "rz_views_read_order")
         return out[:n]

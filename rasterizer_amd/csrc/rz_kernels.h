@@ -54,6 +54,10 @@ struct ViewsParams {
   // Optional block-loop counters of the launch (visited, rejected by hiZ or
   // the edge tests, written), added to stats[0..2].
   unsigned long long* stats;
+  // Optional labelled draw: per-batch counts of the written blocks in which
+  // each batch wins a pixel (zeroed by the caller), with each view's
+  // per-pixel winners in views[v].winner.
+  uint32_t* labels;
 };
 
 // queryVisibility over the queries of every view of a multi-view launch.
@@ -168,15 +172,18 @@ struct LearnParams {
 };
 constexpr int kFeatures = 16;
 
-// Labels of the finished buffers: per-pixel earliest matching batch, then
-// per batch the number of blocks it keeps (see rz_learn.hip).
-hipError_t launch_labels(const LearnParams& p, uint32_t maxBlocks, hipStream_t stream);
 hipError_t launch_features(const LearnParams& p, uint32_t nBatches, hipStream_t stream);
 // Batch items in the order `order` (per view segment, batch numbers): item k
 // draws batch order[k] with its occluder's status and the batch's footprint.
 hipError_t launch_batch_items(const LearnParams& p, const uint32_t* order, uint32_t nBatches, ItemDesc* outItems,
                               ItemState* outState, hipStream_t stream);
 hipError_t launch_fill_u32(uint32_t* p, uint32_t n, uint32_t v, hipStream_t stream);
+// Occluder-granular learned order: each item's score is the largest score of
+// its batches; then items are copied in the order `order` (per view segment).
+hipError_t launch_item_scores(const LearnParams& p, const float* batchScores, float* itemScores,
+                              hipStream_t stream);
+hipError_t launch_item_reorder(const LearnParams& p, const uint32_t* order, ItemDesc* outItems, ItemState* outState,
+                               hipStream_t stream);
 
 // ---------------------------------------------------------------------------
 // Batch-scoring MLP 16 -> 64 -> 64 -> 1, GELU, BF16 operands on MFMA with FP32

@@ -609,6 +609,7 @@ struct ItemLocal {
 
 constexpr uint32_t kItemWindow = 128;
 constexpr uint32_t kQuadChunk = 64;   // records staged in LDS at a time
+constexpr uint32_t kNoWinner = 0xFFFFFFFFu;  // pixel winner of labelled draws: none
 // A scan pass tests one quad per thread: 64 * NW quads.
 
 template <int TBX, int TBY>
@@ -618,6 +619,7 @@ struct TileSmem {
   uint16_t hiz[TBX * TBY];
   ItemLocal items[kItemWindow];    // this window's items that touch the tile
   QuadRec recs[kQuadChunk];        // setup records of the quads being drawn
+  uint32_t qid[kQuadChunk];        // their frame quad numbers (labelled draws)
   // Block-major draw (8x8-block tiles): per staged quad its rectangle in the
   // tile and origin, and per tile row the folded edge offsets and depth
   // samples at the rectangle's first column in the tile.
@@ -816,10 +818,14 @@ __device__ __forceinline__ void raster_quad_wave(const QuadRec& R, TileSmem<TBX,
 // One block of one quad with the block's hiZ and depth in registers (the
 // block-major draw).  Same arithmetic as raster_block with g = 0: lanes 0-3
 // of `s` hold the folded edge offsets, lanes 4-11 the depth samples.
-template <bool STATS>
+//
+// LABELS: `win` is the lane's pixel's winner, the batch whose write produced
+// its depth: the first write of a cleared block (hz == 1) sets it (none where
+// the written depth is 0), later writes only where they raise the depth.
+template <bool STATS, bool LABELS>
 __device__ __forceinline__ void raster_block_reg(const uint64_t* __restrict__ cov, float s, uint32_t lane, int kLane,
                                                  const QuadConst& Q, uint32_t& hz, uint32_t& dval, bool& dirty,
-                                                 BlockCounters& bc) {
+                                                 BlockCounters& bc, uint32_t& win, uint32_t batch) {
   if (STATS) bc.visited++;
   if (hz >= Q.maxZ) {
     if (STATS) bc.rejected++;
@@ -868,6 +874,7 @@ __device__ __forceinline__ void raster_block_reg(const uint64_t* __restrict__ co
   uint32_t d = (j & 2u) ? a23 : a01;
   const uint32_t maskBit = 8u * px + ((py & 1u) ? (py >> 1) : 4u + (py >> 1));
   d = ((mask >> maskBit) & 1ull) ? d : 0u;
+  if (LABELS) win = hz == 1u ? (d != 0u ? batch : kNoWinner) : (d > dval ? batch : win);
   if (hz != 1u) d = max(d, dval);
   dval = d;
   hz = __ockl_wfred_min_u32(d);
@@ -915,9 +922,14 @@ __device__ __forceinline__ void quad_component(const QuadRec& R, uint32_t e, flo
 // fast-cleared, since hiZ only grows once written), and the survivors are
 // drawn in submission order with the block's hiZ and depth in registers,
 // each finishing its fold with the column steps left.
-template <int TBX, int TBY, int NW, bool STATS>
+//
+// LABELS: each block's pixel winners (`winner`, the view's block-linear
+// array) are loaded with its depth unless the block is cleared (its first
+// write sets them all) and stored with it; quad q's batch is qid[q] / 8.
+template <int TBX, int TBY, int NW, bool STATS, bool LABELS>
 __device__ void draw_chunk_block_major(TileSmem<TBX, TBY>& sm, uint32_t n, int tbx0, int tbx1, int tby0, int tby1,
-                                       int bX, float invBX, uint32_t wave, uint32_t lane, BlockCounters& bc) {
+                                       int bX, float invBX, uint32_t wave, uint32_t lane, BlockCounters& bc,
+                                       uint32_t* __restrict__ winner) {
   const uint32_t g = lane / 12u, e = lane - 12u * g;
   for (uint32_t q = wave; q < n; q += NW) {
     const QuadRec& R = sm.recs[q];
@@ -978,6 +990,9 @@ __device__ void draw_chunk_block_major(TileSmem<TBX, TBY>& sm, uint32_t n, int t
     if (m == 0ull) continue;
     uint32_t dval = sm.depth[lb * 64 + int(lane)];
     bool dirty = false;
+    const size_t wIdx = (size_t(tby0 + lby) * size_t(bX) + size_t(tbx0 + lbx)) * 64u + lane;
+    uint32_t win = kNoWinner;
+    if (LABELS && hz != 1u) win = winner[wIdx];
     while (m) {
       const uint32_t q = uint32_t(__builtin_ctzll(m));
       m &= m - 1ull;
@@ -1000,11 +1015,13 @@ __device__ void draw_chunk_block_major(TileSmem<TBX, TBY>& sm, uint32_t n, int t
         const float stepX = ce < 4u ? (ce == 0 ? R.nx[0] : ce == 1 ? R.nx[1] : ce == 2 ? R.nx[2] : R.nx[3]) : R.dDx;
         sv = fold_n(sm.entry[(q * TBY + uint32_t(lby)) * 12u + ce], stepX, lbx - x0);
       }
-      raster_block_reg<STATS>(sm.cov, sv, lane, kLane, Q, hz, dval, dirty, bc);
+      raster_block_reg<STATS, LABELS>(sm.cov, sv, lane, kLane, Q, hz, dval, dirty, bc, win,
+                                      LABELS ? sm.qid[q] >> 3 : 0u);
     }
     if (dirty) {
       sm.depth[lb * 64 + int(lane)] = uint16_t(dval);
       if (lane == 0) sm.hiz[lb] = uint16_t(hz);
+      if (LABELS) winner[wIdx] = win;
     }
   }
 }
@@ -1121,12 +1138,18 @@ struct TileJob {
   uint32_t* done;
   uint32_t* err;
   uint64_t timeoutTicks;
+  uint32_t* winner;  // labelled draws: per-pixel winners (block-linear), and
+  uint32_t* labels;  // per-batch counts of written blocks each batch wins
 };
 
-template <int TBX, int TBY, int NW, bool STATS>
+// LABELS (8x8-block tiles): the draw tracks each pixel's winning batch, and
+// once the tile is final every written block adds one to the label of each
+// distinct batch that wins one of its pixels.
+template <int TBX, int TBY, int NW, bool STATS, bool LABELS>
 __device__ __forceinline__ void process_tile(const TileJob& J, TileSmem<TBX, TBY>& sm,
                                              const uint64_t* __restrict__ cov, bool& covLoaded,
                                              uint64_t (&ctr)[kFrameStatWords]) {
+  static_assert(!LABELS || (TBX == 8 && TBY == 8), "labelled draws use the block-major draw");
   auto stamp = []() -> uint64_t { return STATS ? __builtin_amdgcn_s_memrealtime() : 0; };
   const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
   const uint32_t tX = J.tX, tY = J.tY;
@@ -1313,14 +1336,21 @@ __device__ __forceinline__ void process_tile(const TileJob& J, TileSmem<TBX, TBY
         if (STATS && tid == 0) ctr[kStatScanTicks] += stamp() - tScan;
         // The first 64 hits are staged from registers; later batches (rare)
         // reload their records.
-        if (hit && pos < kQuadChunk) sm.recs[pos] = rec;
+        if (hit && pos < kQuadChunk) {
+          sm.recs[pos] = rec;
+          if (LABELS) sm.qid[pos] = L.quadBase + base + tid;
+        }
         for (uint32_t k0 = 0; k0 < total; k0 += kQuadChunk) {
           const uint32_t n = min(kQuadChunk, total - k0);
-          if (k0 > 0 && hit && pos >= k0 && pos < k0 + n) sm.recs[pos - k0] = J.recs[L.quadBase + base + tid];
+          if (k0 > 0 && hit && pos >= k0 && pos < k0 + n) {
+            sm.recs[pos - k0] = J.recs[L.quadBase + base + tid];
+            if (LABELS) sm.qid[pos - k0] = L.quadBase + base + tid;
+          }
           __syncthreads();
           const uint64_t tDraw = stamp();
           if constexpr (TBX == 8 && TBY == 8) {
-            draw_chunk_block_major<TBX, TBY, NW, STATS>(sm, n, tbx0, tbx1, tby0, tby1, bX, invBX, wave, lane, bc);
+            draw_chunk_block_major<TBX, TBY, NW, STATS, LABELS>(sm, n, tbx0, tbx1, tby0, tby1, bX, invBX, wave, lane,
+                                                                bc, J.winner);
           } else {
             for (uint32_t q = 0; q < n; ++q)
               raster_quad_wave<TBX, TBY, NW, STATS>(sm.recs[q], sm, tbx0, tbx1, tby0, tby1, bX, invBX, wave, lane, bc);
@@ -1347,6 +1377,25 @@ __device__ __forceinline__ void process_tile(const TileJob& J, TileSmem<TBX, TBY
       ctr[kStatBlocksVisited] += sm.blockStats[0];
       ctr[kStatBlocksRejected] += sm.blockStats[1];
       ctr[kStatBlocksWritten] += sm.blockStats[2];
+    }
+  }
+  if constexpr (LABELS) {
+    // Each block is counted by the wave that drew it, which wrote its
+    // winners; blocks still cleared were not written this frame.
+    for (int lb = 0; lb < NB; ++lb) {
+      const int lbx = lb % TBX, lby = lb / TBX;
+      if (block_owner<NW>(lbx, lby) != wave) continue;
+      const int gx = tbx0 + lbx, gy = tby0 + lby;
+      if (gx > tbx1 || gy > tby1 || sm.hiz[lb] == 1u) continue;
+      const uint32_t w = J.winner[(size_t(gy) * size_t(bX) + size_t(gx)) * 64u + lane];
+      bool active = w != kNoWinner;
+      while (true) {
+        const uint64_t m = __ballot(active);
+        if (m == 0ull) break;
+        const uint32_t v = uint32_t(__builtin_amdgcn_readlane(int(w), int(__builtin_ctzll(m))));
+        if (active && w == v) active = false;
+        if (lane == 0) atomicAdd(&J.labels[v], 1u);
+      }
     }
   }
   // Write the tile back.
@@ -1398,8 +1447,10 @@ __global__ void __launch_bounds__(64 * NW) rz_frame_kernel(FrameParams p) {
   J.done = p.done;
   J.err = p.err;
   J.timeoutTicks = p.timeoutTicks;
+  J.winner = nullptr;
+  J.labels = nullptr;
   bool covLoaded = false;
-  process_tile<TBX, TBY, NW, STATS>(J, sm, p.cov, covLoaded, ctr);
+  process_tile<TBX, TBY, NW, STATS, false>(J, sm, p.cov, covLoaded, ctr);
   if (STATS && threadIdx.x == 0) {
     ctr[kStatTotalTicks] = __builtin_amdgcn_s_memrealtime() - tEntry;
     for (int k = 0; k < kFrameStatWords; ++k) p.stats[size_t(blockIdx.x) * kFrameStatWords + k] = ctr[k];
@@ -1410,7 +1461,7 @@ __global__ void __launch_bounds__(64 * NW) rz_frame_kernel(FrameParams p) {
 // p.queue until the queue is exhausted.  Units are handed out in view order
 // and no unit waits on another, so every work-group exits after at most
 // nUnits queue reads.
-template <int TBX, int TBY, int NW, bool STATS>
+template <int TBX, int TBY, int NW, bool STATS, bool LABELS>
 __global__ void __launch_bounds__(64 * NW) rz_views_kernel(ViewsParams p) {
   __shared__ TileSmem<TBX, TBY> sm;
   __shared__ uint32_t unitSh;
@@ -1449,7 +1500,9 @@ __global__ void __launch_bounds__(64 * NW) rz_views_kernel(ViewsParams p) {
     J.done = nullptr;
     J.err = nullptr;
     J.timeoutTicks = 0;
-    process_tile<TBX, TBY, NW, STATS>(J, sm, p.cov, covLoaded, ctr);
+    J.winner = V.winner;
+    J.labels = p.labels;
+    process_tile<TBX, TBY, NW, STATS, LABELS>(J, sm, p.cov, covLoaded, ctr);
     __syncthreads();
   }
   if (STATS && threadIdx.x == 0) {
@@ -1594,144 +1647,6 @@ __device__ __forceinline__ uint32_t item_of_quad(const ItemDesc* __restrict__ it
   return lo;
 }
 
-// One block of a quad sampled alone, as the block loop samples it (coverage
-// mask, rows 0/1 and 8/9 of the plane, packed averages; no hiZ test, no
-// merge), compared with the finished buffer: a covered pixel of a written
-// block whose sample equals the buffer's non-zero depth names `batch`.
-__device__ __forceinline__ void label_block(const uint64_t* __restrict__ cov, const uint16_t* __restrict__ depth,
-                                            const uint16_t* __restrict__ hiz, uint32_t* __restrict__ winner,
-                                            uint32_t blk, float s, int g, uint32_t lane, int kLane,
-                                            const QuadConst& Q, uint32_t batch) {
-  float o[4];
-  uint32_t idx[4];
-  bool outside = false;
-#pragma unroll
-  for (int e = 0; e < 4; ++e) {
-    o[e] = readlane_f(s, 12 * g + e);
-    const int32_t t = cvtt_x86(o[e]);
-    if (Q.mode == kModeConvex) {
-      outside |= o[e] >= 63.0f;
-      idx[e] = (Q.sl[e] | uint32_t(max(t, 0))) & 4095u;
-    } else {
-      idx[e] = (Q.sl[e] | uint32_t(min(max(t, 0), 63))) & 4095u;
-    }
-  }
-  if (outside) return;
-  const uint64_t A = cov[idx[0]], B = cov[idx[1]], C = cov[idx[2]], D = cov[idx[3]];
-  uint64_t mask;
-  if (Q.mode == kModeConvex) mask = (A & B) & (C & D);
-  else if (Q.mode == kModeTriangle0) mask = A & B & C;
-  else if (Q.mode == kModeTriangle1) mask = A & C & D;
-  else if (Q.mode == kModeConcaveRight) mask = (A | D) & (B & C);
-  else if (Q.mode == kModeConcaveCenter) mask = (A & B) | (C & D);
-  else mask = (A & D) & (B | C);
-  if (mask == 0) return;
-  float v = __shfl(s, 12 * g + 4 + kLane);
-  const uint32_t px = lane & 7u, py = lane >> 3;
-  if (px >= 4u) v = fmaf(Q.dDx, 0.5f, v);
-  const uint32_t lo = pack_depth(v);
-  const uint32_t hi = pack_depth(Q.dDy + v);
-  const uint32_t mid = avg_u16(lo, hi);
-  const uint32_t q1 = avg_u16(lo, mid), q3 = avg_u16(mid, hi);
-  const uint32_t j = py >> 1;
-  const uint32_t a01 = (j & 1u) ? q1 : lo, a23 = (j & 1u) ? q3 : mid;
-  const uint32_t d = (j & 2u) ? a23 : a01;
-  const uint32_t maskBit = 8u * px + ((py & 1u) ? (py >> 1) : 4u + (py >> 1));
-  if (((mask >> maskBit) & 1ull) && d != 0u && hiz[blk] != 1u && uint32_t(depth[size_t(blk) * 64 + lane]) == d)
-    atomicMin(&winner[size_t(blk) * 64 + lane], batch);
-}
-
-// Every block of the quad's rectangle, with the block loop's folds (row
-// steps, then column steps, each add rounded).
-__device__ void label_quad_wave(const QuadRec& R, const uint64_t* __restrict__ cov, const ViewDesc& V,
-                                uint32_t batch, uint32_t lane) {
-  const int fb = int(R.fb), bX = int(V.bX);
-  const int rangeX = int(R.range & 0xFFFFu), rangeY = int(R.range >> 16);
-  QuadConst Q;
-  Q.mode = R.modeZ & 0xFFu;
-  Q.maxZ = R.modeZ >> 16;
-  Q.sl[0] = R.slope01 & 0xFFFFu;
-  Q.sl[1] = R.slope01 >> 16;
-  Q.sl[2] = R.slope23 & 0xFFFFu;
-  Q.sl[3] = R.slope23 >> 16;
-  Q.dDx = R.dDx;
-  Q.dDy = R.dDy;
-  const float invBX = 1.0f / float(bX);
-  const int minY = div_bx(fb, bX, invBX), minX = fb - minY * bX;
-  const int kLane = int(((lane & 7u) & 3u) + 4u * ((lane >> 3) & 1u));
-  const uint32_t g = lane / 12u, e = lane - 12u * g;
-  float base, stepY, stepX;
-  if (e < 4) {
-    base = e == 0 ? R.off[0] : e == 1 ? R.off[1] : e == 2 ? R.off[2] : R.off[3];
-    stepY = e == 0 ? R.ny[0] : e == 1 ? R.ny[1] : e == 2 ? R.ny[2] : R.ny[3];
-    stepX = e == 0 ? R.nx[0] : e == 1 ? R.nx[1] : e == 2 ? R.nx[2] : R.nx[3];
-  } else {
-    const uint32_t k = e - 4;
-    const float spx = -0.4375f + 0.125f * float(k & 3u);
-    const float spy = -0.4375f + 0.125f * float(k >> 2);
-    base = fmaf(Q.dDx, spx, fmaf(Q.dDy, spy, R.dp0));
-    stepY = Q.dDy;
-    stepX = Q.dDx;
-  }
-  if (minX + rangeX > bX) {
-    // Wrapped first-block index: blocks fb + r*bX + c, each folded from the origin.
-    for (int r = 0; r < rangeY; ++r)
-      for (int c = 0; c < rangeX; ++c) {
-        const float s = fold_n(fold_n(base, stepY, r), stepX, c);
-        label_block(cov, V.depth, V.hiz, V.winner, uint32_t(fb + r * bX + c), s, 0, lane, kLane, Q, batch);
-      }
-    return;
-  }
-  float line = base;
-#pragma unroll
-  for (uint32_t i = 0; i < 4; ++i)
-    if (i < g) line = line + stepY;
-  for (int r = 0; r < rangeY; r += 5) {
-    float s = line;
-    for (int c = 0; c < rangeX; ++c, s = stepX + s)
-      for (int gg = 0; gg < 5 && r + gg < rangeY; ++gg)
-        label_block(cov, V.depth, V.hiz, V.winner, uint32_t(fb + (r + gg) * bX + c), s, gg, lane, kLane, Q, batch);
-    line = fold_n(line, stepY, 5);
-  }
-}
-
-// One wave per drawn quad; work-groups loop over the quads so that each
-// stages the coverage table once.
-__global__ void __launch_bounds__(256) rz_label_kernel(LearnParams p) {
-  __shared__ uint64_t cov[4096];
-  for (uint32_t i = threadIdx.x; i < 4096; i += 256) cov[i] = p.cov[i];
-  __syncthreads();
-  const uint32_t lane = threadIdx.x & 63u;
-  const uint32_t waves = gridDim.x * 4u;
-  for (uint32_t g = blockIdx.x * 4u + (threadIdx.x >> 6); g < p.nQuads; g += waves) {
-    const QuadRec R = p.recs[g];
-    if ((R.range & 0xFFFFu) == 0u || (R.range >> 16) == 0u) continue;
-    const uint32_t item = item_of_quad(p.items, p.nItems, g);
-    const ViewDesc& V = p.views[p.items[item].view];
-    label_quad_wave(R, cov, V, g >> 3, lane);
-  }
-}
-
-// Per block (blockIdx.y = view): each distinct batch kept by a pixel of the
-// block gets one count; the winner words are reset for the next step.
-__global__ void __launch_bounds__(256) rz_label_count_kernel(LearnParams p) {
-  const ViewDesc& V = p.views[blockIdx.y];
-  const uint32_t lane = threadIdx.x & 63u;
-  const uint32_t blk = blockIdx.x * 4u + (threadIdx.x >> 6);
-  if (blk >= V.bX * V.bY) return;
-  uint32_t* wp = &V.winner[size_t(blk) * 64 + lane];
-  const uint32_t w = *wp;
-  bool active = w != 0xFFFFFFFFu;
-  while (true) {
-    const uint64_t m = __ballot(active);
-    if (m == 0ull) break;
-    const uint32_t v = uint32_t(__builtin_amdgcn_readlane(int(w), int(__builtin_ctzll(m))));
-    if (active && w == v) active = false;
-    if (lane == 0) atomicAdd(&p.labels[v], 1u);
-  }
-  *wp = 0xFFFFFFFFu;
-}
-
 // 16 features per batch, from its quads' setup records and packets and the
 // labels of the previous step:
 //  0 drawn quads / 8          1 log2(1 + union block-box area)
@@ -1863,6 +1778,26 @@ __global__ void __launch_bounds__(256) rz_batch_items_kernel(LearnParams p, cons
   outState[k] = S;
 }
 
+__global__ void __launch_bounds__(256) rz_item_scores_kernel(LearnParams p, const float* __restrict__ batchScores,
+                                                              float* __restrict__ itemScores) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= p.nItems) return;
+  const ItemDesc& D = p.items[i];
+  float best = -3.402823466e+38f;
+  for (uint32_t b = D.quadBase / 8u; b < (D.quadBase + D.quadCount) / 8u; ++b) best = fmaxf(best, batchScores[b]);
+  itemScores[i] = best;
+}
+
+__global__ void __launch_bounds__(256) rz_item_reorder_kernel(LearnParams p, const uint32_t* __restrict__ order,
+                                                               ItemDesc* __restrict__ outItems,
+                                                               ItemState* __restrict__ outState) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= p.nItems) return;
+  const uint32_t i = order[k];
+  outItems[k] = p.items[i];
+  outState[k] = p.state[i];
+}
+
 __global__ void rz_fill_u32_kernel(uint32_t* p, uint32_t n, uint32_t v) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
@@ -1967,14 +1902,18 @@ hipError_t launch_fill_u16(uint16_t* p, uint32_t n, uint16_t v, hipStream_t stre
 }
 
 template <int NW>
-static const void* views_fn(bool stats) {
-  return stats ? reinterpret_cast<const void*>(&rz_views_kernel<8, 8, NW, true>)
-               : reinterpret_cast<const void*>(&rz_views_kernel<8, 8, NW, false>);
+static const void* views_fn(bool stats, bool labels) {
+  if (labels)
+    return stats ? reinterpret_cast<const void*>(&rz_views_kernel<8, 8, NW, true, true>)
+                 : reinterpret_cast<const void*>(&rz_views_kernel<8, 8, NW, false, true>);
+  return stats ? reinterpret_cast<const void*>(&rz_views_kernel<8, 8, NW, true, false>)
+               : reinterpret_cast<const void*>(&rz_views_kernel<8, 8, NW, false, false>);
 }
 
 hipError_t launch_views(const ViewsParams& p, uint32_t nGroups, hipStream_t stream) {
   if (p.nUnits == 0 || nGroups == 0) return hipSuccess;
-  const void* fn = p.waves == 8 ? views_fn<8>(p.stats != nullptr) : views_fn<4>(p.stats != nullptr);
+  const bool st = p.stats != nullptr, lab = p.labels != nullptr;
+  const void* fn = p.waves == 8 ? views_fn<8>(st, lab) : views_fn<4>(st, lab);
   ViewsParams local = p;
   void* args[] = {&local};
   return hipLaunchKernel(fn, dim3(nGroups), dim3(64 * (p.waves == 8 ? 8 : 4)), args, 0, stream);
@@ -1987,9 +1926,10 @@ hipError_t views_max_groups(int waves, int* maxGroups) {
   e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
   if (e != hipSuccess) return e;
   int perCU = 1 << 30;
-  for (bool stats : {false, true}) {
+  for (int variant = 0; variant < 4; ++variant) {
     int n = 0;
-    const void* fn = waves == 8 ? views_fn<8>(stats) : views_fn<4>(stats);
+    const bool stats = variant & 1, labels = variant & 2;
+    const void* fn = waves == 8 ? views_fn<8>(stats, labels) : views_fn<4>(stats, labels);
     e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fn, 64 * (waves == 8 ? 8 : 4), 0);
     if (e != hipSuccess) return e;
     perCU = n < perCU ? n : perCU;
@@ -2004,15 +1944,6 @@ hipError_t launch_views_query(const ViewsQueryParams& p, hipStream_t stream) {
   return hipGetLastError();
 }
 
-hipError_t launch_labels(const LearnParams& p, uint32_t maxBlocks, hipStream_t stream) {
-  if (p.nQuads == 0 || p.nItems == 0) return hipSuccess;
-  hipLaunchKernelGGL(rz_label_kernel, dim3(std::min<uint32_t>(groups(p.nQuads, 4), 4096)), dim3(256), 0, stream, p);
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(rz_label_count_kernel, dim3(groups(maxBlocks, 4), p.nViews), dim3(256), 0, stream, p);
-  return hipGetLastError();
-}
-
 hipError_t launch_features(const LearnParams& p, uint32_t nBatches, hipStream_t stream) {
   if (nBatches == 0 || p.nItems == 0) return hipSuccess;
   hipLaunchKernelGGL(rz_features_kernel, dim3(groups(nBatches, 256)), dim3(256), 0, stream, p, nBatches);
@@ -2024,6 +1955,22 @@ hipError_t launch_batch_items(const LearnParams& p, const uint32_t* order, uint3
   if (nBatches == 0 || p.nItems == 0) return hipSuccess;
   hipLaunchKernelGGL(rz_batch_items_kernel, dim3(groups(nBatches, 256)), dim3(256), 0, stream, p, order, nBatches,
                      outItems, outState);
+  return hipGetLastError();
+}
+
+hipError_t launch_item_scores(const LearnParams& p, const float* batchScores, float* itemScores,
+                              hipStream_t stream) {
+  if (p.nItems == 0) return hipSuccess;
+  hipLaunchKernelGGL(rz_item_scores_kernel, dim3(groups(p.nItems, 256)), dim3(256), 0, stream, p, batchScores,
+                     itemScores);
+  return hipGetLastError();
+}
+
+hipError_t launch_item_reorder(const LearnParams& p, const uint32_t* order, ItemDesc* outItems, ItemState* outState,
+                               hipStream_t stream) {
+  if (p.nItems == 0) return hipSuccess;
+  hipLaunchKernelGGL(rz_item_reorder_kernel, dim3(groups(p.nItems, 256)), dim3(256), 0, stream, p, order, outItems,
+                     outState);
   return hipGetLastError();
 }
 

@@ -80,6 +80,14 @@ struct rz_views {
   ItemState* d_bstate = nullptr;
   // learned order: scores of the batches by the model, sorted per view
   rz_mlp* model = nullptr;
+  bool byOccluder = false;  // learned order of whole occluders (by their best batch) instead of batches
+  float* d_itemScores = nullptr;
+  float* d_itemScoresSorted = nullptr;
+  uint32_t* d_itemIota = nullptr;
+  uint32_t* d_itemOrder = nullptr;
+  uint32_t* d_itemSegments = nullptr;
+  ItemDesc* d_ritems = nullptr;
+  ItemState* d_rstate = nullptr;
   float* d_scores = nullptr;
   float* d_sortedScores = nullptr;
   uint32_t* d_iota = nullptr;
@@ -118,6 +126,17 @@ static void free_plan(rz_views* v) {
   hipFree(v->d_order);
   hipFree(v->d_bitems);
   hipFree(v->d_bstate);
+  hipFree(v->d_itemScores);
+  hipFree(v->d_itemScoresSorted);
+  hipFree(v->d_itemIota);
+  hipFree(v->d_itemOrder);
+  hipFree(v->d_itemSegments);
+  hipFree(v->d_ritems);
+  hipFree(v->d_rstate);
+  v->d_itemScores = v->d_itemScoresSorted = nullptr;
+  v->d_itemIota = v->d_itemOrder = v->d_itemSegments = nullptr;
+  v->d_ritems = nullptr;
+  v->d_rstate = nullptr;
   hipFree(v->d_scores);
   hipFree(v->d_sortedScores);
   hipFree(v->d_iota);
@@ -352,8 +371,22 @@ static int build_plan(rz_views* v) {
   RZ_HIP(hipMemset(v->d_features, 0, nbt * kFeatures * sizeof(float)));
   // An explicit batch order in any view switches the step to batch items;
   // views without one keep their batches in occluder order.
-  v->batchOrder = v->model != nullptr;
+  v->batchOrder = v->model != nullptr && !v->byOccluder;
   for (const ViewHost& h : v->views) v->batchOrder |= !h.order.empty();
+  if (v->model && v->byOccluder) {
+    const size_t ni = std::max<size_t>(v->nItems, 1);
+    std::vector<uint32_t> iota(v->nItems), seg(nv + 1);
+    for (uint32_t i = 0; i < v->nItems; ++i) iota[i] = i;
+    for (uint32_t i = 0; i < nv; ++i) seg[i] = v->h_views[i].itemBase;
+    seg[nv] = v->nItems;
+    if (dev_alloc(&v->d_itemScores, ni) != hipSuccess || dev_alloc(&v->d_itemScoresSorted, ni) != hipSuccess ||
+        dev_alloc(&v->d_itemIota, ni) != hipSuccess || dev_alloc(&v->d_itemOrder, ni) != hipSuccess ||
+        dev_alloc(&v->d_itemSegments, nv + 1) != hipSuccess || dev_alloc(&v->d_ritems, ni) != hipSuccess ||
+        dev_alloc(&v->d_rstate, ni) != hipSuccess)
+      return RZ_ERR_OUT_OF_MEMORY;
+    if (v->nItems) RZ_HIP(hipMemcpy(v->d_itemIota, iota.data(), iota.size() * 4, hipMemcpyHostToDevice));
+    RZ_HIP(hipMemcpy(v->d_itemSegments, seg.data(), seg.size() * 4, hipMemcpyHostToDevice));
+  }
   if (v->model) {
     std::vector<uint32_t> iota(v->nBatches), seg(nv + 1);
     for (uint32_t b = 0; b < v->nBatches; ++b) iota[b] = b;
@@ -364,10 +397,16 @@ static int build_plan(rz_views* v) {
       return RZ_ERR_OUT_OF_MEMORY;
     if (v->nBatches) RZ_HIP(hipMemcpy(v->d_iota, iota.data(), iota.size() * 4, hipMemcpyHostToDevice));
     RZ_HIP(hipMemcpy(v->d_segments, seg.data(), seg.size() * 4, hipMemcpyHostToDevice));
-    size_t bytes = 0;
+    size_t bytes = 0, itemBytes = 0;
     RZ_HIP(hipcub::DeviceSegmentedRadixSort::SortPairsDescending(
         nullptr, bytes, v->d_scores, v->d_sortedScores, v->d_iota, v->d_iota, int(v->nBatches), int(nv),
         v->d_segments, v->d_segments + 1, 0, 32, v->stream));
+    if (v->byOccluder) {
+      RZ_HIP(hipcub::DeviceSegmentedRadixSort::SortPairsDescending(
+          nullptr, itemBytes, v->d_itemScores, v->d_itemScoresSorted, v->d_itemIota, v->d_itemOrder, int(v->nItems),
+          int(nv), v->d_itemSegments, v->d_itemSegments + 1, 0, 32, v->stream));
+      bytes = std::max(bytes, itemBytes);
+    }
     if (hipMalloc(&v->d_sortTemp, std::max<size_t>(bytes, 16)) != hipSuccess) return RZ_ERR_OUT_OF_MEMORY;
     v->sortTempBytes = bytes;
   }
@@ -451,14 +490,24 @@ static int enqueue_step(rz_views* v) {
       mp.scores = v->d_scores;
       RZ_HIP(launch_mlp_forward(mp, v->stream));
       size_t bytes = v->sortTempBytes;
-      RZ_HIP(hipcub::DeviceSegmentedRadixSort::SortPairsDescending(
-          v->d_sortTemp, bytes, v->d_scores, v->d_sortedScores, v->d_iota, v->d_order, int(v->nBatches), int(nv),
-          v->d_segments, v->d_segments + 1, 0, 32, v->stream));
+      if (v->byOccluder) {
+        // Whole occluders by their best batch score, stable within a view.
+        RZ_HIP(launch_item_scores(lp, v->d_scores, v->d_itemScores, v->stream));
+        RZ_HIP(hipcub::DeviceSegmentedRadixSort::SortPairsDescending(
+            v->d_sortTemp, bytes, v->d_itemScores, v->d_itemScoresSorted, v->d_itemIota, v->d_itemOrder,
+            int(v->nItems), int(nv), v->d_itemSegments, v->d_itemSegments + 1, 0, 32, v->stream));
+        RZ_HIP(launch_item_reorder(lp, v->d_itemOrder, v->d_ritems, v->d_rstate, v->stream));
+      } else {
+        RZ_HIP(hipcub::DeviceSegmentedRadixSort::SortPairsDescending(
+            v->d_sortTemp, bytes, v->d_scores, v->d_sortedScores, v->d_iota, v->d_order, int(v->nBatches), int(nv),
+            v->d_segments, v->d_segments + 1, 0, 32, v->stream));
+      }
     }
     if (v->batchOrder) RZ_HIP(launch_batch_items(lp, v->d_order, v->nBatches, v->d_bitems, v->d_bstate, v->stream));
     ViewsParams rp{};
-    rp.items = v->batchOrder ? v->d_bitems : v->d_items;
-    rp.state = v->batchOrder ? v->d_bstate : v->d_state;
+    const bool reordered = v->model && v->byOccluder && v->nItems;
+    rp.items = v->batchOrder ? v->d_bitems : (reordered ? v->d_ritems : v->d_items);
+    rp.state = v->batchOrder ? v->d_bstate : (reordered ? v->d_rstate : v->d_state);
     rp.batchItems = v->batchOrder ? 1u : 0u;
     rp.views = v->d_views;
     rp.nViews = nv;
@@ -470,11 +519,11 @@ static int enqueue_step(rz_views* v) {
     rp.waves = uint32_t(v->waves);
     rp.stats = v->stats ? v->d_stats : nullptr;
     if (v->stats) RZ_HIP(launch_fill_u32(reinterpret_cast<uint32_t*>(v->d_stats), 6, 0u, v->stream));
+    // Labels are counted by the draw itself (after this step's features
+    // have read the previous step's).
+    rp.labels = (v->labels || v->model) ? v->d_labels : nullptr;
+    if (rp.labels) RZ_HIP(launch_fill_u32(v->d_labels, v->nBatches, 0u, v->stream));
     RZ_HIP(launch_views(rp, v->nGroups, v->stream));
-    if (v->labels || v->model) {
-      RZ_HIP(launch_fill_u32(v->d_labels, v->nBatches, 0u, v->stream));
-      RZ_HIP(launch_labels(lp, v->maxBlocks, v->stream));
-    }
   }
   ViewsQueryParams qp{};
   qp.views = v->d_views;
@@ -578,7 +627,7 @@ extern "C" int rz_views_enable_labels(rz_views* v, int enable) {
   return RZ_OK;
 }
 
-extern "C" int rz_views_set_model(rz_views* v, rz_mlp* model) {
+extern "C" int rz_views_set_model(rz_views* v, rz_mlp* model, int byOccluder) {
   if (!v) return RZ_ERR_INVALID_ARGUMENT;
   int s = wait_last_step(v);
   if (s != RZ_OK) return s;
@@ -587,15 +636,21 @@ extern "C" int rz_views_set_model(rz_views* v, rz_mlp* model) {
     if (s != RZ_OK) return s;
   }
   v->model = model;
+  v->byOccluder = byOccluder != 0;
   v->dirty = true;
   return RZ_OK;
 }
 
 extern "C" int rz_views_read_order(rz_views* v, uint32_t view, uint32_t* order) {
-  if (!v || !order || view >= v->views.size() || v->dirty || !v->h_views || !v->batchOrder)
-    return RZ_ERR_INVALID_ARGUMENT;
+  if (!v || !order || view >= v->views.size() || v->dirty || !v->h_views) return RZ_ERR_INVALID_ARGUMENT;
   RZ_HIP(hipStreamSynchronize(v->stream));
   const ViewDesc& d = v->h_views[view];
+  if (v->model && v->byOccluder) {
+    if (d.nItems) RZ_HIP(hipMemcpy(order, v->d_itemOrder + d.itemBase, d.nItems * 4, hipMemcpyDeviceToHost));
+    for (uint32_t k = 0; k < d.nItems; ++k) order[k] -= d.itemBase;
+    return RZ_OK;
+  }
+  if (!v->batchOrder) return RZ_ERR_INVALID_ARGUMENT;
   if (d.nBatches) RZ_HIP(hipMemcpy(order, v->d_order + d.batchBase, d.nBatches * 4, hipMemcpyDeviceToHost));
   for (uint32_t k = 0; k < d.nBatches; ++k) order[k] -= d.batchBase;
   return RZ_OK;

@@ -63,25 +63,32 @@ def test_labels_match_oracle(oracle_mod, scene):
         V.set_mvp(k, mvp)
         V.set_occluders(k, occ)
     V.enable_labels(True)
-    V.run(graph=True)
-    V.sync()
-    for k, (w, h) in enumerate(SIZES):
-        mvp, occ = setup[k]
-        o = oracle_mod.OracleRasterizer(w, h)
-        o.clear()
-        o.setModelViewProjection(mvp)
-        st = o.draw_frame(occ, gated=False)
-        want = o.labels(occ, st)
-        got, _ = V.labels(k)
-        assert got.shape == want.shape
-        assert np.array_equal(got, want), f"view {k}: {int((got != want).sum())} labels differ"
-        # Every written block with a non-zero pixel has at least one winner.
-        d, hz = o.raw()
-        nb = (w // 8) * (h // 8)
-        live = (hz[:nb] != 1) & (d.reshape(nb, 64).max(axis=1) > 0)
-        assert want.sum() >= live.sum()
-        print(f"view {k}: {want.size} batches, {int((want > 0).sum())} labelled, {int(live.sum())} live blocks",
-              flush=True)
+    # Two steps with different cameras: the second step's draw starts from
+    # the first step's depth and winners, which its first writes replace.
+    second = _setup(scene, 11)
+    oracles = [oracle_mod.OracleRasterizer(w, h) for w, h in SIZES]
+    for step, cams in enumerate((setup, second)):
+        for k, (mvp, _) in enumerate(cams):
+            V.set_mvp(k, mvp)
+        V.run(graph=True)
+        V.sync()
+        for k, (w, h) in enumerate(SIZES):
+            mvp, occ = cams[k][0], setup[k][1]
+            o = oracles[k]
+            o.clear()
+            o.setModelViewProjection(mvp)
+            st, want = o.frame_labels(occ)
+            assert np.array_equal(V.status(k), st), f"step {step} view {k}: statuses"
+            got, _ = V.labels(k)
+            assert got.shape == want.shape
+            assert np.array_equal(got, want), f"step {step} view {k}: {int((got != want).sum())} labels differ"
+            # Every written block with a non-zero pixel has at least one winner.
+            d, hz = o.raw()
+            nb = (w // 8) * (h // 8)
+            live = (hz[:nb] != 1) & (d.reshape(nb, 64).max(axis=1) > 0)
+            assert want.sum() >= live.sum()
+            print(f"step {step} view {k}: {want.size} batches, {int((want > 0).sum())} labelled, "
+                  f"{int(live.sum())} live blocks", flush=True)
 
 
 def test_batch_order_matches_oracle(oracle_mod, scene, report):
@@ -254,6 +261,54 @@ def test_learned_order_is_score_order_and_exact(oracle_mod, scene, report):
         diff = np.abs(dg.astype(int) - d0.astype(int))[written]
         print(f"view {k}: learned vs occluder order {int((diff > 0).sum())} px differ, max "
               f"{int(diff.max()) if diff.size else 0} LSB", flush=True)
+
+
+def test_learned_occluder_order_is_score_order_and_exact(oracle_mod, scene, report):
+    """Ordering whole occluders: each occluder's score is its best batch's
+    score, occluders are drawn by descending score (ties in occluder order),
+    and the buffer equals the oracle drawing the occluders in that order; the
+    statuses stay in occluder order."""
+    from rasterizer_amd import Mlp
+
+    owner = Rasterizer(64, 64)
+    V = Views(owner, SIZES)
+    setup = _setup(scene, 5)
+    for k, (mvp, occ) in enumerate(setup):
+        V.set_mvp(k, mvp)
+        V.set_occluders(k, occ)
+    m = Mlp(seed=12)
+    V.set_model(m, by_occluder=True)
+    for _ in range(2):
+        V.run(graph=True)
+        V.sync()
+    for k, (w, h) in enumerate(SIZES):
+        mvp, occ = setup[k]
+        _, feats = V.labels(k)
+        scores = m.forward(feats)
+        nb = [int(o.packets.size) // 32 for o in occ]
+        bounds = np.concatenate([[0], np.cumsum(nb)]).astype(int)
+        best = np.array([scores[bounds[i]:bounds[i + 1]].max() if nb[i] else -np.float32(3.402823466e38)
+                         for i in range(len(occ))], np.float32)
+        want = np.lexsort((np.arange(best.size), -best))
+        got = V.order(k)
+        assert np.array_equal(got, want), f"view {k}: occluder order is not descending best-batch score"
+        o = oracle_mod.OracleRasterizer(w, h)
+        o.clear()
+        o.setModelViewProjection(mvp)
+        st_sorted = o.draw_frame([occ[i] for i in got], gated=False)
+        st = np.empty_like(st_sorted)
+        st[got] = st_sorted
+        assert np.array_equal(V.status(k), st), f"view {k}: statuses"
+        dg, hg = V.raw(k)
+        do, ho = o.raw()
+        written = np.repeat(ho[: (w // 8) * (h // 8)] != 1, 64)
+        assert np.array_equal(hg, ho), f"view {k}: hiZ {report(hg, ho)}"
+        assert np.array_equal(dg[written], do[written]), f"view {k}: depth {report(dg[written], do[written])}"
+    V.set_model(None)
+    V.run(graph=True)
+    V.sync()
+    with pytest.raises(Exception):
+        V.order(0)
 
 
 def test_trained_order_gains_rejects(scene):
