@@ -152,6 +152,12 @@ def views_bench(steps: int, warmup: int, seed: int = 3) -> dict:
         V.enable_stats(False)
         return {**st, "reject_rate": st["rejected"] / max(st["visited"], 1)}
 
+    # Launch configuration by the autotuner (identical bits under every
+    # candidate; untimed), on this workload's first matrices.
+    batched_step(0)
+    V.sync()
+    tuned = V.autotune(reps=3)
+    log(f"views autotune: {tuned}")
     # Occluder order: each view's subset front to back for its eye.
     t_b, gpu_b = time_batched("batched, occluder order")
     cnt_b = block_counters()
@@ -208,6 +214,7 @@ def views_bench(steps: int, warmup: int, seed: int = 3) -> dict:
         "speedup_vs_plain": (steps * total_quads / t_b) / plain_qps,
         "steps": steps,
         "plain_steps": plain_steps,
+        "launch_config": tuned,
         "learned_order": {
             "granularity": "occluder (best batch score)",
             "weights": "rasterizer_amd/order_mlp.npz",

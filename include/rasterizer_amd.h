@@ -37,6 +37,7 @@ extern "C" {
 #define RZ_ERR_TIMEOUT (-4)
 #define RZ_ERR_UNSUPPORTED_HOST (-5)
 #define RZ_ERR_TOO_LARGE (-6)
+#define RZ_ERR_NOT_IDENTICAL (-7)
 
 typedef struct rz_context rz_context;
 typedef struct rz_occluder rz_occluder;
@@ -170,6 +171,36 @@ int rz_profile_read(rz_context* ctx, double ms[4], uint32_t counts[2]);
 int rz_frame_stats_enable(rz_context* ctx, int enable);
 int rz_frame_stats_read(rz_context* ctx, uint64_t* out, uint32_t* count);
 
+/* ---------------------------------------------------------------------------
+ * Launch configuration and autotuning (no counterpart in the reference, whose
+ * raster is one CPU thread).  Every configuration computes identical bits;
+ * only time differs.
+ * ------------------------------------------------------------------------- */
+typedef struct rz_frame_config {
+  uint32_t tileW; /* tile edge in 8x8 blocks: 8 (64-pixel bins) or 16 (128-pixel bins) */
+  uint32_t waves; /* wave64s per tile work-group: 4 or 8 */
+  float ms;       /* rz_tune_frame: median time of one rz_draw_frame; -1 if the grid cannot be co-resident */
+} rz_frame_config;
+
+int rz_get_config(rz_context* ctx, rz_frame_config* config);
+/* RZ_ERR_TOO_LARGE (configuration unchanged) when the tile grid cannot be
+ * co-resident on the device. */
+int rz_set_config(rz_context* ctx, uint32_t tileW, uint32_t waves);
+/* Autotuner: for every configuration (tileW x waves) that fits, clears and
+ * draws the frame of occ (rz_draw_frame with the context's matrix), requires
+ * identical depth, hiZ and statuses from all of them (else
+ * RZ_ERR_NOT_IDENTICAL, configuration unchanged), and times `reps` draws
+ * (HIP events, median).  Applies the fastest and, with a non-NULL cachePath,
+ * records it there for this device and buffer size.  candidates (optional, 4
+ * entries) receives every configuration tried.  Leaves the drawn frame in
+ * the buffer.  Blocks. */
+int rz_tune_frame(rz_context* ctx, uint32_t n, const rz_occluder* const* occ, uint32_t reps, const char* cachePath,
+                  rz_frame_config candidates[4], uint32_t* nCandidates);
+/* Applies the configuration cachePath records for this device and buffer
+ * size; *found = 0 when it has none.  rz_create does this itself when the
+ * environment variable RZ_TUNE_CACHE names a cache file. */
+int rz_load_config(rz_context* ctx, const char* cachePath, int* found);
+
 /* Device memory for callers without an allocator of their own (e.g. the
  * inputs of rz_query_batch_device): rz_device_alloc allocates `bytes` on the
  * context's device and copies `init` into it when non-NULL; rz_device_read
@@ -218,6 +249,27 @@ int rz_views_read_status(rz_views* views, uint32_t view, uint8_t* status);
 /* info[0..7]: views, items, quads, (view, tile) units, queries, raster
  * work-groups, graph captured, plan out of date. */
 int rz_views_get_info(rz_views* views, uint32_t info[8]);
+
+/* Launch configuration of the multi-view raster kernel (no counterpart in the
+ * reference): wave64s per (view, tile) work-group, 4 or 8, and persistent
+ * work-groups per CU (0: as many as fit). */
+typedef struct rz_views_config {
+  uint32_t waves;
+  uint32_t groupsPerCU;
+  float ms; /* rz_views_tune: median step time; -1 if it does not fit */
+} rz_views_config;
+int rz_views_get_config(rz_views* views, rz_views_config* config);
+int rz_views_set_config(rz_views* views, uint32_t waves, uint32_t groupsPerCU);
+/* Autotuner over waves {4, 8} x groupsPerCU {1, all that fit}: runs the
+ * current step (matrices, occluders, queries, order) under each, requires
+ * identical buffers, query bits and labels from all (else
+ * RZ_ERR_NOT_IDENTICAL, configuration unchanged), times `reps` steps each,
+ * applies the fastest and records it in cachePath (if not NULL) for this
+ * device and set of view sizes.  Blocks. */
+int rz_views_tune(rz_views* views, uint32_t reps, const char* cachePath, rz_views_config candidates[4],
+                  uint32_t* nCandidates);
+/* As rz_load_config; rz_views_create does it itself with RZ_TUNE_CACHE. */
+int rz_views_load_config(rz_views* views, const char* cachePath, int* found);
 
 /* Draw-order learning (no counterpart in the reference).  With labels on,
  * every step also computes, per 8-quad batch (the k-th batch of a view is the

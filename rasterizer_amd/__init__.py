@@ -28,6 +28,7 @@ _STATUS = {
     -4: "device-side wait timed out",
     -5: "host CPU RCPPS has an unsupported structure",
     -6: "buffer too large for one co-resident frame launch",
+    -7: "launch configurations disagree (autotuner check)",
 }
 
 
@@ -117,7 +118,29 @@ _SIGNATURES = {
     "rz_views_read_order": ([_P, _U32, _P], ctypes.c_int),
     "rz_views_enable_stats": ([_P, ctypes.c_int], ctypes.c_int),
     "rz_views_read_stats": ([_P, _P], ctypes.c_int),
+    "rz_get_config": ([_P, _P], ctypes.c_int),
+    "rz_set_config": ([_P, _U32, _U32], ctypes.c_int),
+    "rz_tune_frame": ([_P, _U32, _P, _U32, ctypes.c_char_p, _P, _P], ctypes.c_int),
+    "rz_load_config": ([_P, ctypes.c_char_p, _P], ctypes.c_int),
+    "rz_views_get_config": ([_P, _P], ctypes.c_int),
+    "rz_views_set_config": ([_P, _U32, _U32], ctypes.c_int),
+    "rz_views_tune": ([_P, _U32, ctypes.c_char_p, _P, _P], ctypes.c_int),
+    "rz_views_load_config": ([_P, ctypes.c_char_p, _P], ctypes.c_int),
 }
+
+
+class FrameConfig(ctypes.Structure):
+    """rz_frame_config: tile edge in 8x8 blocks, wave64s per tile, tuned ms."""
+    _fields_ = [("tileW", ctypes.c_uint32), ("waves", ctypes.c_uint32), ("ms", ctypes.c_float)]
+
+
+class ViewsConfig(ctypes.Structure):
+    """rz_views_config: wave64s per work-group, work-groups per CU (0: all that fit), tuned ms."""
+    _fields_ = [("waves", ctypes.c_uint32), ("groupsPerCU", ctypes.c_uint32), ("ms", ctypes.c_float)]
+
+
+def _cfg_dict(c) -> dict:
+    return {name: (float(getattr(c, name)) if name == "ms" else int(getattr(c, name))) for name, _ in c._fields_}
 
 
 def lib() -> ctypes.CDLL:
@@ -324,6 +347,31 @@ class Rasterizer:
     def handle_array(self, occluders):
         return (ctypes.c_void_p * max(len(occluders), 1))(*[o.device(self._ctx).value for o in occluders])
 
+    def config(self) -> dict:
+        c = FrameConfig()
+        _check(lib().rz_get_config(self._ctx, ctypes.byref(c)), "rz_get_config")
+        return {"tileW": c.tileW, "waves": c.waves}
+
+    def set_config(self, tileW: int, waves: int) -> None:
+        _check(lib().rz_set_config(self._ctx, tileW, waves), "rz_set_config")
+
+    def autotune(self, occluders, reps: int = 5, cache: "str | os.PathLike | None" = None) -> dict:
+        """rz_tune_frame: times every frame-kernel configuration on this frame
+        (with identical bits required) and keeps the fastest."""
+        n = len(occluders)
+        handles = self.handle_array(occluders)
+        cands = (FrameConfig * 4)()
+        k = ctypes.c_uint32(0)
+        _check(lib().rz_tune_frame(self._ctx, n, ctypes.cast(handles, ctypes.c_void_p), reps,
+                                   os.fsencode(cache) if cache is not None else None, cands, ctypes.byref(k)),
+               "rz_tune_frame")
+        return {"chosen": self.config(), "candidates": [_cfg_dict(cands[i]) for i in range(k.value)]}
+
+    def load_config(self, cache) -> bool:
+        found = ctypes.c_int(0)
+        _check(lib().rz_load_config(self._ctx, os.fsencode(cache), ctypes.byref(found)), "rz_load_config")
+        return bool(found.value)
+
     def query_batch_device(self, n: int, d_bounds: "DeviceBuffer", d_visible: "DeviceBuffer",
                            d_clip: "DeviceBuffer | None" = None) -> None:
         _check(lib().rz_query_batch_device(self._ctx, n, d_bounds.ptr, d_visible.ptr,
@@ -508,6 +556,28 @@ class Views:
         out = np.zeros(max(n, 1), np.uint32)
         _check(lib().rz_views_read_order(self._v, view, _ptr(out)), "rz_views_read_order")
         return out[:n]
+
+    def config(self) -> dict:
+        c = ViewsConfig()
+        _check(lib().rz_views_get_config(self._v, ctypes.byref(c)), "rz_views_get_config")
+        return {"waves": c.waves, "groupsPerCU": c.groupsPerCU}
+
+    def set_config(self, waves: int, groups_per_cu: int = 0) -> None:
+        _check(lib().rz_views_set_config(self._v, waves, groups_per_cu), "rz_views_set_config")
+
+    def autotune(self, reps: int = 5, cache: "str | os.PathLike | None" = None) -> dict:
+        """rz_views_tune: times the current step under every configuration
+        (identical bits required) and keeps the fastest."""
+        cands = (ViewsConfig * 4)()
+        k = ctypes.c_uint32(0)
+        _check(lib().rz_views_tune(self._v, reps, os.fsencode(cache) if cache is not None else None, cands,
+                                   ctypes.byref(k)), "rz_views_tune")
+        return {"chosen": self.config(), "candidates": [_cfg_dict(cands[i]) for i in range(k.value)]}
+
+    def load_config(self, cache) -> bool:
+        found = ctypes.c_int(0)
+        _check(lib().rz_views_load_config(self._v, os.fsencode(cache), ctypes.byref(found)), "rz_views_load_config")
+        return bool(found.value)
 
     def info(self) -> dict:
         a = np.zeros(8, np.uint32)

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <mutex>
@@ -114,6 +115,7 @@ extern "C" const char* rz_status_string(int s) {
     case RZ_ERR_TIMEOUT: return "device-side wait timed out";
     case RZ_ERR_UNSUPPORTED_HOST: return "host CPU RCPPS has an unsupported structure";
     case RZ_ERR_TOO_LARGE: return "buffer too large for one co-resident frame launch";
+    case RZ_ERR_NOT_IDENTICAL: return "launch configurations disagree (autotuner check)";
     default: return "unknown status";
   }
 }
@@ -210,6 +212,10 @@ extern "C" int rz_create(uint32_t width, uint32_t height, rz_context** out) {
     if (hipEventCreateWithFlags(&c->ev_items[i], hipEventDisableTiming) != hipSuccess) return fail(RZ_ERR_DEVICE);
   if (hipEventCreate(&c->t0) != hipSuccess || hipEventCreate(&c->t1) != hipSuccess) return fail(RZ_ERR_DEVICE);
   if (const char* mode = std::getenv("RZ_FRAME_LAUNCH")) c->cooperative = std::strcmp(mode, "plain") != 0;
+  if (const char* cache = rzi::tune_cache_env()) {
+    int found = 0;
+    (void)rz_load_config(c, cache, &found);  // a stale or unusable entry leaves the default
+  }
   // Identity MVP until the caller sets one.
   const float ident[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
   rzh::prebake_mvp(width, height, ident, c->mvp.prebaked, c->mvp.raw);
@@ -759,5 +765,116 @@ extern "C" int rz_frame_stats_read(rz_context* ctx, uint64_t* out, uint32_t* cou
     if (s != RZ_OK) return s;
   }
   *count = tiles;
+  return RZ_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Launch configuration and autotuning.
+// ---------------------------------------------------------------------------
+static std::string frame_key(const rz_context* c) {
+  return rzi::device_key(c->device) + ",frame," + std::to_string(c->width) + "x" + std::to_string(c->height);
+}
+
+// Switches the frame kernel to (tileW, waves) if its tile grid can be
+// co-resident; reallocates the per-tile counters when the tile count changes.
+static int apply_config(rz_context* c, int tw, int waves) {
+  if ((tw != 8 && tw != 16) || (waves != 4 && waves != 8)) return RZ_ERR_INVALID_ARGUMENT;
+  int maxGroups = 0;
+  RZ_HIP(frame_max_groups(tw, waves, &maxGroups));
+  const int tx = int((c->bX + tw - 1) / tw), ty = int((c->bY + tw - 1) / tw);
+  if (tx * ty > maxGroups) return RZ_ERR_TOO_LARGE;
+  RZ_HIP(hipStreamSynchronize(c->stream));
+  const bool stats = c->d_frameStats != nullptr;
+  if (stats) {
+    const int s = rz_frame_stats_enable(c, 0);
+    if (s != RZ_OK) return s;
+  }
+  c->tileW = tw;
+  c->waves = waves;
+  c->tilesX = tx;
+  c->tilesY = ty;
+  return stats ? rz_frame_stats_enable(c, 1) : RZ_OK;
+}
+
+extern "C" int rz_get_config(rz_context* ctx, rz_frame_config* config) {
+  if (!ctx || !config) return RZ_ERR_INVALID_ARGUMENT;
+  config->tileW = uint32_t(ctx->tileW);
+  config->waves = uint32_t(ctx->waves);
+  config->ms = -1.0f;
+  return RZ_OK;
+}
+
+extern "C" int rz_set_config(rz_context* ctx, uint32_t tileW, uint32_t waves) {
+  if (!ctx) return RZ_ERR_INVALID_ARGUMENT;
+  return apply_config(ctx, int(tileW), int(waves));
+}
+
+extern "C" int rz_load_config(rz_context* ctx, const char* cachePath, int* found) {
+  if (!ctx || !cachePath || !found) return RZ_ERR_INVALID_ARGUMENT;
+  *found = 0;
+  std::string v;
+  if (!rzi::tune_lookup(cachePath, frame_key(ctx), v)) return RZ_OK;
+  int tw = 0, waves = 0;
+  if (std::sscanf(v.c_str(), "tileW=%d waves=%d", &tw, &waves) != 2) return RZ_OK;
+  const int s = apply_config(ctx, tw, waves);
+  if (s == RZ_OK) *found = 1;
+  return (s == RZ_ERR_TOO_LARGE || s == RZ_ERR_INVALID_ARGUMENT) ? RZ_OK : s;
+}
+
+extern "C" int rz_tune_frame(rz_context* ctx, uint32_t n, const rz_occluder* const* occ, uint32_t reps,
+                             const char* cachePath, rz_frame_config candidates[4], uint32_t* nCandidates) {
+  if (!ctx || (n && !occ) || reps == 0) return RZ_ERR_INVALID_ARGUMENT;
+  rz_context* c = ctx;
+  const int oldTw = c->tileW, oldWaves = c->waves;
+  auto restore = [&](int code) {
+    (void)apply_config(c, oldTw, oldWaves);
+    return code;
+  };
+  const size_t nb = size_t(c->bX) * c->bY;
+  std::vector<uint16_t> d(nb * 64), h(nb + 8), refD, refH;
+  std::vector<uint8_t> st(std::max<uint32_t>(n, 1)), refS;
+  rz_frame_config all[4];
+  uint32_t k = 0;
+  int best = -1;
+  for (int tw : {8, 16})
+    for (int waves : {4, 8}) {
+      rz_frame_config& cf = all[k];
+      cf.tileW = uint32_t(tw);
+      cf.waves = uint32_t(waves);
+      cf.ms = -1.0f;
+      const uint32_t idx = k++;
+      int s = apply_config(c, tw, waves);
+      if (s == RZ_ERR_TOO_LARGE) continue;
+      if (s != RZ_OK) return restore(s);
+      if ((s = rz_clear(c)) != RZ_OK || (s = rz_draw_frame(c, n, occ, st.data())) != RZ_OK ||
+          (s = rz_sync(c)) != RZ_OK || (s = rz_readback_raw(c, d.data(), h.data())) != RZ_OK)
+        return restore(s);
+      if (refD.empty()) {
+        refD = d;
+        refH = h;
+        refS = st;
+      } else if (d != refD || h != refH || st != refS) {
+        return restore(RZ_ERR_NOT_IDENTICAL);
+      }
+      std::vector<float> t(reps);
+      for (uint32_t r = 0; r < reps; ++r) {
+        if ((s = rz_clear(c)) != RZ_OK || (s = rz_timer_start(c)) != RZ_OK ||
+            (s = rz_draw_frame(c, n, occ, nullptr)) != RZ_OK || (s = rz_timer_stop(c, &t[r])) != RZ_OK)
+          return restore(s);
+      }
+      std::sort(t.begin(), t.end());
+      cf.ms = t[reps / 2];
+      if (best < 0 || cf.ms < all[best].ms) best = int(idx);
+    }
+  if (best < 0) return restore(RZ_ERR_TOO_LARGE);
+  int s = apply_config(c, int(all[best].tileW), int(all[best].waves));
+  if (s != RZ_OK) return restore(s);
+  if (cachePath) {
+    const std::string v = "tileW=" + std::to_string(all[best].tileW) + " waves=" + std::to_string(all[best].waves);
+    if (!rzi::tune_store(cachePath, frame_key(c), v)) return RZ_ERR_INVALID_ARGUMENT;
+  }
+  if (candidates)
+    for (uint32_t i = 0; i < k; ++i) candidates[i] = all[i];
+  if (nCandidates) *nCandidates = k;
   return RZ_OK;
 }

@@ -4,6 +4,9 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <cstdlib>
+#include <string>
+
 #include "rasterizer_amd.h"
 
 // A baked occluder's device copy: Occluder::m_vertexData (Occluder.h:17-18)
@@ -42,5 +45,15 @@ struct HostTables {
   uint32_t rcp[4096];
 };
 const HostTables& host_tables();
+
+// Autotuning cache (rz_tune.cpp): "<device>,<shape>" -> configuration text.
+std::string device_key(int device);
+bool tune_lookup(const char* path, const std::string& key, std::string& value);
+bool tune_store(const char* path, const std::string& key, const std::string& value);
+// Cache path from the environment (RZ_TUNE_CACHE), or null.
+inline const char* tune_cache_env() {
+  const char* p = std::getenv("RZ_TUNE_CACHE");
+  return (p && *p) ? p : nullptr;
+}
 
 }  // namespace rzi

@@ -14,7 +14,9 @@
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
+#include <cstdio>
 #include <cstring>
+#include <string>
 #include <new>
 #include <vector>
 
@@ -95,6 +97,7 @@ struct rz_views {
   void* d_sortTemp = nullptr;
   size_t sortTempBytes = 0;
   int waves = 4;  // wave64s per raster work-group
+  int groupsPerCU = 0;  // persistent raster work-groups per CU (0: as many as fit)
   unsigned long long* d_stats = nullptr;  // block-loop counters of the last step, when on
   bool stats = false;
   hipGraph_t graph = nullptr;
@@ -232,6 +235,10 @@ extern "C" int rz_views_create(uint32_t nViews, const uint32_t* widths, const ui
       return fail(RZ_ERR_DEVICE);
     rzh::prebake_mvp(h.width, h.height, ident, h.mvp.prebaked, h.mvp.raw);
   }
+  if (const char* cache = rzi::tune_cache_env()) {
+    int found = 0;
+    (void)rz_views_load_config(v, cache, &found);  // a stale or unusable entry leaves the default
+  }
   *out = v;
   return RZ_OK;
 }
@@ -354,6 +361,12 @@ static int build_plan(rz_views* v) {
   v->maxClear = maxClear;
   int maxGroups = 0;
   RZ_HIP(views_max_groups(v->waves, &maxGroups));
+  if (v->groupsPerCU > 0) {
+    int dev = 0, cus = 0;
+    RZ_HIP(hipGetDevice(&dev));
+    RZ_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    maxGroups = std::min(maxGroups, v->groupsPerCU * cus);
+  }
   v->nGroups = std::min<uint32_t>(units, uint32_t(std::max(maxGroups, 1)));
   const size_t ni = std::max<size_t>(items.size(), 1);
   if (dev_alloc(&v->d_items, ni) != hipSuccess || dev_alloc(&v->d_state, ni) != hipSuccess ||
@@ -705,5 +718,131 @@ extern "C" int rz_views_get_info(rz_views* v, uint32_t info[8]) {
   info[5] = v->nGroups;
   info[6] = v->exec ? 1u : 0u;
   info[7] = v->dirty ? 1u : 0u;
+  return RZ_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Launch configuration and autotuning.
+// ---------------------------------------------------------------------------
+static std::string views_key(const rz_views* v) {
+  uint64_t px = 0;
+  for (const ViewHost& h : v->views) px += uint64_t(h.width) * h.height;
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  return rzi::device_key(dev) + ",views," + std::to_string(v->views.size()) + "," + std::to_string(px);
+}
+
+extern "C" int rz_views_get_config(rz_views* v, rz_views_config* config) {
+  if (!v || !config) return RZ_ERR_INVALID_ARGUMENT;
+  config->waves = uint32_t(v->waves);
+  config->groupsPerCU = uint32_t(v->groupsPerCU);
+  config->ms = -1.0f;
+  return RZ_OK;
+}
+
+extern "C" int rz_views_set_config(rz_views* v, uint32_t waves, uint32_t groupsPerCU) {
+  if (!v || (waves != 4 && waves != 8) || groupsPerCU > 64) return RZ_ERR_INVALID_ARGUMENT;
+  int s = wait_last_step(v);
+  if (s != RZ_OK) return s;
+  v->waves = int(waves);
+  v->groupsPerCU = int(groupsPerCU);
+  v->dirty = true;
+  return RZ_OK;
+}
+
+extern "C" int rz_views_load_config(rz_views* v, const char* cachePath, int* found) {
+  if (!v || !cachePath || !found) return RZ_ERR_INVALID_ARGUMENT;
+  *found = 0;
+  std::string val;
+  if (!rzi::tune_lookup(cachePath, views_key(v), val)) return RZ_OK;
+  unsigned waves = 0, gpc = 0;
+  if (std::sscanf(val.c_str(), "waves=%u groupsPerCU=%u", &waves, &gpc) != 2) return RZ_OK;
+  const int s = rz_views_set_config(v, waves, gpc);
+  if (s == RZ_OK) *found = 1;
+  return s == RZ_ERR_INVALID_ARGUMENT ? RZ_OK : s;
+}
+
+// Everything a step produces, for the autotuner's identity check.
+static int step_outputs(rz_views* v, std::vector<uint8_t>& out) {
+  out.clear();
+  for (uint32_t i = 0; i < v->views.size(); ++i) {
+    const ViewHost& h = v->views[i];
+    const size_t nb = size_t(h.bX) * h.bY;
+    const size_t at = out.size();
+    out.resize(at + (nb * 64 + nb + 8) * 2 + 2 * size_t(h.nQueries));
+    uint16_t* d = reinterpret_cast<uint16_t*>(out.data() + at);
+    int s = rz_views_read_raw(v, i, d, d + nb * 64);
+    if (s == RZ_OK && h.nQueries) {
+      uint8_t* q = out.data() + at + (nb * 64 + nb + 8) * 2;
+      s = rz_views_read_queries(v, i, q, q + h.nQueries);
+    }
+    if (s != RZ_OK) return s;
+    if (v->labels && v->h_views && v->h_views[i].nBatches) {
+      const size_t at2 = out.size();
+      out.resize(at2 + size_t(v->h_views[i].nBatches) * 4);
+      s = rz_views_read_labels(v, i, reinterpret_cast<uint32_t*>(out.data() + at2), nullptr);
+      if (s != RZ_OK) return s;
+    }
+  }
+  return RZ_OK;
+}
+
+extern "C" int rz_views_tune(rz_views* v, uint32_t reps, const char* cachePath, rz_views_config candidates[4],
+                             uint32_t* nCandidates) {
+  if (!v || reps == 0) return RZ_ERR_INVALID_ARGUMENT;
+  const int oldWaves = v->waves, oldGpc = v->groupsPerCU;
+  auto restore = [&](int code) {
+    (void)rz_views_set_config(v, uint32_t(oldWaves), uint32_t(oldGpc));
+    return code;
+  };
+  std::vector<uint8_t> ref, got;
+  rz_views_config all[4];
+  uint32_t k = 0;
+  int best = -1;
+  for (uint32_t waves : {4u, 8u}) {
+    int maxGroups = 0, dev = 0, cus = 1;
+    RZ_HIP(views_max_groups(int(waves), &maxGroups));
+    RZ_HIP(hipGetDevice(&dev));
+    RZ_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    const uint32_t perCU = uint32_t(std::max(maxGroups / std::max(cus, 1), 1));
+    for (uint32_t gpc : {1u, 0u}) {
+      rz_views_config& cf = all[k];
+      cf.waves = waves;
+      cf.groupsPerCU = gpc;
+      cf.ms = -1.0f;
+      const uint32_t idx = k++;
+      if (gpc == 0 && perCU == 1) continue;  // same launch as one group per CU
+      // Each step is preceded by one with the same inputs, so that every
+      // configuration's step starts from the same buffers (and labels).
+      int s = rz_views_set_config(v, waves, gpc);
+      float ms = 0.0f;
+      for (int warm = 0; warm < 2 && s == RZ_OK; ++warm) {
+        s = rz_views_run(v, 0);
+        if (s == RZ_OK) s = rz_views_sync(v, &ms);
+      }
+      if (s == RZ_OK) s = step_outputs(v, got);
+      if (s != RZ_OK) return restore(s);
+      if (ref.empty()) ref.swap(got);
+      else if (got != ref) return restore(RZ_ERR_NOT_IDENTICAL);
+      std::vector<float> t(reps);
+      for (uint32_t r = 0; r < reps; ++r) {
+        if ((s = rz_views_run(v, 1)) != RZ_OK || (s = rz_views_sync(v, &t[r])) != RZ_OK) return restore(s);
+      }
+      std::sort(t.begin(), t.end());
+      cf.ms = t[reps / 2];
+      if (best < 0 || cf.ms < all[best].ms) best = int(idx);
+    }
+  }
+  if (best < 0) return restore(RZ_ERR_DEVICE);
+  int s = rz_views_set_config(v, all[best].waves, all[best].groupsPerCU);
+  if (s != RZ_OK) return restore(s);
+  if (cachePath) {
+    const std::string val =
+        "waves=" + std::to_string(all[best].waves) + " groupsPerCU=" + std::to_string(all[best].groupsPerCU);
+    if (!rzi::tune_store(cachePath, views_key(v), val)) return RZ_ERR_INVALID_ARGUMENT;
+  }
+  if (candidates)
+    for (uint32_t i = 0; i < k; ++i) candidates[i] = all[i];
+  if (nCandidates) *nCandidates = k;
   return RZ_OK;
 }

@@ -63,7 +63,7 @@ def test_dropin_classes_exported():
 
 
 def test_status_strings(rzlib):
-    for code in (0, -1, -2, -3, -4, -5, -6):
+    for code in (0, -1, -2, -3, -4, -5, -6, -7):
         s = rzlib.rz_status_string(code)
         assert s and len(s) >= 2
     assert rzlib.rz_status_string(12345)
@@ -76,6 +76,14 @@ def test_null_arguments_rejected(rzlib):
     assert rzlib.rz_rasterize(None, None, 0) == -1
     assert rzlib.rz_bake(None, 0, None, None, None, None, None, None) == -1
     assert rzlib.rz_create(12, 8, None) == -1
+    assert rzlib.rz_get_config(None, None) == -1
+    assert rzlib.rz_set_config(None, 8, 8) == -1
+    assert rzlib.rz_tune_frame(None, 0, None, 1, None, None, None) == -1
+    assert rzlib.rz_load_config(None, None, None) == -1
+    assert rzlib.rz_views_get_config(None, None) == -1
+    assert rzlib.rz_views_set_config(None, 4, 0) == -1
+    assert rzlib.rz_views_tune(None, 1, None, None, None) == -1
+    assert rzlib.rz_views_load_config(None, None, None) == -1
 
 
 def _rand_quads(rng, n):
