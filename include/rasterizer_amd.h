@@ -246,6 +246,27 @@ int rz_views_sync(rz_views* views, float* ms);
 int rz_views_read_raw(rz_views* views, uint32_t view, uint16_t* depth, uint16_t* hiz);
 int rz_views_read_queries(rz_views* views, uint32_t view, uint8_t* visible, uint8_t* needsClipping);
 int rz_views_read_status(rz_views* views, uint32_t view, uint8_t* status);
+/* Switches every view to the reference's query-gated draw loop (Main.cpp:
+ * 192-206): per occluder in the view's order, queryVisibility (Rasterizer.cpp:
+ * 123-281) against the view's buffer as drawn so far, then
+ * rasterize<needsClipping> (:606-1295) if visible; statuses as rz_draw_frame.
+ * Each view equals rz_draw_frame on a Rasterizer whose buffer holds what the
+ * view's buffer held (clear leaves depth stale, and query2D reads stale depth
+ * in partly covered cleared blocks, Rasterizer.cpp:283-349).  Tiles of a view wait for
+ * each other's query answers, so a step needs every view's tiles (8x8-block
+ * tiles) to fit co-resident (else run returns RZ_ERR_TOO_LARGE), is launched
+ * cooperatively and is not captured into a graph; it cannot be combined with
+ * a batch order or a model (RZ_ERR_INVALID_ARGUMENT).  0 restores
+ * classify-then-draw. */
+int rz_views_set_query_gated(rz_views* views, int gated);
+
+/* Phase timing with HIP events on the views stream (steps then run without
+ * graph capture while it is on): rz_views_profile_read blocks and returns the
+ * milliseconds of the last step's clear + prep + setup (+ learned-order work),
+ * of the raster launch alone, and of the query launch. */
+int rz_views_profile_enable(rz_views* views, int enable);
+int rz_views_profile_read(rz_views* views, float ms[3]);
+
 /* info[0..7]: views, items, quads, (view, tile) units, queries, raster
  * work-groups, graph captured, plan out of date. */
 int rz_views_get_info(rz_views* views, uint32_t info[8]);
@@ -263,7 +284,8 @@ int rz_views_set_config(rz_views* views, uint32_t waves, uint32_t groupsPerCU);
 /* Autotuner over waves {4, 8} x groupsPerCU {1, all that fit}: runs the
  * current step (matrices, occluders, queries, order) under each, requires
  * identical buffers, query bits and labels from all (else
- * RZ_ERR_NOT_IDENTICAL, configuration unchanged), times `reps` steps each,
+ * RZ_ERR_NOT_IDENTICAL, configuration unchanged; configurations whose grid
+ * cannot hold a query-gated view's tiles are skipped), times `reps` steps each,
  * applies the fastest and records it in cachePath (if not NULL) for this
  * device and set of view sizes.  Blocks. */
 int rz_views_tune(rz_views* views, uint32_t reps, const char* cachePath, rz_views_config candidates[4],

@@ -126,6 +126,9 @@ _SIGNATURES = {
     "rz_views_set_config": ([_P, _U32, _U32], ctypes.c_int),
     "rz_views_tune": ([_P, _U32, ctypes.c_char_p, _P, _P], ctypes.c_int),
     "rz_views_load_config": ([_P, ctypes.c_char_p, _P], ctypes.c_int),
+    "rz_views_set_query_gated": ([_P, ctypes.c_int], ctypes.c_int),
+    "rz_views_profile_enable": ([_P, ctypes.c_int], ctypes.c_int),
+    "rz_views_profile_read": ([_P, _P], ctypes.c_int),
 }
 
 
@@ -556,6 +559,20 @@ class Views:
         out = np.zeros(max(n, 1), np.uint32)
         _check(lib().rz_views_read_order(self._v, view, _ptr(out)), "rz_views_read_order")
         return out[:n]
+
+    def profile(self, enable: bool = True) -> None:
+        _check(lib().rz_views_profile_enable(self._v, 1 if enable else 0), "rz_views_profile_enable")
+
+    def profile_read(self) -> dict:
+        """Milliseconds of the last step's phases: setup (clear, prep, setup
+        and any ordering work), raster (the persistent launch) and queries."""
+        a = np.zeros(3, np.float32)
+        _check(lib().rz_views_profile_read(self._v, _ptr(a)), "rz_views_profile_read")
+        return {"setup_ms": float(a[0]), "raster_ms": float(a[1]), "query_ms": float(a[2])}
+
+    def set_query_gated(self, gated: bool = True) -> None:
+        """Every view runs the reference's query-gated draw loop (as rz_draw_frame)."""
+        _check(lib().rz_views_set_query_gated(self._v, 1 if gated else 0), "rz_views_set_query_gated")
 
     def config(self) -> dict:
         c = ViewsConfig()

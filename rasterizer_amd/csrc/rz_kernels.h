@@ -37,7 +37,8 @@ struct SetupParams {
 };
 
 // Persistent multi-view raster launch: work units are (view, tile) pairs taken
-// from an atomic queue in view order; no unit waits on another.
+// from an atomic queue in view order; units wait only on tiles of their own
+// view, and only for query-gated items.
 struct ViewsParams {
   const ItemDesc* items;
   const ItemState* state;
@@ -58,6 +59,12 @@ struct ViewsParams {
   // each batch wins a pixel (zeroed by the caller), with each view's
   // per-pixel winners in views[v].winner.
   uint32_t* labels;
+  // Query-gated items (kItemQuery): per-item done words (query partials and
+  // visible tiles), the error word and the bounded wait, as the frame
+  // kernel's.  Null done: no gated items.
+  uint32_t* done;
+  uint32_t* err;
+  uint64_t timeoutTicks;
 };
 
 // queryVisibility over the queries of every view of a multi-view launch.
@@ -147,7 +154,7 @@ hipError_t launch_readback(const uint16_t* depth, const uint16_t* hiz, uint32_t 
 hipError_t launch_fill_u16(uint16_t* p, uint32_t n, uint16_t v, hipStream_t stream);
 // Multi-view: grid of `groups` persistent work-groups over p.nUnits units of
 // 8x8-block tiles.
-hipError_t launch_views(const ViewsParams& p, uint32_t groups, hipStream_t stream);
+hipError_t launch_views(const ViewsParams& p, uint32_t groups, hipStream_t stream, bool cooperative = false);
 hipError_t views_max_groups(int waves, int* maxGroups);
 hipError_t launch_views_query(const ViewsQueryParams& p, hipStream_t stream);
 // hiZ := 1 (whole 8-entry groups) and tile min := 1 for every view.

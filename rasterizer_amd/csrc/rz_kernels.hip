@@ -1276,7 +1276,10 @@ __device__ __forceinline__ void process_tile(const TileJob& J, TileSmem<TBX, TBY
           while (true) {
             d = __hip_atomic_load(&J.done[item], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
             if ((d & kDoneCountMask) >= L.expected || d >= kDoneVisibleOne) break;
-            if (__builtin_amdgcn_s_memrealtime() - t0 > J.timeoutTicks) {
+            // After one wait of the launch has timed out, the others stop
+            // waiting too (the launch's result is reported as an error).
+            if (__builtin_amdgcn_s_memrealtime() - t0 > J.timeoutTicks ||
+                __hip_atomic_load(J.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) {
               __hip_atomic_fetch_or(J.err, uint32_t(kErrTimeout), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
               break;
             }
@@ -1458,11 +1461,16 @@ __global__ void __launch_bounds__(64 * NW) rz_frame_kernel(FrameParams p) {
 }
 
 // Multi-view persistent raster: each work-group takes (view, tile) units from
-// p.queue until the queue is exhausted.  Units are handed out in view order
-// and no unit waits on another, so every work-group exits after at most
-// nUnits queue reads.
+// p.queue until the queue is exhausted.  Units are handed out in view order.
+// Classified items never wait.  Query-gated items (p.done) wait only on
+// other tiles of the same view; when every view's tiles fit in the grid and
+// the grid is co-resident (the host checks both), the earliest unfinished
+// view always has all its tiles running, so every wait ends (and each is
+// bounded by p.timeoutTicks in any case).
+// Two work-groups per CU (the LDS allows two): with eight waves each, every
+// SIMD must hold four waves, i.e. at most 128 VGPRs per lane.
 template <int TBX, int TBY, int NW, bool STATS, bool LABELS>
-__global__ void __launch_bounds__(64 * NW) rz_views_kernel(ViewsParams p) {
+__global__ void __launch_bounds__(64 * NW, NW == 8 ? 4 : 2) rz_views_kernel(ViewsParams p) {
   __shared__ TileSmem<TBX, TBY> sm;
   __shared__ uint32_t unitSh;
   __shared__ uint64_t ctr[kFrameStatWords];
@@ -1497,9 +1505,9 @@ __global__ void __launch_bounds__(64 * NW) rz_views_kernel(ViewsParams p) {
     J.hiz = V.hiz;
     J.tileMin = V.tileMin;
     J.pyrX = V.pyrX;
-    J.done = nullptr;
-    J.err = nullptr;
-    J.timeoutTicks = 0;
+    J.done = p.done;
+    J.err = p.err;
+    J.timeoutTicks = p.timeoutTicks;
     J.winner = V.winner;
     J.labels = p.labels;
     process_tile<TBX, TBY, NW, STATS, LABELS>(J, sm, p.cov, covLoaded, ctr);
@@ -1910,13 +1918,15 @@ static const void* views_fn(bool stats, bool labels) {
                : reinterpret_cast<const void*>(&rz_views_kernel<8, 8, NW, false, false>);
 }
 
-hipError_t launch_views(const ViewsParams& p, uint32_t nGroups, hipStream_t stream) {
+hipError_t launch_views(const ViewsParams& p, uint32_t nGroups, hipStream_t stream, bool cooperative) {
   if (p.nUnits == 0 || nGroups == 0) return hipSuccess;
   const bool st = p.stats != nullptr, lab = p.labels != nullptr;
   const void* fn = p.waves == 8 ? views_fn<8>(st, lab) : views_fn<4>(st, lab);
   ViewsParams local = p;
   void* args[] = {&local};
-  return hipLaunchKernel(fn, dim3(nGroups), dim3(64 * (p.waves == 8 ? 8 : 4)), args, 0, stream);
+  const dim3 grid(nGroups), block(64 * (p.waves == 8 ? 8 : 4));
+  if (cooperative) return hipLaunchCooperativeKernel(fn, grid, block, args, 0, stream);
+  return hipLaunchKernel(fn, grid, block, args, 0, stream);
 }
 
 hipError_t views_max_groups(int waves, int* maxGroups) {

@@ -25,6 +25,10 @@
 #include "rz_internal.h"
 #include "rz_kernels.h"
 
+namespace {
+constexpr uint64_t kGatedTimeoutTicks = 200000000ull;  // 2 s of the 100 MHz s_memrealtime clock
+}
+
 using namespace rzd;
 
 namespace {
@@ -98,11 +102,19 @@ struct rz_views {
   size_t sortTempBytes = 0;
   int waves = 4;  // wave64s per raster work-group
   int groupsPerCU = 0;  // persistent raster work-groups per CU (0: as many as fit)
+  bool gated = false;   // query-gated draw loop per view (Main.cpp:192-206) instead of classify-then-draw
+  bool cooperative = true;  // gated launches: cooperative dispatch (RZ_FRAME_LAUNCH=plain: plain)
+  uint32_t* d_err = nullptr;  // device error word of gated waits
   unsigned long long* d_stats = nullptr;  // block-loop counters of the last step, when on
   bool stats = false;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   hipEvent_t t0 = nullptr, t1 = nullptr;
+  // Phase timing of direct (not captured) steps, when on: events before the
+  // clear, before the raster launch, after it and after the queries.
+  bool profile = false;
+  hipEvent_t ph[4] = {nullptr, nullptr, nullptr, nullptr};
+  bool phRecorded = false;
   bool ran = false;
 };
 
@@ -168,6 +180,8 @@ static void free_views(rz_views* v) {
   if (!v) return;
   if (v->stream) (void)hipStreamSynchronize(v->stream);
   free_plan(v);
+  hipFree(v->d_err);
+  v->d_err = nullptr;
   for (ViewHost& h : v->views) {
     hipFree(h.d_depth);
     hipFree(h.d_hiz);
@@ -183,6 +197,8 @@ static void free_views(rz_views* v) {
   hipFree(v->d_stats);
   if (v->t0) (void)hipEventDestroy(v->t0);
   if (v->t1) (void)hipEventDestroy(v->t1);
+  for (hipEvent_t e : v->ph)
+    if (e) (void)hipEventDestroy(e);
   if (v->stream) (void)hipStreamDestroy(v->stream);
   delete v;
 }
@@ -207,9 +223,13 @@ extern "C" int rz_views_create(uint32_t nViews, const uint32_t* widths, const ui
   };
   if (hipStreamCreateWithFlags(&v->stream, hipStreamNonBlocking) != hipSuccess) return fail(RZ_ERR_DEVICE);
   if (hipEventCreate(&v->t0) != hipSuccess || hipEventCreate(&v->t1) != hipSuccess) return fail(RZ_ERR_DEVICE);
+  for (hipEvent_t& e : v->ph)
+    if (hipEventCreate(&e) != hipSuccess) return fail(RZ_ERR_DEVICE);
   if (dev_alloc(&v->d_cov, 4096) != hipSuccess || dev_alloc(&v->d_rcp, 4096) != hipSuccess ||
-      dev_alloc(&v->d_queue, 1) != hipSuccess)
+      dev_alloc(&v->d_queue, 1) != hipSuccess || dev_alloc(&v->d_err, 1) != hipSuccess)
     return fail(RZ_ERR_OUT_OF_MEMORY);
+  if (hipMemset(v->d_err, 0, sizeof(uint32_t)) != hipSuccess) return fail(RZ_ERR_DEVICE);
+  if (const char* mode = std::getenv("RZ_FRAME_LAUNCH")) v->cooperative = std::strcmp(mode, "plain") != 0;
   if (hipMemcpy(v->d_cov, tabs.cov, sizeof tabs.cov, hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(v->d_rcp, tabs.rcp, sizeof tabs.rcp, hipMemcpyHostToDevice) != hipSuccess)
     return fail(RZ_ERR_DEVICE);
@@ -338,7 +358,7 @@ static int build_plan(rz_views* v) {
       it.packets = o->d_packets;
       it.quadBase = uint32_t(quads);
       it.quadCount = (o->packetCount / 4) * 8;
-      it.kind = kItemClassify;
+      it.kind = v->gated ? kItemQuery : kItemClassify;
       it.view = i;
       std::memcpy(it.refMin, o->refMin, 16);
       std::memcpy(it.refMax, o->refMax, 16);
@@ -368,6 +388,16 @@ static int build_plan(rz_views* v) {
     maxGroups = std::min(maxGroups, v->groupsPerCU * cus);
   }
   v->nGroups = std::min<uint32_t>(units, uint32_t(std::max(maxGroups, 1)));
+  if (v->gated) {
+    // Every wait of a gated view ends only if all of the view's tiles can run
+    // at once (nGroups <= units, and units count every view's tiles); batch
+    // orders would query per batch, which the reference's loop does not.
+    bool order = v->model != nullptr;
+    for (const ViewHost& h : v->views) order |= !h.order.empty();
+    if (order) return RZ_ERR_INVALID_ARGUMENT;
+    for (const ViewHost& h : v->views)
+      if (h.tilesX * h.tilesY > v->nGroups) return RZ_ERR_TOO_LARGE;
+  }
   const size_t ni = std::max<size_t>(items.size(), 1);
   if (dev_alloc(&v->d_items, ni) != hipSuccess || dev_alloc(&v->d_state, ni) != hipSuccess ||
       dev_alloc(&v->d_done, ni) != hipSuccess || dev_alloc(&v->d_visible, ni) != hipSuccess ||
@@ -460,9 +490,11 @@ static LearnParams learn_params(rz_views* v) {
 }
 
 // The launches of one step, on v->stream.
-static int enqueue_step(rz_views* v) {
+static int enqueue_step(rz_views* v, bool capturing) {
   const uint32_t nv = uint32_t(v->views.size());
+  const bool prof = v->profile && !capturing;
   RZ_HIP(hipMemcpyAsync(v->d_views, v->h_views, nv * sizeof(ViewDesc), hipMemcpyHostToDevice, v->stream));
+  if (prof) RZ_HIP(hipEventRecord(v->ph[0], v->stream));
   RZ_HIP(launch_views_clear(v->d_views, nv, v->maxClear, v->stream));
   PrepParams pp{};
   pp.items = v->d_items;
@@ -536,7 +568,12 @@ static int enqueue_step(rz_views* v) {
     // have read the previous step's).
     rp.labels = (v->labels || v->model) ? v->d_labels : nullptr;
     if (rp.labels) RZ_HIP(launch_fill_u32(v->d_labels, v->nBatches, 0u, v->stream));
-    RZ_HIP(launch_views(rp, v->nGroups, v->stream));
+    rp.done = v->gated ? v->d_done : nullptr;
+    rp.err = v->d_err;
+    rp.timeoutTicks = kGatedTimeoutTicks;
+    if (prof) RZ_HIP(hipEventRecord(v->ph[1], v->stream));
+    RZ_HIP(launch_views(rp, v->nGroups, v->stream, v->gated && v->cooperative));
+    if (prof) RZ_HIP(hipEventRecord(v->ph[2], v->stream));
   }
   ViewsQueryParams qp{};
   qp.views = v->d_views;
@@ -544,6 +581,14 @@ static int enqueue_step(rz_views* v) {
   qp.nQueries = v->nQueries;
   qp.rcp = v->d_rcp;
   RZ_HIP(launch_views_query(qp, v->stream));
+  if (prof) {
+    if (v->nItems == 0) {
+      RZ_HIP(hipEventRecord(v->ph[1], v->stream));
+      RZ_HIP(hipEventRecord(v->ph[2], v->stream));
+    }
+    RZ_HIP(hipEventRecord(v->ph[3], v->stream));
+  }
+  v->phRecorded = prof;
   return RZ_OK;
 }
 
@@ -556,9 +601,12 @@ extern "C" int rz_views_run(rz_views* v, int useGraph) {
     if (s != RZ_OK) return s;
   }
   for (uint32_t i = 0; i < v->views.size(); ++i) v->h_views[i].mvp = v->views[i].mvp;
+  // Gated steps launch cooperatively (the grid must be co-resident), which
+  // is not captured: they are enqueued directly.
+  if ((v->gated && v->cooperative) || v->profile) useGraph = 0;
   if (useGraph && !v->exec) {
     RZ_HIP(hipStreamBeginCapture(v->stream, hipStreamCaptureModeThreadLocal));
-    s = enqueue_step(v);
+    s = enqueue_step(v, true);
     hipGraph_t g = nullptr;
     const hipError_t e = hipStreamEndCapture(v->stream, &g);
     if (s != RZ_OK || e != hipSuccess) {
@@ -572,7 +620,7 @@ extern "C" int rz_views_run(rz_views* v, int useGraph) {
   if (useGraph) {
     RZ_HIP(hipGraphLaunch(v->exec, v->stream));
   } else {
-    s = enqueue_step(v);
+    s = enqueue_step(v, false);
     if (s != RZ_OK) return s;
   }
   RZ_HIP(hipEventRecord(v->t1, v->stream));
@@ -586,6 +634,12 @@ extern "C" int rz_views_sync(rz_views* v, float* ms) {
   if (ms) {
     *ms = 0.0f;
     if (v->ran) RZ_HIP(hipEventElapsedTime(ms, v->t0, v->t1));
+  }
+  uint32_t err = 0;
+  RZ_HIP(hipMemcpy(&err, v->d_err, sizeof err, hipMemcpyDeviceToHost));
+  if (err) {
+    RZ_HIP(hipMemset(v->d_err, 0, sizeof(uint32_t)));
+    return (err & kErrTimeout) ? RZ_ERR_TIMEOUT : RZ_ERR_DEVICE;
   }
   return RZ_OK;
 }
@@ -618,7 +672,23 @@ extern "C" int rz_views_read_status(rz_views* v, uint32_t view, uint8_t* status)
   std::vector<ItemState> st(d.nItems);
   if (d.nItems)
     RZ_HIP(hipMemcpy(st.data(), v->d_state + d.itemBase, d.nItems * sizeof(ItemState), hipMemcpyDeviceToHost));
-  for (uint32_t i = 0; i < d.nItems; ++i) status[i] = uint8_t(st[i].status);
+  std::vector<uint32_t> done(v->gated ? d.nItems : 0);
+  if (v->gated && d.nItems)
+    RZ_HIP(hipMemcpy(done.data(), v->d_done + d.itemBase, d.nItems * sizeof(uint32_t), hipMemcpyDeviceToHost));
+  for (uint32_t i = 0; i < d.nItems; ++i) {
+    const uint32_t k = st[i].status;
+    // A gated item was drawn (with rasterize<false>) when some tile saw it visible.
+    status[i] = uint8_t(k == kStatusTest ? ((done[i] >> 16) != 0u ? 1u : 0u) : k);
+  }
+  return RZ_OK;
+}
+
+extern "C" int rz_views_set_query_gated(rz_views* v, int gated) {
+  if (!v) return RZ_ERR_INVALID_ARGUMENT;
+  int s = wait_last_step(v);
+  if (s != RZ_OK) return s;
+  v->gated = gated != 0;
+  v->dirty = true;
   return RZ_OK;
 }
 
@@ -820,6 +890,7 @@ extern "C" int rz_views_tune(rz_views* v, uint32_t reps, const char* cachePath, 
         s = rz_views_run(v, 0);
         if (s == RZ_OK) s = rz_views_sync(v, &ms);
       }
+      if (s == RZ_ERR_TOO_LARGE) continue;  // a gated view's tiles do not fit this grid
       if (s == RZ_OK) s = step_outputs(v, got);
       if (s != RZ_OK) return restore(s);
       if (ref.empty()) ref.swap(got);
@@ -844,5 +915,22 @@ extern "C" int rz_views_tune(rz_views* v, uint32_t reps, const char* cachePath, 
   if (candidates)
     for (uint32_t i = 0; i < k; ++i) candidates[i] = all[i];
   if (nCandidates) *nCandidates = k;
+  return RZ_OK;
+}
+
+extern "C" int rz_views_profile_enable(rz_views* v, int enable) {
+  if (!v) return RZ_ERR_INVALID_ARGUMENT;
+  int s = wait_last_step(v);
+  if (s != RZ_OK) return s;
+  v->profile = enable != 0;
+  v->phRecorded = false;
+  return RZ_OK;
+}
+
+extern "C" int rz_views_profile_read(rz_views* v, float ms[3]) {
+  if (!v || !ms) return RZ_ERR_INVALID_ARGUMENT;
+  RZ_HIP(hipStreamSynchronize(v->stream));
+  if (!v->phRecorded) return RZ_ERR_INVALID_ARGUMENT;
+  for (int k = 0; k < 3; ++k) RZ_HIP(hipEventElapsedTime(&ms[k], v->ph[k], v->ph[k + 1]));
   return RZ_OK;
 }

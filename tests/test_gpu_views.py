@@ -134,3 +134,64 @@ def test_views_empty_lists():
     assert np.all(h[: 64] == 1) and np.all(d == 0)
     vis, clip = V.queries(1)
     assert vis.shape == (1,)
+
+
+def test_views_query_gated_match_draw_frame(oracle_mod, scene, report):
+    """Query-gated views: each view equals the reference's frame loop
+    (clear, setModelViewProjection, queryVisibility-gated rasterize of every
+    occluder in order), as the oracle's draw_frame, over two steps."""
+    sizes = [s for s in SIZES if (s[0] // 64 + (s[0] % 64 > 0)) * (s[1] // 64 + (s[1] % 64 > 0)) <= 510]
+    owner = Rasterizer(64, 64)
+    V = Views(owner, sizes)
+    V.set_query_gated(True)
+    oracles = [oracle_mod.OracleRasterizer(w, h) for w, h in sizes]
+    drawn = skipped = 0
+    for step in range(2):
+        setup = _views_setup(scene, 20 + step)
+        for k, (mvp, occ, q) in enumerate(setup[: len(sizes)]):
+            V.set_mvp(k, mvp)
+            V.set_occluders(k, occ)
+            V.set_queries(k, q)
+        print(f"gated step {step}: {V.info()}", flush=True)
+        V.run(graph=True)
+        V.sync()
+        for k, (w, h) in enumerate(sizes):
+            mvp, occ, q = setup[k]
+            o = oracles[k]
+            o.clear()
+            o.setModelViewProjection(mvp)
+            st_o = o.draw_frame(occ, gated=True)
+            st_g = V.status(k)
+            assert np.array_equal(st_g, st_o), f"step {step} view {k}: statuses differ at {np.nonzero(st_g != st_o)[0][:10]}"
+            dg, hg = V.raw(k)
+            do, ho = o.raw()
+            assert np.array_equal(hg, ho), f"step {step} view {k}: hiZ {report(hg, ho)}"
+            assert np.array_equal(dg, do), f"step {step} view {k}: depth {report(dg, do)}"
+            vg, cg = V.queries(k)
+            vo, co = o.query_batch(q)
+            assert np.array_equal(vg, vo) and np.array_equal(cg, co)
+            drawn += int((st_o > 0).sum())
+            skipped += int((st_o == 0).sum())
+    print(f"gated: {drawn} drawn, {skipped} not drawn", flush=True)
+    assert drawn > 20 and skipped > 20
+
+
+def test_views_query_gated_limits(scene):
+    from rasterizer_amd import Mlp, RasterizerError
+
+    owner = Rasterizer(64, 64)
+    # 32 x 32 tiles cannot all be co-resident: refused, not run.
+    V = Views(owner, [(2048, 2048)])
+    V.set_query_gated(True)
+    V.set_occluders(0, scene.occluders[:10])
+    with pytest.raises(RasterizerError):
+        V.run(graph=False)
+    V2 = Views(owner, [(320, 192)])
+    V2.set_occluders(0, scene.occluders[:10])
+    V2.set_query_gated(True)
+    V2.set_model(Mlp(seed=1))
+    with pytest.raises(RasterizerError):
+        V2.run(graph=False)
+    V2.set_model(None)
+    V2.run(graph=False)
+    V2.sync()
