@@ -162,12 +162,15 @@ int rz_profile_read(rz_context* ctx, double ms[4], uint32_t counts[2]);
 /* Per-tile counters of the frame kernel for the most recent draw
  * (observability).  rz_frame_stats_enable(ctx, 1) switches draws to the
  * counting variant of the kernel.  rz_frame_stats_read blocks and copies
- * tilesX*tilesY records of 16 u64 into out (when out is non-NULL and *count
+ * tilesX*tilesY records of 22 u64 into out (when out is non-NULL and *count
  * is large enough) and sets *count to the number of records: total ticks,
  * ticks waiting on other tiles, waits, ticks in local query2D work, query2D
  * contributions, ticks drawing (scan + draw), quads drawn, items touching the
  * tile, scan ticks, scan passes, tile load ticks, item-loop ticks, draw
- * ticks, 3 reserved (ticks of the 100 MHz s_memrealtime clock). */
+ * ticks, (quad, block) pairs visited, of which rejected, of which written,
+ * draw ticks staging per-row folds, draw ticks in the per-block loops, chunks
+ * drawn, fold additions staged, query partials evaluated ahead of their turn
+ * and of those contributed early (ticks of the 100 MHz s_memrealtime clock). */
 int rz_frame_stats_enable(rz_context* ctx, int enable);
 int rz_frame_stats_read(rz_context* ctx, uint64_t* out, uint32_t* count);
 

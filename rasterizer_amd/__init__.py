@@ -432,12 +432,15 @@ class Rasterizer:
         _check(lib().rz_frame_stats_enable(self._ctx, 1 if enable else 0), "rz_frame_stats_enable")
 
     def frame_stats_read(self) -> np.ndarray:
-        """Per-tile counters of the last draw, shape (tiles, 16): total ticks, wait
+        """Per-tile counters of the last draw, shape (tiles, 22): total ticks, wait
         ticks, waits, query ticks, queries, raster ticks, quads, items, scan
-        ticks, scan passes, load ticks, item-loop ticks, draw ticks, 3 spare."""
+        ticks, scan passes, load ticks, item-loop ticks, draw ticks, visited,
+        rejected and written (quad, block) pairs, fold-staging ticks,
+        block-loop ticks, chunks, fold additions, early query partials tried
+        and answered."""
         n = _U32(0)
         _check(lib().rz_frame_stats_read(self._ctx, None, ctypes.byref(n)), "rz_frame_stats_read")
-        out = np.zeros((max(n.value, 1), 16), np.uint64)
+        out = np.zeros((max(n.value, 1), 22), np.uint64)
         _check(lib().rz_frame_stats_read(self._ctx, _ptr(out), ctypes.byref(n)), "rz_frame_stats_read")
         return out[: n.value]
 

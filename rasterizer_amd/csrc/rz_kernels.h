@@ -112,7 +112,13 @@ enum FrameStat : int {
   kStatBlocksVisited = 13,   // (quad, block) pairs visited by the block loop
   kStatBlocksRejected = 14,  // of which rejected by hiZ or the edge tests
   kStatBlocksWritten = 15,   // of which written
-  kFrameStatWords = 16,
+  kStatPhaseATicks = 16,     // block-major draw: staging the per-row folds (with its barrier)
+  kStatPhaseBTicks = 17,     // block-major draw: the per-block candidate loops (with its barrier)
+  kStatChunks = 18,          // staged chunks drawn
+  kStatFoldSteps = 19,       // sequential fold additions made by phase A (all waves)
+  kStatEarlyTried = 20,      // query partials evaluated ahead of their item's turn
+  kStatEarlyAnswered = 21,   // of which contributed early (provably not visible)
+  kFrameStatWords = 22,
 };
 
 struct QueryParams {

@@ -627,8 +627,14 @@ struct TileSmem {
   float entry[(TBX == 8 && TBY == 8) ? kQuadChunk * TBY * 12 : 1];
   uint32_t wcnt[8];
   uint32_t part[2][8];
-  uint32_t blockStats[3];  // observability builds: visited, rejected, written
-  uint32_t vis[2];
+  // Early query answers (query-gated launches): per window item, a bit when
+  // this tile has already contributed its partial, and the tile's draw count
+  // at which an early evaluation last failed; per-wave results; wait state.
+  uint32_t spec[kItemWindow / 32];
+  uint32_t tried[kItemWindow];
+  uint32_t spart[2][8];
+  uint32_t wstate[2];
+  uint32_t blockStats[4];  // observability builds: visited, rejected, written, fold steps
 };
 
 // Wave that owns tile-local block (lbx, lby) while a chunk of quads is drawn:
@@ -666,7 +672,7 @@ struct QuadConst {
 // Per-wave counters of the block loop (observability builds only): blocks
 // visited, rejected by hiZ or by the edge tests, and written.
 struct BlockCounters {
-  uint32_t visited = 0, rejected = 0, written = 0;
+  uint32_t visited = 0, rejected = 0, written = 0, foldSteps = 0;
 };
 
 template <int TBX, int TBY, bool STATS>
@@ -816,52 +822,44 @@ __device__ __forceinline__ void raster_quad_wave(const QuadRec& R, TileSmem<TBX,
 }
 
 // One block of one quad with the block's hiZ and depth in registers (the
-// block-major draw).  Same arithmetic as raster_block with g = 0: lanes 0-3
-// of `s` hold the folded edge offsets, lanes 4-11 the depth samples.
+// block-major draw), in two parts.  sample_block_reg: what the block loop
+// computes for the quad alone (Rasterizer.cpp:1148-1290: the four edge
+// lookups and the coverage mask, Convex blocks with an offset >= 63 and other
+// modes' empty masks marked `skip`, depth rows 0/1 and 8/9 from the plane,
+// packed averages between, 0 outside the mask), with the same arithmetic as
+// raster_block with g = 0: lanes 0-3 of `s` hold the folded edge offsets,
+// lanes 4-11 the depth samples.  It depends on nothing the previous quads
+// wrote, and is written without branches so that two candidates' samples
+// interleave.  merge_block_reg: the hiZ reject and the merge, in submission
+// order.
 //
 // LABELS: `win` is the lane's pixel's winner, the batch whose write produced
 // its depth: the first write of a cleared block (hz == 1) sets it (none where
 // the written depth is 0), later writes only where they raise the depth.
-template <bool STATS, bool LABELS>
-__device__ __forceinline__ void raster_block_reg(const uint64_t* __restrict__ cov, float s, uint32_t lane, int kLane,
-                                                 const QuadConst& Q, uint32_t& hz, uint32_t& dval, bool& dirty,
-                                                 BlockCounters& bc, uint32_t& win, uint32_t batch) {
-  if (STATS) bc.visited++;
-  if (hz >= Q.maxZ) {
-    if (STATS) bc.rejected++;
-    return;
-  }
-  float o[4];
-  uint32_t idx[4];
+__device__ __forceinline__ void sample_block_reg(const uint64_t* __restrict__ cov, float s, uint32_t lane,
+                                                 int kLane, const QuadConst& Q, uint32_t& dOut, bool& skip) {
+  const bool convex = Q.mode == kModeConvex;
   bool outside = false;
+  uint32_t idx[4];
 #pragma unroll
   for (int e = 0; e < 4; ++e) {
-    o[e] = readlane_f(s, e);
-    const int32_t t = cvtt_x86(o[e]);
-    if (Q.mode == kModeConvex) {
-      outside |= o[e] >= 63.0f;
-      idx[e] = (Q.sl[e] | uint32_t(max(t, 0))) & 4095u;
-    } else {
-      idx[e] = (Q.sl[e] | uint32_t(min(max(t, 0), 63))) & 4095u;
-    }
-  }
-  if (outside) {
-    if (STATS) bc.rejected++;
-    return;
+    const float o = readlane_f(s, e);
+    const int32_t t = cvtt_x86(o);
+    outside |= o >= 63.0f;
+    const uint32_t tc = convex ? uint32_t(max(t, 0)) : uint32_t(min(max(t, 0), 63));
+    idx[e] = (Q.sl[e] | tc) & 4095u;
   }
   const uint64_t A = cov[idx[0]], B = cov[idx[1]], C = cov[idx[2]], D = cov[idx[3]];
-  uint64_t mask;
-  if (Q.mode == kModeConvex) mask = (A & B) & (C & D);
-  else if (Q.mode == kModeTriangle0) mask = A & B & C;
-  else if (Q.mode == kModeTriangle1) mask = A & C & D;
-  else if (Q.mode == kModeConcaveRight) mask = (A | D) & (B & C);
-  else if (Q.mode == kModeConcaveCenter) mask = (A & B) | (C & D);
-  else mask = (A & D) & (B | C);
-  if (Q.mode != kModeConvex && mask == 0) {
-    if (STATS) bc.rejected++;
-    return;
-  }
-  if (STATS) bc.written++;
+  const uint64_t mConvex = (A & B) & (C & D);
+  const uint64_t mTri0 = A & B & C, mTri1 = A & C & D;
+  const uint64_t mRight = (A | D) & (B & C), mCenter = (A & B) | (C & D), mLeft = (A & D) & (B | C);
+  const uint64_t mask = convex ? mConvex
+                        : Q.mode == kModeTriangle0 ? mTri0
+                        : Q.mode == kModeTriangle1 ? mTri1
+                        : Q.mode == kModeConcaveRight ? mRight
+                        : Q.mode == kModeConcaveCenter ? mCenter
+                                                       : mLeft;
+  skip = convex ? outside : mask == 0ull;
   float v = __shfl(s, 4 + kLane);
   const uint32_t px = lane & 7u, py = lane >> 3;
   if (px >= 4u) v = fmaf(Q.dDx, 0.5f, v);
@@ -871,14 +869,32 @@ __device__ __forceinline__ void raster_block_reg(const uint64_t* __restrict__ co
   const uint32_t q1 = avg_u16(lo, mid), q3 = avg_u16(mid, hi);
   const uint32_t j = py >> 1;
   const uint32_t a01 = (j & 1u) ? q1 : lo, a23 = (j & 1u) ? q3 : mid;
-  uint32_t d = (j & 2u) ? a23 : a01;
+  const uint32_t d = (j & 2u) ? a23 : a01;
   const uint32_t maskBit = 8u * px + ((py & 1u) ? (py >> 1) : 4u + (py >> 1));
-  d = ((mask >> maskBit) & 1ull) ? d : 0u;
+  dOut = ((mask >> maskBit) & 1ull) ? d : 0u;
+}
+
+template <bool STATS, bool LABELS>
+__device__ __forceinline__ void merge_block_reg(uint32_t d, bool skip, uint32_t maxZ, uint32_t& hz, uint32_t& dval,
+                                                bool& dirty, BlockCounters& bc, uint32_t& win, uint32_t batch) {
+  if (STATS) bc.visited++;
+  if (hz >= maxZ || skip) {
+    if (STATS) bc.rejected++;
+    return;
+  }
+  if (STATS) bc.written++;
   if (LABELS) win = hz == 1u ? (d != 0u ? batch : kNoWinner) : (d > dval ? batch : win);
   if (hz != 1u) d = max(d, dval);
   dval = d;
   hz = __ockl_wfred_min_u32(d);
   dirty = true;
+}
+
+// s + step, n times (0 <= n <= 7), each add rounded, without branches.
+__device__ __forceinline__ float fold_upto7(float s, float step, int n) {
+#pragma unroll
+  for (int k = 0; k < 7; ++k) s = k < n ? s + step : s;
+  return s;
 }
 
 __device__ __forceinline__ QuadConst quad_const(const QuadRec& R) {
@@ -926,10 +942,13 @@ __device__ __forceinline__ void quad_component(const QuadRec& R, uint32_t e, flo
 // LABELS: each block's pixel winners (`winner`, the view's block-linear
 // array) are loaded with its depth unless the block is cleared (its first
 // write sets them all) and stored with it; quad q's batch is qid[q] / 8.
+//
+// Returns (observability builds) the s_memrealtime stamp at which phase B
+// started.
 template <int TBX, int TBY, int NW, bool STATS, bool LABELS>
-__device__ void draw_chunk_block_major(TileSmem<TBX, TBY>& sm, uint32_t n, int tbx0, int tbx1, int tby0, int tby1,
-                                       int bX, float invBX, uint32_t wave, uint32_t lane, BlockCounters& bc,
-                                       uint32_t* __restrict__ winner) {
+__device__ uint64_t draw_chunk_block_major(TileSmem<TBX, TBY>& sm, uint32_t n, int tbx0, int tbx1, int tby0, int tby1,
+                                           int bX, float invBX, uint32_t wave, uint32_t lane, BlockCounters& bc,
+                                           uint32_t* __restrict__ winner) {
   const uint32_t g = lane / 12u, e = lane - 12u * g;
   for (uint32_t q = wave; q < n; q += NW) {
     const QuadRec& R = sm.recs[q];
@@ -955,6 +974,7 @@ __device__ void draw_chunk_block_major(TileSmem<TBX, TBY>& sm, uint32_t n, int t
     float base, stepY, stepX;
     quad_component(R, e, base, stepY, stepX);
     const int cLo = x0 + tbx0 - minX;
+    if (STATS && lane == 0) bc.foldSteps += uint32_t(y0 + tby0 - minY + ((y1 - y0) / 5 + 1) * (cLo + 5));
     float line = fold_n(base, stepY, y0 + tby0 - minY);
 #pragma unroll
     for (uint32_t i = 0; i < 4; ++i)
@@ -967,6 +987,7 @@ __device__ void draw_chunk_block_major(TileSmem<TBX, TBY>& sm, uint32_t n, int t
     }
   }
   __syncthreads();
+  const uint64_t tB = STATS ? __builtin_amdgcn_s_memrealtime() : 0;
 
   const int kLane = int(((lane & 7u) & 3u) + 4u * ((lane >> 3) & 1u));
   const bool valid = lane < n;
@@ -993,30 +1014,48 @@ __device__ void draw_chunk_block_major(TileSmem<TBX, TBY>& sm, uint32_t n, int t
     const size_t wIdx = (size_t(tby0 + lby) * size_t(bX) + size_t(tbx0 + lbx)) * 64u + lane;
     uint32_t win = kNoWinner;
     if (LABELS && hz != 1u) win = winner[wIdx];
-    while (m) {
-      const uint32_t q = uint32_t(__builtin_ctzll(m));
-      m &= m - 1ull;
+    // This block's twelve components of staged quad q: false when a quad
+    // with a wrapped first-block index does not cover the block.
+    auto component = [&](uint32_t q, float& sv) -> bool {
       const QuadRec& R = sm.recs[q];
-      const QuadConst Q = quad_const(R);
-      float sv;
       if (sm.qinfo[q][3] & 1u) {
         // Wrapped first-block index: this block's (row, column) in the quad's
         // block numbering, folded from the origin.
         const int d = (tby0 + lby) * bX + tbx0 + lbx - int(R.fb);
         const int rangeX = int(R.range & 0xFFFFu), rangeY = int(R.range >> 16);
-        if (d < 0) continue;
+        if (d < 0) return false;
         const int r = div_bx(d, bX, invBX), c = d - r * bX;
-        if (r >= rangeY || c >= rangeX) continue;
+        if (r >= rangeY || c >= rangeX) return false;
         float base, stepY, stepX;
         quad_component(R, ce, base, stepY, stepX);
         sv = fold_n(fold_n(base, stepY, r), stepX, c);
-      } else {
-        const int x0 = int(sm.qinfo[q][0] & 0xFFu);
-        const float stepX = ce < 4u ? (ce == 0 ? R.nx[0] : ce == 1 ? R.nx[1] : ce == 2 ? R.nx[2] : R.nx[3]) : R.dDx;
-        sv = fold_n(sm.entry[(q * TBY + uint32_t(lby)) * 12u + ce], stepX, lbx - x0);
+        return true;
       }
-      raster_block_reg<STATS, LABELS>(sm.cov, sv, lane, kLane, Q, hz, dval, dirty, bc, win,
-                                      LABELS ? sm.qid[q] >> 3 : 0u);
+      const int x0 = int(sm.qinfo[q][0] & 0xFFu);
+      const float stepX = ce < 4u ? (ce == 0 ? R.nx[0] : ce == 1 ? R.nx[1] : ce == 2 ? R.nx[2] : R.nx[3]) : R.dDx;
+      sv = fold_upto7(sm.entry[(q * TBY + uint32_t(lby)) * 12u + ce], stepX, lbx - x0);
+      return true;
+    };
+    // Candidates two at a time: both samples first (independent), then the
+    // merges in submission order.
+    while (m) {
+      const uint32_t qa = uint32_t(__builtin_ctzll(m));
+      m &= m - 1ull;
+      const bool two = m != 0ull;
+      const uint32_t qb = two ? uint32_t(__builtin_ctzll(m)) : qa;
+      if (two) m &= m - 1ull;
+      float sa = 0.0f, sb = 0.0f;
+      const bool va = component(qa, sa);
+      const bool vb = two && component(qb, sb);
+      const QuadConst Qa = quad_const(sm.recs[qa]), Qb = quad_const(sm.recs[qb]);
+      uint32_t da, db;
+      bool ka, kb;
+      sample_block_reg(sm.cov, sa, lane, kLane, Qa, da, ka);
+      sample_block_reg(sm.cov, sb, lane, kLane, Qb, db, kb);
+      if (va)
+        merge_block_reg<STATS, LABELS>(da, ka, Qa.maxZ, hz, dval, dirty, bc, win, LABELS ? sm.qid[qa] >> 3 : 0u);
+      if (vb)
+        merge_block_reg<STATS, LABELS>(db, kb, Qb.maxZ, hz, dval, dirty, bc, win, LABELS ? sm.qid[qb] >> 3 : 0u);
     }
     if (dirty) {
       sm.depth[lb * 64 + int(lane)] = uint16_t(dval);
@@ -1024,6 +1063,7 @@ __device__ void draw_chunk_block_major(TileSmem<TBX, TBY>& sm, uint32_t n, int t
       if (LABELS) winner[wIdx] = win;
     }
   }
+  return tB;
 }
 
 // query2D restricted to this wave's rows of the tile (Rasterizer.cpp:283-349).
@@ -1051,6 +1091,43 @@ __device__ __forceinline__ bool query_wave(const ItemLocal& S, TileSmem<TBX, TBY
     }
   }
   return false;
+}
+
+// query2D of this wave's rows evaluated ahead of the item's turn: bit 0 when
+// a pixel is visible, bit 1 when a block of the rectangle has hiZ 0 or 1.
+// A block with hiZ >= 2 can never read as cleared (hiZ == 1) again, since its
+// minimum only grows: every later write max-merges, so its depth and hiZ only
+// grow, and every test of query2D (the hiZ skip and the row test) can only
+// move from visible to not visible.  So a result of 0 from every wave is the
+// answer the tile gives at the item's turn.  A cleared block's next write
+// replaces its stale depth, and a block at hiZ 0 could reach exactly 1, so
+// neither proves anything.
+template <int TBX, int TBY, int NW>
+__device__ __forceinline__ uint32_t spec_query_wave(const ItemLocal& S, TileSmem<TBX, TBY>& sm, int tbx0, int tbx1,
+                                                    int tby0, int tby1, uint32_t wave, uint32_t lane) {
+  const int qbx0 = int(S.qMinX >> 3), qbx1 = int(S.qMaxX >> 3), qby0 = int(S.qMinY >> 3), qby1 = int(S.qMaxY >> 3);
+  const uint32_t maxZ = S.qMaxZ;
+  const int px = int(lane & 7u), py = int(lane >> 3);
+  for (int lr = int(wave); lr < TBY; lr += NW) {
+    const int by = tby0 + lr;
+    if (by > tby1) break;
+    if (by < qby0 || by > qby1) continue;
+    const int sy = int(S.qMinY) - 8 * by, ey = int(S.qMaxY) - 8 * by;
+    const int startY = max(sy, 0), endY = min(ey, 7);
+    const int cx0 = max(tbx0, qbx0), cx1 = min(tbx1, qbx1);
+    for (int bx = cx0; bx <= cx1; ++bx) {
+      const int lb = lr * TBX + (bx - tbx0);
+      const uint32_t hz = sm.hiz[lb];
+      if (hz <= 1u) return 2u;
+      if (maxZ <= hz) continue;
+      const int startX = max(int(S.qMinX) - 8 * bx, 0), endX = min(int(S.qMaxX) - 8 * bx, 7);
+      if (startY == 0 && endY == 7 && startX == 0 && endX == 7) return 1u;
+      const bool mine = py >= startY && py <= endY && px >= startX && px <= endX &&
+                        uint32_t(sm.depth[lb * 64 + int(lane)]) < maxZ;
+      if (__ballot(mine) != 0ull) return 1u;
+    }
+  }
+  return 0u;
 }
 
 __device__ __forceinline__ bool box_touches(uint64_t bb, int tbx0, int tbx1, int tby0, int tby1) {
@@ -1194,10 +1271,42 @@ __device__ __forceinline__ void process_tile(const TileJob& J, TileSmem<TBX, TBY
     sm.hiz[lb] = (gx <= tbx1 && gy <= tby1) ? J.hiz[size_t(gy) * bX + gx] : uint16_t(0xFFFF);
   }
 
-  uint32_t qPar = 0, vPar = 0;
+  uint32_t qPar = 0;
   BlockCounters bc;
+  // Early "not visible" answers for later query items of the window (see
+  // spec_query_wave), in query-gated launches only.
+  const bool gated = J.done != nullptr;
+  uint32_t drawVersion = 1, sPar = 0, wPar = 0, nRelW = 0;
+  auto answered_early = [&](uint32_t k) -> bool { return (sm.spec[k >> 5] >> (k & 31u)) & 1u; };
+  auto next_early = [&](uint32_t from) -> uint32_t {
+    for (uint32_t c = from; c < nRelW; ++c)
+      if ((sm.items[c].flags & 1u) && !answered_early(c) && sm.tried[c] != drawVersion) return c;
+    return nRelW;
+  };
+  auto try_early = [&](uint32_t c) {
+    const ItemLocal& L = sm.items[c];
+    const uint32_t r = spec_query_wave<TBX, TBY, NW>(L, sm, tbx0, tbx1, tby0, tby1, wave, lane);
+    if (lane == 0) sm.spart[sPar][wave] = r;
+    __syncthreads();
+    if (tid == 0) {
+      uint32_t any = 0;
+#pragma unroll
+      for (int w = 0; w < NW; ++w) any |= sm.spart[sPar][w];
+      if (any == 0u) {
+        __hip_atomic_fetch_add(&J.done[L.index], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        sm.spec[c >> 5] |= 1u << (c & 31u);
+      } else {
+        sm.tried[c] = drawVersion;
+      }
+      if (STATS) {
+        ctr[kStatEarlyTried]++;
+        if (any == 0u) ctr[kStatEarlyAnswered]++;
+      }
+    }
+    sPar ^= 1u;
+  };
   if (STATS) {
-    if (tid < 3) sm.blockStats[tid] = 0;
+    if (tid < 4) sm.blockStats[tid] = 0;
     __syncthreads();
     if (tid == 0) ctr[kStatLoadTicks] += stamp() - tLoad;
   }
@@ -1241,13 +1350,31 @@ __device__ __forceinline__ void process_tile(const TileJob& J, TileSmem<TBX, TBY
       }
       L.groups = groups;
     }
+    if (gated) {
+      if (tid < kItemWindow / 32) sm.spec[tid] = 0u;
+      if (tid < kItemWindow) sm.tried[tid] = 0u;
+    }
     __syncthreads();
+    nRelW = nRel;
+    if (gated && nRel > 0) {
+      // Answer what the tile can already prove; skipped while no block of
+      // the tile has hiZ >= 2 (nothing can be proven then).
+      int written = 0;
+      for (int lb = int(tid); lb < NB && !written; lb += int(NT)) {
+        const int gx = tbx0 + lb % TBX, gy = tby0 + lb / TBX;
+        written = gx <= tbx1 && gy <= tby1 && sm.hiz[lb] > 1u;
+      }
+      if (__syncthreads_or(written)) {
+        for (uint32_t c = next_early(0); c < nRel; c = next_early(c + 1)) try_early(c);
+        __syncthreads();
+      }
+    }
 
     for (uint32_t k = 0; k < nRel; ++k) {
       const ItemLocal& L = sm.items[k];
       const uint32_t item = L.index, flags = L.flags, status = flags >> 2;
       if (STATS && tid == 0) ctr[kStatItems]++;
-      if (flags & 1u) {
+      if ((flags & 1u) && !(gated && answered_early(k))) {
         // This tile's share of query2D for the item, at its exact turn.
         const uint64_t tq = stamp();
         const bool found = query_wave<TBX, TBY, NW>(L, sm, tbx0, tbx1, tby0, tby1, wave, lane);
@@ -1269,31 +1396,45 @@ __device__ __forceinline__ void process_tile(const TileJob& J, TileSmem<TBX, TBY
       if (!(flags & 2u)) continue;
       if (status == kStatusTest) {
         // Wait until every tile of the rectangle has answered, or one of them
-        // has already seen a visible pixel.
-        if (tid == 0) {
-          const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
-          uint32_t d;
-          while (true) {
-            d = __hip_atomic_load(&J.done[item], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-            if ((d & kDoneCountMask) >= L.expected || d >= kDoneVisibleOne) break;
-            // After one wait of the launch has timed out, the others stop
-            // waiting too (the launch's result is reported as an error).
-            if (__builtin_amdgcn_s_memrealtime() - t0 > J.timeoutTicks ||
-                __hip_atomic_load(J.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) {
+        // has already seen a visible pixel; meanwhile answer later items of
+        // the window early where the tile's state already proves them.
+        const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
+        uint32_t c = k + 1;
+        bool vis;
+        while (true) {
+          if (tid == 0) {
+            const uint32_t d = __hip_atomic_load(&J.done[item], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+            uint32_t st = 0;
+            if ((d & kDoneCountMask) >= L.expected || d >= kDoneVisibleOne) {
+              st = d >= kDoneVisibleOne ? 2u : 1u;
+            } else if (__builtin_amdgcn_s_memrealtime() - t0 > J.timeoutTicks ||
+                       __hip_atomic_load(J.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) {
+              // After one wait of the launch has timed out, the others stop
+              // waiting too (the launch's result is reported as an error).
               __hip_atomic_fetch_or(J.err, uint32_t(kErrTimeout), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-              break;
+              st = d >= kDoneVisibleOne ? 2u : 1u;
             }
+            sm.wstate[wPar] = st;
+          }
+          __syncthreads();
+          const uint32_t st = sm.wstate[wPar];
+          wPar ^= 1u;
+          if (st != 0u) {
+            vis = st == 2u;
+            break;
+          }
+          c = next_early(c);
+          if (c < nRel) {
+            try_early(c);
+            ++c;
+          } else if (tid == 0) {
             __builtin_amdgcn_s_sleep(1);
           }
-          sm.vis[vPar] = d >= kDoneVisibleOne ? 1u : 0u;
-          if (STATS) {
-            ctr[kStatWaitTicks] += stamp() - t0;
-            ctr[kStatWaits]++;
-          }
         }
-        __syncthreads();
-        const bool vis = sm.vis[vPar] != 0u;
-        vPar ^= 1u;
+        if (STATS && tid == 0) {
+          ctr[kStatWaitTicks] += stamp() - t0;
+          ctr[kStatWaits]++;
+        }
         if (!vis) continue;
       }
       // Draw the item's quads that touch the tile, in submission order: each
@@ -1351,19 +1492,29 @@ __device__ __forceinline__ void process_tile(const TileJob& J, TileSmem<TBX, TBY
           }
           __syncthreads();
           const uint64_t tDraw = stamp();
+          uint64_t tB = 0;
           if constexpr (TBX == 8 && TBY == 8) {
-            draw_chunk_block_major<TBX, TBY, NW, STATS, LABELS>(sm, n, tbx0, tbx1, tby0, tby1, bX, invBX, wave, lane,
-                                                                bc, J.winner);
+            tB = draw_chunk_block_major<TBX, TBY, NW, STATS, LABELS>(sm, n, tbx0, tbx1, tby0, tby1, bX, invBX, wave,
+                                                                     lane, bc, J.winner);
           } else {
             for (uint32_t q = 0; q < n; ++q)
               raster_quad_wave<TBX, TBY, NW, STATS>(sm.recs[q], sm, tbx0, tbx1, tby0, tby1, bX, invBX, wave, lane, bc);
           }
           if (STATS && tid == 0) ctr[kStatQuads] += n;
           __syncthreads();
-          if (STATS && tid == 0) ctr[kStatDrawTicks] += stamp() - tDraw;
+          if (STATS && tid == 0) {
+            const uint64_t t = stamp();
+            ctr[kStatDrawTicks] += t - tDraw;
+            ctr[kStatChunks]++;
+            if (TBX == 8 && TBY == 8) {
+              ctr[kStatPhaseATicks] += tB - tDraw;
+              ctr[kStatPhaseBTicks] += t - tB;
+            }
+          }
         }
       }
       if (STATS && tid == 0) ctr[kStatRasterTicks] += stamp() - tr;
+      ++drawVersion;  // the tile's state changed: early answers that failed may now succeed
     }
     __syncthreads();
   }
@@ -1374,12 +1525,14 @@ __device__ __forceinline__ void process_tile(const TileJob& J, TileSmem<TBX, TBY
       atomicAdd(&sm.blockStats[0], bc.visited);
       atomicAdd(&sm.blockStats[1], bc.rejected);
       atomicAdd(&sm.blockStats[2], bc.written);
+      atomicAdd(&sm.blockStats[3], bc.foldSteps);
     }
     __syncthreads();
     if (tid == 0) {
       ctr[kStatBlocksVisited] += sm.blockStats[0];
       ctr[kStatBlocksRejected] += sm.blockStats[1];
       ctr[kStatBlocksWritten] += sm.blockStats[2];
+      ctr[kStatFoldSteps] += sm.blockStats[3];
     }
   }
   if constexpr (LABELS) {
@@ -1425,8 +1578,10 @@ __device__ __forceinline__ void process_tile(const TileJob& J, TileSmem<TBX, TBY
   }
 }
 
+// Two work-groups per CU, as the multi-view kernel (at most 128 VGPRs with
+// eight waves per group).
 template <int TBX, int TBY, int NW, bool STATS>
-__global__ void __launch_bounds__(64 * NW) rz_frame_kernel(FrameParams p) {
+__global__ void __launch_bounds__(64 * NW, NW == 8 ? 4 : 2) rz_frame_kernel(FrameParams p) {
   __shared__ TileSmem<TBX, TBY> sm;
   // Observability counters, kept in LDS by thread 0 (unused unless STATS).
   __shared__ uint64_t ctr[kFrameStatWords];

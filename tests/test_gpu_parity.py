@@ -109,3 +109,28 @@ def test_config2_camera_path(oracle_mod, scene2, report):
             assert np.array_equal(g.readBackDepth(), o.readBackDepth()), f"frame {f}: readback differs"
     print(f"config2: {nf} frames, {drawn} occluder draws, {clipped} with rasterize<true>", flush=True)
     assert clipped > 0, "the path should exercise the near-clipped raster path"
+
+
+def test_counting_variant_matches_oracle(oracle_mod, scene1, report):
+    """The observability build of the frame kernel (per-tile counters) draws
+    the same bits as the oracle, and its counters are consistent: every
+    visited (quad, block) pair is either rejected or written, early query
+    answers are a subset of the early evaluations, and every tile's total
+    covers its item loop."""
+    m = scene1.meta
+    g, o = _pair(oracle_mod, m["width"], m["height"], m["mvp"])
+    occ = [scene1.occluders[i] for i in scenes.front_to_back(scene1.occluders, m["eye"])]
+    g.frame_stats(True)
+    st_g = g.draw_frame(occ)
+    st_o = o.draw_frame(occ)
+    assert np.array_equal(st_g, st_o)
+    _assert_same_buffers(g, o, report, "counting variant")
+    s = g.frame_stats_read().astype(np.int64)
+    tw = g.config()["tileW"]
+    assert s.shape == (-(-g.blocks_x // tw) * -(-g.blocks_y // tw), 22)
+    assert np.array_equal(s[:, 13], s[:, 14] + s[:, 15])
+    assert np.all(s[:, 21] <= s[:, 20])
+    assert np.all(s[:, 0] >= s[:, 11]) and s[:, 6].sum() > 0
+    print(f"visited {s[:, 13].sum()}, written {s[:, 15].sum()}, early tried {s[:, 20].sum()} "
+          f"answered {s[:, 21].sum()}", flush=True)
+    g.frame_stats(False)
