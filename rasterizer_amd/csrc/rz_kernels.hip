@@ -829,9 +829,9 @@ __device__ __forceinline__ void raster_quad_wave(const QuadRec& R, TileSmem<TBX,
 // packed averages between, 0 outside the mask), with the same arithmetic as
 // raster_block with g = 0: lanes 0-3 of `s` hold the folded edge offsets,
 // lanes 4-11 the depth samples.  It depends on nothing the previous quads
-// wrote, and is written without branches so that two candidates' samples
-// interleave.  merge_block_reg: the hiZ reject and the merge, in submission
-// order.
+// wrote.  merge_block_reg: the hiZ reject and the merge, in submission order
+// (the caller tests hiZ before sampling as well, since most visits end
+// there).
 //
 // LABELS: `win` is the lane's pixel's winner, the batch whose write produced
 // its depth: the first write of a cleared block (hz == 1) sets it (none where
@@ -1036,26 +1036,25 @@ __device__ uint64_t draw_chunk_block_major(TileSmem<TBX, TBY>& sm, uint32_t n, i
       sv = fold_upto7(sm.entry[(q * TBY + uint32_t(lby)) * 12u + ce], stepX, lbx - x0);
       return true;
     };
-    // Candidates two at a time: both samples first (independent), then the
-    // merges in submission order.
+    // Candidates in submission order; the hiZ test comes first, since most
+    // (quad, block) visits end there.
     while (m) {
-      const uint32_t qa = uint32_t(__builtin_ctzll(m));
+      const uint32_t q = uint32_t(__builtin_ctzll(m));
       m &= m - 1ull;
-      const bool two = m != 0ull;
-      const uint32_t qb = two ? uint32_t(__builtin_ctzll(m)) : qa;
-      if (two) m &= m - 1ull;
-      float sa = 0.0f, sb = 0.0f;
-      const bool va = component(qa, sa);
-      const bool vb = two && component(qb, sb);
-      const QuadConst Qa = quad_const(sm.recs[qa]), Qb = quad_const(sm.recs[qb]);
-      uint32_t da, db;
-      bool ka, kb;
-      sample_block_reg(sm.cov, sa, lane, kLane, Qa, da, ka);
-      sample_block_reg(sm.cov, sb, lane, kLane, Qb, db, kb);
-      if (va)
-        merge_block_reg<STATS, LABELS>(da, ka, Qa.maxZ, hz, dval, dirty, bc, win, LABELS ? sm.qid[qa] >> 3 : 0u);
-      if (vb)
-        merge_block_reg<STATS, LABELS>(db, kb, Qb.maxZ, hz, dval, dirty, bc, win, LABELS ? sm.qid[qb] >> 3 : 0u);
+      float sv = 0.0f;
+      if (!component(q, sv)) continue;
+      const QuadConst Q = quad_const(sm.recs[q]);
+      if (hz >= Q.maxZ) {
+        if (STATS) {
+          bc.visited++;
+          bc.rejected++;
+        }
+        continue;
+      }
+      uint32_t d;
+      bool k;
+      sample_block_reg(sm.cov, sv, lane, kLane, Q, d, k);
+      merge_block_reg<STATS, LABELS>(d, k, Q.maxZ, hz, dval, dirty, bc, win, LABELS ? sm.qid[q] >> 3 : 0u);
     }
     if (dirty) {
       sm.depth[lb * 64 + int(lane)] = uint16_t(dval);
