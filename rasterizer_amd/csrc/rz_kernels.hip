@@ -1437,9 +1437,9 @@ __device__ __forceinline__ void process_tile(const TileJob& J, TileSmem<TBX, TBY
         if (!vis) continue;
       }
       // Draw the item's quads that touch the tile, in submission order: each
-      // pass tests 256 quads (one per thread; passes and waves whose 64-quad
-      // groups miss the tile are skipped), each hit keeping its setup record
-      // in registers until it is staged into LDS and drawn, 64 at a time.
+      // pass tests 64 * NW quads (one per thread; passes and waves whose
+      // 64-quad groups miss the tile are skipped); hits are staged into LDS
+      // and drawn 64 at a time.
       const uint64_t tr = stamp();
       const uint32_t gBase = L.quadBase >> 6;
       for (uint32_t base = 0; base < L.quadCount; base += NT) {
@@ -1477,10 +1477,10 @@ __device__ __forceinline__ void process_tile(const TileJob& J, TileSmem<TBX, TBY
         }
         const uint32_t pos = hoff + hpre;
         if (STATS && tid == 0) ctr[kStatScanTicks] += stamp() - tScan;
-        // The first 64 hits are staged from registers; later batches (rare)
-        // reload their records.
+        // Hits reload their record (just read, so from cache) rather than
+        // hold it across the compaction, which made the compiler spill it.
         if (hit && pos < kQuadChunk) {
-          sm.recs[pos] = rec;
+          sm.recs[pos] = J.recs[L.quadBase + base + tid];
           if (LABELS) sm.qid[pos] = L.quadBase + base + tid;
         }
         for (uint32_t k0 = 0; k0 < total; k0 += kQuadChunk) {
