@@ -233,6 +233,101 @@ def views_bench(steps: int, warmup: int, seed: int = 3) -> dict:
     }
 
 
+def stress_bench(steps: int, warmup: int, seed: int = 5) -> dict:
+    """The stress configuration: 50,000 icosphere instances (80 quads each,
+    one occluder each) on a 2 km terrain, 4.26M quads, seen by a 3840x2160
+    street-level view and 15 overhead 1024^2 views, each submitting the whole
+    scene in front-to-back order, 65,536 queries per view.  Batched (one
+    classify-then-draw graph replay per step, occluder order and learned
+    order) and plain (each view through rz_draw_frame on one context per
+    resolution, then its queries).  Units: submitted real quads per second."""
+    from rasterizer_amd import DeviceBuffer, Rasterizer, Views, learn, scenes
+
+    t0 = time.perf_counter()
+    sc = scenes.config5(seed=seed)
+    views = sc.meta["views"]
+    nv = len(views)
+    orders = [scenes.front_to_back(sc.occluders, v["eye"]) for v in views]
+    total_quads = nv * sc.quad_count
+    total_q = sum(v["queries"].shape[0] for v in views)
+    log(f"stress: {len(sc.occluders)} occluders, {sc.quad_count:,} quads, {nv} views "
+        f"({time.perf_counter() - t0:.1f} s)")
+    sizes = [v["size"] for v in views]
+    ctx = {s: Rasterizer(*s) for s in sorted(set(sizes))}
+    owner = ctx[sizes[0]]
+    V = Views(owner, sizes)
+    for k, v in enumerate(views):
+        V.set_mvp(k, v["mvp"])
+        V.set_occluders(k, [sc.occluders[i] for i in orders[k]])
+        V.set_queries(k, v["queries"])
+    t0 = time.perf_counter()
+    V.run(graph=True)
+    V.sync()
+    log(f"stress: plan built and run ({time.perf_counter() - t0:.1f} s), {V.info()}")
+    tuned = V.autotune(reps=2)
+    log(f"stress autotune: {tuned}")
+
+    def timed(what: str):
+        for _ in range(warmup):
+            V.run(graph=True)
+        V.sync()
+        t = time.perf_counter()
+        for _ in range(steps):
+            V.run(graph=True)
+        V.sync()
+        t = time.perf_counter() - t
+        log(f"stress {what}: {t * 1e3 / steps:.3f} ms/step, {steps * total_quads / t / 1e6:.1f} M quads/s")
+        return t
+
+    t_b = timed("batched, occluder order")
+    V.set_model(learn.load(), by_occluder=True)
+    t_l = timed("batched, learned order")
+    V.set_model(None)
+
+    handles = [ctx[sizes[k]].handle_array([sc.occluders[i] for i in orders[k]]) for k in range(nv)]
+    dq = [(DeviceBuffer(ctx[sizes[k]], v["queries"].nbytes, v["queries"]),
+           DeviceBuffer(ctx[sizes[k]], v["queries"].shape[0]), DeviceBuffer(ctx[sizes[k]], v["queries"].shape[0]))
+          for k, v in enumerate(views)]
+
+    def plain_step() -> None:
+        for k, v in enumerate(views):
+            R = ctx[sizes[k]]
+            R.clear()
+            R.setModelViewProjection(v["mvp"])
+            R.draw_frame_handles(handles[k], len(sc.occluders))
+            R.query_batch_device(v["queries"].shape[0], *dq[k])
+
+    plain_step()
+    for R in ctx.values():
+        R.sync()
+    plain_steps = max(1, steps // 4)
+    t0 = time.perf_counter()
+    for _ in range(plain_steps):
+        plain_step()
+    for R in ctx.values():
+        R.sync()
+    t_p = time.perf_counter() - t0
+    log(f"stress plain per-view: {t_p * 1e3 / plain_steps:.3f} ms/step")
+    plain_qps = plain_steps * total_quads / t_p
+    return {
+        "workload": (f"{nv} views (1x3840x2160 street level, 15x1024^2 overhead), {sc.quad_count:,} quads "
+                     f"({sc.meta['instances']:,} icosphere instances + terrain) submitted per view, "
+                     f"{total_q:,} AABB queries per step"),
+        "batched_quads_per_s": steps * total_quads / t_b,
+        "batched_ms_per_step": t_b * 1e3 / steps,
+        "learned_quads_per_s": steps * total_quads / t_l,
+        "learned_ms_per_step": t_l * 1e3 / steps,
+        "plain_quads_per_s": plain_qps,
+        "plain_ms_per_step": t_p * 1e3 / plain_steps,
+        "speedup_vs_plain": (steps * total_quads / t_b) / plain_qps,
+        "learned_speedup_vs_plain": (steps * total_quads / t_l) / plain_qps,
+        "queries_per_s_batched": steps * total_q / t_b,
+        "steps": steps,
+        "plain_steps": plain_steps,
+        "launch_config": tuned,
+    }
+
+
 def frames_per_launch(k: int, cap: int) -> int:
     """Largest divisor of k not above cap: K frames in equal launches."""
     f = max(1, min(k, cap))
@@ -252,6 +347,8 @@ def main() -> int:
     ap.add_argument("--no-views", action="store_true", help="skip the batched multi-view measurement")
     ap.add_argument("--views-steps", type=int, default=32)
     ap.add_argument("--frames-in-flight", type=int, default=128, help="frames per persistent launch (at most)")
+    ap.add_argument("--no-stress", action="store_true", help="skip the stress configuration")
+    ap.add_argument("--stress-steps", type=int, default=8)
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="first copy this many MiB device-to-device 3 times (a kernel of known traffic for "
                          "scaling the FETCH_SIZE / WRITE_SIZE counters)")
@@ -454,6 +551,8 @@ def main() -> int:
     }
     if not args.no_views:
         out["batched"] = views_bench(args.views_steps, min(args.warmup, 4))
+    if not args.no_stress:
+        out["stress"] = stress_bench(args.stress_steps, 2)
     if not args.no_cpu:
         threads = min(len(os.sched_getaffinity(0)), 16)
         t0 = time.perf_counter()

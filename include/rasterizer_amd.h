@@ -263,6 +263,14 @@ int rz_views_read_status(rz_views* views, uint32_t view, uint8_t* status);
  * classify-then-draw. */
 int rz_views_set_query_gated(rz_views* views, int gated);
 
+/* Observability of the last step (no counterpart in the reference): the
+ * view's per-occluder prologue state (128 B records: the folded matrix T[4][4]
+ * and c0, c1 of Rasterizer.cpp:616-655, then the draw status) and its quads'
+ * setup records (80 B each, in the view's occluder order: edge normals and
+ * offsets, depth plane, first block, block ranges, mode | packed maxZ << 16,
+ * slope lookups; Rasterizer.cpp:657-1081).  Either pointer may be NULL. */
+int rz_views_read_items(rz_views* views, uint32_t view, void* state, void* setup);
+
 /* Phase timing with HIP events on the views stream (steps then run without
  * graph capture while it is on): rz_views_profile_read blocks and returns the
  * milliseconds of the last step's clear + prep + setup (+ learned-order work),

@@ -68,6 +68,7 @@ def lib() -> ctypes.CDLL:
             "or_frame_classified": ([P, U, P, P], None),
             "or_frame_labels": ([P, U, P, P, P, P], None),
             "or_setup_dump": ([P, P, U, P, P, ctypes.c_int, P], ctypes.c_int),
+            "or_prologue": ([P, P, P, P], None),
             "or_bake": ([P, U, P, P, P, P, P, P], ctypes.c_int),
             "or_x86_rcp": ([ctypes.c_float], ctypes.c_float),
             "or_x86_cvtt": ([ctypes.c_float], ctypes.c_int32),
@@ -220,6 +221,12 @@ class OracleRasterizer:
                               _ptr(out))
         del keep
         return status[: len(occluders)], out[:nb]
+
+    def prologue(self, occ) -> np.ndarray:
+        """or_prologue: T (16, row-major) then c0, c1 for an occluder."""
+        out = np.zeros(18, np.float32)
+        lib().or_prologue(self._r, _ptr(_f4(occ.m_refMin)), _ptr(_f4(occ.m_refMax)), _ptr(out))
+        return out
 
     def draw_frame(self, occluders, gated: bool = True) -> np.ndarray:
         """Main.cpp:192-206 (gated) or the batched path's classify-then-draw

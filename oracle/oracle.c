@@ -255,7 +255,10 @@ static void prologue(const OrRaster* r, const float refMin[4], const float refMa
   float Wa = P->T[3][3], Wb = x_dp4(P->T[3], s);
   float q = (Za - Zb) / (Wa - Wb);
   P->c0 = q;
-  P->c1 = fmaf(-q, Wa, Za);
+  /* _mm_fnmadd_ps(q, Wa, Za) as the reference calls it (Rasterizer.cpp:654):
+   * a NaN q (0/0 from a degenerate reference box) propagates with its sign,
+   * where fmaf(-q, ...) would flip it. */
+  P->c1 = _mm_cvtss_f32(_mm_fnmadd_ss(_mm_set_ss(q), _mm_set_ss(Wa), _mm_set_ss(Za)));
 }
 
 typedef struct {
@@ -574,6 +577,15 @@ static void traverse_quad(OrRaster* r, const QuadSetup* Q) {
     for (int k = 0; k < 8; ++k) lineDepth[k] = lineDepth[k] + dDy;
     for (int e = 0; e < 4; ++e) lineOffset[e] = lineOffset[e] + Q->ny[e];
   }
+}
+
+void or_prologue(const OrRaster* r, const float refMin[4], const float refMax[4], float out[18]) {
+  Prologue P;
+  prologue(r, refMin, refMax, &P);
+  for (int o = 0; o < 4; ++o)
+    for (int j = 0; j < 4; ++j) out[4 * o + j] = P.T[o][j];
+  out[16] = P.c0;
+  out[17] = P.c1;
 }
 
 int or_setup_dump(const OrRaster* r, const uint32_t* packets, uint32_t packetCount, const float refMin[4],

@@ -49,6 +49,9 @@ void or_rasterize(OrRaster* r, const uint32_t* packets, uint32_t packetCount,
 /* Debug: per-quad setup of rasterize<clip> (24 floats per quad: valid, mode,
  * minX, minY, rangeX, rangeY, maxZ, dp0, off[4], nx[4], ny[4], dDx, dDy,
  * firstBlock, 0).  Returns the number of quads. */
+/* Rasterizer.cpp:616-655 for one occluder: the folded matrix T[4][4]
+ * (row-major) and c0, c1. */
+void or_prologue(const OrRaster* r, const float refMin[4], const float refMax[4], float out[18]);
 int or_setup_dump(const OrRaster* r, const uint32_t* packets, uint32_t packetCount, const float refMin[4],
                   const float refMax[4], int clip, float* out);
 

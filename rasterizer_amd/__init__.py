@@ -129,6 +129,7 @@ _SIGNATURES = {
     "rz_views_set_query_gated": ([_P, ctypes.c_int], ctypes.c_int),
     "rz_views_profile_enable": ([_P, ctypes.c_int], ctypes.c_int),
     "rz_views_profile_read": ([_P, _P], ctypes.c_int),
+    "rz_views_read_items": ([_P, _U32, _P, _P], ctypes.c_int),
 }
 
 
@@ -562,6 +563,22 @@ class Views:
         out = np.zeros(max(n, 1), np.uint32)
         _check(lib().rz_views_read_order(self._v, view, _ptr(out)), "rz_views_read_order")
         return out[:n]
+
+    STATE_DTYPE = np.dtype([("T", "<f4", (4, 4)), ("c0", "<f4"), ("c1", "<f4"), ("status", "<u4"),
+                            ("q", "<u4", (5,)), ("expected", "<u4"), ("fp", "<u4", (4,)), ("pad", "<u4")])
+    SETUP_DTYPE = np.dtype([("nx", "<f4", (4,)), ("ny", "<f4", (4,)), ("off", "<f4", (4,)), ("dp0", "<f4"),
+                            ("dDx", "<f4"), ("dDy", "<f4"), ("fb", "<u4"), ("range", "<u4"), ("modeZ", "<u4"),
+                            ("slope01", "<u4"), ("slope23", "<u4")])
+
+    def items(self, view: int):
+        """The last step's per-occluder prologue state and per-quad setup
+        records of a view (rz_views_read_items), as structured arrays."""
+        occ = self._occ[view]
+        st = np.zeros(max(len(occ), 1), self.STATE_DTYPE)
+        nq = sum(int(o.packets.size) // 4 for o in occ)
+        rec = np.zeros(max(nq, 1), self.SETUP_DTYPE)
+        _check(lib().rz_views_read_items(self._v, view, _ptr(st), _ptr(rec)), "rz_views_read_items")
+        return st[: len(occ)], rec[:nq]
 
     def profile(self, enable: bool = True) -> None:
         _check(lib().rz_views_profile_enable(self._v, 1 if enable else 0), "rz_views_profile_enable")

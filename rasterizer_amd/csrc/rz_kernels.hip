@@ -257,9 +257,10 @@ __global__ void __launch_bounds__(256) rz_prep_kernel(PrepParams p) {
   }
   const float Za = m3[2], Zb = dp4_x86(m0[2], m1[2], m2[2], m3[2], 2097152.0f, 1024.0f, 1.0f, 1.0f);
   const float Wa = m3[3], Wb = dp4_x86(m0[3], m1[3], m2[3], m3[3], 2097152.0f, 1024.0f, 1.0f, 1.0f);
-  const float q = (Za - Zb) / (Wa - Wb);
+  // x86 NaN rules: a tiny reference box can make both differences 0.
+  const float q = div_x86(Za - Zb, Wa - Wb);
   S.c0 = q;
-  S.c1 = fmaf(-q, Wa, Za);
+  S.c1 = fnmadd_x86(q, Wa, Za);
 
   S.fp[0] = 0xFFFFFFFFu;
   S.fp[1] = 0u;
@@ -404,7 +405,7 @@ __device__ void setup_quad(const ItemState& S, const uint32_t I[4], bool clip, u
 
   float z[4];
 #pragma unroll
-  for (int k = 0; k < 4; ++k) z[k] = fmaf(invW[k], S.c1, S.c0);
+  for (int k = 0; k < 4; ++k) z[k] = fmadd_x86(invW[k], S.c1, S.c0);
   float maxZ = max_x86(max_x86(z[0], z[1]), max_x86(z[2], z[3]));
   if ((ws[0] | ws[1] | ws[2] | ws[3]) != 0u) maxZ = 1.0f;
 

@@ -934,3 +934,19 @@ extern "C" int rz_views_profile_read(rz_views* v, float ms[3]) {
   for (int k = 0; k < 3; ++k) RZ_HIP(hipEventElapsedTime(&ms[k], v->ph[k], v->ph[k + 1]));
   return RZ_OK;
 }
+
+extern "C" int rz_views_read_items(rz_views* v, uint32_t view, void* state, void* setup) {
+  if (!v || view >= v->views.size() || v->dirty || !v->h_views) return RZ_ERR_INVALID_ARGUMENT;
+  RZ_HIP(hipStreamSynchronize(v->stream));
+  const ViewDesc& d = v->h_views[view];
+  if (state && d.nItems)
+    RZ_HIP(hipMemcpy(state, v->d_state + d.itemBase, size_t(d.nItems) * sizeof(ItemState), hipMemcpyDeviceToHost));
+  if (setup && d.nItems) {
+    std::vector<ItemDesc> items(d.nItems);
+    RZ_HIP(hipMemcpy(items.data(), v->d_items + d.itemBase, items.size() * sizeof(ItemDesc), hipMemcpyDeviceToHost));
+    const uint32_t q0 = items.front().quadBase;
+    const uint32_t q1 = items.back().quadBase + items.back().quadCount;
+    RZ_HIP(hipMemcpy(setup, v->d_recs + q0, size_t(q1 - q0) * sizeof(QuadRec), hipMemcpyDeviceToHost));
+  }
+  return RZ_OK;
+}

@@ -12,6 +12,13 @@
 //  - DPPS sums its products as (p0 + p1) + (p2 + p3) with each product and sum
 //    rounded to float.
 //  - PACKUSDW saturates the signed 32-bit input to [0, 65535]; PAVGW rounds up.
+//  - NaNs: an invalid operation (0/0, inf/inf, 0 * inf, inf - inf) gives the
+//    x86 default NaN 0xFFC00000 (negative) where gfx950 gives 0x7FC00000, and
+//    a NaN operand propagates quieted with its sign (FNMADD does not negate
+//    it).  Packed depth reads the sign (a negative NaN packs to 0, a positive
+//    one to 65535), so the divide and FMAs whose results become depth are
+//    written out (div_x86, fmadd_x86, fnmadd_x86).  NaN operands are taken in
+//    argument order.
 // The file is compiled with -ffp-contract=off: a * b + c is two roundings,
 // and fmaf() is used exactly where the reference calls an FMA intrinsic.
 #pragma once
@@ -49,6 +56,36 @@ __device__ __forceinline__ float dp4_x86(float a0, float a1, float a2, float a3,
                                          float b3) {
   const float p0 = a0 * b0, p1 = a1 * b1, p2 = a2 * b2, p3 = a3 * b3;
   return (p0 + p1) + (p2 + p3);
+}
+
+constexpr uint32_t kX86DefaultNaN = 0xFFC00000u;
+__device__ __forceinline__ bool nan_x86(float x) { return (fbits(x) & 0x7FFFFFFFu) > 0x7F800000u; }
+__device__ __forceinline__ float quiet_x86(float x) { return bitsf(fbits(x) | 0x400000u); }
+
+// DIVSS.
+__device__ __forceinline__ float div_x86(float a, float b) {
+  if (nan_x86(a)) return quiet_x86(a);
+  if (nan_x86(b)) return quiet_x86(b);
+  const float r = a / b;
+  return nan_x86(r) ? bitsf(kX86DefaultNaN) : r;
+}
+
+// VFMADD: a * b + c, one rounding.
+__device__ __forceinline__ float fmadd_x86(float a, float b, float c) {
+  if (nan_x86(a)) return quiet_x86(a);
+  if (nan_x86(b)) return quiet_x86(b);
+  if (nan_x86(c)) return quiet_x86(c);
+  const float r = fmaf(a, b, c);
+  return nan_x86(r) ? bitsf(kX86DefaultNaN) : r;
+}
+
+// VFNMADD: -(a * b) + c, one rounding.
+__device__ __forceinline__ float fnmadd_x86(float a, float b, float c) {
+  if (nan_x86(a)) return quiet_x86(a);
+  if (nan_x86(b)) return quiet_x86(b);
+  if (nan_x86(c)) return quiet_x86(c);
+  const float r = fmaf(-a, b, c);
+  return nan_x86(r) ? bitsf(kX86DefaultNaN) : r;
 }
 
 __device__ __forceinline__ uint32_t pack_depth(float d) {

@@ -468,3 +468,131 @@ def front_to_back(occluders, eye) -> np.ndarray:
     p = (d * d).astype(np.float32)
     dist = ((p[:, 0] + p[:, 1]).astype(np.float32) + p[:, 2]).astype(np.float32)
     return np.argsort(dist, kind="stable")
+
+
+def icosphere() -> tuple[np.ndarray, np.ndarray]:
+    """Unit icosphere with 80 triangles: the icosahedron with every face split
+    into four, new vertices pushed to the sphere.  Returns (vertices (42, 3),
+    faces (80, 3)), faces wound as box_quads' faces: front-facing seen from
+    outside under the left-handed view matrices."""
+    t = (1.0 + math.sqrt(5.0)) / 2.0
+    v = [(-1, t, 0), (1, t, 0), (-1, -t, 0), (1, -t, 0), (0, -1, t), (0, 1, t), (0, -1, -t), (0, 1, -t),
+         (t, 0, -1), (t, 0, 1), (-t, 0, -1), (-t, 0, 1)]
+    f = [(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9), (5, 11, 4), (11, 10, 2), (10, 7, 6),
+         (7, 1, 8), (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8), (3, 8, 9), (4, 9, 5), (2, 4, 11), (6, 2, 10),
+         (8, 6, 7), (9, 8, 1)]
+    verts = [np.array(p, np.float64) / np.linalg.norm(p) for p in v]
+    mid = {}
+
+    def midpoint(a, b):
+        k = (min(a, b), max(a, b))
+        if k not in mid:
+            m = verts[a] + verts[b]
+            verts.append(m / np.linalg.norm(m))
+            mid[k] = len(verts) - 1
+        return mid[k]
+
+    faces = []
+    for a, b, c in f:
+        ab, bc, ca = midpoint(a, b), midpoint(b, c), midpoint(c, a)
+        faces += [(a, ca, ab), (b, ab, bc), (c, bc, ca), (ab, ca, bc)]
+    return np.array(verts), np.array(faces)
+
+
+def _random_rotations(rng: np.random.Generator, n: int) -> np.ndarray:
+    """n uniformly random rotation matrices (Shoemake's unit quaternions)."""
+    u1, u2, u3 = rng.random(n), rng.random(n), rng.random(n)
+    a, b = np.sqrt(1 - u1), np.sqrt(u1)
+    x, y = a * np.sin(2 * np.pi * u2), a * np.cos(2 * np.pi * u2)
+    z, w = b * np.sin(2 * np.pi * u3), b * np.cos(2 * np.pi * u3)
+    R = np.empty((n, 3, 3))
+    R[:, 0, 0] = 1 - 2 * (y * y + z * z)
+    R[:, 0, 1] = 2 * (x * y - z * w)
+    R[:, 0, 2] = 2 * (x * z + y * w)
+    R[:, 1, 0] = 2 * (x * y + z * w)
+    R[:, 1, 1] = 1 - 2 * (x * x + z * z)
+    R[:, 1, 2] = 2 * (y * z - x * w)
+    R[:, 2, 0] = 2 * (x * z - y * w)
+    R[:, 2, 1] = 2 * (y * z + x * w)
+    R[:, 2, 2] = 1 - 2 * (x * x + y * y)
+    return R
+
+
+CONFIG5_SIZES = [(3840, 2160)] + [(1024, 1024)] * 15
+
+
+def config5(seed: int = 5, n_instances: int = 50000, terrain_n: int = 512, extent: float = 2000.0,
+            queries_per_view: int = 65536, sizes=None):
+    """Stress scene: n_instances seeded instances of the 80-triangle icosphere
+    (scales log-U[0.5,10], positions U over an extent x extent square resting
+    on the terrain, uniformly random rotations; every triangle one degenerate
+    quad (a, b, c, a), as leftover triangles of the reference's quad
+    decomposition are) on config 2's fBm terrain stretched to the same square.
+    Each instance is one occluder baked against its own bounding box (80
+    quads, 10 batches); the terrain is SAH-batched against its bounds.
+    Views (default 16): one street-level 3840x2160 main view and 15
+    shadow-like 1024^2 views looking down from 300-700 units above the
+    terrain; each gets queries_per_view query boxes (instance bounds near its
+    target plus props on the terrain)."""
+    sizes = list(CONFIG5_SIZES if sizes is None else sizes)
+    rng = np.random.default_rng(seed)
+    spacing = extent / (terrain_n - 1)
+    hf = Heightfield(terrain_n, spacing, 20.0, seed=(seed * 0x2545F4914F6CDD1D) & _M64)
+    terrain = hf.quads()
+    tq = pad_to_batches(terrain)
+    tv = tq.reshape(-1, 4)
+    t_min, t_max = tv.min(axis=0).astype(np.float32), tv.max(axis=0).astype(np.float32)
+    occ = [Occluder.bake(tq[b].reshape(-1, 4), t_min, t_max)
+           for b in sah_batches(tq.min(axis=1), tq.max(axis=1), 512, 8)]
+    n_terrain = len(occ)
+
+    base_v, base_f = icosphere()
+    scale = np.exp(rng.uniform(math.log(0.5), math.log(10.0), n_instances))
+    px = rng.uniform(0.0, extent, n_instances)
+    pz = rng.uniform(0.0, extent, n_instances)
+    py = hf.height(px, pz) + scale
+    centers = np.stack([px, py, pz], axis=1)
+    R = _random_rotations(rng, n_instances)
+    tri = base_v[base_f]                                   # (80, 3, 3)
+    for k in range(n_instances):
+        p = (tri @ R[k].T) * scale[k] + centers[k]         # (80, 3, 3)
+        q = np.ones((80, 4, 4), np.float32)
+        q[:, 0:3, 0:3] = p
+        q[:, 3, 0:3] = p[:, 0]                             # (a, b, c, a)
+        v = q.reshape(-1, 4)
+        occ.append(Occluder.bake(v, v.min(axis=0), v.max(axis=0)))
+    scene = Scene(occ, int(terrain.shape[0] + 80 * n_instances), t_min, t_max)
+
+    views = []
+    for i, (w, h) in enumerate(sizes):
+        if i == 0 and w >= 2048:
+            eye = np.array([0.5 * extent, 0.0, 0.05 * extent])
+            eye[1] = hf.height(eye[0], eye[2]) + 2.0
+            d = np.array([0.0, -0.05, 1.0])
+        else:
+            target = np.array([rng.uniform(0.2, 0.8) * extent, 0.0, rng.uniform(0.2, 0.8) * extent])
+            target[1] = hf.height(target[0], target[2])
+            ang = rng.uniform(0, 2 * np.pi)
+            off = np.array([np.cos(ang), 0.0, np.sin(ang)]) * rng.uniform(100.0, 300.0)
+            eye = target + off + np.array([0.0, rng.uniform(300.0, 700.0), 0.0])
+            d = target - eye
+        d = d / np.linalg.norm(d)
+        up = (0.0, 1.0, 0.0) if abs(d[1]) < 0.95 else (1.0, 0.0, 0.0)
+        # Query boxes: instance bounds nearest the view's line of sight, then
+        # props on the terrain around it.
+        look = eye + d * 200.0
+        dist = np.linalg.norm(centers - look, axis=1)
+        near = np.argsort(dist)[: queries_per_view // 2]
+        n_props = queries_per_view - near.size
+        qx = np.clip(look[0] + rng.uniform(-400, 400, n_props), 0, extent)
+        qz = np.clip(look[2] + rng.uniform(-400, 400, n_props), 0, extent)
+        qh = np.exp(rng.uniform(math.log(0.25), math.log(3.0), (n_props, 3)))
+        qc = np.stack([qx, hf.height(qx, qz) + qh[:, 1], qz], axis=1)
+        boxes = np.concatenate([aabb_bounds(centers[near].astype(np.float32),
+                                            np.repeat(scale[near, None], 3, axis=1).astype(np.float32)),
+                                aabb_bounds(qc.astype(np.float32), qh.astype(np.float32))])
+        views.append(dict(size=(w, h), eye=eye.astype(np.float32), dir=d.astype(np.float32), up=up, queries=boxes,
+                          mvp=view_projection(eye, d, up, math.pi / 3, w / h)))
+    scene.meta.update(views=views, seed=seed, terrain_quads=int(terrain.shape[0]), n_terrain_occluders=n_terrain,
+                      instances=n_instances)
+    return scene

@@ -231,3 +231,32 @@ def test_empty_frame_and_occluder(oracle_mod, report):
     _same(g, o, report, "empty frame")
     with pytest.raises(ValueError):
         Occluder(np.zeros(8, np.uint32), np.zeros(4), np.ones(4), np.zeros(4), np.ones(4))
+
+
+def test_degenerate_reference_box_matches_oracle(oracle_mod, report):
+    """A flat occluder facing an axis-aligned camera, baked against a box 1e-5
+    deep: the prologue's c0 is 0/0 (Rasterizer.cpp:645-655).  x86 gives the
+    negative default NaN, every depth is that NaN and packs to maxZ 0, so no
+    block is written; the GPU must follow x86's NaN rules, not its own."""
+    from rasterizer_amd import Views
+    from test_oracle_pins import _flat_facing_occluder
+
+    occ = _flat_facing_occluder()
+    mvp = scenes.view_projection((0, 0, 0), (0, 0, 1), (0, 1, 0), np.pi / 3, 2.0)
+    g, o = _pair(oracle_mod, 128, 64, mvp)
+    for clip in (False, True):
+        g.rasterize(occ, clip)
+        o.rasterize(occ, clip)
+        _same(g, o, report, f"flat facing occluder rasterize<{clip}>")
+    V = Views(g, [(128, 64)])
+    V.set_mvp(0, mvp)
+    V.set_occluders(0, [occ, occ])
+    V.run(graph=False)
+    V.sync()
+    st, rec = V.items(0)
+    assert st["c0"].view(np.uint32).tolist() == [0xFFC00000] * 2
+    assert st["c1"].view(np.uint32).tolist() == [0xFFC00000] * 2
+    valid = (rec["range"] & 0xFFFF) != 0
+    assert valid.any() and np.all((rec["modeZ"][valid] >> 16) == 0)
+    _, hz = V.raw(0)
+    assert np.all(hz[:128] == 1)

@@ -195,3 +195,30 @@ def test_views_query_gated_limits(scene):
     V2.set_model(None)
     V2.run(graph=False)
     V2.sync()
+
+
+def test_stress_scene_views_match_oracle(oracle_mod, report):
+    """The stress configuration at reduced size: icosphere instances (each its
+    own occluder with its own bounding box) on the terrain, a 3840x2160
+    street-level view (first-block wrap included) and two overhead 1024^2
+    views in one step, each against the oracle's classify-then-draw."""
+    sizes = [(3840, 2160), (1024, 1024), (1024, 1024)]
+    sc = scenes.config5(n_instances=2000, terrain_n=128, queries_per_view=2048, sizes=sizes)
+    views = sc.meta["views"]
+    owner = Rasterizer(64, 64)
+    V = Views(owner, sizes)
+    setup = []
+    for k, vw in enumerate(views):
+        occ = [sc.occluders[i] for i in scenes.front_to_back(sc.occluders, vw["eye"])]
+        V.set_mvp(k, vw["mvp"])
+        V.set_occluders(k, occ)
+        V.set_queries(k, vw["queries"])
+        setup.append(occ)
+    V.run(graph=True)
+    V.sync()
+    drawn = 0
+    for k, vw in enumerate(views):
+        w, h = vw["size"]
+        drawn += _check_view(oracle_mod, V, k, w, h, vw["mvp"], setup[k], vw["queries"], report,
+                             f"stress view {k} {w}x{h}")
+    assert drawn > 500

@@ -723,3 +723,40 @@ def test_bake_quantisation(oracle_mod):
             Z = int(np.trunc(np.float32(np.float64(t[2]) * 1023 + 0.5)))
             want = ((X << 21) | (Y << 10) | Z) & 0xFFFFFFFF
             assert int(words[b, k, l]) == want, (qi, k)
+
+
+def _flat_facing_occluder(half=20.0, dist=100.0):
+    """Eight copies of a quad facing a camera that looks along +z, baked
+    against a reference box 1e-5 deep: the x and y columns of the matrix
+    contribute nothing to Z and W, so the prologue's two Z and two W points
+    (Rasterizer.cpp:645-655) round to the same floats and c0 = 0/0."""
+    from rasterizer_amd import Occluder
+
+    quad = np.array([[-1, -1, 0], [1, -1, 0], [1, 1, 0], [-1, 1, 0]], np.float32) * half + np.float32([0, 0, dist])
+    v = np.tile(np.concatenate([quad, np.ones((4, 1), np.float32)], 1), (8, 1))
+    return Occluder.bake(v, np.float32([-half, -half, dist, 1]), np.float32([half, half, dist + 1e-5, 1]))
+
+
+FACING_CAMERA = dict(eye=(0, 0, 0), d=(0, 0, 1), up=(0, 1, 0))
+
+
+def test_degenerate_reference_box_gives_x86_default_nan(oracle_mod):
+    """DIVSS of 0/0 gives the 'QNaN floating-point indefinite' 0xFFC00000 and
+    VFNMADD passes a NaN operand through unchanged (Intel SDM), so c0 and c1
+    are both that negative NaN; every depth is then that NaN, packs to 0, and
+    the occluder's quads are rejected by any hiZ (a cleared buffer stays
+    cleared)."""
+    from rasterizer_amd import scenes
+
+    occ = _flat_facing_occluder()
+    o = oracle_mod.OracleRasterizer(128, 64)
+    o.setModelViewProjection(scenes.view_projection((0, 0, 0), (0, 0, 1), (0, 1, 0), np.pi / 3, 2.0))
+    pro = o.prologue(occ)
+    assert pro[16:18].view(np.uint32).tolist() == [0xFFC00000, 0xFFC00000]
+    sd = o.setup_dump(occ, False)
+    valid = sd[:, 0] != 0
+    assert valid.any() and np.all(sd[valid, 6] == 0)
+    o.clear()
+    o.rasterize(occ, False)
+    _, hz = o.raw()
+    assert np.all(hz[:16 * 8] == 1)
