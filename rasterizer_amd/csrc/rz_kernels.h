@@ -65,6 +65,10 @@ struct ViewsParams {
   uint32_t* done;
   uint32_t* err;
   uint64_t timeoutTicks;
+  // Optional per-tile item lists (launch_views_bin): tile t of view v walks
+  // lists[views[v].listBase + t * (items in view) ...], listLen[unit] long.
+  uint32_t* lists;
+  uint32_t* listLen;
 };
 
 // queryVisibility over the queries of every view of a multi-view launch.
@@ -162,6 +166,8 @@ hipError_t launch_fill_u16(uint16_t* p, uint32_t n, uint16_t v, hipStream_t stre
 // 8x8-block tiles.
 hipError_t launch_views(const ViewsParams& p, uint32_t groups, hipStream_t stream, bool cooperative = false);
 hipError_t views_max_groups(int waves, int* maxGroups);
+// Builds p.lists / p.listLen from p.items, p.state (after prep and setup).
+hipError_t launch_views_bin(const ViewsParams& p, hipStream_t stream);
 hipError_t launch_views_query(const ViewsQueryParams& p, hipStream_t stream);
 // hiZ := 1 (whole 8-entry groups) and tile min := 1 for every view.
 hipError_t launch_views_clear(const ViewDesc* views, uint32_t nViews, uint32_t maxEntries, hipStream_t stream);

@@ -1188,6 +1188,20 @@ __device__ __forceinline__ bool query_overlaps_tile(uint32_t qMinX, uint32_t qMa
          tY <= (qMaxY >> 3) / TBY;
 }
 
+// Relevance of one item to a tile: bit 0 the query rectangle overlaps it,
+// bit 1 some valid quad's blocks do; status in bits 2-3; 0 when neither or
+// when the item is not drawn.
+template <int TBX, int TBY>
+__device__ __forceinline__ uint32_t tile_relevance(const ItemState& S, uint32_t tX, uint32_t tY, int tbx0, int tbx1,
+                                                   int tby0, int tby1) {
+  const uint32_t status = S.status;
+  if (status == kStatusSkip) return 0u;
+  const bool q = status == kStatusTest && query_overlaps_tile<TBX, TBY>(S.qMinX, S.qMaxX, S.qMinY, S.qMaxY, tX, tY);
+  const bool f = S.fp[0] <= S.fp[1] && int(S.fp[0]) <= tbx1 && int(S.fp[1]) >= tbx0 && int(S.fp[2]) <= tby1 &&
+                 int(S.fp[3]) >= tby0;
+  return (q || f) ? ((q ? 1u : 0u) | (f ? 2u : 0u) | (status << 2)) : 0u;
+}
+
 // done[i] packs, over the tiles of item i's query rectangle, the number of
 // tiles that have contributed (bits 0-15) and how many found a visible pixel
 // (bits 16-31).
@@ -1217,6 +1231,8 @@ struct TileJob {
   uint64_t timeoutTicks;
   uint32_t* winner;  // labelled draws: per-pixel winners (block-linear), and
   uint32_t* labels;  // per-batch counts of written blocks each batch wins
+  const uint32_t* list;  // optional: this tile's relevant items, in submission order
+  uint32_t listLen;
 };
 
 // LABELS (8x8-block tiles): the draw tracks each pixel's winning batch, and
@@ -1235,21 +1251,19 @@ __device__ __forceinline__ void process_tile(const TileJob& J, TileSmem<TBX, TBY
   const int tbx1 = min(tbx0 + TBX, bX) - 1, tby1 = min(tby0 + TBY, bY) - 1;
   const float invBX = 1.0f / float(bX);
 
-  // Relevance of one item to this tile: bit 0 the query rectangle overlaps
-  // it, bit 1 some valid quad's blocks do; status in bits 2-3.
   auto relevance = [&](uint32_t i) -> uint32_t {
-    const ItemState& S = J.state[i];
-    const uint32_t status = S.status;
-    if (status == kStatusSkip) return 0u;
-    const bool q = status == kStatusTest && query_overlaps_tile<TBX, TBY>(S.qMinX, S.qMaxX, S.qMinY, S.qMaxY, tX, tY);
-    const bool f = S.fp[0] <= S.fp[1] && int(S.fp[0]) <= tbx1 && int(S.fp[1]) >= tbx0 && int(S.fp[2]) <= tby1 &&
-                   int(S.fp[3]) >= tby0;
-    return (q || f) ? ((q ? 1u : 0u) | (f ? 2u : 0u) | (status << 2)) : 0u;
+    return tile_relevance<TBX, TBY>(J.state[i], tX, tY, tbx0, tbx1, tby0, tby1);
   };
+  // The tile's items: its binned list (the items relevant to it, in
+  // submission order) when the launch has one, else the whole item range.
+  const uint32_t nList = J.list ? J.listLen : J.itemEnd - J.itemBegin;
+  auto item_at = [&](uint32_t pos) -> uint32_t { return J.list ? J.list[pos] : J.itemBegin + pos; };
 
   int involved = 0;
   constexpr uint32_t NT = 64u * NW;
-  for (uint32_t i = J.itemBegin + tid; i < J.itemEnd && !involved; i += NT) involved = relevance(i) != 0u;
+  if (J.list) involved = nList > 0u;
+  else
+    for (uint32_t i = J.itemBegin + tid; i < J.itemEnd && !involved; i += NT) involved = relevance(i) != 0u;
   if (!__syncthreads_or(involved)) return;
 
   const uint64_t tLoad = stamp();
@@ -1311,10 +1325,11 @@ __device__ __forceinline__ void process_tile(const TileJob& J, TileSmem<TBX, TBY
     if (tid == 0) ctr[kStatLoadTicks] += stamp() - tLoad;
   }
   const uint64_t tLoop = stamp();
-  for (uint32_t wbase = J.itemBegin; wbase < J.itemEnd; wbase += kItemWindow) {
+  for (uint32_t wbase = 0; wbase < nList; wbase += kItemWindow) {
     // Gather this window's items that touch the tile, in draw order.
-    const uint32_t i = wbase + tid;
-    const uint32_t rel = (tid < kItemWindow && i < J.itemEnd) ? relevance(i) : 0u;
+    const bool inWin = tid < kItemWindow && wbase + tid < nList;
+    const uint32_t i = inWin ? item_at(wbase + tid) : 0u;
+    const uint32_t rel = inWin ? relevance(i) : 0u;
     const uint64_t m = __ballot(rel != 0u);
     const uint32_t pre = __builtin_amdgcn_mbcnt_hi(uint32_t(m >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(m), 0u));
     if (lane == 0) sm.wcnt[wave] = uint32_t(__popcll(m));
@@ -1607,6 +1622,8 @@ __global__ void __launch_bounds__(64 * NW, NW == 8 ? 4 : 2) rz_frame_kernel(Fram
   J.timeoutTicks = p.timeoutTicks;
   J.winner = nullptr;
   J.labels = nullptr;
+  J.list = nullptr;
+  J.listLen = 0;
   bool covLoaded = false;
   process_tile<TBX, TBY, NW, STATS, false>(J, sm, p.cov, covLoaded, ctr);
   if (STATS && threadIdx.x == 0) {
@@ -1665,6 +1682,8 @@ __global__ void __launch_bounds__(64 * NW, NW == 8 ? 4 : 2) rz_views_kernel(View
     J.timeoutTicks = p.timeoutTicks;
     J.winner = V.winner;
     J.labels = p.labels;
+    J.list = p.lists ? p.lists + V.listBase + size_t(t) * (J.itemEnd - J.itemBegin) : nullptr;
+    J.listLen = p.lists ? p.listLen[unit] : 0u;
     process_tile<TBX, TBY, NW, STATS, LABELS>(J, sm, p.cov, covLoaded, ctr);
     __syncthreads();
   }
@@ -1673,6 +1692,57 @@ __global__ void __launch_bounds__(64 * NW, NW == 8 ? 4 : 2) rz_views_kernel(View
     atomicAdd(&p.stats[1], (unsigned long long)ctr[kStatBlocksRejected]);
     atomicAdd(&p.stats[2], (unsigned long long)ctr[kStatBlocksWritten]);
   }
+}
+
+// Per-tile item lists of a multi-view launch: for each (view, tile) unit, the
+// indices of the view's items relevant to the tile (tile_relevance), in
+// submission order, at lists[views[v].listBase + tile * (items in view)],
+// and their count in listLen[unit].  One work-group per unit tests the items
+// 256 at a time and compacts them with wave ballots.
+__global__ void __launch_bounds__(256) rz_views_bin_kernel(ViewsParams p) {
+  __shared__ uint32_t wcnt[4];
+  const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+  for (uint32_t unit = blockIdx.x; unit < p.nUnits; unit += gridDim.x) {
+    uint32_t lo = 0, hi = p.nViews - 1;
+    while (lo < hi) {
+      const uint32_t mid = (lo + hi + 1) >> 1;
+      if (p.views[mid].unitBase <= unit) lo = mid;
+      else hi = mid - 1;
+    }
+    const ViewDesc& V = p.views[lo];
+    const uint32_t t = unit - V.unitBase;
+    const uint32_t tX = t % V.tilesX, tY = t / V.tilesX;
+    const int tbx0 = int(tX) * 8, tby0 = int(tY) * 8;
+    const int tbx1 = min(tbx0 + 8, int(V.bX)) - 1, tby1 = min(tby0 + 8, int(V.bY)) - 1;
+    const uint32_t begin = p.batchItems ? V.batchBase : V.itemBase;
+    const uint32_t end = begin + (p.batchItems ? V.nBatches : V.nItems);
+    uint32_t* out = p.lists + V.listBase + size_t(t) * (end - begin);
+    uint32_t n = 0;
+    for (uint32_t base = begin; base < end; base += 256u) {
+      const uint32_t i = base + tid;
+      const bool rel = i < end && tile_relevance<8, 8>(p.state[i], tX, tY, tbx0, tbx1, tby0, tby1) != 0u;
+      const uint64_t m = __ballot(rel);
+      const uint32_t pre = __builtin_amdgcn_mbcnt_hi(uint32_t(m >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(m), 0u));
+      if (lane == 0) wcnt[wave] = uint32_t(__popcll(m));
+      __syncthreads();
+      uint32_t off = n, total = 0;
+#pragma unroll
+      for (uint32_t w = 0; w < 4; ++w) {
+        off += w < wave ? wcnt[w] : 0u;
+        total += wcnt[w];
+      }
+      if (rel) out[off + pre] = i;
+      n += total;
+      __syncthreads();
+    }
+    if (tid == 0) p.listLen[unit] = n;
+  }
+}
+
+hipError_t launch_views_bin(const ViewsParams& p, hipStream_t stream) {
+  if (p.nUnits == 0 || !p.lists) return hipSuccess;
+  hipLaunchKernelGGL(rz_views_bin_kernel, dim3(std::min<uint32_t>(p.nUnits, 65535u)), dim3(256), 0, stream, p);
+  return hipGetLastError();
 }
 
 // Per-item draw status of a frame: 0 not drawn, 1 rasterize<false>,

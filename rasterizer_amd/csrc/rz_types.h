@@ -78,7 +78,8 @@ struct ViewDesc {
   uint8_t* queryVisible;     // nQueries
   uint8_t* queryClip;        // nQueries
   uint32_t* winner;          // per pixel, for draw-order labels (null when labels are off)
-  uint64_t pad;
+  uint32_t listBase;         // first entry of the view's per-tile item lists
+  uint32_t pad;
   Mvp mvp;
 };
 static_assert(sizeof(ViewDesc) == 256, "ViewDesc layout");

@@ -102,6 +102,8 @@ struct rz_views {
   size_t sortTempBytes = 0;
   int waves = 4;  // wave64s per raster work-group
   int groupsPerCU = 0;  // persistent raster work-groups per CU (0: as many as fit)
+  uint32_t* d_lists = nullptr;    // per-tile item lists (null: tiles walk their view's item range)
+  uint32_t* d_listLen = nullptr;  // per (view, tile) unit
   bool gated = false;   // query-gated draw loop per view (Main.cpp:192-206) instead of classify-then-draw
   bool cooperative = true;  // gated launches: cooperative dispatch (RZ_FRAME_LAUNCH=plain: plain)
   uint32_t* d_err = nullptr;  // device error word of gated waits
@@ -141,6 +143,9 @@ static void free_plan(rz_views* v) {
   hipFree(v->d_order);
   hipFree(v->d_bitems);
   hipFree(v->d_bstate);
+  hipFree(v->d_lists);
+  hipFree(v->d_listLen);
+  v->d_lists = v->d_listLen = nullptr;
   hipFree(v->d_itemScores);
   hipFree(v->d_itemScoresSorted);
   hipFree(v->d_itemIota);
@@ -416,6 +421,22 @@ static int build_plan(rz_views* v) {
   // views without one keep their batches in occluder order.
   v->batchOrder = v->model != nullptr && !v->byOccluder;
   for (const ViewHost& h : v->views) v->batchOrder |= !h.order.empty();
+  {
+    // Per-tile item lists: each unit gets room for all of its view's items
+    // (or batches); skipped when that would not fit 32-bit offsets or 4 GB,
+    // in which case tiles walk their view's whole range.
+    uint64_t total = 0;
+    for (uint32_t i = 0; i < nv; ++i) {
+      ViewDesc& d = v->h_views[i];
+      d.listBase = uint32_t(total);
+      total += uint64_t(d.nUnits) * (v->batchOrder ? d.nBatches : d.nItems);
+    }
+    if (total > 0 && total < (1ull << 30)) {
+      if (dev_alloc(&v->d_lists, size_t(total)) != hipSuccess ||
+          dev_alloc(&v->d_listLen, std::max<size_t>(v->nUnits, 1)) != hipSuccess)
+        return RZ_ERR_OUT_OF_MEMORY;
+    }
+  }
   if (v->model && v->byOccluder) {
     const size_t ni = std::max<size_t>(v->nItems, 1);
     std::vector<uint32_t> iota(v->nItems), seg(nv + 1);
@@ -568,6 +589,9 @@ static int enqueue_step(rz_views* v, bool capturing) {
     // have read the previous step's).
     rp.labels = (v->labels || v->model) ? v->d_labels : nullptr;
     if (rp.labels) RZ_HIP(launch_fill_u32(v->d_labels, v->nBatches, 0u, v->stream));
+    rp.lists = v->d_lists;
+    rp.listLen = v->d_listLen;
+    RZ_HIP(launch_views_bin(rp, v->stream));
     rp.done = v->gated ? v->d_done : nullptr;
     rp.err = v->d_err;
     rp.timeoutTicks = kGatedTimeoutTicks;
