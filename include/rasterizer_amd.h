@@ -343,8 +343,11 @@ int rz_mlp_train_step(rz_mlp* mlp, uint32_t n, const float* x, uint32_t nPairs, 
  * batches with the model (from this step's features and the previous step's
  * labels).  byOccluder == 0 draws the batches by descending score; != 0 draws
  * whole occluders by their best batch score (no batch items, so it scales to
- * large occluder sets).  Ties keep occluder order.  NULL restores occluder
- * order.  The model must outlive its use. */
+ * large occluder sets).  Only batches of drawn (not frustum-culled)
+ * occluders are scored; the others score -infinity, so they come last (their
+ * occluders score -FLT_MAX, as occluders without batches).  Ties keep
+ * occluder order.  NULL restores occluder order.  The model must outlive its
+ * use. */
 int rz_views_set_model(rz_views* views, rz_mlp* model, int byOccluder);
 /* The order of the last learned step: view-local batch numbers, or occluder
  * numbers when ordering by occluder. */

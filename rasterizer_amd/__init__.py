@@ -543,7 +543,8 @@ class Views:
 
     def set_model(self, mlp: "Mlp | None", by_occluder: bool = False) -> None:
         """Learned order: batches by descending score, or with by_occluder
-        whole occluders by their best batch's score."""
+        whole occluders by their best batch's score.  Batches of culled
+        occluders score -inf (occluders: -FLT_MAX) and come last."""
         self._model = mlp
         self._by_occluder = by_occluder and mlp is not None
         _check(lib().rz_views_set_model(self._v, mlp._m if mlp is not None else None, 1 if by_occluder else 0),

@@ -166,8 +166,10 @@ hipError_t launch_fill_u16(uint16_t* p, uint32_t n, uint16_t v, hipStream_t stre
 // 8x8-block tiles.
 hipError_t launch_views(const ViewsParams& p, uint32_t groups, hipStream_t stream, bool cooperative = false);
 hipError_t views_max_groups(int waves, int* maxGroups);
-// Builds p.lists / p.listLen from p.items, p.state (after prep and setup).
-hipError_t launch_views_bin(const ViewsParams& p, hipStream_t stream);
+// Builds p.lists / p.listLen from p.state (after prep and setup); tileBox:
+// nItems u64 of scratch (the items of p.items / p.state, batch items
+// included).
+hipError_t launch_views_bin(const ViewsParams& p, uint32_t nItems, uint64_t* tileBox, hipStream_t stream);
 hipError_t launch_views_query(const ViewsQueryParams& p, hipStream_t stream);
 // hiZ := 1 (whole 8-entry groups) and tile min := 1 for every view.
 hipError_t launch_views_clear(const ViewDesc* views, uint32_t nViews, uint32_t maxEntries, hipStream_t stream);
@@ -188,6 +190,8 @@ struct LearnParams {
   const uint64_t* cov;
   uint32_t* labels;    // per batch
   float* features;     // per batch, kFeatures floats
+  uint32_t* drawn;     // optional: the batches whose occluder is drawn, in no particular order
+  uint32_t* nDrawn;    // their count (zero it before the features launch)
 };
 constexpr int kFeatures = 16;
 
@@ -219,6 +223,8 @@ struct MlpParams {
   const float* norm;
   const float* x;       // n x 16
   uint32_t n;
+  const uint32_t* rows;   // optional (forward): score only samples rows[0 .. *nRows), nRows[0] <= n
+  const uint32_t* nRows;
   float* scores;        // forward output
   const float* dscore;  // backward input: dLoss/dscore per sample
   float* grad;          // backward output, accumulated (zero it first)

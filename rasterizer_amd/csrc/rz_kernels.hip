@@ -548,8 +548,13 @@ __global__ void __launch_bounds__(256) rz_setup_kernel(SetupParams p) {
         box = quad_box(o.rec.fb, o.rec.range & 0xFFFFu, o.rec.range >> 16, bX, bY);
       }
     }
-    if (!valid) p.recs[g].range = 0u;  // an empty rectangle marks the quad as not drawn
-    p.bbox[g] = box;
+    // Records and boxes of items that are not drawn are never read (tiles,
+    // binning and features test the item's status first), so only drawn
+    // items write them; an empty rectangle marks a quad as not drawn.
+    if (st != kStatusSkip) {
+      if (!valid) p.recs[g].range = 0u;
+      p.bbox[g] = box;
+    }
   }
   // Footprint of each item's valid quads: one atomic per wave when the whole
   // wave works on one item, else one per valid lane.
@@ -1694,12 +1699,41 @@ __global__ void __launch_bounds__(64 * NW, NW == 8 ? 4 : 2) rz_views_kernel(View
   }
 }
 
+// Per item of a multi-view launch, the box in 8x8-block tiles (u16 x0, x1,
+// y0, y1) that bounds every tile it is relevant to: the union of its
+// footprint's tiles and, for a query-gated item, its query rectangle's;
+// empty (x0 > x1) when the item is not drawn.
+__global__ void __launch_bounds__(256) rz_item_tiles_kernel(ViewsParams p, uint32_t n, uint64_t* __restrict__ out) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const ItemState& S = p.state[i];
+  uint32_t x0 = 0xFFFFu, x1 = 0u, y0 = 0xFFFFu, y1 = 0u;
+  if (S.status != kStatusSkip) {
+    if (S.fp[0] <= S.fp[1]) {
+      x0 = S.fp[0] >> 3;
+      x1 = S.fp[1] >> 3;
+      y0 = S.fp[2] >> 3;
+      y1 = S.fp[3] >> 3;
+    }
+    if (S.status == kStatusTest) {
+      x0 = min(x0, S.qMinX >> 6);
+      x1 = max(x1, S.qMaxX >> 6);
+      y0 = min(y0, S.qMinY >> 6);
+      y1 = max(y1, S.qMaxY >> 6);
+    }
+  }
+  out[i] = uint64_t(min(x0, 0xFFFFu)) | uint64_t(min(x1, 0xFFFFu)) << 16 | uint64_t(min(y0, 0xFFFFu)) << 32 |
+           uint64_t(min(y1, 0xFFFFu)) << 48;
+}
+
 // Per-tile item lists of a multi-view launch: for each (view, tile) unit, the
-// indices of the view's items relevant to the tile (tile_relevance), in
-// submission order, at lists[views[v].listBase + tile * (items in view)],
-// and their count in listLen[unit].  One work-group per unit tests the items
-// 256 at a time and compacts them with wave ballots.
-__global__ void __launch_bounds__(256) rz_views_bin_kernel(ViewsParams p) {
+// indices of the view's items whose tile box (rz_item_tiles_kernel) holds the
+// tile, in submission order, at lists[views[v].listBase + tile * (items in
+// view)], and their count in listLen[unit].  A box can hold a tile the item is
+// not relevant to; the raster kernel's exact test drops those.  One
+// work-group per unit tests the items 256 at a time and compacts them with
+// wave ballots.
+__global__ void __launch_bounds__(256) rz_views_bin_kernel(ViewsParams p, const uint64_t* __restrict__ tileBox) {
   __shared__ uint32_t wcnt[4];
   const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
   for (uint32_t unit = blockIdx.x; unit < p.nUnits; unit += gridDim.x) {
@@ -1720,7 +1754,12 @@ __global__ void __launch_bounds__(256) rz_views_bin_kernel(ViewsParams p) {
     uint32_t n = 0;
     for (uint32_t base = begin; base < end; base += 256u) {
       const uint32_t i = base + tid;
-      const bool rel = i < end && tile_relevance<8, 8>(p.state[i], tX, tY, tbx0, tbx1, tby0, tby1) != 0u;
+      bool rel = false;
+      if (i < end) {
+        const uint64_t b = tileBox[i];
+        rel = uint32_t(b & 0xFFFF) <= tX && tX <= uint32_t((b >> 16) & 0xFFFF) && uint32_t((b >> 32) & 0xFFFF) <= tY &&
+              tY <= uint32_t(b >> 48);
+      }
       const uint64_t m = __ballot(rel);
       const uint32_t pre = __builtin_amdgcn_mbcnt_hi(uint32_t(m >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(m), 0u));
       if (lane == 0) wcnt[wave] = uint32_t(__popcll(m));
@@ -1739,9 +1778,13 @@ __global__ void __launch_bounds__(256) rz_views_bin_kernel(ViewsParams p) {
   }
 }
 
-hipError_t launch_views_bin(const ViewsParams& p, hipStream_t stream) {
+hipError_t launch_views_bin(const ViewsParams& p, uint32_t nItems, uint64_t* tileBox, hipStream_t stream) {
   if (p.nUnits == 0 || !p.lists) return hipSuccess;
-  hipLaunchKernelGGL(rz_views_bin_kernel, dim3(std::min<uint32_t>(p.nUnits, 65535u)), dim3(256), 0, stream, p);
+  if (nItems) hipLaunchKernelGGL(rz_item_tiles_kernel, dim3((nItems + 255) / 256), dim3(256), 0, stream, p, nItems, tileBox);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(rz_views_bin_kernel, dim3(std::min<uint32_t>(p.nUnits, 65535u)), dim3(256), 0, stream, p,
+                     tileBox);
   return hipGetLastError();
 }
 
@@ -1922,6 +1965,7 @@ __global__ void __launch_bounds__(256) rz_features_kernel(LearnParams p, uint32_
 #pragma unroll
       for (int a = 0; a < 3; ++a) cen[a] += P[j][a] * (1.0f / 32.0f);
     }
+    if (S.status == kStatusSkip) continue;  // not drawn: no valid quads (records not written)
     const QuadRec& R = p.recs[g];
     const uint32_t rx = R.range & 0xFFFFu, ry = R.range >> 16;
     if (rx == 0u || ry == 0u) continue;
@@ -1978,6 +2022,19 @@ __global__ void __launch_bounds__(256) rz_features_kernel(LearnParams p, uint32_
   f[14] = V.nBatches > 1 ? float(b - V.batchBase) / float(V.nBatches - 1) : 0.0f;
 #pragma unroll
   for (int k = 0; k < kFeatures; ++k) p.features[size_t(b) * kFeatures + k] = f[k];
+  if (p.drawn) {
+    // Append the batch to the drawn list, one atomic per wave.
+    const bool drawn = S.status != kStatusSkip;
+    const uint64_t m = __ballot(drawn);
+    if (m != 0ull) {
+      const uint32_t lane = threadIdx.x & 63u, leader = uint32_t(__builtin_ctzll(m));
+      uint32_t base = 0;
+      if (lane == leader) base = atomicAdd(p.nDrawn, uint32_t(__popcll(m)));
+      base = uint32_t(__builtin_amdgcn_readlane(int(base), int(leader)));
+      const uint32_t pre = __builtin_amdgcn_mbcnt_hi(uint32_t(m >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(m), 0u));
+      if (drawn) p.drawn[base + pre] = b;
+    }
+  }
 }
 
 // Item k of a reordered draw: batch order[k] alone, with its occluder's
@@ -1994,7 +2051,7 @@ __global__ void __launch_bounds__(256) rz_batch_items_kernel(LearnParams p, cons
   D.quadBase = b * 8u;
   D.quadCount = 8u;
   uint32_t c0 = 0xFFFFFFFFu, c1 = 0, r0 = 0xFFFFFFFFu, r1 = 0;
-  for (uint32_t j = 0; j < 8; ++j) {
+  for (uint32_t j = 0; j < 8 && S.status != kStatusSkip; ++j) {  // no boxes are written for skipped occluders
     const uint64_t bb = p.bbox[b * 8u + j];
     const uint32_t bc0 = uint32_t(bb & 0xFFFF), bc1 = uint32_t((bb >> 16) & 0xFFFF);
     if (bc0 > bc1) continue;  // empty box: quad not drawn

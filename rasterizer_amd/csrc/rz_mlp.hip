@@ -66,17 +66,23 @@ __device__ void stage_weights(MlpWeights& W, const float* __restrict__ w) {
   }
 }
 
-// Normalised inputs of the wave's 32 samples (zeros past n) into xn / xnt.
-__device__ void stage_inputs(MlpWave& S, const MlpParams& p, uint32_t base, uint32_t lane) {
-  const uint32_t s = lane >> 1, f0 = (lane & 1u) * 8u, g = base + s;
+// Normalised inputs of the wave's 32 samples (zeros past n) into xn and,
+// for the backward pass (TRANSPOSED), xnt.  Sample s of the wave is row
+// base + s of x, or row rows[base + s] when a row list is given (n = its
+// length).
+template <bool TRANSPOSED>
+__device__ void stage_inputs(MlpWave& S, const MlpParams& p, uint32_t base, uint32_t n, uint32_t lane) {
+  const uint32_t s = lane >> 1, f0 = (lane & 1u) * 8u;
+  const bool in = base + s < n;
+  const uint32_t g = !in ? 0u : p.rows ? p.rows[base + s] : base + s;
 #pragma unroll
   for (uint32_t j = 0; j < 8; ++j) {
     const uint32_t f = f0 + j;
-    const float x = g < p.n ? (p.x[size_t(g) * 16 + f] - p.norm[f]) * p.norm[16 + f] : 0.0f;
+    const float x = in ? (p.x[size_t(g) * 16 + f] - p.norm[f]) * p.norm[16 + f] : 0.0f;
     const __bf16 b = __bf16(x);
     S.xn[s * kPadK32 + f] = b;
     S.xn[s * kPadK32 + 16 + f] = __bf16(0.0f);
-    S.xnt[f * kPadK32 + s] = b;
+    if (TRANSPOSED) S.xnt[f * kPadK32 + s] = b;
   }
 }
 
@@ -86,7 +92,9 @@ struct Acts {
 };
 
 // Forward of the wave's 32 samples: z1, z2 (pre-activations), h2 in
-// registers, h1 (BF16) in LDS in both layouts.
+// registers, h1 (BF16) in LDS, and for the backward pass (TRANSPOSED) also
+// transposed.
+template <bool TRANSPOSED>
 __device__ void forward_wave(const MlpWeights& W, MlpWave& S, const float* __restrict__ w, uint32_t lane, Acts& a) {
   const uint32_t col = lane & 15u, rq = (lane >> 4) * 4u;
 #pragma unroll
@@ -103,7 +111,7 @@ __device__ void forward_wave(const MlpWeights& W, MlpWave& S, const float* __res
         const __bf16 h = __bf16(gelu(z));
         const uint32_t row = 16u * rt + rq + i, hid = 16u * t + col;
         S.h1[row * kPadK64 + hid] = h;
-        S.h1t[hid * kPadK32 + row] = h;
+        if (TRANSPOSED) S.h1t[hid * kPadK32 + row] = h;
       }
     }
   __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0): the wave's LDS writes are done
@@ -147,12 +155,13 @@ __global__ void __launch_bounds__(256) rz_mlp_forward_kernel(MlpParams p) {
   __syncthreads();
   const uint32_t col = lane & 15u, rq = (lane >> 4) * 4u;
   const float b3 = p.w[kMlpB3];
-  for (uint32_t base0 = blockIdx.x * 128u; base0 < p.n; base0 += gridDim.x * 128u) {
+  const uint32_t n = p.rows ? min(*p.nRows, p.n) : p.n;
+  for (uint32_t base0 = blockIdx.x * 128u; base0 < n; base0 += gridDim.x * 128u) {
     const uint32_t base = base0 + 32u * wave;
-    stage_inputs(S, p, base, lane);
+    stage_inputs<false>(S, p, base, n, lane);
     __syncthreads();
     Acts a;
-    forward_wave(W, S, p.w, lane, a);
+    forward_wave<false>(W, S, p.w, lane, a);
 #pragma unroll
     for (int rt = 0; rt < 2; ++rt)
 #pragma unroll
@@ -161,8 +170,8 @@ __global__ void __launch_bounds__(256) rz_mlp_forward_kernel(MlpParams p) {
 #pragma unroll
         for (int t = 0; t < 4; ++t) part += a.h2[rt][t][i] * p.w[kMlpW3 + 16 * t + col];
         const float sc = sum16(part) + b3;
-        const uint32_t g = base + 16u * rt + rq + i;
-        if (col == 0 && g < p.n) p.scores[g] = sc;
+        const uint32_t s = base + 16u * rt + rq + i;
+        if (col == 0 && s < n) p.scores[p.rows ? p.rows[s] : s] = sc;
       }
     __syncthreads();
   }
@@ -187,10 +196,10 @@ __global__ void __launch_bounds__(256) rz_mlp_backward_kernel(MlpParams p) {
   float db3 = 0.0f;
   for (uint32_t base0 = blockIdx.x * 128u; base0 < p.n; base0 += gridDim.x * 128u) {
     const uint32_t base = base0 + 32u * wave;
-    stage_inputs(S, p, base, lane);
+    stage_inputs<true>(S, p, base, p.n, lane);
     __syncthreads();
     Acts a;
-    forward_wave(W, S, p.w, lane, a);
+    forward_wave<true>(W, S, p.w, lane, a);
     // Layer 3 and layer 2's activation: dz2 = ds * W3 * gelu'(z2).
 #pragma unroll
     for (int rt = 0; rt < 2; ++rt)

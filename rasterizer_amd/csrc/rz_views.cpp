@@ -18,6 +18,7 @@
 #include <cstring>
 #include <string>
 #include <new>
+#include <limits>
 #include <vector>
 
 #include "rasterizer_amd.h"
@@ -98,12 +99,14 @@ struct rz_views {
   float* d_sortedScores = nullptr;
   uint32_t* d_iota = nullptr;
   uint32_t* d_segments = nullptr;  // nViews + 1 batch offsets
+  uint32_t* d_drawn = nullptr;     // batches of drawn occluders (scored by the model), count in d_drawn[nBatches]
   void* d_sortTemp = nullptr;
   size_t sortTempBytes = 0;
   int waves = 4;  // wave64s per raster work-group
   int groupsPerCU = 0;  // persistent raster work-groups per CU (0: as many as fit)
   uint32_t* d_lists = nullptr;    // per-tile item lists (null: tiles walk their view's item range)
   uint32_t* d_listLen = nullptr;  // per (view, tile) unit
+  uint64_t* d_tileBox = nullptr;  // per drawn item (occluder or batch): tile box, for binning
   bool gated = false;   // query-gated draw loop per view (Main.cpp:192-206) instead of classify-then-draw
   bool cooperative = true;  // gated launches: cooperative dispatch (RZ_FRAME_LAUNCH=plain: plain)
   uint32_t* d_err = nullptr;  // device error word of gated waits
@@ -145,7 +148,9 @@ static void free_plan(rz_views* v) {
   hipFree(v->d_bstate);
   hipFree(v->d_lists);
   hipFree(v->d_listLen);
+  hipFree(v->d_tileBox);
   v->d_lists = v->d_listLen = nullptr;
+  v->d_tileBox = nullptr;
   hipFree(v->d_itemScores);
   hipFree(v->d_itemScoresSorted);
   hipFree(v->d_itemIota);
@@ -162,8 +167,9 @@ static void free_plan(rz_views* v) {
   hipFree(v->d_iota);
   hipFree(v->d_segments);
   hipFree(v->d_sortTemp);
+  hipFree(v->d_drawn);
   v->d_scores = v->d_sortedScores = nullptr;
-  v->d_iota = v->d_segments = nullptr;
+  v->d_iota = v->d_segments = v->d_drawn = nullptr;
   v->d_sortTemp = nullptr;
   v->sortTempBytes = 0;
   v->d_labels = nullptr;
@@ -432,8 +438,10 @@ static int build_plan(rz_views* v) {
       total += uint64_t(d.nUnits) * (v->batchOrder ? d.nBatches : d.nItems);
     }
     if (total > 0 && total < (1ull << 30)) {
+      const size_t nDraw = std::max<size_t>(v->batchOrder ? v->nBatches : v->nItems, 1);
       if (dev_alloc(&v->d_lists, size_t(total)) != hipSuccess ||
-          dev_alloc(&v->d_listLen, std::max<size_t>(v->nUnits, 1)) != hipSuccess)
+          dev_alloc(&v->d_listLen, std::max<size_t>(v->nUnits, 1)) != hipSuccess ||
+          dev_alloc(&v->d_tileBox, nDraw) != hipSuccess)
         return RZ_ERR_OUT_OF_MEMORY;
     }
   }
@@ -457,7 +465,8 @@ static int build_plan(rz_views* v) {
     for (uint32_t i = 0; i < nv; ++i) seg[i] = v->h_views[i].batchBase;
     seg[nv] = v->nBatches;
     if (dev_alloc(&v->d_scores, nbt) != hipSuccess || dev_alloc(&v->d_sortedScores, nbt) != hipSuccess ||
-        dev_alloc(&v->d_iota, nbt) != hipSuccess || dev_alloc(&v->d_segments, nv + 1) != hipSuccess)
+        dev_alloc(&v->d_iota, nbt) != hipSuccess || dev_alloc(&v->d_segments, nv + 1) != hipSuccess ||
+        dev_alloc(&v->d_drawn, nbt + 1) != hipSuccess)
       return RZ_ERR_OUT_OF_MEMORY;
     if (v->nBatches) RZ_HIP(hipMemcpy(v->d_iota, iota.data(), iota.size() * 4, hipMemcpyHostToDevice));
     RZ_HIP(hipMemcpy(v->d_segments, seg.data(), seg.size() * 4, hipMemcpyHostToDevice));
@@ -507,6 +516,10 @@ static LearnParams learn_params(rz_views* v) {
   lp.cov = v->d_cov;
   lp.labels = v->d_labels;
   lp.features = v->d_features;
+  if (v->model) {
+    lp.drawn = v->d_drawn;
+    lp.nDrawn = v->d_drawn + v->nBatches;
+  }
   return lp;
 }
 
@@ -544,15 +557,20 @@ static int enqueue_step(rz_views* v, bool capturing) {
     sp.views = v->d_views;
     RZ_HIP(launch_setup(sp, v->stream));
     const LearnParams lp = learn_params(v);
+    if (lp.nDrawn) RZ_HIP(launch_fill_u32(lp.nDrawn, 1, 0u, v->stream));
     if (v->labels || v->model) RZ_HIP(launch_features(lp, v->nBatches, v->stream));
     if (v->model && v->nBatches) {
-      // Score every batch, then order each view's batches by descending
-      // score; the radix sort is stable, so ties keep occluder order.
+      // Score the batches of drawn occluders (the others score -inf), then
+      // order each view's batches by descending score; the radix sort is
+      // stable, so ties keep occluder order.
+      RZ_HIP(launch_fill_f32(v->d_scores, v->nBatches, -std::numeric_limits<float>::infinity(), v->stream));
       MlpParams mp{};
       mp.w = v->model->d_w;
       mp.norm = v->model->d_norm;
       mp.x = v->d_features;
       mp.n = v->nBatches;
+      mp.rows = lp.drawn;
+      mp.nRows = lp.nDrawn;
       mp.scores = v->d_scores;
       RZ_HIP(launch_mlp_forward(mp, v->stream));
       size_t bytes = v->sortTempBytes;
@@ -591,7 +609,7 @@ static int enqueue_step(rz_views* v, bool capturing) {
     if (rp.labels) RZ_HIP(launch_fill_u32(v->d_labels, v->nBatches, 0u, v->stream));
     rp.lists = v->d_lists;
     rp.listLen = v->d_listLen;
-    RZ_HIP(launch_views_bin(rp, v->stream));
+    RZ_HIP(launch_views_bin(rp, v->batchOrder ? v->nBatches : v->nItems, v->d_tileBox, v->stream));
     rp.done = v->gated ? v->d_done : nullptr;
     rp.err = v->d_err;
     rp.timeoutTicks = kGatedTimeoutTicks;

@@ -221,6 +221,15 @@ def test_features_match_definition(oracle_mod, scene):
         assert np.all(err <= 1e-4 + 1e-4 * np.abs(want)), (np.unravel_index(np.argmax(err), err.shape), err.max())
 
 
+def _scores_of_drawn(m, feats, occ, st):
+    """Model scores of each batch of a view; batches of occluders the oracle
+    did not draw (status 0) score -inf."""
+    scores = m.forward(feats)
+    drawn = np.repeat(np.asarray(st) != 0, [int(o.packets.size) // 32 for o in occ])
+    assert drawn.size == scores.size
+    return np.where(drawn, scores, -np.inf).astype(np.float32)
+
+
 def test_learned_order_is_score_order_and_exact(oracle_mod, scene, report):
     """With a model set, each view draws its batches by descending model score
     (ties in occluder order); the buffer equals the oracle drawing the batches
@@ -243,14 +252,14 @@ def test_learned_order_is_score_order_and_exact(oracle_mod, scene, report):
     for k, (w, h) in enumerate(SIZES):
         mvp, occ = setup[k]
         _, feats = V.labels(k)
-        scores = m.forward(feats)
-        want = np.lexsort((np.arange(scores.size), -scores))
-        got = V.order(k)
-        assert np.array_equal(got, want), f"view {k}: order is not descending score"
         o0 = oracle_mod.OracleRasterizer(w, h)
         o0.clear()
         o0.setModelViewProjection(mvp)
         st = o0.draw_frame(occ, gated=False)
+        scores = _scores_of_drawn(m, feats, occ, st)
+        want = np.lexsort((np.arange(scores.size), -scores))
+        got = V.order(k)
+        assert np.array_equal(got, want), f"view {k}: order is not descending score"
         o = _oracle_batch_draw(oracle_mod, w, h, mvp, occ, st, got)
         dg, hg = V.raw(k)
         do, ho = o.raw()
@@ -284,10 +293,14 @@ def test_learned_occluder_order_is_score_order_and_exact(oracle_mod, scene, repo
     for k, (w, h) in enumerate(SIZES):
         mvp, occ = setup[k]
         _, feats = V.labels(k)
-        scores = m.forward(feats)
+        o0 = oracle_mod.OracleRasterizer(w, h)
+        o0.clear()
+        o0.setModelViewProjection(mvp)
+        scores = _scores_of_drawn(m, feats, occ, o0.draw_frame(occ, gated=False))
         nb = [int(o.packets.size) // 32 for o in occ]
         bounds = np.concatenate([[0], np.cumsum(nb)]).astype(int)
-        best = np.array([scores[bounds[i]:bounds[i + 1]].max() if nb[i] else -np.float32(3.402823466e38)
+        fmax = np.float32(3.402823466e38)
+        best = np.array([max(scores[bounds[i]:bounds[i + 1]].max(), -fmax) if nb[i] else -fmax
                          for i in range(len(occ))], np.float32)
         want = np.lexsort((np.arange(best.size), -best))
         got = V.order(k)
