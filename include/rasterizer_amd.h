@@ -357,6 +357,11 @@ int rz_views_read_order(rz_views* views, uint32_t view, uint32_t* order);
  * visited, those rejected by the hiZ or edge tests, and blocks written. */
 int rz_views_enable_stats(rz_views* views, int enable);
 int rz_views_read_stats(rz_views* views, uint64_t out[3]);
+/* The per-tile counters of the last step's raster launch (stats enabled),
+ * summed over all (view, tile) units, in the 22-word layout of
+ * rz_frame_stats_read; word 0 is the sum of the persistent work-groups'
+ * lifetimes, in 100 MHz ticks like the other tick words. */
+int rz_views_read_tile_stats(rz_views* views, uint64_t out[22]);
 
 #ifdef __cplusplus
 }

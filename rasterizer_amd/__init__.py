@@ -118,6 +118,7 @@ _SIGNATURES = {
     "rz_views_read_order": ([_P, _U32, _P], ctypes.c_int),
     "rz_views_enable_stats": ([_P, ctypes.c_int], ctypes.c_int),
     "rz_views_read_stats": ([_P, _P], ctypes.c_int),
+    "rz_views_read_tile_stats": ([_P, _P], ctypes.c_int),
     "rz_get_config": ([_P, _P], ctypes.c_int),
     "rz_set_config": ([_P, _U32, _U32], ctypes.c_int),
     "rz_tune_frame": ([_P, _U32, _P, _U32, ctypes.c_char_p, _P, _P], ctypes.c_int),
@@ -558,6 +559,14 @@ class Views:
         a = np.zeros(3, np.uint64)
         _check(lib().rz_views_read_stats(self._v, _ptr(a)), "rz_views_read_stats")
         return {"visited": int(a[0]), "rejected": int(a[1]), "written": int(a[2])}
+
+    def tile_stats(self) -> np.ndarray:
+        """Per-tile counters of the last step summed over all units (22 words,
+        the layout of Rasterizer.frame_stats_read; word 0: work-group
+        lifetimes)."""
+        a = np.zeros(22, np.uint64)
+        _check(lib().rz_views_read_tile_stats(self._v, _ptr(a)), "rz_views_read_tile_stats")
+        return a
 
     def order(self, view: int) -> np.ndarray:
         n = len(self._occ[view]) if getattr(self, "_by_occluder", False) else self.n_batches(view)

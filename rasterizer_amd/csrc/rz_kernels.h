@@ -34,6 +34,7 @@ struct SetupParams {
   uint64_t* bbox;
   uint64_t* groupBox;  // box of each aligned group of 64 quads (union of valid quads)
   const ViewDesc* views;  // multi-view launches: bX, bY from views[item.view]
+  const uint32_t* groupItem;  // optional: per group of 64 quads, the item of its first quad
 };
 
 // Persistent multi-view raster launch: work units are (view, tile) pairs taken
@@ -190,6 +191,7 @@ struct LearnParams {
   const uint64_t* cov;
   uint32_t* labels;    // per batch
   float* features;     // per batch, kFeatures floats
+  const uint32_t* groupItem;  // optional: per group of 64 quads, the item of its first quad
   uint32_t* drawn;     // optional: the batches whose occluder is drawn, in no particular order
   uint32_t* nDrawn;    // their count (zero it before the features launch)
 };

@@ -514,6 +514,26 @@ __device__ uint64_t quad_box(uint32_t fb, uint32_t rangeX, uint32_t rangeY, uint
   return uint64_t(c0) | (uint64_t(c1) << 16) | (uint64_t(r0) << 32) | (uint64_t(r1) << 48);
 }
 
+// The item that holds quad g: the last item whose quadBase <= g (items are
+// in quad order; empty items share their successor's quadBase).  With a
+// group table (groupItem[k] = the item of quad 64k) the search starts there
+// and steps forward over the items that begin inside the 64-quad group.
+__device__ __forceinline__ uint32_t item_of_quad(const ItemDesc* __restrict__ items, uint32_t nItems, uint32_t g,
+                                                 const uint32_t* __restrict__ groupItem = nullptr) {
+  if (groupItem) {
+    uint32_t i = groupItem[g >> 6];
+    while (i + 1 < nItems && items[i + 1].quadBase <= g) ++i;
+    return i;
+  }
+  uint32_t lo = 0, hi = nItems - 1;
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi + 1) >> 1;
+    if (items[mid].quadBase <= g) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
 __global__ void __launch_bounds__(256) rz_setup_kernel(SetupParams p) {
   const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
   const bool inRange = g < p.nQuads;
@@ -521,13 +541,7 @@ __global__ void __launch_bounds__(256) rz_setup_kernel(SetupParams p) {
   bool valid = false;
   uint64_t box = kEmptyBox;
   if (inRange) {
-    uint32_t lo = 0, hi = p.nItems - 1;
-    while (lo < hi) {
-      const uint32_t mid = (lo + hi + 1) >> 1;
-      if (p.items[mid].quadBase <= g) lo = mid;
-      else hi = mid - 1;
-    }
-    item = lo;
+    item = item_of_quad(p.items, p.nItems, g, p.groupItem);
     const ItemDesc& D = p.items[item];
     const ItemState& S = p.state[item];
     const uint32_t st = S.status;
@@ -1071,29 +1085,56 @@ __device__ uint64_t draw_chunk_block_major(TileSmem<TBX, TBY>& sm, uint32_t n, i
   return tB;
 }
 
-// query2D restricted to this wave's rows of the tile (Rasterizer.cpp:283-349).
+// One row of blocks of query2D inside the tile, one lane per block:
+// returns 1 if the wave finds a visible pixel in row lr, and with SPEC 2 if
+// a block of the rectangle in that row has hiZ 0 or 1; 0 otherwise.  The
+// hiZ tests of the row's blocks run together; only blocks that pass them are
+// tested pixel by pixel.
+template <int TBX, int TBY, bool SPEC>
+__device__ __forceinline__ uint32_t query_row(const ItemLocal& S, const TileSmem<TBX, TBY>& sm, int lr, int by,
+                                              int tbx0, int tbx1, uint32_t lane) {
+  static_assert(TBX <= 64, "one lane per block of a tile row");
+  const uint32_t maxZ = S.qMaxZ;
+  const int qbx0 = int(S.qMinX >> 3), qbx1 = int(S.qMaxX >> 3);
+  const int startY = max(int(S.qMinY) - 8 * by, 0), endY = min(int(S.qMaxY) - 8 * by, 7);
+  const int cx0 = max(tbx0, qbx0), cx1 = min(tbx1, qbx1);
+  const int n = cx1 - cx0 + 1;
+  bool cleared = false, cand = false, full = false;
+  if (int(lane) < n) {
+    const int bx = cx0 + int(lane);
+    const uint32_t hz = sm.hiz[lr * TBX + (bx - tbx0)];
+    cleared = hz <= 1u;
+    cand = maxZ > hz;
+    full = cand && startY == 0 && endY == 7 && int(S.qMinX) <= 8 * bx && int(S.qMaxX) >= 8 * bx + 7;
+  }
+  if (SPEC && __ballot(cleared) != 0ull) return 2u;
+  if (__ballot(full) != 0ull) return 1u;
+  uint64_t m = __ballot(cand);
+  const int px = int(lane & 7u), py = int(lane >> 3);
+  while (m) {
+    const int bx = cx0 + int(__builtin_ctzll(m));
+    m &= m - 1ull;
+    const int lb = lr * TBX + (bx - tbx0);
+    const int startX = max(int(S.qMinX) - 8 * bx, 0), endX = min(int(S.qMaxX) - 8 * bx, 7);
+    const bool mine = py >= startY && py <= endY && px >= startX && px <= endX &&
+                      uint32_t(sm.depth[lb * 64 + int(lane)]) < maxZ;
+    if (__ballot(mine) != 0ull) return 1u;
+  }
+  return 0u;
+}
+
+// query2D restricted to this wave's rows of the tile (Rasterizer.cpp:283-349):
+// true if a pixel of the rectangle in those rows is visible.  Any order of
+// the blocks gives the same answer.
 template <int TBX, int TBY, int NW>
 __device__ __forceinline__ bool query_wave(const ItemLocal& S, TileSmem<TBX, TBY>& sm, int tbx0, int tbx1,
                                            int tby0, int tby1, uint32_t wave, uint32_t lane) {
-  const int qbx0 = int(S.qMinX >> 3), qbx1 = int(S.qMaxX >> 3), qby0 = int(S.qMinY >> 3), qby1 = int(S.qMaxY >> 3);
-  const uint32_t maxZ = S.qMaxZ;
-  const int px = int(lane & 7u), py = int(lane >> 3);
+  const int qby0 = int(S.qMinY >> 3), qby1 = int(S.qMaxY >> 3);
   for (int lr = int(wave); lr < TBY; lr += NW) {
     const int by = tby0 + lr;
     if (by > tby1) break;
     if (by < qby0 || by > qby1) continue;
-    const int sy = int(S.qMinY) - 8 * by, ey = int(S.qMaxY) - 8 * by;
-    const int startY = max(sy, 0), endY = min(ey, 7);
-    const int cx0 = max(tbx0, qbx0), cx1 = min(tbx1, qbx1);
-    for (int bx = cx0; bx <= cx1; ++bx) {
-      const int lb = lr * TBX + (bx - tbx0);
-      if (maxZ <= uint32_t(sm.hiz[lb])) continue;
-      const int startX = max(int(S.qMinX) - 8 * bx, 0), endX = min(int(S.qMaxX) - 8 * bx, 7);
-      if (startY == 0 && endY == 7 && startX == 0 && endX == 7) return true;
-      const bool mine = py >= startY && py <= endY && px >= startX && px <= endX &&
-                        uint32_t(sm.depth[lb * 64 + int(lane)]) < maxZ;
-      if (__ballot(mine) != 0ull) return true;
-    }
+    if (query_row<TBX, TBY, false>(S, sm, lr, by, tbx0, tbx1, lane)) return true;
   }
   return false;
 }
@@ -1110,27 +1151,13 @@ __device__ __forceinline__ bool query_wave(const ItemLocal& S, TileSmem<TBX, TBY
 template <int TBX, int TBY, int NW>
 __device__ __forceinline__ uint32_t spec_query_wave(const ItemLocal& S, TileSmem<TBX, TBY>& sm, int tbx0, int tbx1,
                                                     int tby0, int tby1, uint32_t wave, uint32_t lane) {
-  const int qbx0 = int(S.qMinX >> 3), qbx1 = int(S.qMaxX >> 3), qby0 = int(S.qMinY >> 3), qby1 = int(S.qMaxY >> 3);
-  const uint32_t maxZ = S.qMaxZ;
-  const int px = int(lane & 7u), py = int(lane >> 3);
+  const int qby0 = int(S.qMinY >> 3), qby1 = int(S.qMaxY >> 3);
   for (int lr = int(wave); lr < TBY; lr += NW) {
     const int by = tby0 + lr;
     if (by > tby1) break;
     if (by < qby0 || by > qby1) continue;
-    const int sy = int(S.qMinY) - 8 * by, ey = int(S.qMaxY) - 8 * by;
-    const int startY = max(sy, 0), endY = min(ey, 7);
-    const int cx0 = max(tbx0, qbx0), cx1 = min(tbx1, qbx1);
-    for (int bx = cx0; bx <= cx1; ++bx) {
-      const int lb = lr * TBX + (bx - tbx0);
-      const uint32_t hz = sm.hiz[lb];
-      if (hz <= 1u) return 2u;
-      if (maxZ <= hz) continue;
-      const int startX = max(int(S.qMinX) - 8 * bx, 0), endX = min(int(S.qMaxX) - 8 * bx, 7);
-      if (startY == 0 && endY == 7 && startX == 0 && endX == 7) return 1u;
-      const bool mine = py >= startY && py <= endY && px >= startX && px <= endX &&
-                        uint32_t(sm.depth[lb * 64 + int(lane)]) < maxZ;
-      if (__ballot(mine) != 0ull) return 1u;
-    }
+    const uint32_t r = query_row<TBX, TBY, true>(S, sm, lr, by, tbx0, tbx1, lane);
+    if (r) return r;
   }
   return 0u;
 }
@@ -1652,6 +1679,7 @@ __global__ void __launch_bounds__(64 * NW, NW == 8 ? 4 : 2) rz_views_kernel(View
   __shared__ uint32_t unitSh;
   __shared__ uint64_t ctr[kFrameStatWords];
   if (STATS && threadIdx.x < kFrameStatWords) ctr[threadIdx.x] = 0;
+  const uint64_t tEntry = STATS ? __builtin_amdgcn_s_memrealtime() : 0;
   bool covLoaded = false;
   while (true) {
     if (threadIdx.x == 0) unitSh = __hip_atomic_fetch_add(p.queue, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -1692,10 +1720,20 @@ __global__ void __launch_bounds__(64 * NW, NW == 8 ? 4 : 2) rz_views_kernel(View
     process_tile<TBX, TBY, NW, STATS, LABELS>(J, sm, p.cov, covLoaded, ctr);
     __syncthreads();
   }
-  if (STATS && threadIdx.x == 0) {
-    atomicAdd(&p.stats[0], (unsigned long long)ctr[kStatBlocksVisited]);
-    atomicAdd(&p.stats[1], (unsigned long long)ctr[kStatBlocksRejected]);
-    atomicAdd(&p.stats[2], (unsigned long long)ctr[kStatBlocksWritten]);
+  if (STATS) {
+    // Words 0-2: block-loop counters; 3 onwards: the per-tile counters of
+    // every unit summed, word 3 + kStatTotalTicks being the work-groups'
+    // lifetimes.
+    __syncthreads();
+    if (threadIdx.x == 0) ctr[kStatTotalTicks] = __builtin_amdgcn_s_memrealtime() - tEntry;
+    __syncthreads();
+    if (threadIdx.x < kFrameStatWords)
+      atomicAdd(&p.stats[3 + threadIdx.x], (unsigned long long)ctr[threadIdx.x]);
+    if (threadIdx.x == 0) {
+      atomicAdd(&p.stats[0], (unsigned long long)ctr[kStatBlocksVisited]);
+      atomicAdd(&p.stats[1], (unsigned long long)ctr[kStatBlocksRejected]);
+      atomicAdd(&p.stats[2], (unsigned long long)ctr[kStatBlocksWritten]);
+    }
   }
 }
 
@@ -1913,16 +1951,6 @@ __global__ void rz_fill_u16_kernel(uint16_t* p, uint32_t n, uint16_t v) {
 // Draw-order labels and batch features.
 // ---------------------------------------------------------------------------
 // Item whose quad range holds quad g (items are in quad order).
-__device__ __forceinline__ uint32_t item_of_quad(const ItemDesc* __restrict__ items, uint32_t nItems, uint32_t g) {
-  uint32_t lo = 0, hi = nItems - 1;
-  while (lo < hi) {
-    const uint32_t mid = (lo + hi + 1) >> 1;
-    if (items[mid].quadBase <= g) lo = mid;
-    else hi = mid - 1;
-  }
-  return lo;
-}
-
 // 16 features per batch, from its quads' setup records and packets and the
 // labels of the previous step:
 //  0 drawn quads / 8          1 log2(1 + union block-box area)
@@ -1934,7 +1962,7 @@ __device__ __forceinline__ uint32_t item_of_quad(const ItemDesc* __restrict__ it
 __global__ void __launch_bounds__(256) rz_features_kernel(LearnParams p, uint32_t nBatches) {
   const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= nBatches) return;
-  const uint32_t item = item_of_quad(p.items, p.nItems, b * 8u);
+  const uint32_t item = item_of_quad(p.items, p.nItems, b * 8u, p.groupItem);
   const ItemDesc& D = p.items[item];
   const ItemState& S = p.state[item];
   const ViewDesc& V = p.views[D.view];
@@ -2045,7 +2073,7 @@ __global__ void __launch_bounds__(256) rz_batch_items_kernel(LearnParams p, cons
   const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= nBatches) return;
   const uint32_t b = order[k];
-  const uint32_t item = item_of_quad(p.items, p.nItems, b * 8u);
+  const uint32_t item = item_of_quad(p.items, p.nItems, b * 8u, p.groupItem);
   ItemDesc D = p.items[item];
   ItemState S = p.state[item];
   D.quadBase = b * 8u;

@@ -73,6 +73,7 @@ struct rz_views {
   QuadRec* d_recs = nullptr;
   uint64_t* d_bbox = nullptr;
   uint64_t* d_groupBox = nullptr;
+  uint32_t* d_groupItem = nullptr;  // per group of 64 quads: the item of its first quad
   ViewDesc* d_views = nullptr;
   ViewDesc* h_views = nullptr;  // pinned; copied to d_views by every step
   uint32_t* d_queue = nullptr;
@@ -140,6 +141,8 @@ static void free_plan(rz_views* v) {
   hipFree(v->d_recs);
   hipFree(v->d_bbox);
   hipFree(v->d_groupBox);
+  hipFree(v->d_groupItem);
+  v->d_groupItem = nullptr;
   hipFree(v->d_views);
   hipFree(v->d_labels);
   hipFree(v->d_features);
@@ -415,9 +418,22 @@ static int build_plan(rz_views* v) {
       dev_alloc(&v->d_recs, std::max<size_t>(quads, 1)) != hipSuccess ||
       dev_alloc(&v->d_bbox, std::max<size_t>(quads, 1)) != hipSuccess ||
       dev_alloc(&v->d_groupBox, std::max<size_t>((quads + 63) / 64, 1)) != hipSuccess ||
+      dev_alloc(&v->d_groupItem, std::max<size_t>((quads + 63) / 64, 1)) != hipSuccess ||
       dev_alloc(&v->d_views, nv) != hipSuccess)
     return RZ_ERR_OUT_OF_MEMORY;
-  if (!items.empty()) RZ_HIP(hipMemcpy(v->d_items, items.data(), items.size() * sizeof(ItemDesc), hipMemcpyHostToDevice));
+  if (!items.empty()) {
+    RZ_HIP(hipMemcpy(v->d_items, items.data(), items.size() * sizeof(ItemDesc), hipMemcpyHostToDevice));
+    // Item of each 64-quad group's first quad (the last item starting at or
+    // before it), for the setup and features lookups.
+    std::vector<uint32_t> groupItem((quads + 63) / 64);
+    uint32_t it = 0;
+    for (size_t k = 0; k < groupItem.size(); ++k) {
+      while (it + 1 < items.size() && items[it + 1].quadBase <= k * 64) ++it;
+      groupItem[k] = it;
+    }
+    if (!groupItem.empty())
+      RZ_HIP(hipMemcpy(v->d_groupItem, groupItem.data(), groupItem.size() * 4, hipMemcpyHostToDevice));
+  }
   const size_t nbt = std::max<size_t>(v->nBatches, 1);
   if (dev_alloc(&v->d_labels, nbt) != hipSuccess || dev_alloc(&v->d_features, nbt * kFeatures) != hipSuccess)
     return RZ_ERR_OUT_OF_MEMORY;
@@ -516,6 +532,7 @@ static LearnParams learn_params(rz_views* v) {
   lp.cov = v->d_cov;
   lp.labels = v->d_labels;
   lp.features = v->d_features;
+  lp.groupItem = v->d_groupItem;
   if (v->model) {
     lp.drawn = v->d_drawn;
     lp.nDrawn = v->d_drawn + v->nBatches;
@@ -554,6 +571,7 @@ static int enqueue_step(rz_views* v, bool capturing) {
     sp.recs = v->d_recs;
     sp.bbox = v->d_bbox;
     sp.groupBox = v->d_groupBox;
+    sp.groupItem = v->d_groupItem;
     sp.views = v->d_views;
     RZ_HIP(launch_setup(sp, v->stream));
     const LearnParams lp = learn_params(v);
@@ -602,7 +620,8 @@ static int enqueue_step(rz_views* v, bool capturing) {
     rp.queue = v->d_queue;
     rp.waves = uint32_t(v->waves);
     rp.stats = v->stats ? v->d_stats : nullptr;
-    if (v->stats) RZ_HIP(launch_fill_u32(reinterpret_cast<uint32_t*>(v->d_stats), 6, 0u, v->stream));
+    if (v->stats)
+      RZ_HIP(launch_fill_u32(reinterpret_cast<uint32_t*>(v->d_stats), 2 * (3 + kFrameStatWords), 0u, v->stream));
     // Labels are counted by the draw itself (after this step's features
     // have read the previous step's).
     rp.labels = (v->labels || v->model) ? v->d_labels : nullptr;
@@ -805,8 +824,8 @@ extern "C" int rz_views_enable_stats(rz_views* v, int enable) {
   int s = wait_last_step(v);
   if (s != RZ_OK) return s;
   if (enable && !v->d_stats) {
-    if (dev_alloc(&v->d_stats, 3) != hipSuccess) return RZ_ERR_OUT_OF_MEMORY;
-    RZ_HIP(hipMemset(v->d_stats, 0, 3 * sizeof(unsigned long long)));
+    if (dev_alloc(&v->d_stats, 3 + kFrameStatWords) != hipSuccess) return RZ_ERR_OUT_OF_MEMORY;
+    RZ_HIP(hipMemset(v->d_stats, 0, (3 + kFrameStatWords) * sizeof(unsigned long long)));
   }
   v->stats = enable != 0;
   v->dirty = true;
@@ -817,6 +836,14 @@ extern "C" int rz_views_read_stats(rz_views* v, uint64_t out[3]) {
   if (!v || !out || !v->d_stats) return RZ_ERR_INVALID_ARGUMENT;
   RZ_HIP(hipStreamSynchronize(v->stream));
   RZ_HIP(hipMemcpy(out, v->d_stats, 3 * sizeof(uint64_t), hipMemcpyDeviceToHost));
+  return RZ_OK;
+}
+
+extern "C" int rz_views_read_tile_stats(rz_views* v, uint64_t out[22]) {
+  static_assert(kFrameStatWords == 22, "tile stats layout");
+  if (!v || !out || !v->d_stats) return RZ_ERR_INVALID_ARGUMENT;
+  RZ_HIP(hipStreamSynchronize(v->stream));
+  RZ_HIP(hipMemcpy(out, v->d_stats + 3, kFrameStatWords * sizeof(uint64_t), hipMemcpyDeviceToHost));
   return RZ_OK;
 }
 

@@ -363,6 +363,10 @@ def test_trained_order_gains_rejects(scene):
             V.run(graph=True)
             ms = V.sync()
             st = V.stats()
+            # The summed per-tile counters hold the same block counts.
+            ts = V.tile_stats()
+            assert (int(ts[13]), int(ts[14]), int(ts[15])) == (st["visited"], st["rejected"], st["written"])
+            assert ts[0] > 0 and ts[7] > 0
             rates.append((st["rejected"] / max(st["visited"], 1), st["visited"], ms))
         return rates[-1]
 
