@@ -1016,70 +1016,116 @@ __device__ uint64_t draw_chunk_block_major(TileSmem<TBX, TBY>& sm, uint32_t n, i
   const int qx0 = int(rect & 0xFFu), qx1 = int((rect >> 8) & 0xFFu);
   const int qy0 = int((rect >> 16) & 0xFFu), qy1 = int(rect >> 24);
   const uint32_t ce = lane < 12u ? lane : 0u;
-  for (int lb = 0; lb < TBX * TBY; ++lb) {
-    const int lbx = lb % TBX, lby = lb / TBX;
-    if (block_owner<NW>(lbx, lby) != wave) continue;
-    uint32_t hz = sm.hiz[lb];
-    const bool inRect = valid && lbx >= qx0 && lbx <= qx1 && lby >= qy0 && lby <= qy1;
-    const bool cand = inRect && !(hz != 1u && hz >= maxZq);
-    uint64_t m = __ballot(cand);
-    if (STATS) {
-      const uint32_t pre = uint32_t(__popcll(__ballot(inRect) & ~m));
-      bc.visited += pre;
-      bc.rejected += pre;
-    }
-    if (m == 0ull) continue;
-    uint32_t dval = sm.depth[lb * 64 + int(lane)];
-    bool dirty = false;
-    const size_t wIdx = (size_t(tby0 + lby) * size_t(bX) + size_t(tbx0 + lbx)) * 64u + lane;
-    uint32_t win = kNoWinner;
-    if (LABELS && hz != 1u) win = winner[wIdx];
-    // This block's twelve components of staged quad q: false when a quad
-    // with a wrapped first-block index does not cover the block.
-    auto component = [&](uint32_t q, float& sv) -> bool {
-      const QuadRec& R = sm.recs[q];
-      if (sm.qinfo[q][3] & 1u) {
-        // Wrapped first-block index: this block's (row, column) in the quad's
-        // block numbering, folded from the origin.
-        const int d = (tby0 + lby) * bX + tbx0 + lbx - int(R.fb);
-        const int rangeX = int(R.range & 0xFFFFu), rangeY = int(R.range >> 16);
-        if (d < 0) return false;
-        const int r = div_bx(d, bX, invBX), c = d - r * bX;
-        if (r >= rangeY || c >= rangeX) return false;
-        float base, stepY, stepX;
-        quad_component(R, ce, base, stepY, stepX);
-        sv = fold_n(fold_n(base, stepY, r), stepX, c);
-        return true;
+  // The wave's blocks: (lbx + lby) & (NW - 1) == wave, TBX / NW per row.
+  static_assert(TBX % NW == 0, "diagonal block ownership");
+  constexpr int kPerRow = TBX / NW, kOwned = TBX * TBY / NW;
+  // Two blocks are drawn at a time: their candidate loops are independent
+  // chains, interleaved so that one's latencies overlap the other's.
+  struct Slot {
+    int lb;  // lbx = lb % TBX, lby = lb / TBX
+    uint32_t hz, dval, win;
+    bool dirty;
+    uint64_t m;  // candidates left, in submission order
+  };
+  int nextK = 0;
+  // Next owned block with candidates into S (false when none is left).
+  auto open = [&](Slot& S) -> bool {
+    while (nextK < kOwned) {
+      const int k = nextK++;
+      const int lby = k / kPerRow;
+      const int lbx = int((wave - uint32_t(lby)) & uint32_t(NW - 1)) + NW * (k % kPerRow);
+      const int lb = lby * TBX + lbx;
+      const uint32_t hz = sm.hiz[lb];
+      const bool inRect = valid && lbx >= qx0 && lbx <= qx1 && lby >= qy0 && lby <= qy1;
+      const bool cand = inRect && !(hz != 1u && hz >= maxZq);
+      const uint64_t m = __ballot(cand);
+      if (STATS) {
+        const uint32_t pre = uint32_t(__popcll(__ballot(inRect) & ~m));
+        bc.visited += pre;
+        bc.rejected += pre;
       }
-      const int x0 = int(sm.qinfo[q][0] & 0xFFu);
-      const float stepX = ce < 4u ? (ce == 0 ? R.nx[0] : ce == 1 ? R.nx[1] : ce == 2 ? R.nx[2] : R.nx[3]) : R.dDx;
-      sv = fold_upto7(sm.entry[(q * TBY + uint32_t(lby)) * 12u + ce], stepX, lbx - x0);
+      if (m == 0ull) continue;
+      S.lb = lb;
+      S.hz = hz;
+      S.m = m;
+      S.dval = sm.depth[lb * 64 + int(lane)];
+      S.dirty = false;
+      S.win = kNoWinner;
+      if (LABELS && hz != 1u) S.win = winner[(size_t(tby0 + lby) * size_t(bX) + size_t(tbx0 + lbx)) * 64u + lane];
       return true;
-    };
-    // Candidates in submission order; the hiZ test comes first, since most
-    // (quad, block) visits end there.
-    while (m) {
-      const uint32_t q = uint32_t(__builtin_ctzll(m));
-      m &= m - 1ull;
-      float sv = 0.0f;
-      if (!component(q, sv)) continue;
-      const QuadConst Q = quad_const(sm.recs[q]);
-      if (hz >= Q.maxZ) {
-        if (STATS) {
-          bc.visited++;
-          bc.rejected++;
-        }
-        continue;
-      }
-      uint32_t d;
-      bool k;
-      sample_block_reg(sm.cov, sv, lane, kLane, Q, d, k);
-      merge_block_reg<STATS, LABELS>(d, k, Q.maxZ, hz, dval, dirty, bc, win, LABELS ? sm.qid[q] >> 3 : 0u);
     }
-    if (dirty) {
-      sm.depth[lb * 64 + int(lane)] = uint16_t(dval);
-      if (lane == 0) sm.hiz[lb] = uint16_t(hz);
-      if (LABELS) winner[wIdx] = win;
+    return false;
+  };
+  auto close = [&](const Slot& S) {
+    if (!S.dirty) return;
+    sm.depth[S.lb * 64 + int(lane)] = uint16_t(S.dval);
+    if (lane == 0) sm.hiz[S.lb] = uint16_t(S.hz);
+    if (LABELS)
+      winner[(size_t(tby0 + S.lb / TBX) * size_t(bX) + size_t(tbx0 + S.lb % TBX)) * 64u + lane] = S.win;
+  };
+  // The block's twelve components of staged quad q (lane ce holds component
+  // ce): false when a quad with a wrapped first-block index does not cover
+  // the block.
+  auto component = [&](const Slot& S, uint32_t q, float& sv) -> bool {
+    const QuadRec& R = sm.recs[q];
+    if (sm.qinfo[q][3] & 1u) {
+      // Wrapped first-block index: this block's (row, column) in the quad's
+      // block numbering, folded from the origin.
+      const int d = (tby0 + S.lb / TBX) * bX + tbx0 + S.lb % TBX - int(R.fb);
+      const int rangeX = int(R.range & 0xFFFFu), rangeY = int(R.range >> 16);
+      if (d < 0) return false;
+      const int r = div_bx(d, bX, invBX), c = d - r * bX;
+      if (r >= rangeY || c >= rangeX) return false;
+      float base, stepY, stepX;
+      quad_component(R, ce, base, stepY, stepX);
+      sv = fold_n(fold_n(base, stepY, r), stepX, c);
+      return true;
+    }
+    const int x0 = int(sm.qinfo[q][0] & 0xFFu);
+    const float stepX = ce < 4u ? (ce == 0 ? R.nx[0] : ce == 1 ? R.nx[1] : ce == 2 ? R.nx[2] : R.nx[3]) : R.dDx;
+    sv = fold_upto7(sm.entry[(q * TBY + uint32_t(S.lb / TBX)) * 12u + ce], stepX, S.lb % TBX - x0);
+    return true;
+  };
+  // One candidate visit of slot S: the hiZ test first (most visits end
+  // there), then the block's coverage and depth and the merge.
+  Slot A, B;
+  bool aOn = open(A), bOn = open(B);
+  while (aOn || bOn) {
+    const uint32_t qa = aOn ? uint32_t(__builtin_ctzll(A.m)) : 0u;
+    const uint32_t qb = bOn ? uint32_t(__builtin_ctzll(B.m)) : 0u;
+    if (aOn) A.m &= A.m - 1ull;
+    if (bOn) B.m &= B.m - 1ull;
+    float sa = 0.0f, sb = 0.0f;
+    bool va = aOn && component(A, qa, sa);
+    bool vb = bOn && component(B, qb, sb);
+    const QuadConst Qa = quad_const(sm.recs[qa]), Qb = quad_const(sm.recs[qb]);
+    if (va && A.hz >= Qa.maxZ) {
+      va = false;
+      if (STATS) {
+        bc.visited++;
+        bc.rejected++;
+      }
+    }
+    if (vb && B.hz >= Qb.maxZ) {
+      vb = false;
+      if (STATS) {
+        bc.visited++;
+        bc.rejected++;
+      }
+    }
+    uint32_t da = 0, db = 0;
+    bool ka = true, kb = true;
+    if (va) sample_block_reg(sm.cov, sa, lane, kLane, Qa, da, ka);
+    if (vb) sample_block_reg(sm.cov, sb, lane, kLane, Qb, db, kb);
+    if (va) merge_block_reg<STATS, LABELS>(da, ka, Qa.maxZ, A.hz, A.dval, A.dirty, bc, A.win, LABELS ? sm.qid[qa] >> 3 : 0u);
+    if (vb) merge_block_reg<STATS, LABELS>(db, kb, Qb.maxZ, B.hz, B.dval, B.dirty, bc, B.win, LABELS ? sm.qid[qb] >> 3 : 0u);
+    if (aOn && A.m == 0ull) {
+      close(A);
+      aOn = open(A);
+    }
+    if (bOn && B.m == 0ull) {
+      close(B);
+      bOn = open(B);
     }
   }
   return tB;
