@@ -61,7 +61,9 @@ def test_table_partial_count(table):
 
 def test_table_monotone_in_offset(table):
     """A larger edge offset moves the edge toward the covered side, so each
-    bucket's mask is a subset of the previous bucket's."""
+    bucket's mask is a subset of the previous bucket's.  Equivalently every
+    (slope bucket, pixel) column is a prefix in the offset bucket, which is
+    what lets the kernels hold the table as per-pixel thresholds."""
     t = table.reshape(64, 64)
     assert np.all((t[:, 1:] & ~t[:, :-1]) == 0)
 
