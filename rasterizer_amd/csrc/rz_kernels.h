@@ -207,6 +207,13 @@ hipError_t launch_fill_u32(uint32_t* p, uint32_t n, uint32_t v, hipStream_t stre
 // its batches; then items are copied in the order `order` (per view segment).
 hipError_t launch_item_scores(const LearnParams& p, const float* batchScores, float* itemScores,
                               hipStream_t stream);
+// Radix keys that order n scored elements (batches, or items when items is
+// true) by view, then by descending score: (view << 32) | ~(the score's
+// order-preserving bits).  A stable ascending sort over the low
+// 32 + (bits of nViews - 1) bits then gives each view's elements by
+// descending score, ties in element order.
+hipError_t launch_sort_keys(const ViewDesc* views, uint32_t nViews, const float* scores, uint32_t n, bool items,
+                            uint64_t* keys, hipStream_t stream);
 hipError_t launch_item_reorder(const LearnParams& p, const uint32_t* order, ItemDesc* outItems, ItemState* outState,
                                hipStream_t stream);
 

@@ -2024,14 +2024,27 @@ __global__ void __launch_bounds__(256) rz_features_kernel(LearnParams p, uint32_
   float ext[3];
 #pragma unroll
   for (int j = 0; j < 3; ++j) ext[j] = D.refMax[j] - D.refMin[j];
+  // The batch's 4 packets of 8 quads: 32 contiguous words (word 8j + k is
+  // vertex j of quad k), read as 16-byte loads.
+  uint32_t W[32];
+  {
+    const uint4* src = reinterpret_cast<const uint4*>(D.packets + size_t(b * 8u - D.quadBase) * 4u);
+#pragma unroll
+    for (int v = 0; v < 8; ++v) {
+      const uint4 x = src[v];
+      W[4 * v] = x.x;
+      W[4 * v + 1] = x.y;
+      W[4 * v + 2] = x.z;
+      W[4 * v + 3] = x.w;
+    }
+  }
+#pragma unroll
   for (uint32_t k = 0; k < 8; ++k) {
     const uint32_t g = b * 8u + k;
-    const uint32_t q = g - D.quadBase;
-    const uint32_t* pk = D.packets + (q >> 3) * 32 + (q & 7);
     float P[4][3];
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
-      const uint32_t w = pk[8 * j];
+      const uint32_t w = W[8 * j + k];
       const float X = float((int32_t(w) >> 21) + 1024), Y = float((w >> 10) & 2047u), Z = float(w & 1023u);
       P[j][0] = D.refMin[0] + X * (ext[0] / 2047.0f);
       P[j][1] = D.refMin[1] + Y * (ext[1] / 2047.0f);
@@ -2150,6 +2163,22 @@ __global__ void __launch_bounds__(256) rz_item_scores_kernel(LearnParams p, cons
   float best = -3.402823466e+38f;
   for (uint32_t b = D.quadBase / 8u; b < (D.quadBase + D.quadCount) / 8u; ++b) best = fmaxf(best, batchScores[b]);
   itemScores[i] = best;
+}
+
+__global__ void __launch_bounds__(256) rz_sort_keys_kernel(const ViewDesc* __restrict__ views, uint32_t nViews,
+                                                           const float* __restrict__ scores, uint32_t n, bool items,
+                                                           uint64_t* __restrict__ keys) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint32_t lo = 0, hi = nViews - 1;
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi + 1) >> 1;
+    if ((items ? views[mid].itemBase : views[mid].batchBase) <= i) lo = mid;
+    else hi = mid - 1;
+  }
+  const uint32_t u = __float_as_uint(scores[i]);
+  const uint32_t ordered = (u & 0x80000000u) ? ~u : (u | 0x80000000u);  // ascending with the float order
+  keys[i] = (uint64_t(lo) << 32) | uint64_t(~ordered);
 }
 
 __global__ void __launch_bounds__(256) rz_item_reorder_kernel(LearnParams p, const uint32_t* __restrict__ order,
@@ -2329,6 +2358,14 @@ hipError_t launch_item_scores(const LearnParams& p, const float* batchScores, fl
   if (p.nItems == 0) return hipSuccess;
   hipLaunchKernelGGL(rz_item_scores_kernel, dim3(groups(p.nItems, 256)), dim3(256), 0, stream, p, batchScores,
                      itemScores);
+  return hipGetLastError();
+}
+
+hipError_t launch_sort_keys(const ViewDesc* views, uint32_t nViews, const float* scores, uint32_t n, bool items,
+                            uint64_t* keys, hipStream_t stream) {
+  if (n == 0 || nViews == 0) return hipSuccess;
+  hipLaunchKernelGGL(rz_sort_keys_kernel, dim3(groups(n, 256)), dim3(256), 0, stream, views, nViews, scores, n, items,
+                     keys);
   return hipGetLastError();
 }
 

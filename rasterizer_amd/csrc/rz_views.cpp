@@ -90,16 +90,15 @@ struct rz_views {
   rz_mlp* model = nullptr;
   bool byOccluder = false;  // learned order of whole occluders (by their best batch) instead of batches
   float* d_itemScores = nullptr;
-  float* d_itemScoresSorted = nullptr;
   uint32_t* d_itemIota = nullptr;
   uint32_t* d_itemOrder = nullptr;
-  uint32_t* d_itemSegments = nullptr;
   ItemDesc* d_ritems = nullptr;
   ItemState* d_rstate = nullptr;
   float* d_scores = nullptr;
-  float* d_sortedScores = nullptr;
   uint32_t* d_iota = nullptr;
-  uint32_t* d_segments = nullptr;  // nViews + 1 batch offsets
+  uint64_t* d_sortKeys = nullptr;     // radix keys (view, descending score) of batches or items
+  uint64_t* d_sortKeysOut = nullptr;
+  int sortEndBit = 32;
   uint32_t* d_drawn = nullptr;     // batches of drawn occluders (scored by the model), count in d_drawn[nBatches]
   void* d_sortTemp = nullptr;
   size_t sortTempBytes = 0;
@@ -155,24 +154,23 @@ static void free_plan(rz_views* v) {
   v->d_lists = v->d_listLen = nullptr;
   v->d_tileBox = nullptr;
   hipFree(v->d_itemScores);
-  hipFree(v->d_itemScoresSorted);
   hipFree(v->d_itemIota);
   hipFree(v->d_itemOrder);
-  hipFree(v->d_itemSegments);
   hipFree(v->d_ritems);
   hipFree(v->d_rstate);
-  v->d_itemScores = v->d_itemScoresSorted = nullptr;
-  v->d_itemIota = v->d_itemOrder = v->d_itemSegments = nullptr;
+  v->d_itemScores = nullptr;
+  v->d_itemIota = v->d_itemOrder = nullptr;
   v->d_ritems = nullptr;
   v->d_rstate = nullptr;
   hipFree(v->d_scores);
-  hipFree(v->d_sortedScores);
+  hipFree(v->d_sortKeys);
+  hipFree(v->d_sortKeysOut);
   hipFree(v->d_iota);
-  hipFree(v->d_segments);
   hipFree(v->d_sortTemp);
   hipFree(v->d_drawn);
-  v->d_scores = v->d_sortedScores = nullptr;
-  v->d_iota = v->d_segments = v->d_drawn = nullptr;
+  v->d_scores = nullptr;
+  v->d_sortKeys = v->d_sortKeysOut = nullptr;
+  v->d_iota = v->d_drawn = nullptr;
   v->d_sortTemp = nullptr;
   v->sortTempBytes = 0;
   v->d_labels = nullptr;
@@ -463,37 +461,34 @@ static int build_plan(rz_views* v) {
   }
   if (v->model && v->byOccluder) {
     const size_t ni = std::max<size_t>(v->nItems, 1);
-    std::vector<uint32_t> iota(v->nItems), seg(nv + 1);
+    std::vector<uint32_t> iota(v->nItems);
     for (uint32_t i = 0; i < v->nItems; ++i) iota[i] = i;
-    for (uint32_t i = 0; i < nv; ++i) seg[i] = v->h_views[i].itemBase;
-    seg[nv] = v->nItems;
-    if (dev_alloc(&v->d_itemScores, ni) != hipSuccess || dev_alloc(&v->d_itemScoresSorted, ni) != hipSuccess ||
-        dev_alloc(&v->d_itemIota, ni) != hipSuccess || dev_alloc(&v->d_itemOrder, ni) != hipSuccess ||
-        dev_alloc(&v->d_itemSegments, nv + 1) != hipSuccess || dev_alloc(&v->d_ritems, ni) != hipSuccess ||
+    if (dev_alloc(&v->d_itemScores, ni) != hipSuccess || dev_alloc(&v->d_itemIota, ni) != hipSuccess ||
+        dev_alloc(&v->d_itemOrder, ni) != hipSuccess || dev_alloc(&v->d_ritems, ni) != hipSuccess ||
         dev_alloc(&v->d_rstate, ni) != hipSuccess)
       return RZ_ERR_OUT_OF_MEMORY;
     if (v->nItems) RZ_HIP(hipMemcpy(v->d_itemIota, iota.data(), iota.size() * 4, hipMemcpyHostToDevice));
-    RZ_HIP(hipMemcpy(v->d_itemSegments, seg.data(), seg.size() * 4, hipMemcpyHostToDevice));
   }
   if (v->model) {
-    std::vector<uint32_t> iota(v->nBatches), seg(nv + 1);
+    std::vector<uint32_t> iota(v->nBatches);
     for (uint32_t b = 0; b < v->nBatches; ++b) iota[b] = b;
-    for (uint32_t i = 0; i < nv; ++i) seg[i] = v->h_views[i].batchBase;
-    seg[nv] = v->nBatches;
-    if (dev_alloc(&v->d_scores, nbt) != hipSuccess || dev_alloc(&v->d_sortedScores, nbt) != hipSuccess ||
-        dev_alloc(&v->d_iota, nbt) != hipSuccess || dev_alloc(&v->d_segments, nv + 1) != hipSuccess ||
+    const size_t nk = std::max<size_t>(std::max(v->nBatches, v->byOccluder ? v->nItems : 0u), 1);
+    if (dev_alloc(&v->d_scores, nbt) != hipSuccess || dev_alloc(&v->d_iota, nbt) != hipSuccess ||
+        dev_alloc(&v->d_sortKeys, nk) != hipSuccess || dev_alloc(&v->d_sortKeysOut, nk) != hipSuccess ||
         dev_alloc(&v->d_drawn, nbt + 1) != hipSuccess)
       return RZ_ERR_OUT_OF_MEMORY;
     if (v->nBatches) RZ_HIP(hipMemcpy(v->d_iota, iota.data(), iota.size() * 4, hipMemcpyHostToDevice));
-    RZ_HIP(hipMemcpy(v->d_segments, seg.data(), seg.size() * 4, hipMemcpyHostToDevice));
+    // One device-wide stable radix sort on (view, descending score) keys;
+    // the view index needs the bits of nv - 1 above the 32 score bits.
+    int viewBits = 0;
+    while ((1u << viewBits) < nv) ++viewBits;
+    v->sortEndBit = 32 + viewBits;
     size_t bytes = 0, itemBytes = 0;
-    RZ_HIP(hipcub::DeviceSegmentedRadixSort::SortPairsDescending(
-        nullptr, bytes, v->d_scores, v->d_sortedScores, v->d_iota, v->d_iota, int(v->nBatches), int(nv),
-        v->d_segments, v->d_segments + 1, 0, 32, v->stream));
+    RZ_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, v->d_sortKeys, v->d_sortKeysOut, v->d_iota, v->d_iota,
+                                              int(v->nBatches), 0, v->sortEndBit, v->stream));
     if (v->byOccluder) {
-      RZ_HIP(hipcub::DeviceSegmentedRadixSort::SortPairsDescending(
-          nullptr, itemBytes, v->d_itemScores, v->d_itemScoresSorted, v->d_itemIota, v->d_itemOrder, int(v->nItems),
-          int(nv), v->d_itemSegments, v->d_itemSegments + 1, 0, 32, v->stream));
+      RZ_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, itemBytes, v->d_sortKeys, v->d_sortKeysOut, v->d_itemIota,
+                                                v->d_itemOrder, int(v->nItems), 0, v->sortEndBit, v->stream));
       bytes = std::max(bytes, itemBytes);
     }
     if (hipMalloc(&v->d_sortTemp, std::max<size_t>(bytes, 16)) != hipSuccess) return RZ_ERR_OUT_OF_MEMORY;
@@ -595,14 +590,15 @@ static int enqueue_step(rz_views* v, bool capturing) {
       if (v->byOccluder) {
         // Whole occluders by their best batch score, stable within a view.
         RZ_HIP(launch_item_scores(lp, v->d_scores, v->d_itemScores, v->stream));
-        RZ_HIP(hipcub::DeviceSegmentedRadixSort::SortPairsDescending(
-            v->d_sortTemp, bytes, v->d_itemScores, v->d_itemScoresSorted, v->d_itemIota, v->d_itemOrder,
-            int(v->nItems), int(nv), v->d_itemSegments, v->d_itemSegments + 1, 0, 32, v->stream));
+        RZ_HIP(launch_sort_keys(v->d_views, nv, v->d_itemScores, v->nItems, true, v->d_sortKeys, v->stream));
+        RZ_HIP(hipcub::DeviceRadixSort::SortPairs(v->d_sortTemp, bytes, v->d_sortKeys, v->d_sortKeysOut,
+                                                  v->d_itemIota, v->d_itemOrder, int(v->nItems), 0, v->sortEndBit,
+                                                  v->stream));
         RZ_HIP(launch_item_reorder(lp, v->d_itemOrder, v->d_ritems, v->d_rstate, v->stream));
       } else {
-        RZ_HIP(hipcub::DeviceSegmentedRadixSort::SortPairsDescending(
-            v->d_sortTemp, bytes, v->d_scores, v->d_sortedScores, v->d_iota, v->d_order, int(v->nBatches), int(nv),
-            v->d_segments, v->d_segments + 1, 0, 32, v->stream));
+        RZ_HIP(launch_sort_keys(v->d_views, nv, v->d_scores, v->nBatches, false, v->d_sortKeys, v->stream));
+        RZ_HIP(hipcub::DeviceRadixSort::SortPairs(v->d_sortTemp, bytes, v->d_sortKeys, v->d_sortKeysOut, v->d_iota,
+                                                  v->d_order, int(v->nBatches), 0, v->sortEndBit, v->stream));
       }
     }
     if (v->batchOrder) RZ_HIP(launch_batch_items(lp, v->d_order, v->nBatches, v->d_bitems, v->d_bstate, v->stream));
