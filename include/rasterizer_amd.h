@@ -349,9 +349,18 @@ int rz_mlp_train_step(rz_mlp* mlp, uint32_t n, const float* x, uint32_t nPairs, 
  * occluder order.  NULL restores occluder order.  The model must outlive its
  * use. */
 int rz_views_set_model(rz_views* views, rz_mlp* model, int byOccluder);
-/* The order of the last learned step: view-local batch numbers, or occluder
- * numbers when ordering by occluder. */
+/* The order of the last learned or front-to-back step: view-local batch
+ * numbers, or occluder numbers when ordering by occluder or front to back. */
 int rz_views_read_order(rz_views* views, uint32_t view, uint32_t* order);
+/* Front-to-back draw order computed on the device every step, replacing
+ * Main.cpp:184-190's std::sort: the view's occluders by ascending squared
+ * distance from eye to their centre ((boundsMin + boundsMax) * 0.5,
+ * Occluder.cpp:178), float32 summed as DPPS sums it ((dx^2 + dy^2) + dz^2);
+ * ties keep submission order (std::sort leaves them unspecified).  Works in
+ * both launch modes; statuses and query results stay in submission order.
+ * NULL restores submission order.  Cannot be combined with a learned model or
+ * an explicit batch order (RZ_ERR_INVALID_ARGUMENT at the next run). */
+int rz_views_set_eye(rz_views* views, uint32_t view, const float eye[3]);
 /* Block-loop counters of each step (observability): rz_views_read_stats
  * returns, for the last step over all views, the (quad, block) pairs
  * visited, those rejected by the hiZ or edge tests, and blocks written. */

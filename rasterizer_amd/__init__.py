@@ -119,6 +119,7 @@ _SIGNATURES = {
     "rz_views_enable_stats": ([_P, ctypes.c_int], ctypes.c_int),
     "rz_views_read_stats": ([_P, _P], ctypes.c_int),
     "rz_views_read_tile_stats": ([_P, _P], ctypes.c_int),
+    "rz_views_set_eye": ([_P, ctypes.c_uint32, _P], ctypes.c_int),
     "rz_get_config": ([_P, _P], ctypes.c_int),
     "rz_set_config": ([_P, _U32, _U32], ctypes.c_int),
     "rz_tune_frame": ([_P, _U32, _P, _U32, ctypes.c_char_p, _P, _P], ctypes.c_int),
@@ -568,8 +569,23 @@ class Views:
         _check(lib().rz_views_read_tile_stats(self._v, _ptr(a)), "rz_views_read_tile_stats")
         return a
 
+    def set_eye(self, view: int, eye) -> None:
+        """Front-to-back order from `eye` computed on the device every step
+        (Main.cpp:184-190, ties in submission order); None restores
+        submission order."""
+        if eye is None:
+            _check(lib().rz_views_set_eye(self._v, view, None), "rz_views_set_eye")
+            getattr(self, "_eyes", set()).discard(view)
+            return
+        e = np.ascontiguousarray(np.asarray(eye, np.float32)[:3])
+        _check(lib().rz_views_set_eye(self._v, view, _ptr(e)), "rz_views_set_eye")
+        if not hasattr(self, "_eyes"):
+            self._eyes = set()
+        self._eyes.add(view)
+
     def order(self, view: int) -> np.ndarray:
-        n = len(self._occ[view]) if getattr(self, "_by_occluder", False) else self.n_batches(view)
+        by_occ = getattr(self, "_by_occluder", False) or bool(getattr(self, "_eyes", set()))
+        n = len(self._occ[view]) if by_occ else self.n_batches(view)
         out = np.zeros(max(n, 1), np.uint32)
         _check(lib().rz_views_read_order(self._v, view, _ptr(out)), "rz_views_read_order")
         return out[:n]

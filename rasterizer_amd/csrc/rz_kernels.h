@@ -214,6 +214,12 @@ hipError_t launch_item_scores(const LearnParams& p, const float* batchScores, fl
 // descending score, ties in element order.
 hipError_t launch_sort_keys(const ViewDesc* views, uint32_t nViews, const float* scores, uint32_t n, bool items,
                             uint64_t* keys, hipStream_t stream);
+// Front-to-back keys (Main.cpp:184-190) of n items: (view << 32) | the bits
+// of the squared distance from eyes[view] (x, y, z; w != 0 when the view is
+// sorted) to the item's centre (boundsMin + boundsMax) * 0.5
+// (Occluder.cpp:178), summed as DPPS sums it; views without an eye get key
+// (view << 32), so a stable sort keeps their submission order.
+hipError_t launch_f2b_keys(const ItemDesc* items, uint32_t n, const float* eyes, uint64_t* keys, hipStream_t stream);
 hipError_t launch_item_reorder(const LearnParams& p, const uint32_t* order, ItemDesc* outItems, ItemState* outState,
                                hipStream_t stream);
 

@@ -2181,6 +2181,29 @@ __global__ void __launch_bounds__(256) rz_sort_keys_kernel(const ViewDesc* __res
   keys[i] = (uint64_t(lo) << 32) | uint64_t(~ordered);
 }
 
+// Squared distance of Main.cpp:184-190: _mm_dp_ps(d, d, 0x7f) sums the x
+// and y products, then adds z's (each rounded; compiled without
+// contraction, so no fused multiply-add).
+__global__ void __launch_bounds__(256) rz_f2b_keys_kernel(const ItemDesc* __restrict__ items, uint32_t n,
+                                                          const float* __restrict__ eyes, uint64_t* __restrict__ keys) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const ItemDesc& D = items[i];
+  const float* e = eyes + 4 * D.view;
+  uint32_t bits = 0;
+  if (e[3] != 0.0f) {
+    float p[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      const float c = (D.boundsMax[a] + D.boundsMin[a]) * 0.5f;
+      const float d = c - e[a];
+      p[a] = d * d;
+    }
+    bits = __float_as_uint((p[0] + p[1]) + p[2]);  // >= +0 (or NaN, last): its bits order as the value
+  }
+  keys[i] = (uint64_t(D.view) << 32) | uint64_t(bits);
+}
+
 __global__ void __launch_bounds__(256) rz_item_reorder_kernel(LearnParams p, const uint32_t* __restrict__ order,
                                                                ItemDesc* __restrict__ outItems,
                                                                ItemState* __restrict__ outState) {
@@ -2366,6 +2389,12 @@ hipError_t launch_sort_keys(const ViewDesc* views, uint32_t nViews, const float*
   if (n == 0 || nViews == 0) return hipSuccess;
   hipLaunchKernelGGL(rz_sort_keys_kernel, dim3(groups(n, 256)), dim3(256), 0, stream, views, nViews, scores, n, items,
                      keys);
+  return hipGetLastError();
+}
+
+hipError_t launch_f2b_keys(const ItemDesc* items, uint32_t n, const float* eyes, uint64_t* keys, hipStream_t stream) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(rz_f2b_keys_kernel, dim3(groups(n, 256)), dim3(256), 0, stream, items, n, eyes, keys);
   return hipGetLastError();
 }
 

@@ -47,6 +47,7 @@ struct ViewHost {
   uint32_t* d_winner = nullptr;  // per pixel, when labels are on
   std::vector<uint32_t> order;   // explicit batch order (view-local batch numbers), empty = occluder order
   Mvp mvp{};
+  float eye[4] = {0.0f, 0.0f, 0.0f, 0.0f};  // front-to-back order from this eye when eye[3] != 0
 };
 
 template <typename T>
@@ -76,6 +77,9 @@ struct rz_views {
   uint32_t* d_groupItem = nullptr;  // per group of 64 quads: the item of its first quad
   ViewDesc* d_views = nullptr;
   ViewDesc* h_views = nullptr;  // pinned; copied to d_views by every step
+  float* h_eyes = nullptr;      // pinned, 4 floats per view; copied to d_eyes by front-to-back steps
+  float* d_eyes = nullptr;
+  bool f2b = false;             // some view is drawn front to back (order computed on the device)
   uint32_t* d_queue = nullptr;
   // draw-order learning
   bool labels = false;    // compute features before and labels after the draw
@@ -179,6 +183,10 @@ static void free_plan(rz_views* v) {
   v->d_bitems = nullptr;
   v->d_bstate = nullptr;
   if (v->h_views) (void)hipHostFree(v->h_views);
+  if (v->h_eyes) (void)hipHostFree(v->h_eyes);
+  hipFree(v->d_eyes);
+  v->h_eyes = nullptr;
+  v->d_eyes = nullptr;
   v->d_items = nullptr;
   v->d_state = nullptr;
   v->d_done = v->d_visible = nullptr;
@@ -441,6 +449,16 @@ static int build_plan(rz_views* v) {
   // views without one keep their batches in occluder order.
   v->batchOrder = v->model != nullptr && !v->byOccluder;
   for (const ViewHost& h : v->views) v->batchOrder |= !h.order.empty();
+  // Front-to-back views: the device sorts whole occluders, which neither a
+  // model's order nor an explicit batch order can be combined with.
+  v->f2b = false;
+  for (const ViewHost& h : v->views) v->f2b |= h.eye[3] != 0.0f;
+  if (v->f2b && (v->model || v->batchOrder)) return RZ_ERR_INVALID_ARGUMENT;
+  if (v->f2b) {
+    if (hipHostMalloc(reinterpret_cast<void**>(&v->h_eyes), size_t(nv) * 4 * sizeof(float)) != hipSuccess ||
+        dev_alloc(&v->d_eyes, size_t(nv) * 4) != hipSuccess)
+      return RZ_ERR_OUT_OF_MEMORY;
+  }
   {
     // Per-tile item lists: each unit gets room for all of its view's items
     // (or batches); skipped when that would not fit 32-bit offsets or 4 GB,
@@ -459,15 +477,31 @@ static int build_plan(rz_views* v) {
         return RZ_ERR_OUT_OF_MEMORY;
     }
   }
-  if (v->model && v->byOccluder) {
+  // Occluder-granular reorders (learned by occluder, front to back): items
+  // and their states are copied in the sorted order after setup.
+  const bool itemReorder = (v->model && v->byOccluder) || v->f2b;
+  int viewBits = 0;
+  while ((1u << viewBits) < nv) ++viewBits;
+  v->sortEndBit = 32 + viewBits;  // view index above the 32 key bits
+  if (itemReorder) {
     const size_t ni = std::max<size_t>(v->nItems, 1);
     std::vector<uint32_t> iota(v->nItems);
     for (uint32_t i = 0; i < v->nItems; ++i) iota[i] = i;
-    if (dev_alloc(&v->d_itemScores, ni) != hipSuccess || dev_alloc(&v->d_itemIota, ni) != hipSuccess ||
+    if ((v->model && dev_alloc(&v->d_itemScores, ni) != hipSuccess) || dev_alloc(&v->d_itemIota, ni) != hipSuccess ||
         dev_alloc(&v->d_itemOrder, ni) != hipSuccess || dev_alloc(&v->d_ritems, ni) != hipSuccess ||
         dev_alloc(&v->d_rstate, ni) != hipSuccess)
       return RZ_ERR_OUT_OF_MEMORY;
     if (v->nItems) RZ_HIP(hipMemcpy(v->d_itemIota, iota.data(), iota.size() * 4, hipMemcpyHostToDevice));
+  }
+  if (v->f2b) {
+    size_t bytes = 0;
+    if (dev_alloc(&v->d_sortKeys, std::max<size_t>(v->nItems, 1)) != hipSuccess ||
+        dev_alloc(&v->d_sortKeysOut, std::max<size_t>(v->nItems, 1)) != hipSuccess)
+      return RZ_ERR_OUT_OF_MEMORY;
+    RZ_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, v->d_sortKeys, v->d_sortKeysOut, v->d_itemIota,
+                                              v->d_itemOrder, int(v->nItems), 0, v->sortEndBit, v->stream));
+    if (hipMalloc(&v->d_sortTemp, std::max<size_t>(bytes, 16)) != hipSuccess) return RZ_ERR_OUT_OF_MEMORY;
+    v->sortTempBytes = bytes;
   }
   if (v->model) {
     std::vector<uint32_t> iota(v->nBatches);
@@ -478,11 +512,7 @@ static int build_plan(rz_views* v) {
         dev_alloc(&v->d_drawn, nbt + 1) != hipSuccess)
       return RZ_ERR_OUT_OF_MEMORY;
     if (v->nBatches) RZ_HIP(hipMemcpy(v->d_iota, iota.data(), iota.size() * 4, hipMemcpyHostToDevice));
-    // One device-wide stable radix sort on (view, descending score) keys;
-    // the view index needs the bits of nv - 1 above the 32 score bits.
-    int viewBits = 0;
-    while ((1u << viewBits) < nv) ++viewBits;
-    v->sortEndBit = 32 + viewBits;
+    // One device-wide stable radix sort on (view, descending score) keys.
     size_t bytes = 0, itemBytes = 0;
     RZ_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, v->d_sortKeys, v->d_sortKeysOut, v->d_iota, v->d_iota,
                                               int(v->nBatches), 0, v->sortEndBit, v->stream));
@@ -601,9 +631,18 @@ static int enqueue_step(rz_views* v, bool capturing) {
                                                   v->d_order, int(v->nBatches), 0, v->sortEndBit, v->stream));
       }
     }
+    if (v->f2b) {
+      // Front to back (Main.cpp:184-190), stable: ties keep submission order.
+      size_t bytes = v->sortTempBytes;
+      RZ_HIP(hipMemcpyAsync(v->d_eyes, v->h_eyes, nv * 4 * sizeof(float), hipMemcpyHostToDevice, v->stream));
+      RZ_HIP(launch_f2b_keys(v->d_items, v->nItems, v->d_eyes, v->d_sortKeys, v->stream));
+      RZ_HIP(hipcub::DeviceRadixSort::SortPairs(v->d_sortTemp, bytes, v->d_sortKeys, v->d_sortKeysOut, v->d_itemIota,
+                                                v->d_itemOrder, int(v->nItems), 0, v->sortEndBit, v->stream));
+      RZ_HIP(launch_item_reorder(lp, v->d_itemOrder, v->d_ritems, v->d_rstate, v->stream));
+    }
     if (v->batchOrder) RZ_HIP(launch_batch_items(lp, v->d_order, v->nBatches, v->d_bitems, v->d_bstate, v->stream));
     ViewsParams rp{};
-    const bool reordered = v->model && v->byOccluder && v->nItems;
+    const bool reordered = ((v->model && v->byOccluder) || v->f2b) && v->nItems;
     rp.items = v->batchOrder ? v->d_bitems : (reordered ? v->d_ritems : v->d_items);
     rp.state = v->batchOrder ? v->d_bstate : (reordered ? v->d_rstate : v->d_state);
     rp.batchItems = v->batchOrder ? 1u : 0u;
@@ -657,7 +696,10 @@ extern "C" int rz_views_run(rz_views* v, int useGraph) {
     s = build_plan(v);
     if (s != RZ_OK) return s;
   }
-  for (uint32_t i = 0; i < v->views.size(); ++i) v->h_views[i].mvp = v->views[i].mvp;
+  for (uint32_t i = 0; i < v->views.size(); ++i) {
+    v->h_views[i].mvp = v->views[i].mvp;
+    if (v->h_eyes) std::memcpy(v->h_eyes + 4 * i, v->views[i].eye, 4 * sizeof(float));
+  }
   // Gated steps launch cooperatively (the grid must be co-resident), which
   // is not captured: they are enqueued directly.
   if ((v->gated && v->cooperative) || v->profile) useGraph = 0;
@@ -730,8 +772,16 @@ extern "C" int rz_views_read_status(rz_views* v, uint32_t view, uint8_t* status)
   if (d.nItems)
     RZ_HIP(hipMemcpy(st.data(), v->d_state + d.itemBase, d.nItems * sizeof(ItemState), hipMemcpyDeviceToHost));
   std::vector<uint32_t> done(v->gated ? d.nItems : 0);
-  if (v->gated && d.nItems)
+  if (v->gated && d.nItems) {
     RZ_HIP(hipMemcpy(done.data(), v->d_done + d.itemBase, d.nItems * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    if (v->f2b) {
+      // The launch drew the items in sorted order: done word k is item order[k]'s.
+      std::vector<uint32_t> order(d.nItems), byItem(d.nItems);
+      RZ_HIP(hipMemcpy(order.data(), v->d_itemOrder + d.itemBase, d.nItems * 4, hipMemcpyDeviceToHost));
+      for (uint32_t k = 0; k < d.nItems; ++k) byItem[order[k] - d.itemBase] = done[k];
+      done.swap(byItem);
+    }
+  }
   for (uint32_t i = 0; i < d.nItems; ++i) {
     const uint32_t k = st[i].status;
     // A gated item was drawn (with rasterize<false>) when some tile saw it visible.
@@ -785,7 +835,7 @@ extern "C" int rz_views_read_order(rz_views* v, uint32_t view, uint32_t* order) 
   if (!v || !order || view >= v->views.size() || v->dirty || !v->h_views) return RZ_ERR_INVALID_ARGUMENT;
   RZ_HIP(hipStreamSynchronize(v->stream));
   const ViewDesc& d = v->h_views[view];
-  if (v->model && v->byOccluder) {
+  if ((v->model && v->byOccluder) || v->f2b) {
     if (d.nItems) RZ_HIP(hipMemcpy(order, v->d_itemOrder + d.itemBase, d.nItems * 4, hipMemcpyDeviceToHost));
     for (uint32_t k = 0; k < d.nItems; ++k) order[k] -= d.itemBase;
     return RZ_OK;
@@ -793,6 +843,22 @@ extern "C" int rz_views_read_order(rz_views* v, uint32_t view, uint32_t* order) 
   if (!v->batchOrder) return RZ_ERR_INVALID_ARGUMENT;
   if (d.nBatches) RZ_HIP(hipMemcpy(order, v->d_order + d.batchBase, d.nBatches * 4, hipMemcpyDeviceToHost));
   for (uint32_t k = 0; k < d.nBatches; ++k) order[k] -= d.batchBase;
+  return RZ_OK;
+}
+
+extern "C" int rz_views_set_eye(rz_views* v, uint32_t view, const float eye[3]) {
+  if (!v || view >= v->views.size()) return RZ_ERR_INVALID_ARGUMENT;
+  int s = wait_last_step(v);
+  if (s != RZ_OK) return s;
+  ViewHost& h = v->views[view];
+  const bool on = eye != nullptr;
+  if (on != (h.eye[3] != 0.0f)) v->dirty = true;  // switching the mode rebuilds the plan
+  if (on) {
+    h.eye[0] = eye[0];
+    h.eye[1] = eye[1];
+    h.eye[2] = eye[2];
+  }
+  h.eye[3] = on ? 1.0f : 0.0f;
   return RZ_OK;
 }
 
