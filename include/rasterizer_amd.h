@@ -38,6 +38,7 @@ extern "C" {
 #define RZ_ERR_UNSUPPORTED_HOST (-5)
 #define RZ_ERR_TOO_LARGE (-6)
 #define RZ_ERR_NOT_IDENTICAL (-7)
+#define RZ_ERR_IO (-8)
 
 typedef struct rz_context rz_context;
 typedef struct rz_occluder rz_occluder;
@@ -371,6 +372,37 @@ int rz_views_read_stats(rz_views* views, uint64_t out[3]);
  * rz_frame_stats_read; word 0 is the sum of the persistent work-groups'
  * lifetimes, in 100 MHz ticks like the other tick words. */
 int rz_views_read_tile_stats(rz_views* views, uint64_t out[22]);
+
+/* On-disk formats (no counterpart in the reference, which re-bakes its
+ * scene at every start): versioned little-endian files with a magic, the
+ * payload size and an FNV-1a 64 checksum of the payload.  Host only; no
+ * device needed.  RZ_ERR_IO when a file cannot be written or read, or is not
+ * this format, version or checksum. */
+
+/* A baked occluder as the reference holds it (Occluder.h:7-21,
+ * Occluder.cpp:153-176): packetCount packets of 8 words (m_vertexData) and
+ * its reference box and bounds. */
+typedef struct rz_occluder_data {
+  const uint32_t* packets;
+  uint32_t packetCount; /* multiple of 4 (whole 8-quad batches) */
+  float refMin[4], refMax[4], boundsMin[4], boundsMax[4];
+} rz_occluder_data;
+int rz_write_occluders(const char* path, uint32_t n, const rz_occluder_data* occluders);
+/* With occluders == NULL: sets *n and *totalPackets from the file.  Else *n
+ * and *totalPackets give the capacity of occluders and packetWords
+ * (totalPackets * 8 words); the descriptors are filled with their packets
+ * pointing into packetWords, and *n, *totalPackets are set to the file's. */
+int rz_read_occluders(const char* path, uint32_t* n, uint32_t* totalPackets, rz_occluder_data* occluders,
+                      uint32_t* packetWords);
+/* A frame dump: raw depth and hiZ as rz_readback_raw returns them
+ * ((width/8)*(height/8)*64 and (width/8)*(height/8) + 8 u16) and the query
+ * bits of nQueries queries (NULL when nQueries is 0).  rz_read_frame sets
+ * the sizes and fills the arrays that are not NULL (call it with NULL arrays
+ * first to learn the sizes). */
+int rz_write_frame(const char* path, uint32_t width, uint32_t height, const uint16_t* depth, const uint16_t* hiz,
+                   uint32_t nQueries, const uint8_t* visible, const uint8_t* needsClipping);
+int rz_read_frame(const char* path, uint32_t* width, uint32_t* height, uint32_t* nQueries, uint16_t* depth,
+                  uint16_t* hiz, uint8_t* visible, uint8_t* needsClipping);
 
 #ifdef __cplusplus
 }

@@ -29,6 +29,7 @@ _STATUS = {
     -5: "host CPU RCPPS has an unsupported structure",
     -6: "buffer too large for one co-resident frame launch",
     -7: "launch configurations disagree (autotuner check)",
+    -8: "file cannot be written or read, or is not this format, version or checksum",
 }
 
 
@@ -132,7 +133,18 @@ _SIGNATURES = {
     "rz_views_profile_enable": ([_P, ctypes.c_int], ctypes.c_int),
     "rz_views_profile_read": ([_P, _P], ctypes.c_int),
     "rz_views_read_items": ([_P, _U32, _P, _P], ctypes.c_int),
+    "rz_write_occluders": ([ctypes.c_char_p, _U32, _P], ctypes.c_int),
+    "rz_read_occluders": ([ctypes.c_char_p, _P, _P, _P, _P], ctypes.c_int),
+    "rz_write_frame": ([ctypes.c_char_p, _U32, _U32, _P, _P, _U32, _P, _P], ctypes.c_int),
+    "rz_read_frame": ([ctypes.c_char_p, _P, _P, _P, _P, _P, _P, _P], ctypes.c_int),
 }
+
+
+class _OccluderData(ctypes.Structure):
+    """rz_occluder_data: a baked occluder's packets, reference box and bounds."""
+    _fields_ = [("packets", ctypes.POINTER(ctypes.c_uint32)), ("packetCount", ctypes.c_uint32),
+                ("refMin", ctypes.c_float * 4), ("refMax", ctypes.c_float * 4),
+                ("boundsMin", ctypes.c_float * 4), ("boundsMax", ctypes.c_float * 4)]
 
 
 class FrameConfig(ctypes.Structure):
@@ -239,6 +251,72 @@ class Occluder:
             self.release()
         except Exception:
             pass
+
+
+def save_occluders(path, occluders) -> None:
+    """Writes baked occluders (packets, reference boxes, bounds) to a
+    versioned, checksummed file (rz_write_occluders)."""
+    occluders = list(occluders)
+    arr = (_OccluderData * max(len(occluders), 1))()
+    for d, o in zip(arr, occluders):
+        d.packets = o.packets.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))
+        d.packetCount = o.packets.size // 8
+        for name, v in (("refMin", o.m_refMin), ("refMax", o.m_refMax), ("boundsMin", o.m_boundsMin),
+                        ("boundsMax", o.m_boundsMax)):
+            getattr(d, name)[:] = [float(x) for x in v]
+    _check(lib().rz_write_occluders(str(path).encode(), len(occluders), ctypes.cast(arr, _P)), "rz_write_occluders")
+
+
+def load_occluders(path) -> list:
+    """Reads a file written by save_occluders into Occluder objects."""
+    n, total = ctypes.c_uint32(0), ctypes.c_uint32(0)
+    p = str(path).encode()
+    _check(lib().rz_read_occluders(p, ctypes.byref(n), ctypes.byref(total), None, None), "rz_read_occluders")
+    arr = (_OccluderData * max(n.value, 1))()
+    words = np.zeros(max(total.value * 8, 1), np.uint32)
+    _check(lib().rz_read_occluders(p, ctypes.byref(n), ctypes.byref(total), ctypes.cast(arr, _P), _ptr(words)),
+           "rz_read_occluders")
+    out, at = [], 0
+    for d in arr[: n.value]:
+        k = d.packetCount * 8
+        out.append(Occluder(words[at:at + k].copy(), list(d.refMin), list(d.refMax), list(d.boundsMin),
+                            list(d.boundsMax)))
+        at += k
+    return out
+
+
+def save_frame(path, width: int, height: int, depth: np.ndarray, hiz: np.ndarray, visible=None, clip=None) -> None:
+    """Writes a frame dump: raw depth and hiZ (as Rasterizer.raw returns
+    them) and optionally the query bits (rz_write_frame)."""
+    depth = np.ascontiguousarray(depth, np.uint16)
+    hiz = np.ascontiguousarray(hiz, np.uint16)
+    nb = (width // 8) * (height // 8)
+    if depth.size != nb * 64 or hiz.size != nb + 8:
+        raise ValueError("depth must hold (w/8)*(h/8)*64 and hiz (w/8)*(h/8) + 8 values")
+    nq = 0 if visible is None else len(visible)
+    vis = np.ascontiguousarray(visible if visible is not None else np.zeros(1), np.uint8)
+    clp = np.ascontiguousarray(clip if clip is not None else np.zeros(max(nq, 1)), np.uint8)
+    if clp.size < nq:
+        raise ValueError("clip must have one byte per query")
+    _check(lib().rz_write_frame(str(path).encode(), width, height, _ptr(depth), _ptr(hiz), nq, _ptr(vis), _ptr(clp)),
+           "rz_write_frame")
+
+
+def load_frame(path) -> dict:
+    """Reads a frame dump: {width, height, depth, hiz, visible, clip}."""
+    w, h, nq = ctypes.c_uint32(0), ctypes.c_uint32(0), ctypes.c_uint32(0)
+    p = str(path).encode()
+    _check(lib().rz_read_frame(p, ctypes.byref(w), ctypes.byref(h), ctypes.byref(nq), None, None, None, None),
+           "rz_read_frame")
+    nb = (w.value // 8) * (h.value // 8)
+    depth = np.zeros(nb * 64, np.uint16)
+    hiz = np.zeros(nb + 8, np.uint16)
+    vis = np.zeros(max(nq.value, 1), np.uint8)
+    clp = np.zeros(max(nq.value, 1), np.uint8)
+    _check(lib().rz_read_frame(p, ctypes.byref(w), ctypes.byref(h), ctypes.byref(nq), _ptr(depth), _ptr(hiz),
+                               _ptr(vis), _ptr(clp)), "rz_read_frame")
+    return {"width": w.value, "height": h.value, "depth": depth, "hiz": hiz, "visible": vis[: nq.value],
+            "clip": clp[: nq.value]}
 
 
 class DeviceBuffer:

@@ -61,6 +61,7 @@ def build(verbose: bool = False) -> Path:
     LIBDIR.mkdir(exist_ok=True)
     objs = {
         "rz_host.o": ["g++", *HOST_FLAGS, "-c", str(CSRC / "rz_host.cpp")],
+        "rz_io.o": ["g++", *HOST_FLAGS, "-c", str(CSRC / "rz_io.cpp")],
         "dropin.o": ["g++", *HOST_FLAGS, "-c", str(CSRC / "dropin.cpp")],
         "rz_kernels.o": [hipcc, *HIP_FLAGS, "-c", str(CSRC / "rz_kernels.hip")],
         "rz_api.o": [hipcc, *HIP_FLAGS, "-c", str(CSRC / "rz_api.cpp")],

@@ -116,6 +116,7 @@ extern "C" const char* rz_status_string(int s) {
     case RZ_ERR_UNSUPPORTED_HOST: return "host CPU RCPPS has an unsupported structure";
     case RZ_ERR_TOO_LARGE: return "buffer too large for one co-resident frame launch";
     case RZ_ERR_NOT_IDENTICAL: return "launch configurations disagree (autotuner check)";
+    case RZ_ERR_IO: return "file cannot be written or read, or is not this format, version or checksum";
     default: return "unknown status";
   }
 }

@@ -63,7 +63,7 @@ def test_dropin_classes_exported():
 
 
 def test_status_strings(rzlib):
-    for code in (0, -1, -2, -3, -4, -5, -6, -7):
+    for code in (0, -1, -2, -3, -4, -5, -6, -7, -8):
         s = rzlib.rz_status_string(code)
         assert s and len(s) >= 2
     assert rzlib.rz_status_string(12345)
