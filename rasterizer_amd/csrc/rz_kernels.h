@@ -70,6 +70,11 @@ struct ViewsParams {
   // lists[views[v].listBase + t * (items in view) ...], listLen[unit] long.
   uint32_t* lists;
   uint32_t* listLen;
+  // Optional (with lists): the units that have items, in unit order, as
+  // view << 20 | tile, and their count; the queue hands out only these.
+  uint32_t* unitList;
+  uint32_t* nActive;
+  uint32_t* unitCounts;  // scratch: per view
 };
 
 // queryVisibility over the queries of every view of a multi-view launch.
@@ -171,6 +176,9 @@ hipError_t views_max_groups(int waves, int* maxGroups);
 // nItems u64 of scratch (the items of p.items / p.state, batch items
 // included).
 hipError_t launch_views_bin(const ViewsParams& p, uint32_t nItems, uint64_t* tileBox, hipStream_t stream);
+// Builds p.unitList / p.nActive from p.listLen (after launch_views_bin);
+// unitTmp: nUnits words of scratch.
+hipError_t launch_views_compact(const ViewsParams& p, uint32_t* unitTmp, hipStream_t stream);
 hipError_t launch_views_query(const ViewsQueryParams& p, hipStream_t stream);
 // hiZ := 1 (whole 8-entry groups) and tile min := 1 for every view.
 hipError_t launch_views_clear(const ViewDesc* views, uint32_t nViews, uint32_t maxEntries, hipStream_t stream);

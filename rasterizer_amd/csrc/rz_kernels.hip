@@ -1727,20 +1727,33 @@ __global__ void __launch_bounds__(64 * NW, NW == 8 ? 4 : 2) rz_views_kernel(View
   if (STATS && threadIdx.x < kFrameStatWords) ctr[threadIdx.x] = 0;
   const uint64_t tEntry = STATS ? __builtin_amdgcn_s_memrealtime() : 0;
   bool covLoaded = false;
+  // With a compacted unit list, only units that have items are handed out,
+  // each carrying its view; else every unit, its view found by search.
+  const uint32_t nHanded = p.unitList ? *p.nActive : p.nUnits;
   while (true) {
-    if (threadIdx.x == 0) unitSh = __hip_atomic_fetch_add(p.queue, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (threadIdx.x == 0) {
+      const uint32_t u = __hip_atomic_fetch_add(p.queue, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      unitSh = u >= nHanded ? 0xFFFFFFFFu : p.unitList ? p.unitList[u] : u;
+    }
     __syncthreads();
-    const uint32_t unit = unitSh;
+    const uint32_t entry = unitSh;
     __syncthreads();
-    if (unit >= p.nUnits) break;
-    uint32_t lo = 0, hi = p.nViews - 1;
-    while (lo < hi) {
-      const uint32_t mid = (lo + hi + 1) >> 1;
-      if (p.views[mid].unitBase <= unit) lo = mid;
-      else hi = mid - 1;
+    if (entry == 0xFFFFFFFFu) break;
+    uint32_t lo = 0, t = 0;
+    if (p.unitList) {
+      lo = entry >> 20;
+      t = entry & 0xFFFFFu;
+    } else {
+      uint32_t hi = p.nViews - 1;
+      while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1) >> 1;
+        if (p.views[mid].unitBase <= entry) lo = mid;
+        else hi = mid - 1;
+      }
+      t = entry - p.views[lo].unitBase;
     }
     const ViewDesc& V = p.views[lo];
-    const uint32_t t = unit - V.unitBase;
+    const uint32_t unit = V.unitBase + t;
     TileJob J;
     J.items = p.items;
     J.state = p.state;
@@ -1781,6 +1794,47 @@ __global__ void __launch_bounds__(64 * NW, NW == 8 ? 4 : 2) rz_views_kernel(View
       atomicAdd(&p.stats[2], (unsigned long long)ctr[kStatBlocksWritten]);
     }
   }
+}
+
+// The units of one view (one work-group per view) that have items, in tile
+// order, as view << 20 | tile, at unitTmp[view's unitBase ...], and their
+// count in p.unitCounts[view].
+__global__ void __launch_bounds__(256) rz_views_compact_kernel(ViewsParams p, uint32_t* __restrict__ unitTmp) {
+  __shared__ uint32_t wcnt[4];
+  const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+  const uint32_t v = blockIdx.x;
+  const ViewDesc& V = p.views[v];
+  uint32_t n = 0;
+  for (uint32_t base = 0; base < V.nUnits; base += 256u) {
+    const uint32_t t = base + tid;
+    const bool has = t < V.nUnits && p.listLen[V.unitBase + t] != 0u;
+    const uint64_t m = __ballot(has);
+    const uint32_t pre = __builtin_amdgcn_mbcnt_hi(uint32_t(m >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(m), 0u));
+    if (lane == 0) wcnt[wave] = uint32_t(__popcll(m));
+    __syncthreads();
+    uint32_t off = n, total = 0;
+#pragma unroll
+    for (uint32_t w = 0; w < 4; ++w) {
+      off += w < wave ? wcnt[w] : 0u;
+      total += wcnt[w];
+    }
+    if (has) unitTmp[V.unitBase + off + pre] = (v << 20) | t;
+    n += total;
+    __syncthreads();
+  }
+  if (tid == 0) p.unitCounts[v] = n;
+}
+
+// Concatenates the views' compacted units in view order into p.unitList
+// (one work-group per view); the last view also writes the total.
+__global__ void __launch_bounds__(256) rz_views_gather_kernel(ViewsParams p, const uint32_t* __restrict__ unitTmp) {
+  const uint32_t v = blockIdx.x;
+  uint32_t off = 0;
+  for (uint32_t w = 0; w < v; ++w) off += p.unitCounts[w];
+  const uint32_t n = p.unitCounts[v];
+  const uint32_t src = p.views[v].unitBase;
+  for (uint32_t k = threadIdx.x; k < n; k += blockDim.x) p.unitList[off + k] = unitTmp[src + k];
+  if (v == p.nViews - 1 && threadIdx.x == 0) *p.nActive = off + n;
 }
 
 // Per item of a multi-view launch, the box in 8x8-block tiles (u16 x0, x1,
@@ -1869,6 +1923,15 @@ hipError_t launch_views_bin(const ViewsParams& p, uint32_t nItems, uint64_t* til
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(rz_views_bin_kernel, dim3(std::min<uint32_t>(p.nUnits, 65535u)), dim3(256), 0, stream, p,
                      tileBox);
+  return hipGetLastError();
+}
+
+hipError_t launch_views_compact(const ViewsParams& p, uint32_t* unitTmp, hipStream_t stream) {
+  if (p.nUnits == 0 || !p.unitList) return hipSuccess;
+  hipLaunchKernelGGL(rz_views_compact_kernel, dim3(p.nViews), dim3(256), 0, stream, p, unitTmp);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(rz_views_gather_kernel, dim3(p.nViews), dim3(256), 0, stream, p, static_cast<const uint32_t*>(unitTmp));
   return hipGetLastError();
 }
 

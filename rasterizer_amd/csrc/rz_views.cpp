@@ -111,6 +111,10 @@ struct rz_views {
   uint32_t* d_lists = nullptr;    // per-tile item lists (null: tiles walk their view's item range)
   uint32_t* d_listLen = nullptr;  // per (view, tile) unit
   uint64_t* d_tileBox = nullptr;  // per drawn item (occluder or batch): tile box, for binning
+  uint32_t* d_unitList = nullptr;  // units with items, as view << 20 | tile (with lists)
+  uint32_t* d_unitTmp = nullptr;
+  uint32_t* d_unitCounts = nullptr;  // per view
+  uint32_t* d_nActive = nullptr;
   bool gated = false;   // query-gated draw loop per view (Main.cpp:192-206) instead of classify-then-draw
   bool cooperative = true;  // gated launches: cooperative dispatch (RZ_FRAME_LAUNCH=plain: plain)
   uint32_t* d_err = nullptr;  // device error word of gated waits
@@ -155,8 +159,13 @@ static void free_plan(rz_views* v) {
   hipFree(v->d_lists);
   hipFree(v->d_listLen);
   hipFree(v->d_tileBox);
+  hipFree(v->d_unitList);
+  hipFree(v->d_unitTmp);
+  hipFree(v->d_unitCounts);
+  hipFree(v->d_nActive);
   v->d_lists = v->d_listLen = nullptr;
   v->d_tileBox = nullptr;
+  v->d_unitList = v->d_unitTmp = v->d_unitCounts = v->d_nActive = nullptr;
   hipFree(v->d_itemScores);
   hipFree(v->d_itemIota);
   hipFree(v->d_itemOrder);
@@ -475,6 +484,14 @@ static int build_plan(rz_views* v) {
           dev_alloc(&v->d_listLen, std::max<size_t>(v->nUnits, 1)) != hipSuccess ||
           dev_alloc(&v->d_tileBox, nDraw) != hipSuccess)
         return RZ_ERR_OUT_OF_MEMORY;
+      // Compacted unit list (view << 20 | tile): fewer than 4096 views of
+      // fewer than 2^20 tiles each.
+      bool fits = nv < 4096;
+      for (uint32_t i = 0; i < nv; ++i) fits &= v->h_views[i].nUnits < (1u << 20);
+      if (fits && (dev_alloc(&v->d_unitList, std::max<size_t>(v->nUnits, 1)) != hipSuccess ||
+                   dev_alloc(&v->d_unitTmp, std::max<size_t>(v->nUnits, 1)) != hipSuccess ||
+                   dev_alloc(&v->d_unitCounts, nv) != hipSuccess || dev_alloc(&v->d_nActive, 1) != hipSuccess))
+        return RZ_ERR_OUT_OF_MEMORY;
     }
   }
   // Occluder-granular reorders (learned by occluder, front to back): items
@@ -664,6 +681,10 @@ static int enqueue_step(rz_views* v, bool capturing) {
     rp.lists = v->d_lists;
     rp.listLen = v->d_listLen;
     RZ_HIP(launch_views_bin(rp, v->batchOrder ? v->nBatches : v->nItems, v->d_tileBox, v->stream));
+    rp.unitList = v->d_unitList;
+    rp.nActive = v->d_nActive;
+    rp.unitCounts = v->d_unitCounts;
+    RZ_HIP(launch_views_compact(rp, v->d_unitTmp, v->stream));
     rp.done = v->gated ? v->d_done : nullptr;
     rp.err = v->d_err;
     rp.timeoutTicks = kGatedTimeoutTicks;
