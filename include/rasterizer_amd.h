@@ -145,6 +145,17 @@ int rz_get_mvp(rz_context* ctx, float prebaked[16], float raw[16]);
  * center as m_boundsMin, m_boundsMax, m_center. */
 int rz_bake(const float* vertices, uint32_t n, const float refMin[4], const float refMax[4], uint32_t* packets,
             float boundsMin[4], float boundsMax[4], float center[4]);
+/* Occluder::bake on the device, for many occluders at once (occluders built
+ * at run time): occluder i's vertices are d_vertices[4 * vertexOffsets[i] ..
+ * 4 * vertexOffsets[i + 1]) (device, xyzw floats; a positive multiple of 32
+ * vertices, i.e. whole 8-quad batches; offsets multiples of 32), its
+ * reference box refMinMax[8i .. 8i + 8) (host: min xyzw, max xyzw).  Writes
+ * the packets, one word per vertex at the same offsets, to d_packets, and
+ * boundsMin, boundsMax and centre (12 floats per occluder) to d_bounds.
+ * Bit-identical to rz_bake; RZ_ERR_UNSUPPORTED_HOST if this CPU's RSQRTPS
+ * cannot be tabulated.  vertexOffsets: n + 1 host words.  Blocks. */
+int rz_bake_device(rz_context* ctx, uint32_t n, const uint32_t* vertexOffsets, const float* d_vertices,
+                   const float* refMinMax, uint32_t* d_packets, float* d_bounds);
 
 /* Timing helpers on the context stream (HIP events): rz_timer_start records
  * a start event, rz_timer_stop a stop event and returns (after waiting for

@@ -85,6 +85,7 @@ _SIGNATURES = {
     "rz_get_tables": ([_P, _P, _P], ctypes.c_int),
     "rz_get_mvp": ([_P, _P, _P], ctypes.c_int),
     "rz_bake": ([_P, _U32, _P, _P, _P, _P, _P, _P], ctypes.c_int),
+    "rz_bake_device": ([_P, _U32, _P, _P, _P, _P, _P], ctypes.c_int),
     "rz_timer_start": ([_P], ctypes.c_int),
     "rz_timer_stop": ([_P, ctypes.POINTER(ctypes.c_float)], ctypes.c_int),
     "rz_profile_enable": ([_P, ctypes.c_int], ctypes.c_int),
@@ -455,6 +456,33 @@ class Rasterizer:
         found = ctypes.c_int(0)
         _check(lib().rz_load_config(self._ctx, os.fsencode(cache), ctypes.byref(found)), "rz_load_config")
         return bool(found.value)
+
+    def bake_device(self, meshes, ref_mins, ref_maxs) -> list:
+        """Occluder::bake on the device for several occluders at once
+        (rz_bake_device): meshes[i] is an (n_i, 4) vertex array (n_i a
+        positive multiple of 32), with its reference box; returns Occluder
+        objects, bit-identical to Occluder.bake."""
+        meshes = [np.ascontiguousarray(np.asarray(m, np.float32).reshape(-1, 4)) for m in meshes]
+        n = len(meshes)
+        if n == 0:
+            return []
+        offsets = np.zeros(n + 1, np.uint32)
+        offsets[1:] = np.cumsum([m.shape[0] for m in meshes])
+        verts = np.concatenate(meshes)
+        ref = np.concatenate([np.concatenate([_vec4(a), _vec4(b)]) for a, b in zip(ref_mins, ref_maxs)]).astype(np.float32)
+        d_v = DeviceBuffer(self, verts.nbytes, verts)
+        d_p = DeviceBuffer(self, int(offsets[-1]) * 4)
+        d_b = DeviceBuffer(self, n * 48)
+        try:
+            _check(lib().rz_bake_device(self.ctx, n, _ptr(offsets), d_v.ptr, _ptr(ref), d_p.ptr, d_b.ptr),
+                   "rz_bake_device")
+            packets = d_p.read(np.uint32, int(offsets[-1]))
+            bounds = d_b.read(np.float32, n * 12).reshape(n, 12)
+        finally:
+            for b in (d_v, d_p, d_b):
+                b.free()
+        return [Occluder(packets[offsets[i]:offsets[i + 1]].copy(), ref[8 * i:8 * i + 4], ref[8 * i + 4:8 * i + 8],
+                         bounds[i, 0:4], bounds[i, 4:8], bounds[i, 8:12]) for i in range(n)]
 
     def query_batch_device(self, n: int, d_bounds: "DeviceBuffer", d_visible: "DeviceBuffer",
                            d_clip: "DeviceBuffer | None" = None) -> None:

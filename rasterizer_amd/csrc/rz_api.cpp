@@ -42,6 +42,7 @@ const HostTables& host_tables() {
   std::call_once(once, [] {
     if (!rzh::build_coverage_table(t.cov)) t.status = RZ_ERR_UNSUPPORTED_HOST;
     else if (rzh::build_rcp_table(t.rcp) != 0) t.status = RZ_ERR_UNSUPPORTED_HOST;
+    if (rzh::build_rsqrt_table(t.rsqrt) != 0) t.rsqrtStatus = RZ_ERR_UNSUPPORTED_HOST;
   });
   return t;
 }
@@ -670,6 +671,54 @@ extern "C" int rz_bake(const float* vertices, uint32_t n, const float refMin[4],
     return RZ_ERR_INVALID_ARGUMENT;
   return rzh::bake(vertices, n, refMin, refMax, packets, boundsMin, boundsMax, center) == 0 ? RZ_OK
                                                                                              : RZ_ERR_INVALID_ARGUMENT;
+}
+
+extern "C" int rz_bake_device(rz_context* ctx, uint32_t n, const uint32_t* vertexOffsets, const float* d_vertices,
+                              const float* refMinMax, uint32_t* d_packets, float* d_bounds) {
+  if (!ctx || (n && (!vertexOffsets || !d_vertices || !refMinMax || !d_packets || !d_bounds)))
+    return RZ_ERR_INVALID_ARGUMENT;
+  if (n == 0) return RZ_OK;
+  for (uint32_t i = 0; i < n; ++i) {
+    const uint32_t a = vertexOffsets[i], b = vertexOffsets[i + 1];
+    if (b <= a || (b - a) % 32 || a % 32) return RZ_ERR_INVALID_ARGUMENT;
+  }
+  if (vertexOffsets[n] >= (1u << 31)) return RZ_ERR_TOO_LARGE;
+  const rzi::HostTables& tabs = rzi::host_tables();
+  if (tabs.rsqrtStatus != RZ_OK) return tabs.rsqrtStatus;
+  const size_t quads = vertexOffsets[n] / 4;
+  BakeParams p{};
+  uint32_t* d_offsets = nullptr;
+  float* d_ref = nullptr;
+  uint32_t* d_rsqrt = nullptr;
+  int status = RZ_OK;
+  if (dev_alloc(&d_offsets, size_t(n) + 1) != hipSuccess || dev_alloc(&d_ref, size_t(n) * 8) != hipSuccess ||
+      dev_alloc(&d_rsqrt, 8192) != hipSuccess || dev_alloc(&p.normals, quads) != hipSuccess ||
+      dev_alloc(&p.cluster, quads) != hipSuccess || dev_alloc(&p.order, quads) != hipSuccess) {
+    status = RZ_ERR_OUT_OF_MEMORY;
+  } else if (hipMemcpyAsync(d_offsets, vertexOffsets, (size_t(n) + 1) * 4, hipMemcpyHostToDevice, ctx->stream) !=
+                 hipSuccess ||
+             hipMemcpyAsync(d_ref, refMinMax, size_t(n) * 32, hipMemcpyHostToDevice, ctx->stream) != hipSuccess ||
+             hipMemcpyAsync(d_rsqrt, tabs.rsqrt, sizeof tabs.rsqrt, hipMemcpyHostToDevice, ctx->stream) !=
+                 hipSuccess) {
+    status = RZ_ERR_DEVICE;
+  } else {
+    p.vertices = d_vertices;
+    p.offsets = d_offsets;
+    p.ref = d_ref;
+    p.packets = d_packets;
+    p.bounds = d_bounds;
+    p.rsqrt = d_rsqrt;
+    if (launch_bake(p, n, ctx->stream) != hipSuccess || hipStreamSynchronize(ctx->stream) != hipSuccess)
+      status = RZ_ERR_DEVICE;
+  }
+  (void)hipStreamSynchronize(ctx->stream);
+  hipFree(d_offsets);
+  hipFree(d_ref);
+  hipFree(d_rsqrt);
+  hipFree(p.normals);
+  hipFree(p.cluster);
+  hipFree(p.order);
+  return status;
 }
 
 extern "C" int rz_timer_start(rz_context* ctx) {

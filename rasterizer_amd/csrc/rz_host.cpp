@@ -127,6 +127,33 @@ int build_rcp_table(uint32_t* table) {
   return 0;
 }
 
+int build_rsqrt_table(uint32_t* table) {
+  auto rsq = [](float x) { return as_u32(_mm_cvtss_f32(_mm_rsqrt_ps(_mm_set1_ps(x)))); };
+  for (uint32_t p = 0; p < 2; ++p)
+    for (uint32_t i = 0; i < 4096; ++i) table[p * 4096 + i] = rsq(as_f32(((127u + p) << 23) | (i << 11)));
+  for (uint32_t p = 0; p < 2; ++p)
+    for (uint32_t m = 0; m < (1u << 23); ++m)
+      if (rsq(as_f32(((127u + p) << 23) | m)) != table[p * 4096 + (m >> 11)]) return -1;
+  auto predict = [&](uint32_t u) -> uint32_t {
+    const uint32_t s = u & 0x80000000u, e = (u >> 23) & 0xFFu, m = u & 0x7FFFFFu;
+    if (e == 0xFF) return m ? (u | 0x400000u) : (s ? 0xFFC00000u : 0u);
+    if (e == 0) return s | 0x7F800000u;
+    if (s) return 0xFFC00000u;
+    const int E = int(e) - 127, p = E & 1, k = (E - p) / 2;
+    const uint32_t t = table[uint32_t(p) * 4096 + (m >> 11)];
+    return (uint32_t(int((t >> 23) & 0xFF) - k) << 23) | (t & 0x7FFFFFu);
+  };
+  for (uint32_t e = 0; e < 256; ++e)
+    for (uint32_t m = 0; m < (1u << 23); m += 4093) {
+      for (uint32_t s = 0; s < 2; ++s) {
+        uint32_t u = (s << 31) | (e << 23) | m;
+        if (e == 0xFF && m) u |= 0x400000u;  // probe quiet NaNs only
+        if (rsq(as_f32(u)) != predict(u)) return -1;
+      }
+    }
+  return 0;
+}
+
 // setModelViewProjection (Rasterizer.cpp:76-105): raw rows are the transpose
 // of the row-vector input; the prebaked rows fold the viewport (x, y mapped
 // to [0, W-8] x [0, H-8]) and the depth mapping ([-1,1] -> [bias, 0]); both

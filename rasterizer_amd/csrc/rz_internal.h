@@ -43,6 +43,8 @@ struct HostTables {
   int status = RZ_OK;
   uint64_t cov[4096];
   uint32_t rcp[4096];
+  int rsqrtStatus = RZ_OK;  // the device bake needs this host's RSQRTPS structure
+  uint32_t rsqrt[8192];
 };
 const HostTables& host_tables();
 

@@ -253,6 +253,24 @@ struct MlpParams {
   float* grad;          // backward output, accumulated (zero it first)
 };
 hipError_t launch_mlp_forward(const MlpParams& p, hipStream_t stream);
+
+// Occluder::bake on the device (rz_bake.hip): occluder o's vertices are
+// vertices[offsets[o] .. offsets[o + 1]) (4 floats each, a multiple of 32),
+// its reference box ref[8o .. 8o + 8) (min xyzw, max xyzw); packets get one
+// word per vertex at the same offsets, bounds 12 floats per occluder
+// (boundsMin, boundsMax, centre).  Scratch per quad: normals, cluster, order.
+struct BakeParams {
+  const float* vertices;
+  const uint32_t* offsets;
+  const float* ref;
+  uint32_t* packets;
+  float* bounds;
+  float4* normals;
+  uint32_t* cluster;
+  uint32_t* order;
+  const uint32_t* rsqrt;  // 8192-entry host RSQRTPS table
+};
+hipError_t launch_bake(const BakeParams& p, uint32_t nOccluders, hipStream_t stream);
 hipError_t launch_mlp_backward(const MlpParams& p, hipStream_t stream);
 // Pairwise ranking loss over pairs (i, j) whose sample i should rank above j:
 // mean softplus(s_j - s_i); adds its gradient to dscore and its value to *loss.

@@ -45,6 +45,21 @@ __device__ __forceinline__ float rcp_x86(float x, const uint32_t* __restrict__ t
   return bitsf(s | (uint32_t(re) << 23) | (t & 0x7FFFFFu));
 }
 
+// RSQRTPS from the host's table (8192 entries: exponent parity p, the top 12
+// mantissa bits; see rzh::build_rsqrt_table).
+__device__ __forceinline__ float rsqrt_x86(float x, const uint32_t* __restrict__ tab) {
+  const uint32_t u = fbits(x);
+  const uint32_t s = u & 0x80000000u;
+  const uint32_t e = (u >> 23) & 0xFFu;
+  const uint32_t m = u & 0x7FFFFFu;
+  if (e == 0xFFu) return bitsf(m ? (u | 0x400000u) : (s ? 0xFFC00000u : 0u));
+  if (e == 0u) return bitsf(s | 0x7F800000u);
+  if (s) return bitsf(0xFFC00000u);
+  const int E = int(e) - 127, p = E & 1, k = (E - p) / 2;
+  const uint32_t t = tab[uint32_t(p) * 4096u + (m >> 11)];
+  return bitsf((uint32_t(int((t >> 23) & 0xFFu) - k) << 23) | (t & 0x7FFFFFu));
+}
+
 __device__ __forceinline__ int32_t cvtt_x86(float x) {
   return (__builtin_fabsf(x) < 2147483648.0f) ? int32_t(x) : int32_t(0x80000000u);
 }
