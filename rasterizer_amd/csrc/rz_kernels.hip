@@ -1974,19 +1974,12 @@ __global__ void __launch_bounds__(256) rz_query_kernel(QueryParams p) {
 }
 
 // queryVisibility over the queries of every view of a multi-view launch, one
-// wave per box; the view is found from the box index.
+// wave per box; blockIdx.y is the view.
 __global__ void __launch_bounds__(256) rz_views_query_kernel(ViewsQueryParams p) {
   const uint32_t lane = threadIdx.x & 63u;
-  const uint32_t q = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  if (q >= p.nQueries) return;
-  uint32_t lo = 0, hi = p.nViews - 1;
-  while (lo < hi) {
-    const uint32_t mid = (lo + hi + 1) >> 1;
-    if (p.views[mid].queryBase <= q) lo = mid;
-    else hi = mid - 1;
-  }
-  const ViewDesc& V = p.views[lo];
-  const uint32_t k = q - V.queryBase;
+  const ViewDesc& V = p.views[blockIdx.y];
+  const uint32_t k = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  if (k >= V.nQueries) return;
   const float* b = V.queryBounds + 8 * size_t(k);
   const float bmin[4] = {b[0], b[1], b[2], b[3]};
   const float bmax[4] = {b[4], b[5], b[6], b[7]};
@@ -2421,7 +2414,7 @@ hipError_t views_max_groups(int waves, int* maxGroups) {
 
 hipError_t launch_views_query(const ViewsQueryParams& p, hipStream_t stream) {
   if (p.nQueries == 0) return hipSuccess;
-  hipLaunchKernelGGL(rz_views_query_kernel, dim3(groups(p.nQueries, 4)), dim3(256), 0, stream, p);
+  hipLaunchKernelGGL(rz_views_query_kernel, dim3(groups(p.maxPerView, 4), p.nViews), dim3(256), 0, stream, p);
   return hipGetLastError();
 }
 

@@ -696,6 +696,7 @@ static int enqueue_step(rz_views* v, bool capturing) {
   qp.views = v->d_views;
   qp.nViews = nv;
   qp.nQueries = v->nQueries;
+  for (const ViewHost& h : v->views) qp.maxPerView = std::max(qp.maxPerView, h.nQueries);
   qp.rcp = v->d_rcp;
   RZ_HIP(launch_views_query(qp, v->stream));
   if (prof) {
