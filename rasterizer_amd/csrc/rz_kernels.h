@@ -81,7 +81,7 @@ struct ViewsParams {
 struct ViewsQueryParams {
   const ViewDesc* views;
   uint32_t nViews, nQueries;
-  uint32_t maxPerView;  // most queries of any view (grid x extent, 4 per work-group)
+  uint32_t maxPerView;  // most queries of any view (grid x extent, 256 per work-group)
   const uint32_t* rcp;
 };
 

@@ -1973,26 +1973,40 @@ __global__ void __launch_bounds__(256) rz_query_kernel(QueryParams p) {
   }
 }
 
-// queryVisibility over the queries of every view of a multi-view launch, one
-// wave per box; blockIdx.y is the view.
+// queryVisibility over the queries of every view of a multi-view launch;
+// blockIdx.y is the view.  A step holds many boxes, so one lane per box does
+// the depth-independent part (projection, frustum, near plane, rectangle)
+// and the wave then runs query2D for its boxes that need it.  (The
+// single-view kernel keeps one wave per box: its batches are too small to
+// fill the GPU with 64 boxes per wave.)
 __global__ void __launch_bounds__(256) rz_views_query_kernel(ViewsQueryParams p) {
   const uint32_t lane = threadIdx.x & 63u;
   const ViewDesc& V = p.views[blockIdx.y];
-  const uint32_t k = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  if (k >= V.nQueries) return;
-  const float* b = V.queryBounds + 8 * size_t(k);
-  const float bmin[4] = {b[0], b[1], b[2], b[3]};
-  const float bmax[4] = {b[4], b[5], b[6], b[7]};
-  const QueryPre Q = query_pre(V.mvp, bmin, bmax, V.width, V.height, p.rcp);
-  uint32_t vis = 0, clip = 0;
-  if (Q.kind == 1) {
-    vis = 1;
-    clip = 1;
-  } else if (Q.kind == 3) {
-    vis = query2d_wave(Q.minX, Q.maxX, Q.minY, Q.maxY, Q.maxZ, V.depth, V.hiz, V.tileMin, V.bX, V.pyrX, lane) ? 1u
-                                                                                                            : 0u;
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;  // 64 boxes per wave
+  const bool valid = k < V.nQueries;
+  QueryPre Q = {0, 0, 0, 0, 0, 0};
+  if (valid) {
+    const float* b = V.queryBounds + 8 * size_t(k);
+    const float bmin[4] = {b[0], b[1], b[2], b[3]};
+    const float bmax[4] = {b[4], b[5], b[6], b[7]};
+    Q = query_pre(V.mvp, bmin, bmax, V.width, V.height, p.rcp);
   }
-  if (lane == 0) {
+  uint32_t vis = Q.kind == 1 ? 1u : 0u;
+  const uint32_t clip = vis;
+  // The boxes that need query2D, one after another, each by the whole wave.
+  uint64_t m = __ballot(valid && Q.kind == 3);
+  while (m) {
+    const int j = int(__builtin_ctzll(m));
+    m &= m - 1ull;
+    const uint32_t minX = uint32_t(__builtin_amdgcn_readlane(int(Q.minX), j));
+    const uint32_t maxX = uint32_t(__builtin_amdgcn_readlane(int(Q.maxX), j));
+    const uint32_t minY = uint32_t(__builtin_amdgcn_readlane(int(Q.minY), j));
+    const uint32_t maxY = uint32_t(__builtin_amdgcn_readlane(int(Q.maxY), j));
+    const uint32_t maxZ = uint32_t(__builtin_amdgcn_readlane(int(Q.maxZ), j));
+    const bool v = query2d_wave(minX, maxX, minY, maxY, maxZ, V.depth, V.hiz, V.tileMin, V.bX, V.pyrX, lane);
+    if (int(lane) == j) vis = v ? 1u : 0u;
+  }
+  if (valid) {
     V.queryVisible[k] = uint8_t(vis);
     V.queryClip[k] = uint8_t(clip);
   }
@@ -2414,7 +2428,7 @@ hipError_t views_max_groups(int waves, int* maxGroups) {
 
 hipError_t launch_views_query(const ViewsQueryParams& p, hipStream_t stream) {
   if (p.nQueries == 0) return hipSuccess;
-  hipLaunchKernelGGL(rz_views_query_kernel, dim3(groups(p.maxPerView, 4), p.nViews), dim3(256), 0, stream, p);
+  hipLaunchKernelGGL(rz_views_query_kernel, dim3(groups(p.maxPerView, 256), p.nViews), dim3(256), 0, stream, p);
   return hipGetLastError();
 }
 
