@@ -156,6 +156,21 @@ int rz_bake(const float* vertices, uint32_t n, const float refMin[4], const floa
  * cannot be tabulated.  vertexOffsets: n + 1 host words.  Blocks. */
 int rz_bake_device(rz_context* ctx, uint32_t n, const uint32_t* vertexOffsets, const float* d_vertices,
                    const float* refMinMax, uint32_t* d_packets, float* d_bounds);
+/* SurfaceAreaHeuristic::generateBatches(aabbs, targetSize, splitGranularity)
+ * (SurfaceAreaHeuristic.h, SurfaceAreaHeuristic.cpp:96-104) on the device.
+ * d_boxes: n quad boxes as std::vector<Aabb> lays them out (min xyzw, max
+ * xyzw; w ignored), finite, min <= max and |coordinate| <= 2^40.  Writes the
+ * batches' quad indices, batch after batch, to d_indices (n words, device)
+ * and the batch boundaries to batchOffsets (host, *nBatches + 1 words; room
+ * for n + 1 suffices), in the reference's depth-first order, bit-identical
+ * to its splits: stable sorts by box centre (-0 equal to +0), DPPS surface
+ * areas, the first strict minimum of the cost over (axis, split index).
+ * Needs splitGranularity >= 1 and targetSize > 2 * splitGranularity (else a
+ * child could have no split position); a set of at most 2 * splitGranularity
+ * quads, where the reference's split is undefined, is one batch in input
+ * order.  Boxes outside the stated range: RZ_ERR_INVALID_ARGUMENT.  Blocks. */
+int rz_sah_batches(rz_context* ctx, uint32_t n, const float* d_boxes, uint32_t targetSize, uint32_t splitGranularity,
+                   uint32_t* d_indices, uint32_t* batchOffsets, uint32_t* nBatches);
 
 /* Timing helpers on the context stream (HIP events): rz_timer_start records
  * a start event, rz_timer_stop a stop event and returns (after waiting for

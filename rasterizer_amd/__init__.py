@@ -86,6 +86,7 @@ _SIGNATURES = {
     "rz_get_mvp": ([_P, _P, _P], ctypes.c_int),
     "rz_bake": ([_P, _U32, _P, _P, _P, _P, _P, _P], ctypes.c_int),
     "rz_bake_device": ([_P, _U32, _P, _P, _P, _P, _P], ctypes.c_int),
+    "rz_sah_batches": ([_P, _U32, _P, _U32, _U32, _P, _P, ctypes.POINTER(_U32)], ctypes.c_int),
     "rz_timer_start": ([_P], ctypes.c_int),
     "rz_timer_stop": ([_P, ctypes.POINTER(ctypes.c_float)], ctypes.c_int),
     "rz_profile_enable": ([_P, ctypes.c_int], ctypes.c_int),
@@ -483,6 +484,34 @@ class Rasterizer:
                 b.free()
         return [Occluder(packets[offsets[i]:offsets[i + 1]].copy(), ref[8 * i:8 * i + 4], ref[8 * i + 4:8 * i + 8],
                          bounds[i, 0:4], bounds[i, 4:8], bounds[i, 8:12]) for i in range(n)]
+
+    def sah_batches_device(self, quad_min, quad_max, target: int = 512, gran: int = 8) -> list:
+        """SurfaceAreaHeuristic::generateBatches on the device
+        (rz_sah_batches): quad_min and quad_max are (n, 3) or (n, 4) box
+        corners; returns the batches' quad-index arrays in the reference's
+        order, identical to scenes.sah_batches."""
+        mn = np.atleast_2d(np.asarray(quad_min, np.float32))
+        mx = np.atleast_2d(np.asarray(quad_max, np.float32))
+        n = mn.shape[0]
+        boxes = np.zeros((n, 8), np.float32)
+        boxes[:, 0:3] = mn[:, :3]
+        boxes[:, 4:7] = mx[:, :3]
+        offsets = np.zeros(n + 1, np.uint32)
+        count = _U32(0)
+        if n == 0:
+            _check(lib().rz_sah_batches(self.ctx, 0, None, target, gran, None, _ptr(offsets), ctypes.byref(count)),
+                   "rz_sah_batches")
+            return []
+        d_b = DeviceBuffer(self, boxes.nbytes, boxes)
+        d_i = DeviceBuffer(self, n * 4)
+        try:
+            _check(lib().rz_sah_batches(self.ctx, n, d_b.ptr, target, gran, d_i.ptr, _ptr(offsets),
+                                        ctypes.byref(count)), "rz_sah_batches")
+            idx = d_i.read(np.uint32, n).astype(np.int64)
+        finally:
+            d_b.free()
+            d_i.free()
+        return [idx[offsets[k]:offsets[k + 1]] for k in range(count.value)]
 
     def query_batch_device(self, n: int, d_bounds: "DeviceBuffer", d_visible: "DeviceBuffer",
                            d_clip: "DeviceBuffer | None" = None) -> None:

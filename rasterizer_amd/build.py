@@ -69,6 +69,7 @@ def build(verbose: bool = False) -> Path:
         "rz_learn.o": [hipcc, *HIP_FLAGS, "-c", str(CSRC / "rz_learn.cpp")],
         "rz_mlp.o": [hipcc, *HIP_FLAGS, "-c", str(CSRC / "rz_mlp.hip")],
         "rz_bake.o": [hipcc, *HIP_FLAGS, "-c", str(CSRC / "rz_bake.hip")],
+        "rz_sah.o": [hipcc, *HIP_FLAGS, "-c", str(CSRC / "rz_sah.hip")],
         "rz_tune.o": [hipcc, *HIP_FLAGS, "-c", str(CSRC / "rz_tune.cpp")],
     }
     cmds = [cmd + ["-o", str(BUILD / name)] for name, cmd in objs.items()]

@@ -721,6 +721,13 @@ extern "C" int rz_bake_device(rz_context* ctx, uint32_t n, const uint32_t* verte
   return status;
 }
 
+extern "C" int rz_sah_batches(rz_context* ctx, uint32_t n, const float* d_boxes, uint32_t targetSize,
+                              uint32_t splitGranularity, uint32_t* d_indices, uint32_t* batchOffsets,
+                              uint32_t* nBatches) {
+  if (!ctx || !batchOffsets || !nBatches || (n && (!d_boxes || !d_indices))) return RZ_ERR_INVALID_ARGUMENT;
+  return sah_batches_run(ctx->stream, n, d_boxes, targetSize, splitGranularity, d_indices, batchOffsets, nBatches);
+}
+
 extern "C" int rz_timer_start(rz_context* ctx) {
   if (!ctx) return RZ_ERR_INVALID_ARGUMENT;
   RZ_HIP(hipEventRecord(ctx->t0, ctx->stream));

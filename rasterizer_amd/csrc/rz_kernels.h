@@ -272,6 +272,9 @@ struct BakeParams {
   const uint32_t* rsqrt;  // 8192-entry host RSQRTPS table
 };
 hipError_t launch_bake(const BakeParams& p, uint32_t nOccluders, hipStream_t stream);
+// SAH batching of n quad boxes on the stream (rz_sah.hip); an RZ_* status.
+int sah_batches_run(hipStream_t stream, uint32_t n, const float* d_boxes, uint32_t target, uint32_t gran,
+                    uint32_t* d_indices, uint32_t* batchOffsets, uint32_t* nBatches);
 hipError_t launch_mlp_backward(const MlpParams& p, hipStream_t stream);
 // Pairwise ranking loss over pairs (i, j) whose sample i should rank above j:
 // mean softplus(s_j - s_i); adds its gradient to dscore and its value to *loss.

@@ -159,35 +159,41 @@ def _sah_split(cmin, cmax, centers, idx, gran):
         k = int(np.argmin(cost))
         if cost[k] < best_cost:
             best_cost, best_axis, best_index = cost[k], axis, int(splits[k])
-    if best_axis < 0:
-        return idx, n
+    assert best_axis >= 0, "a part that is split has more than 2 * gran quads"
     idx = idx[np.argsort(centers[idx, best_axis], kind="stable")]
     return idx, best_index
 
 
 def sah_batches(quad_min: np.ndarray, quad_max: np.ndarray, target: int = 512, gran: int = 8) -> list:
     """SurfaceAreaHeuristic::generateBatches (SurfaceAreaHeuristic.cpp:75-104):
-    recursive SAH splits at multiples of `gran` until every batch holds fewer
-    than `target` quads.  Returns lists of quad indices in batch order."""
+    recursive SAH splits at multiples of `gran`.  The whole set is always
+    split (generateBatches calls the recursion on it directly); a part with
+    fewer than `target` quads is a batch, a larger one is split again.
+    Returns lists of quad indices in batch order.  Needs target > 2 * gran
+    (so that every part that is split again has a split position); a set of
+    at most 2 * gran quads, where the reference's split is undefined, is one
+    batch in input order."""
+    if gran < 1 or target <= 2 * gran:
+        raise ValueError("sah_batches needs gran >= 1 and target > 2 * gran")
     cmin = np.asarray(quad_min, np.float32)[:, :3]
     cmax = np.asarray(quad_max, np.float32)[:, :3]
+    n = cmin.shape[0]
+    if n == 0:
+        return []
+    if n <= 2 * gran:
+        return [np.arange(n)]
     centers = (cmin + cmax).astype(np.float32)
     out = []
 
     def rec(idx):
-        if idx.size < target:
-            out.append(idx)
-            return
         idx, split = _sah_split(cmin, cmax, centers, idx, gran)
         for part in (idx[:split], idx[split:]):
-            if part.size == 0:
-                continue
             if part.size < target:
                 out.append(part)
             else:
                 rec(part)
 
-    rec(np.arange(cmin.shape[0]))
+    rec(np.arange(n))
     return out
 
 

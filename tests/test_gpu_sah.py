@@ -1,0 +1,87 @@
+"""SurfaceAreaHeuristic::generateBatches on the device (rz_sah_batches)
+against scenes.sah_batches (its restatement, pinned by tests/test_sah.py):
+the same batches, in the same order, with the same quad order inside each,
+over random boxes with many equal centres and signed zeros, a symmetric grid
+(ties between axes and between split positions), a scene's quads, sets below
+the target and at the granularity limit, and the argument checks."""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from rasterizer_amd import Rasterizer, RasterizerError, scenes
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def rast():
+    return Rasterizer(64, 64)
+
+
+def _same(a, b):
+    return len(a) == len(b) and all(np.array_equal(x, y) for x, y in zip(a, b))
+
+
+def _random(seed, n, grid):
+    rng = np.random.default_rng(seed)
+    c = rng.integers(-grid, grid + 1, size=(n, 3)).astype(np.float32)
+    h = rng.integers(0, 3, size=(n, 3)).astype(np.float32)
+    mn, mx = c - h, c + h
+    z = rng.random((n, 3)) < 0.1
+    mn[z & (mn == 0)] = -0.0
+    mx[z & (mx == 0)] = -0.0
+    return mn, mx
+
+
+@pytest.mark.parametrize("seed,n,target,gran,grid", [
+    (1, 5000, 512, 8, 20), (2, 5000, 64, 8, 3), (3, 20000, 512, 8, 1000), (4, 3000, 40, 4, 2),
+    (5, 1500, 33, 16, 6)])
+def test_random_boxes(rast, seed, n, target, gran, grid):
+    mn, mx = _random(seed, n, grid)
+    print(f"sah random seed={seed} n={n} target={target} gran={gran}", flush=True)
+    assert _same(rast.sah_batches_device(mn, mx, target, gran), scenes.sah_batches(mn, mx, target, gran))
+
+
+@pytest.mark.parametrize("side,target,gran", [(8, 20, 4), (32, 100, 8), (64, 512, 8)])
+def test_symmetric_grid(rast, side, target, gran):
+    i, j = np.meshgrid(np.arange(side), np.arange(side), indexing="ij")
+    c = np.stack([i.ravel(), np.zeros(side * side), j.ravel()], axis=1).astype(np.float32) * 3
+    mn, mx = c - 1, c + 1
+    print(f"sah grid side={side}", flush=True)
+    assert _same(rast.sah_batches_device(mn, mx, target, gran), scenes.sah_batches(mn, mx, target, gran))
+
+
+def test_scene_quads(rast):
+    """The quads of a padded box scene, as Main.cpp:96-107 batches them."""
+    rng = scenes.SplitMix64(11)
+    c, h = scenes.random_boxes(rng, 2048, -50.0, 50.0, 0.5, 5.0)
+    quads = scenes.pad_to_batches(np.concatenate([scenes.box_quads(c[i], h[i]) for i in range(2048)], axis=0))
+    mn, mx = quads.min(axis=1), quads.max(axis=1)
+    print(f"sah scene quads={quads.shape[0]}", flush=True)
+    got = rast.sah_batches_device(mn, mx)
+    assert _same(got, scenes.sah_batches(mn, mx))
+    assert sorted(np.concatenate(got).tolist()) == list(range(quads.shape[0]))
+
+
+@pytest.mark.parametrize("n", [0, 1, 16, 17, 24, 300, 511, 512, 513])
+def test_small_sets(rast, n):
+    mn, mx = _random(100 + n, n, 4)
+    assert _same(rast.sah_batches_device(mn, mx, 512, 8), scenes.sah_batches(mn, mx, 512, 8))
+
+
+def test_arguments(rast):
+    mn, mx = _random(7, 100, 4)
+    for target, gran in ((16, 8), (512, 0)):
+        with pytest.raises(RasterizerError):
+            rast.sah_batches_device(mn, mx, target, gran)
+    for bad in (np.nan, np.inf, 2.0 ** 41):
+        m = mn.copy()
+        m[37, 1] = bad
+        with pytest.raises(RasterizerError):
+            rast.sah_batches_device(m, mx, 512, 8)
+    m = mn.copy()
+    m[5, 2] = mx[5, 2] + 1  # min > max
+    with pytest.raises(RasterizerError):
+        rast.sah_batches_device(m, mx, 512, 8)
