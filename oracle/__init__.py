@@ -34,10 +34,48 @@ class _Occ(ctypes.Structure):
     ]
 
 
+REF_DIR = Path("/root/reference")
+SAH_REF_PATH = HERE / "_ref" / "libsah_ref.so"
+
+
 def build() -> Path:
-    """Compiles the oracle with its Makefile (gcc, -ffp-contract=off)."""
+    """Compiles the oracle with its Makefile (gcc, -ffp-contract=off) and,
+    where the reference's sources are present, the reference's own SAH
+    batching into oracle/_ref/."""
     subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
+    if (REF_DIR / "SoftwareRasterizer" / "SurfaceAreaHeuristic.cpp").exists():
+        subprocess.run(["make", "-s", "-C", str(HERE), "ref", f"REF={REF_DIR}"], check=True)
     return HERE / "_build" / "liboracle.so"
+
+
+def sah_reference():
+    """The reference's SurfaceAreaHeuristic::generateBatches as built into
+    oracle/_ref/ (None if that build is absent): a function
+    (quad_min, quad_max, target, gran) -> list of index arrays, or None where
+    the reference's behaviour is undefined."""
+    if not SAH_REF_PATH.exists():
+        return None
+    lib = ctypes.CDLL(str(SAH_REF_PATH))
+    fn = lib.sah_ref_batches
+    fn.restype = ctypes.c_int
+    fn.argtypes = [ctypes.c_uint32, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_void_p,
+                   ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint32)]
+
+    def run(quad_min, quad_max, target, gran):
+        mn = np.atleast_2d(np.asarray(quad_min, np.float32))
+        mx = np.atleast_2d(np.asarray(quad_max, np.float32))
+        n = mn.shape[0]
+        boxes = np.zeros((max(n, 1), 8), np.float32)
+        boxes[:n, 0:3] = mn[:, :3]
+        boxes[:n, 4:7] = mx[:, :3]
+        idx = np.zeros(max(n, 1), np.uint32)
+        off = np.zeros(n + 2, np.uint32)
+        cnt = ctypes.c_uint32(0)
+        if fn(n, boxes.ctypes.data, target, gran, idx.ctypes.data, off.ctypes.data, ctypes.byref(cnt)) != 0:
+            return None
+        return [idx[off[b]:off[b + 1]].astype(np.int64) for b in range(cnt.value)]
+
+    return run
 
 
 def lib() -> ctypes.CDLL:

@@ -126,3 +126,51 @@ def test_small_sets_and_arguments():
         scenes.sah_batches(mn, mx, 16, 8)
     with pytest.raises(ValueError):
         scenes.sah_batches(mn, mx, 16, 0)
+
+
+def _reference_or_skip():
+    import oracle
+
+    ref = oracle.sah_reference()
+    if ref is None:
+        pytest.skip("oracle/_ref/libsah_ref.so is not built (the reference's sources are not on this machine)")
+    return ref
+
+
+@pytest.mark.parametrize("seed,n,target,gran,grid", [
+    (11, 40, 12, 4, 3), (12, 300, 512, 8, 5), (13, 5000, 512, 8, 20), (14, 5000, 64, 8, 3),
+    (15, 20000, 512, 8, 1000), (16, 3000, 40, 4, 2), (17, 1500, 33, 16, 6), (18, 17, 512, 8, 2)])
+def test_restatement_matches_reference_build(seed, n, target, gran, grid):
+    """scenes.sah_batches against the reference's own generateBatches, built
+    from its unmodified source by oracle/Makefile."""
+    ref = _reference_or_skip()
+    mn, mx = _boxes(seed, n, grid)
+    want = ref(mn, mx, target, gran)
+    got = scenes.sah_batches(mn, mx, target, gran)
+    assert len(got) == len(want) and all(np.array_equal(a, b) for a, b in zip(got, want))
+
+
+@pytest.mark.parametrize("side,target,gran", [(8, 20, 4), (32, 100, 8), (64, 512, 8)])
+def test_symmetric_grid_matches_reference_build(side, target, gran):
+    ref = _reference_or_skip()
+    i, j = np.meshgrid(np.arange(side), np.arange(side), indexing="ij")
+    c = np.stack([i.ravel(), np.zeros(side * side), j.ravel()], axis=1).astype(np.float32) * 3
+    mn, mx = c - 1, c + 1
+    want = ref(mn, mx, target, gran)
+    got = scenes.sah_batches(mn, mx, target, gran)
+    assert len(got) == len(want) and all(np.array_equal(a, b) for a, b in zip(got, want))
+
+
+def test_scene_quads_match_reference_build():
+    """A padded box scene and a terrain heightfield, batched as Main.cpp:96-107
+    does (target 512, granularity 8)."""
+    ref = _reference_or_skip()
+    rng = scenes.SplitMix64(11)
+    c, h = scenes.random_boxes(rng, 2048, -50.0, 50.0, 0.5, 5.0)
+    boxes = scenes.pad_to_batches(np.concatenate([scenes.box_quads(c[i], h[i]) for i in range(2048)], axis=0))
+    terrain = scenes.pad_to_batches(scenes.Heightfield(129, 4.0, 20.0, seed=5).quads())
+    for quads in (boxes, terrain):
+        mn, mx = quads.min(axis=1), quads.max(axis=1)
+        want = ref(mn, mx, 512, 8)
+        got = scenes.sah_batches(mn, mx, 512, 8)
+        assert len(got) == len(want) and all(np.array_equal(a, b) for a, b in zip(got, want))
