@@ -85,3 +85,17 @@ def test_arguments(rast):
     m[5, 2] = mx[5, 2] + 1  # min > max
     with pytest.raises(RasterizerError):
         rast.sah_batches_device(m, mx, 512, 8)
+
+
+@pytest.mark.parametrize("seed,n,target,gran,grid", [(21, 300, 512, 8, 5), (22, 20000, 512, 8, 1000),
+                                                      (23, 5000, 64, 8, 3)])
+def test_matches_reference_build(rast, seed, n, target, gran, grid):
+    """The device batches against the reference's own generateBatches (built
+    from its unmodified source into oracle/_ref/ where the sources were)."""
+    import oracle
+
+    ref = oracle.sah_reference()
+    if ref is None:
+        pytest.skip("oracle/_ref/libsah_ref.so is not built")
+    mn, mx = _random(seed, n, grid)
+    assert _same(rast.sah_batches_device(mn, mx, target, gran), ref(mn, mx, target, gran))
