@@ -135,11 +135,24 @@ __global__ void __launch_bounds__(256) sah_gather_kernel(const float* __restrict
   revSeg[n - 1 - i] = segOf[i];
 }
 
+__device__ __forceinline__ uint64_t wave_min_u64(uint64_t v) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) {
+    const uint32_t lo = uint32_t(__shfl_xor(int(uint32_t(v)), off));
+    const uint32_t hi = uint32_t(__shfl_xor(int(uint32_t(v >> 32)), off));
+    const uint64_t o = (uint64_t(hi) << 32) | lo;
+    v = o < v ? o : v;
+  }
+  return v;
+}
+
 // sahSplit's cost loop (SurfaceAreaHeuristic.cpp:46-64) for one axis: split
 // position k of a segment of len items costs area(items [0, k)) * k +
 // area(items [k, len)) * (len - k), for k = gran, 2 gran, ... < len - gran.
 // The key orders by cost (non-negative, -0 folded into +0), then axis, then
-// k, so the minimum is the reference's first strict minimum.
+// k, so the minimum is the reference's first strict minimum.  Keys are
+// reduced per wave and per work-group while they share a segment, so a large
+// segment takes one atomic per work-group rather than one per candidate.
 __global__ void __launch_bounds__(256) sah_cost_kernel(const Ext6* __restrict__ fwdScan,
                                                        const Ext6* __restrict__ revScan,
                                                        const uint32_t* __restrict__ segOf,
@@ -147,18 +160,52 @@ __global__ void __launch_bounds__(256) sah_cost_kernel(const Ext6* __restrict__ 
                                                        const uint32_t* __restrict__ segLen,
                                                        const uint32_t* __restrict__ segOpen, uint32_t n,
                                                        uint32_t gran, uint32_t axis, uint64_t* __restrict__ segBest) {
+  __shared__ uint64_t waveBest[4];
+  __shared__ uint32_t waveSeg[4];
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const uint32_t s = segOf[i];
-  if (!segOpen[s]) return;
-  const uint32_t k = i - segStart[s], len = segLen[s];
-  if (k < gran || k % gran != 0u || k + gran >= len) return;
-  const float left = surface_area(fwdScan[i - 1]) * float(int(k));
-  const float right = surface_area(revScan[n - 1 - i]) * float(int(len - k));
-  float cost = left + right;
-  if (cost == 0.0f) cost = 0.0f;
-  const uint64_t key = (uint64_t(__float_as_uint(cost)) << 32) | (uint64_t(axis) << 30) | k;
-  atomicMin(reinterpret_cast<unsigned long long*>(&segBest[s]), static_cast<unsigned long long>(key));
+  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+  const bool inRange = i < n;
+  const uint32_t s = inRange ? segOf[i] : 0u;
+  uint64_t key = ~0ull;
+  if (inRange && segOpen[s]) {
+    const uint32_t k = i - segStart[s], len = segLen[s];
+    if (k >= gran && k % gran == 0u && k + gran < len) {
+      const float left = surface_area(fwdScan[i - 1]) * float(int(k));
+      const float right = surface_area(revScan[n - 1 - i]) * float(int(len - k));
+      float cost = left + right;
+      if (cost == 0.0f) cost = 0.0f;
+      key = (uint64_t(__float_as_uint(cost)) << 32) | (uint64_t(axis) << 30) | k;
+    }
+  }
+  // Lanes are consecutive positions, so lane 0 is in range whenever any is.
+  const uint32_t s0 = uint32_t(__builtin_amdgcn_readfirstlane(int(s)));
+  const bool uniform = __ballot(inRange && s != s0) == 0ull;
+  uint64_t wk = ~0ull;
+  if (uniform) {
+    wk = wave_min_u64(key);
+  } else if (key != ~0ull) {
+    atomicMin(reinterpret_cast<unsigned long long*>(&segBest[s]), static_cast<unsigned long long>(key));
+  }
+  if (lane == 0) {
+    waveBest[wave] = wk;
+    waveSeg[wave] = s0;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint64_t cur = ~0ull;
+    uint32_t curSeg = 0;
+    for (uint32_t w = 0; w < 4; ++w) {
+      if (waveBest[w] == ~0ull) continue;
+      if (cur != ~0ull && waveSeg[w] != curSeg) {
+        atomicMin(reinterpret_cast<unsigned long long*>(&segBest[curSeg]), static_cast<unsigned long long>(cur));
+        cur = ~0ull;
+      }
+      curSeg = waveSeg[w];
+      cur = waveBest[w] < cur ? waveBest[w] : cur;
+    }
+    if (cur != ~0ull)
+      atomicMin(reinterpret_cast<unsigned long long*>(&segBest[curSeg]), static_cast<unsigned long long>(cur));
+  }
 }
 
 uint32_t grid_of(uint32_t n) { return (n + 255u) / 256u; }
