@@ -159,7 +159,8 @@ int rz_bake_device(rz_context* ctx, uint32_t n, const uint32_t* vertexOffsets, c
 /* SurfaceAreaHeuristic::generateBatches(aabbs, targetSize, splitGranularity)
  * (SurfaceAreaHeuristic.h, SurfaceAreaHeuristic.cpp:96-104) on the device.
  * d_boxes: n quad boxes as std::vector<Aabb> lays them out (min xyzw, max
- * xyzw; w ignored), finite, min <= max and |coordinate| <= 2^40.  Writes the
+ * xyzw; w ignored; 16-byte aligned), finite, min <= max and
+ * |coordinate| <= 2^40.  Writes the
  * batches' quad indices, batch after batch, to d_indices (n words, device)
  * and the batch boundaries to batchOffsets (host, *nBatches + 1 words; room
  * for n + 1 suffices), in the reference's depth-first order, bit-identical

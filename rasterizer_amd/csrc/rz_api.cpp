@@ -725,6 +725,7 @@ extern "C" int rz_sah_batches(rz_context* ctx, uint32_t n, const float* d_boxes,
                               uint32_t splitGranularity, uint32_t* d_indices, uint32_t* batchOffsets,
                               uint32_t* nBatches) {
   if (!ctx || !batchOffsets || !nBatches || (n && (!d_boxes || !d_indices))) return RZ_ERR_INVALID_ARGUMENT;
+  if (reinterpret_cast<uintptr_t>(d_boxes) % 16u != 0u) return RZ_ERR_INVALID_ARGUMENT;  // read as float4
   return sah_batches_run(ctx->stream, n, d_boxes, targetSize, splitGranularity, d_indices, batchOffsets, nBatches);
 }
 
