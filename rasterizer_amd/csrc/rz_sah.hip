@@ -8,8 +8,9 @@
 // recursion is one round over all open ranges ("segments") at once:
 //   - each stable sort is one device-wide radix sort on (segment, centre
 //     key), so segments stay where they are and sort independently;
-//   - prefix and suffix boxes are segmented scans (min and max are exact, so
-//     the scan order does not change them);
+//   - prefix and suffix boxes are segmented scans over the boxes of chunks
+//     of gran positions, since every split falls between chunks (min and max
+//     are exact, so neither the chunking nor the scan order changes them);
 //   - the winning (cost, axis, index) of a segment is a 64-bit atomicMin,
 //     which keeps the reference's first strict minimum in (axis, index) order.
 // Segments stay in the index array in the order the depth-first recursion
@@ -120,19 +121,27 @@ __global__ void __launch_bounds__(256) sah_keys_kernel(const float* __restrict__
   keys[i] = (uint64_t(s) << 32) | low;
 }
 
-// Boxes in sorted order, forwards and reversed (the suffix scan runs over the
-// reversed array).
-__global__ void __launch_bounds__(256) sah_gather_kernel(const float* __restrict__ boxes,
+// Boxes of each chunk of gran consecutive positions, in sorted order,
+// forwards and reversed (the suffix scan runs over the reversed array).
+// Every segment starts at a multiple of gran (the root at 0, each split gran
+// times an integer further on), so no chunk spans two segments.
+__global__ void __launch_bounds__(256) sah_chunks_kernel(const float* __restrict__ boxes,
                                                          const uint32_t* __restrict__ idx,
                                                          const uint32_t* __restrict__ segOf, uint32_t n,
-                                                         Ext6* __restrict__ fwd, Ext6* __restrict__ rev,
+                                                         uint32_t gran, uint32_t nChunks, Ext6* __restrict__ fwd,
+                                                         Ext6* __restrict__ rev, uint32_t* __restrict__ chunkSeg,
                                                          uint32_t* __restrict__ revSeg) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const Ext6 b = load_ext(boxes, idx[i]);
-  fwd[i] = b;
-  rev[n - 1 - i] = b;
-  revSeg[n - 1 - i] = segOf[i];
+  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nChunks) return;
+  const uint32_t i0 = c * gran, i1 = min(i0 + gran, n);
+  Ext6 b = load_ext(boxes, idx[i0]);
+  const MergeExt merge;
+  for (uint32_t i = i0 + 1; i < i1; ++i) b = merge(b, load_ext(boxes, idx[i]));
+  const uint32_t sg = segOf[i0];
+  fwd[c] = b;
+  rev[nChunks - 1 - c] = b;
+  chunkSeg[c] = sg;
+  revSeg[nChunks - 1 - c] = sg;
 }
 
 __device__ __forceinline__ uint64_t wave_min_u64(uint64_t v) {
@@ -155,29 +164,31 @@ __device__ __forceinline__ uint64_t wave_min_u64(uint64_t v) {
 // segment takes one atomic per work-group rather than one per candidate.
 __global__ void __launch_bounds__(256) sah_cost_kernel(const Ext6* __restrict__ fwdScan,
                                                        const Ext6* __restrict__ revScan,
-                                                       const uint32_t* __restrict__ segOf,
+                                                       const uint32_t* __restrict__ chunkSeg,
                                                        const uint32_t* __restrict__ segStart,
                                                        const uint32_t* __restrict__ segLen,
-                                                       const uint32_t* __restrict__ segOpen, uint32_t n,
+                                                       const uint32_t* __restrict__ segOpen, uint32_t nChunks,
                                                        uint32_t gran, uint32_t axis, uint64_t* __restrict__ segBest) {
   __shared__ uint64_t waveBest[4];
   __shared__ uint32_t waveSeg[4];
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  // One thread per chunk c: the split before position c * gran, whose left
+  // part ends with chunk c - 1 and whose right part starts with chunk c.
+  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
-  const bool inRange = i < n;
-  const uint32_t s = inRange ? segOf[i] : 0u;
+  const bool inRange = c < nChunks;
+  const uint32_t s = inRange ? chunkSeg[c] : 0u;
   uint64_t key = ~0ull;
   if (inRange && segOpen[s]) {
-    const uint32_t k = i - segStart[s], len = segLen[s];
-    if (k >= gran && k % gran == 0u && k + gran < len) {
-      const float left = surface_area(fwdScan[i - 1]) * float(int(k));
-      const float right = surface_area(revScan[n - 1 - i]) * float(int(len - k));
+    const uint32_t k = c * gran - segStart[s], len = segLen[s];
+    if (k >= gran && k + gran < len) {
+      const float left = surface_area(fwdScan[c - 1]) * float(int(k));
+      const float right = surface_area(revScan[nChunks - 1 - c]) * float(int(len - k));
       float cost = left + right;
       if (cost == 0.0f) cost = 0.0f;
       key = (uint64_t(__float_as_uint(cost)) << 32) | (uint64_t(axis) << 30) | k;
     }
   }
-  // Lanes are consecutive positions, so lane 0 is in range whenever any is.
+  // Lanes are consecutive chunks, so lane 0 is in range whenever any is.
   const uint32_t s0 = uint32_t(__builtin_amdgcn_readfirstlane(int(s)));
   const bool uniform = __ballot(inRange && s != s0) == 0ull;
   uint64_t wk = ~0ull;
@@ -219,13 +230,15 @@ hipError_t alloc(T** p, size_t count) {
 struct SahBuffers {
   uint32_t* idx[2] = {nullptr, nullptr};
   uint64_t* keys[2] = {nullptr, nullptr};
-  uint32_t *segOf = nullptr, *revSeg = nullptr, *segStart = nullptr, *segLen = nullptr, *segOpen = nullptr;
+  uint32_t *segOf = nullptr, *chunkSeg = nullptr, *revSeg = nullptr, *segStart = nullptr, *segLen = nullptr,
+           *segOpen = nullptr;
   uint32_t* bad = nullptr;
   uint64_t* segBest = nullptr;
   Ext6 *fwd = nullptr, *fwdScan = nullptr, *rev = nullptr, *revScan = nullptr;
   void* temp = nullptr;
   ~SahBuffers() {
-    for (void* p : {(void*)idx[0], (void*)idx[1], (void*)keys[0], (void*)keys[1], (void*)segOf, (void*)revSeg,
+    for (void* p : {(void*)idx[0], (void*)idx[1], (void*)keys[0], (void*)keys[1], (void*)segOf, (void*)chunkSeg,
+                    (void*)revSeg,
                     (void*)segStart, (void*)segLen, (void*)segOpen, (void*)bad, (void*)segBest, (void*)fwd,
                     (void*)fwdScan, (void*)rev, (void*)revScan, temp})
       if (p) (void)hipFree(p);
@@ -269,18 +282,20 @@ int sah_batches_run(hipStream_t stream, uint32_t n, const float* d_boxes, uint32
   }
   // At most n / gran + 1 segments: every closed piece holds gran items or more.
   const size_t maxSeg = size_t(n) / gran + 2;
+  const uint32_t nChunks = (n + gran - 1) / gran;
   if (alloc(&B.idx[0], n) != hipSuccess || alloc(&B.idx[1], n) != hipSuccess || alloc(&B.keys[0], n) != hipSuccess ||
-      alloc(&B.keys[1], n) != hipSuccess || alloc(&B.segOf, n) != hipSuccess || alloc(&B.revSeg, n) != hipSuccess ||
+      alloc(&B.keys[1], n) != hipSuccess || alloc(&B.segOf, n) != hipSuccess ||
+      alloc(&B.chunkSeg, nChunks) != hipSuccess || alloc(&B.revSeg, nChunks) != hipSuccess ||
       alloc(&B.segStart, maxSeg) != hipSuccess || alloc(&B.segLen, maxSeg) != hipSuccess ||
       alloc(&B.segOpen, maxSeg) != hipSuccess || alloc(&B.segBest, maxSeg) != hipSuccess ||
-      alloc(&B.fwd, n) != hipSuccess || alloc(&B.fwdScan, n) != hipSuccess || alloc(&B.rev, n) != hipSuccess ||
-      alloc(&B.revScan, n) != hipSuccess)
+      alloc(&B.fwd, nChunks) != hipSuccess || alloc(&B.fwdScan, nChunks) != hipSuccess ||
+      alloc(&B.rev, nChunks) != hipSuccess || alloc(&B.revScan, nChunks) != hipSuccess)
     return RZ_ERR_OUT_OF_MEMORY;
   size_t sortBytes = 0, scanBytes = 0;
   SAH_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, sortBytes, B.keys[0], B.keys[1], B.idx[0], B.idx[1], n, 0, 64,
                                              stream));
-  SAH_HIP(hipcub::DeviceScan::InclusiveScanByKey(nullptr, scanBytes, B.segOf, B.fwd, B.fwdScan, MergeExt(), n,
-                                                 hipcub::Equality(), stream));
+  SAH_HIP(hipcub::DeviceScan::InclusiveScanByKey(nullptr, scanBytes, B.chunkSeg, B.fwd, B.fwdScan, MergeExt(),
+                                                 nChunks, hipcub::Equality(), stream));
   size_t tempBytes = std::max(sortBytes, scanBytes);
   if (alloc(reinterpret_cast<uint8_t**>(&B.temp), tempBytes) != hipSuccess) return RZ_ERR_OUT_OF_MEMORY;
 
@@ -312,16 +327,17 @@ int sah_batches_run(hipStream_t stream, uint32_t n, const float* d_boxes, uint32
                                                  0, int(32 + bits), stream));
       cur ^= 1;
       if (axis == 3) break;
-      sah_gather_kernel<<<grid_of(n), 256, 0, stream>>>(d_boxes, B.idx[cur], B.segOf, n, B.fwd, B.rev, B.revSeg);
+      sah_chunks_kernel<<<grid_of(nChunks), 256, 0, stream>>>(d_boxes, B.idx[cur], B.segOf, n, gran, nChunks,
+                                                              B.fwd, B.rev, B.chunkSeg, B.revSeg);
       SAH_HIP(hipGetLastError());
       bytes = tempBytes;
-      SAH_HIP(hipcub::DeviceScan::InclusiveScanByKey(B.temp, bytes, B.segOf, B.fwd, B.fwdScan, MergeExt(), n,
+      SAH_HIP(hipcub::DeviceScan::InclusiveScanByKey(B.temp, bytes, B.chunkSeg, B.fwd, B.fwdScan, MergeExt(), nChunks,
                                                      hipcub::Equality(), stream));
       bytes = tempBytes;
-      SAH_HIP(hipcub::DeviceScan::InclusiveScanByKey(B.temp, bytes, B.revSeg, B.rev, B.revScan, MergeExt(), n,
+      SAH_HIP(hipcub::DeviceScan::InclusiveScanByKey(B.temp, bytes, B.revSeg, B.rev, B.revScan, MergeExt(), nChunks,
                                                      hipcub::Equality(), stream));
-      sah_cost_kernel<<<grid_of(n), 256, 0, stream>>>(B.fwdScan, B.revScan, B.segOf, B.segStart, B.segLen, B.segOpen,
-                                                      n, gran, uint32_t(axis), B.segBest);
+      sah_cost_kernel<<<grid_of(nChunks), 256, 0, stream>>>(B.fwdScan, B.revScan, B.chunkSeg, B.segStart, B.segLen,
+                                                            B.segOpen, nChunks, gran, uint32_t(axis), B.segBest);
       SAH_HIP(hipGetLastError());
     }
     best.resize(S);
