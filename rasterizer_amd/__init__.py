@@ -513,6 +513,22 @@ class Rasterizer:
             d_i.free()
         return [idx[offsets[k]:offsets[k + 1]] for k in range(count.value)]
 
+    def bake_scene_device(self, quads, target: int = 512, gran: int = 8) -> list:
+        """Main.cpp:86-128 with both scene-preparation steps on the device:
+        pad to whole 8-quad batches, SAH-batch the quads (rz_sah_batches),
+        then bake every batch against the scene box (rz_bake_device).
+        quads: (n, 4, 4) vertices.  Returns the Occluders, identical to
+        scenes.bake_scene(quads).occluders."""
+        from . import scenes
+
+        q = scenes.pad_to_batches(np.asarray(quads, np.float32))
+        verts = q.reshape(-1, 4)
+        ref_min = verts.min(axis=0).astype(np.float32)
+        ref_max = verts.max(axis=0).astype(np.float32)
+        batches = self.sah_batches_device(q.min(axis=1), q.max(axis=1), target, gran)
+        meshes = [q[b].reshape(-1, 4) for b in batches]
+        return self.bake_device(meshes, [ref_min] * len(meshes), [ref_max] * len(meshes))
+
     def query_batch_device(self, n: int, d_bounds: "DeviceBuffer", d_visible: "DeviceBuffer",
                            d_clip: "DeviceBuffer | None" = None) -> None:
         _check(lib().rz_query_batch_device(self._ctx, n, d_bounds.ptr, d_visible.ptr,

@@ -99,3 +99,26 @@ def test_matches_reference_build(rast, seed, n, target, gran, grid):
         pytest.skip("oracle/_ref/libsah_ref.so is not built")
     mn, mx = _random(seed, n, grid)
     assert _same(rast.sah_batches_device(mn, mx, target, gran), ref(mn, mx, target, gran))
+
+
+@pytest.mark.parametrize("which", ["boxes", "city"])
+def test_scene_prepared_on_device(rast, which):
+    """Main.cpp:86-128 with SAH batching and the bake both on the device
+    (Rasterizer.bake_scene_device) against the host preparation
+    (scenes.bake_scene): the same occluders, packets, bounds and centres bit
+    for bit."""
+    if which == "boxes":
+        rng = scenes.SplitMix64(13)
+        c, h = scenes.random_boxes(rng, 1500, -50.0, 50.0, 0.5, 5.0)
+        quads = np.concatenate([scenes.box_quads(c[i], h[i]) for i in range(1500)], axis=0)
+    else:
+        quads = scenes.Heightfield(97, 4.0, 20.0, seed=3).quads()
+    print(f"scene prep {which} quads={quads.shape[0]}", flush=True)
+    want = scenes.bake_scene(quads).occluders
+    got = rast.bake_scene_device(quads)
+    assert len(got) == len(want)
+    for a, b in zip(got, want):
+        assert np.array_equal(a.packets, b.packets)
+        assert np.array_equal(a.m_boundsMin, b.m_boundsMin) and np.array_equal(a.m_boundsMax, b.m_boundsMax)
+        assert np.array_equal(a.m_center, b.m_center)
+        assert np.array_equal(a.m_refMin, b.m_refMin) and np.array_equal(a.m_refMax, b.m_refMax)
