@@ -172,6 +172,26 @@ int rz_bake_device(rz_context* ctx, uint32_t n, const uint32_t* vertexOffsets, c
  * order.  Boxes outside the stated range: RZ_ERR_INVALID_ARGUMENT.  Blocks. */
 int rz_sah_batches(rz_context* ctx, uint32_t n, const float* d_boxes, uint32_t targetSize, uint32_t splitGranularity,
                    uint32_t* d_indices, uint32_t* batchOffsets, uint32_t* nBatches);
+/* Main.cpp:86-128 on the device, for occluder sets that live in HBM: pad to
+ * whole 8-quad batches with copies of vertex 0 (:90-94), one Aabb per quad
+ * (:96-105), SurfaceAreaHeuristic::generateBatches (:107, as
+ * rz_sah_batches), the scene box over every vertex (:109-113) and
+ * Occluder::bake of each batch against it (:115-128, as rz_bake_device).
+ * d_vertices: nQuads * 4 vertices, xyzw floats, 16-byte aligned, device.
+ * Writes occluder i's packets, one word per vertex, to
+ * d_packets[4 * quadOffsets[i] .. 4 * quadOffsets[i + 1]) (room for
+ * 4 * nQuads rounded up to 32 words), its boundsMin, boundsMax and centre to
+ * d_bounds[12 i ..] (room for 12 * (nQuads + 7) / 8 floats), the batch
+ * boundaries in padded quads to quadOffsets (host; room for
+ * (nQuads + 7) / 8 + 1 words), the count to *nOccluders and the scene box
+ * (min xyzw, max xyzw) to refMinMax.  Bit-identical to the host preparation
+ * (rz_bake per batch of the reference's SAH batches); where a coordinate's
+ * extreme is a zero of both signs, the scene box takes -0 as its minimum and
+ * +0 as its maximum.  splitGranularity must be a multiple of 8 and
+ * targetSize > 2 * splitGranularity.  Blocks. */
+int rz_prepare_scene_device(rz_context* ctx, uint32_t nQuads, const float* d_vertices, uint32_t targetSize,
+                            uint32_t splitGranularity, uint32_t* d_packets, float* d_bounds, uint32_t* quadOffsets,
+                            uint32_t* nOccluders, float refMinMax[8]);
 
 /* Timing helpers on the context stream (HIP events): rz_timer_start records
  * a start event, rz_timer_stop a stop event and returns (after waiting for

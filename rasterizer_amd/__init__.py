@@ -87,6 +87,7 @@ _SIGNATURES = {
     "rz_bake": ([_P, _U32, _P, _P, _P, _P, _P, _P], ctypes.c_int),
     "rz_bake_device": ([_P, _U32, _P, _P, _P, _P, _P], ctypes.c_int),
     "rz_sah_batches": ([_P, _U32, _P, _U32, _U32, _P, _P, ctypes.POINTER(_U32)], ctypes.c_int),
+    "rz_prepare_scene_device": ([_P, _U32, _P, _U32, _U32, _P, _P, _P, ctypes.POINTER(_U32), _P], ctypes.c_int),
     "rz_timer_start": ([_P], ctypes.c_int),
     "rz_timer_stop": ([_P, ctypes.POINTER(ctypes.c_float)], ctypes.c_int),
     "rz_profile_enable": ([_P, ctypes.c_int], ctypes.c_int),
@@ -528,6 +529,35 @@ class Rasterizer:
         batches = self.sah_batches_device(q.min(axis=1), q.max(axis=1), target, gran)
         meshes = [q[b].reshape(-1, 4) for b in batches]
         return self.bake_device(meshes, [ref_min] * len(meshes), [ref_max] * len(meshes))
+
+    def prepare_scene_device(self, quads, target: int = 512, gran: int = 8) -> list:
+        """rz_prepare_scene_device: Main.cpp:86-128 entirely on the device
+        from (n, 4, 4) quad vertices uploaded once (padding, SAH batching,
+        scene box, bake).  Returns the Occluders, identical to
+        scenes.bake_scene(quads).occluders."""
+        q = np.ascontiguousarray(np.asarray(quads, np.float32).reshape(-1, 4, 4))
+        n = q.shape[0]
+        if n == 0:
+            return []
+        n_pad = (n + 7) // 8 * 8
+        max_occ = n_pad // 8
+        offsets = np.zeros(max_occ + 1, np.uint32)
+        ref = np.zeros(8, np.float32)
+        count = _U32(0)
+        d_v = DeviceBuffer(self, q.nbytes, q)
+        d_p = DeviceBuffer(self, n_pad * 16)
+        d_b = DeviceBuffer(self, max_occ * 48)
+        try:
+            _check(lib().rz_prepare_scene_device(self.ctx, n, d_v.ptr, target, gran, d_p.ptr, d_b.ptr, _ptr(offsets),
+                                                 ctypes.byref(count), _ptr(ref)), "rz_prepare_scene_device")
+            k = count.value
+            packets = d_p.read(np.uint32, n_pad * 4)
+            bounds = d_b.read(np.float32, max_occ * 12).reshape(max_occ, 12)
+        finally:
+            for b in (d_v, d_p, d_b):
+                b.free()
+        return [Occluder(packets[4 * offsets[i]:4 * offsets[i + 1]].copy(), ref[0:4], ref[4:8], bounds[i, 0:4],
+                         bounds[i, 4:8], bounds[i, 8:12]) for i in range(k)]
 
     def query_batch_device(self, n: int, d_bounds: "DeviceBuffer", d_visible: "DeviceBuffer",
                            d_clip: "DeviceBuffer | None" = None) -> None:

@@ -729,6 +729,65 @@ extern "C" int rz_sah_batches(rz_context* ctx, uint32_t n, const float* d_boxes,
   return sah_batches_run(ctx->stream, n, d_boxes, targetSize, splitGranularity, d_indices, batchOffsets, nBatches);
 }
 
+extern "C" int rz_prepare_scene_device(rz_context* ctx, uint32_t nQuads, const float* d_vertices,
+                                       uint32_t targetSize, uint32_t splitGranularity, uint32_t* d_packets,
+                                       float* d_bounds, uint32_t* quadOffsets, uint32_t* nOccluders,
+                                       float refMinMax[8]) {
+  if (!ctx || !quadOffsets || !nOccluders || !refMinMax || (nQuads && (!d_vertices || !d_packets || !d_bounds)))
+    return RZ_ERR_INVALID_ARGUMENT;
+  // Occluder::bake takes whole 8-quad batches, so every split must fall on one.
+  if (splitGranularity == 0 || splitGranularity % 8u != 0u) return RZ_ERR_INVALID_ARGUMENT;
+  if (reinterpret_cast<uintptr_t>(d_vertices) % 16u != 0u) return RZ_ERR_INVALID_ARGUMENT;
+  quadOffsets[0] = 0;
+  *nOccluders = 0;
+  if (nQuads == 0) return RZ_OK;
+  if (nQuads > (1u << 28)) return RZ_ERR_TOO_LARGE;
+  const rzi::HostTables& tabs = rzi::host_tables();
+  if (tabs.rsqrtStatus != RZ_OK) return tabs.rsqrtStatus;
+  const uint32_t nPad = (nQuads + 7u) & ~7u;  // Main.cpp:90-94
+  float* d_boxes = nullptr;
+  float* d_sorted = nullptr;
+  uint32_t* d_order = nullptr;
+  uint32_t* d_keys = nullptr;
+  int status = RZ_OK;
+  uint32_t keys[8] = {~0u, ~0u, ~0u, ~0u, 0u, 0u, 0u, 0u};
+  if (dev_alloc(&d_boxes, size_t(nPad) * 8) != hipSuccess || dev_alloc(&d_sorted, size_t(nPad) * 16) != hipSuccess ||
+      dev_alloc(&d_order, nPad) != hipSuccess || dev_alloc(&d_keys, 8) != hipSuccess) {
+    status = RZ_ERR_OUT_OF_MEMORY;
+  } else if (hipMemcpyAsync(d_keys, keys, sizeof keys, hipMemcpyHostToDevice, ctx->stream) != hipSuccess ||
+             launch_scene_boxes(d_vertices, nQuads, nPad, d_boxes, d_keys, ctx->stream) != hipSuccess ||
+             hipMemcpyAsync(keys, d_keys, sizeof keys, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
+             hipStreamSynchronize(ctx->stream) != hipSuccess) {
+    status = RZ_ERR_DEVICE;
+  } else {
+    for (int l = 0; l < 8; ++l) {
+      const uint32_t k = keys[l];
+      const uint32_t u = (k & 0x80000000u) ? (k & 0x7FFFFFFFu) : ~k;
+      std::memcpy(&refMinMax[l], &u, 4);
+    }
+    uint32_t nOcc = 0;
+    status = sah_batches_run(ctx->stream, nPad, d_boxes, targetSize, splitGranularity, d_order, quadOffsets, &nOcc);
+    if (status == RZ_OK) {
+      if (launch_scene_gather(d_vertices, nQuads, nPad, d_order, d_sorted, ctx->stream) != hipSuccess) {
+        status = RZ_ERR_DEVICE;
+      } else {
+        std::vector<uint32_t> vertexOffsets(size_t(nOcc) + 1);
+        std::vector<float> refs(size_t(nOcc) * 8);
+        for (uint32_t i = 0; i <= nOcc; ++i) vertexOffsets[i] = quadOffsets[i] * 4u;
+        for (uint32_t i = 0; i < nOcc; ++i) std::memcpy(&refs[size_t(i) * 8], refMinMax, 32);
+        status = rz_bake_device(ctx, nOcc, vertexOffsets.data(), d_sorted, refs.data(), d_packets, d_bounds);
+        if (status == RZ_OK) *nOccluders = nOcc;
+      }
+    }
+  }
+  (void)hipStreamSynchronize(ctx->stream);
+  hipFree(d_boxes);
+  hipFree(d_sorted);
+  hipFree(d_order);
+  hipFree(d_keys);
+  return status;
+}
+
 extern "C" int rz_timer_start(rz_context* ctx) {
   if (!ctx) return RZ_ERR_INVALID_ARGUMENT;
   RZ_HIP(hipEventRecord(ctx->t0, ctx->stream));

@@ -275,6 +275,13 @@ hipError_t launch_bake(const BakeParams& p, uint32_t nOccluders, hipStream_t str
 // SAH batching of n quad boxes on the stream (rz_sah.hip); an RZ_* status.
 int sah_batches_run(hipStream_t stream, uint32_t n, const float* d_boxes, uint32_t target, uint32_t gran,
                     uint32_t* d_indices, uint32_t* batchOffsets, uint32_t* nBatches);
+// Scene preparation around it (rz_sah.hip): boxes of nPad padded quads and
+// the scene box as ordered-float keys (min xyzw, max xyzw; the caller sets
+// them to ~0 and 0 first); the padded quads' vertices gathered in `order`.
+hipError_t launch_scene_boxes(const float* vertices, uint32_t nQuads, uint32_t nPad, float* boxes,
+                              uint32_t* sceneKeys, hipStream_t stream);
+hipError_t launch_scene_gather(const float* vertices, uint32_t nQuads, uint32_t nPad, const uint32_t* order,
+                               float* out, hipStream_t stream);
 hipError_t launch_mlp_backward(const MlpParams& p, hipStream_t stream);
 // Pairwise ranking loss over pairs (i, j) whose sample i should rank above j:
 // mean softplus(s_j - s_i); adds its gradient to dscore and its value to *loss.

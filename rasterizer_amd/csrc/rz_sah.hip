@@ -26,6 +26,9 @@
 #include "rasterizer_amd.h"
 #include "rz_kernels.h"
 
+extern "C" __device__ uint32_t __ockl_wfred_min_u32(uint32_t);
+extern "C" __device__ uint32_t __ockl_wfred_max_u32(uint32_t);
+
 namespace rzd {
 
 namespace {
@@ -375,6 +378,84 @@ int sah_batches_run(hipStream_t stream, uint32_t n, const float* d_boxes, uint32
   for (size_t s = 0; s < start.size(); ++s) batchOffsets[s + 1] = start[s] + len[s];
   *nBatches = uint32_t(start.size());
   return RZ_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Scene preparation around the SAH pass (Main.cpp:86-128): padded quad
+// boxes, the scene box, and the vertices gathered into batch order.
+// ---------------------------------------------------------------------------
+namespace {
+
+// Ascending-order key of a float (for exact, order-independent min and max
+// with atomics; -0 sorts below +0).
+__device__ __forceinline__ uint32_t ordered_key(float f) {
+  const uint32_t u = __float_as_uint(f);
+  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+
+// Quad j < nPad: padded quads (j >= nQuads) are four copies of vertex 0, as
+// Main.cpp:90-94 pads the index buffer with indices[0].
+__device__ __forceinline__ float4 padded_vertex(const float4* __restrict__ v, uint32_t nQuads, uint32_t j, uint32_t k) {
+  return j < nQuads ? v[4 * size_t(j) + k] : v[0];
+}
+
+__global__ void __launch_bounds__(256) scene_boxes_kernel(const float4* __restrict__ vertices, uint32_t nQuads,
+                                                          uint32_t nPad, float4* __restrict__ boxes,
+                                                          uint32_t* __restrict__ sceneKeys) {
+  const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  float4 mn = make_float4(INFINITY, INFINITY, INFINITY, INFINITY);
+  float4 mx = make_float4(-INFINITY, -INFINITY, -INFINITY, -INFINITY);
+  uint32_t kmn[4] = {~0u, ~0u, ~0u, ~0u}, kmx[4] = {0u, 0u, 0u, 0u};
+  if (j < nPad) {
+    // Main.cpp:96-105: the quad's Aabb over its four vertices (all lanes).
+#pragma unroll
+    for (uint32_t k = 0; k < 4; ++k) {
+      const float4 v = padded_vertex(vertices, nQuads, j, k);
+      mn = make_float4(fminf(mn.x, v.x), fminf(mn.y, v.y), fminf(mn.z, v.z), fminf(mn.w, v.w));
+      mx = make_float4(fmaxf(mx.x, v.x), fmaxf(mx.y, v.y), fmaxf(mx.z, v.z), fmaxf(mx.w, v.w));
+      const float c[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+      for (int l = 0; l < 4; ++l) {
+        kmn[l] = min(kmn[l], ordered_key(c[l]));
+        kmx[l] = max(kmx[l], ordered_key(c[l]));
+      }
+    }
+    boxes[2 * size_t(j)] = mn;
+    boxes[2 * size_t(j) + 1] = mx;
+  }
+  // Main.cpp:109-113: the scene box over every vertex, as ordered keys.
+#pragma unroll
+  for (int l = 0; l < 4; ++l) {
+    const uint32_t a = __ockl_wfred_min_u32(kmn[l]), b = __ockl_wfred_max_u32(kmx[l]);
+    if ((threadIdx.x & 63u) == 0u) {
+      atomicMin(&sceneKeys[l], a);
+      atomicMax(&sceneKeys[4 + l], b);
+    }
+  }
+}
+
+__global__ void __launch_bounds__(256) scene_gather_kernel(const float4* __restrict__ vertices, uint32_t nQuads,
+                                                           uint32_t nPad, const uint32_t* __restrict__ order,
+                                                           float4* __restrict__ out) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 4 * nPad) return;
+  out[t] = padded_vertex(vertices, nQuads, order[t >> 2], t & 3u);
+}
+
+}  // namespace
+
+hipError_t launch_scene_boxes(const float* vertices, uint32_t nQuads, uint32_t nPad, float* boxes,
+                              uint32_t* sceneKeys, hipStream_t stream) {
+  scene_boxes_kernel<<<grid_of(nPad), 256, 0, stream>>>(reinterpret_cast<const float4*>(vertices), nQuads, nPad,
+                                                        reinterpret_cast<float4*>(boxes), sceneKeys);
+  return hipGetLastError();
+}
+
+hipError_t launch_scene_gather(const float* vertices, uint32_t nQuads, uint32_t nPad, const uint32_t* order,
+                               float* out, hipStream_t stream) {
+  scene_gather_kernel<<<grid_of(4 * nPad), 256, 0, stream>>>(reinterpret_cast<const float4*>(vertices), nQuads, nPad,
+                                                             order, reinterpret_cast<float4*>(out));
+  return hipGetLastError();
 }
 
 }  // namespace rzd

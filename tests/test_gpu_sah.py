@@ -101,6 +101,41 @@ def test_matches_reference_build(rast, seed, n, target, gran, grid):
     assert _same(rast.sah_batches_device(mn, mx, target, gran), ref(mn, mx, target, gran))
 
 
+def _scene_quads(which):
+    if which == "boxes":  # 1,501 boxes: 9,006 quads, padded to 9,008
+        rng = scenes.SplitMix64(13)
+        c, h = scenes.random_boxes(rng, 1501, -50.0, 50.0, 0.5, 5.0)
+        return np.concatenate([scenes.box_quads(c[i], h[i]) for i in range(1501)], axis=0)
+    return scenes.Heightfield(97, 4.0, 20.0, seed=3).quads()
+
+
+def _same_occluders(got, want):
+    assert len(got) == len(want)
+    for a, b in zip(got, want):
+        assert np.array_equal(a.packets, b.packets)
+        assert np.array_equal(a.m_boundsMin, b.m_boundsMin) and np.array_equal(a.m_boundsMax, b.m_boundsMax)
+        assert np.array_equal(a.m_center, b.m_center)
+        assert np.array_equal(a.m_refMin, b.m_refMin) and np.array_equal(a.m_refMax, b.m_refMax)
+
+
+@pytest.mark.parametrize("which", ["boxes", "city"])
+def test_scene_prepared_entirely_on_device(rast, which):
+    """rz_prepare_scene_device (padding, quad boxes, scene box, SAH batches,
+    gather and bake in HBM) against the host preparation, bit for bit."""
+    quads = _scene_quads(which)
+    print(f"prepare_scene_device {which} quads={quads.shape[0]}", flush=True)
+    _same_occluders(rast.prepare_scene_device(quads), scenes.bake_scene(quads).occluders)
+
+
+def test_prepare_scene_arguments(rast):
+    quads = _scene_quads("city")[:100]
+    for target, gran in ((512, 4), (512, 12), (16, 8)):
+        with pytest.raises(RasterizerError):
+            rast.prepare_scene_device(quads, target, gran)
+    # 5 quads pad to 8, one batch in input order on both sides.
+    _same_occluders(rast.prepare_scene_device(quads[:5]), scenes.bake_scene(quads[:5]).occluders)
+
+
 @pytest.mark.parametrize("which", ["boxes", "city"])
 def test_scene_prepared_on_device(rast, which):
     """Main.cpp:86-128 with SAH batching and the bake both on the device
