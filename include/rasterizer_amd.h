@@ -71,6 +71,12 @@ int rz_clear(rz_context* ctx);
 int rz_occluder_create(rz_context* ctx, const uint32_t* packets, uint32_t packetCount, const float refMin[4],
                        const float refMax[4], const float boundsMin[4], const float boundsMax[4],
                        rz_occluder** out);
+/* The same from packets already in device memory (d_packets, e.g. written by
+ * rz_bake_device or rz_prepare_scene_device): copied device to device; the
+ * boxes are host arrays. */
+int rz_occluder_create_device(rz_context* ctx, const uint32_t* d_packets, uint32_t packetCount, const float refMin[4],
+                              const float refMax[4], const float boundsMin[4], const float boundsMax[4],
+                              rz_occluder** out);
 int rz_occluder_destroy(rz_occluder* occluder);
 
 /* template<bool possiblyNearClipped> Rasterizer::rasterize(const Occluder&)

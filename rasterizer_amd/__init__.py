@@ -69,6 +69,7 @@ _SIGNATURES = {
     "rz_set_mvp": ([_P, _P], ctypes.c_int),
     "rz_clear": ([_P], ctypes.c_int),
     "rz_occluder_create": ([_P, _P, _U32, _P, _P, _P, _P, ctypes.POINTER(_P)], ctypes.c_int),
+    "rz_occluder_create_device": ([_P, _P, _U32, _P, _P, _P, _P, ctypes.POINTER(_P)], ctypes.c_int),
     "rz_occluder_destroy": ([_P], ctypes.c_int),
     "rz_rasterize": ([_P, _P, ctypes.c_int], ctypes.c_int),
     "rz_draw_frame": ([_P, _U32, _P, _P], ctypes.c_int),

@@ -246,9 +246,32 @@ extern "C" int rz_clear(rz_context* ctx) {
   return RZ_OK;
 }
 
+namespace {
+
+int occluder_create(rz_context* ctx, const uint32_t* packets, uint32_t packetCount, const float refMin[4],
+                    const float refMax[4], const float boundsMin[4], const float boundsMax[4], hipMemcpyKind kind,
+                    rz_occluder** out);
+
+}  // namespace
+
 extern "C" int rz_occluder_create(rz_context* ctx, const uint32_t* packets, uint32_t packetCount,
                                   const float refMin[4], const float refMax[4], const float boundsMin[4],
                                   const float boundsMax[4], rz_occluder** out) {
+  return occluder_create(ctx, packets, packetCount, refMin, refMax, boundsMin, boundsMax, hipMemcpyHostToDevice, out);
+}
+
+extern "C" int rz_occluder_create_device(rz_context* ctx, const uint32_t* d_packets, uint32_t packetCount,
+                                         const float refMin[4], const float refMax[4], const float boundsMin[4],
+                                         const float boundsMax[4], rz_occluder** out) {
+  return occluder_create(ctx, d_packets, packetCount, refMin, refMax, boundsMin, boundsMax, hipMemcpyDeviceToDevice,
+                         out);
+}
+
+namespace {
+
+int occluder_create(rz_context* ctx, const uint32_t* packets, uint32_t packetCount, const float refMin[4],
+                    const float refMax[4], const float boundsMin[4], const float boundsMax[4], hipMemcpyKind kind,
+                    rz_occluder** out) {
   if (!ctx || !out || !refMin || !refMax || !boundsMin || !boundsMax || (packetCount % 4) != 0 ||
       (packetCount && !packets))
     return RZ_ERR_INVALID_ARGUMENT;
@@ -267,7 +290,7 @@ extern "C" int rz_occluder_create(rz_context* ctx, const uint32_t* packets, uint
       delete o;
       return RZ_ERR_OUT_OF_MEMORY;
     }
-    if (hipMemcpy(o->d_packets, packets, size_t(packetCount) * 32, hipMemcpyHostToDevice) != hipSuccess) {
+    if (hipMemcpy(o->d_packets, packets, size_t(packetCount) * 32, kind) != hipSuccess) {
       hipFree(o->d_packets);
       delete o;
       return RZ_ERR_DEVICE;
@@ -276,6 +299,8 @@ extern "C" int rz_occluder_create(rz_context* ctx, const uint32_t* packets, uint
   *out = o;
   return RZ_OK;
 }
+
+}  // namespace
 
 extern "C" int rz_occluder_destroy(rz_occluder* o) {
   if (!o) return RZ_ERR_INVALID_ARGUMENT;

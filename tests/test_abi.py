@@ -87,6 +87,7 @@ def test_null_arguments_rejected(rzlib):
     assert rzlib.rz_views_read_tile_stats(None, None) == -1
     assert rzlib.rz_sah_batches(None, 0, None, 512, 8, None, None, None) == -1
     assert rzlib.rz_prepare_scene_device(None, 0, None, 512, 8, None, None, None, None, None) == -1
+    assert rzlib.rz_occluder_create_device(None, None, 0, None, None, None, None, None) == -1
 
 
 def _rand_quads(rng, n):

@@ -157,3 +157,73 @@ def test_scene_prepared_on_device(rast, which):
         assert np.array_equal(a.m_boundsMin, b.m_boundsMin) and np.array_equal(a.m_boundsMax, b.m_boundsMax)
         assert np.array_equal(a.m_center, b.m_center)
         assert np.array_equal(a.m_refMin, b.m_refMin) and np.array_equal(a.m_refMax, b.m_refMax)
+
+
+def test_draw_from_device_prepared_occluders():
+    """Occluders prepared on the device (rz_prepare_scene_device) and wrapped
+    by rz_occluder_create_device without leaving HBM draw the same
+    query-gated frame, bit for bit, as the host-prepared ones."""
+    import ctypes
+    import math
+
+    from rasterizer_amd import DeviceBuffer, lib
+
+    W, H = 320, 192
+    R = Rasterizer(W, H)
+    rng = scenes.SplitMix64(8)
+    c, h = scenes.random_boxes(rng, 600, -50.0, 50.0, 0.5, 5.0)
+    quads = np.ascontiguousarray(np.concatenate([scenes.box_quads(c[i], h[i]) for i in range(600)], axis=0),
+                                 dtype=np.float32)
+    eye = np.zeros(3, np.float32)
+    mvp = scenes.view_projection(eye, (0, 0, 1), (0, 1, 0), math.pi / 3, W / H)
+    host = scenes.bake_scene(quads).occluders
+    order = scenes.front_to_back(host, eye)
+
+    n = quads.shape[0]
+    n_pad = (n + 7) // 8 * 8
+    max_occ = n_pad // 8
+    offs = np.zeros(max_occ + 1, np.uint32)
+    ref = np.zeros(8, np.float32)
+    cnt = ctypes.c_uint32(0)
+    d_v = DeviceBuffer(R, quads.nbytes, quads)
+    d_p = DeviceBuffer(R, n_pad * 16)
+    d_b = DeviceBuffer(R, max_occ * 48)
+    handles = []
+    try:
+        assert lib().rz_prepare_scene_device(R.ctx, n, d_v.ptr, 512, 8, d_p.ptr, d_b.ptr, offs.ctypes.data,
+                                             ctypes.byref(cnt), ref.ctypes.data) == 0
+        k = cnt.value
+        assert k == len(host)
+        bounds = d_b.read(np.float32, max_occ * 12).reshape(max_occ, 12)
+        for i in range(k):
+            bmin, bmax = bounds[i, 0:4].copy(), bounds[i, 4:8].copy()
+            rmin, rmax = ref[0:4].copy(), ref[4:8].copy()
+            hnd = ctypes.c_void_p()
+            start = ctypes.c_void_p(d_p.ptr.value + 16 * int(offs[i]))
+            assert lib().rz_occluder_create_device(R.ctx, start, (int(offs[i + 1]) - int(offs[i])) // 2,
+                                                   rmin.ctypes.data, rmax.ctypes.data, bmin.ctypes.data,
+                                                   bmax.ctypes.data, ctypes.byref(hnd)) == 0
+            handles.append(hnd)
+        zeros_d = np.zeros(R.blocks_x * R.blocks_y * 64, np.uint16)
+        zeros_h = np.zeros(R.blocks_x * R.blocks_y + 8, np.uint16)
+        R.setModelViewProjection(mvp)
+
+        R.upload_raw(zeros_d, zeros_h)
+        R.clear()
+        arr = (ctypes.c_void_p * k)(*[handles[j].value for j in order])
+        R.draw_frame_handles(arr, k)
+        R.sync()
+        dev_depth, dev_hiz = R.raw()
+
+        R.upload_raw(zeros_d, zeros_h)
+        R.clear()
+        R.draw_frame([host[j] for j in order])
+        host_depth, host_hiz = R.raw()
+        assert np.array_equal(dev_depth, host_depth) and np.array_equal(dev_hiz, host_hiz)
+        assert np.count_nonzero(host_hiz[:-8] != 1) > 0  # something was drawn
+    finally:
+        R.sync()
+        for hnd in handles:
+            lib().rz_occluder_destroy(hnd)
+        for b in (d_v, d_p, d_b):
+            b.free()
