@@ -227,3 +227,24 @@ def test_draw_from_device_prepared_occluders():
             lib().rz_occluder_destroy(hnd)
         for b in (d_v, d_p, d_b):
             b.free()
+
+
+def test_one_million_quads(rast):
+    """1,000,000 random quad boxes at (512, 8): a partition into batches below
+    the target, and, where the reference's own build is present, its exact
+    batches (about 4 s of single-threaded reference work)."""
+    import oracle
+
+    n = 1_000_000
+    rng = np.random.default_rng(n)
+    c = rng.uniform(-1000, 1000, (n, 3)).astype(np.float32)
+    h = np.exp(rng.uniform(np.log(0.5), np.log(5), (n, 3))).astype(np.float32)
+    mn, mx = c - h, c + h
+    print("sah 1M quads", flush=True)
+    got = rast.sah_batches_device(mn, mx)
+    sizes = np.array([b.size for b in got])
+    assert np.all((sizes > 0) & (sizes < 512) & (sizes % 8 == 0))
+    assert np.array_equal(np.sort(np.concatenate(got)), np.arange(n))
+    ref = oracle.sah_reference()
+    if ref is not None:
+        assert _same(got, ref(mn, mx, 512, 8))
