@@ -539,6 +539,11 @@ class Rasterizer:
         q = np.ascontiguousarray(np.asarray(quads, np.float32).reshape(-1, 4, 4))
         n = q.shape[0]
         if n == 0:
+            offsets = np.zeros(1, np.uint32)
+            ref = np.zeros(8, np.float32)
+            count = _U32(0)
+            _check(lib().rz_prepare_scene_device(self.ctx, 0, None, target, gran, None, None, _ptr(offsets),
+                                                 ctypes.byref(count), _ptr(ref)), "rz_prepare_scene_device")
             return []
         n_pad = (n + 7) // 8 * 8
         max_occ = n_pad // 8

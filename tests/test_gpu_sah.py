@@ -132,6 +132,7 @@ def test_prepare_scene_arguments(rast):
     for target, gran in ((512, 4), (512, 12), (16, 8)):
         with pytest.raises(RasterizerError):
             rast.prepare_scene_device(quads, target, gran)
+    assert rast.prepare_scene_device(quads[:0]) == []
     # 5 quads pad to 8, one batch in input order on both sides.
     _same_occluders(rast.prepare_scene_device(quads[:5]), scenes.bake_scene(quads[:5]).occluders)
 
