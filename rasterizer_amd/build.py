@@ -82,6 +82,15 @@ def build(verbose: bool = False) -> Path:
             *[str(BUILD / n) for n in objs], "-lpthread"]
     _run_all([link])
     os.replace(tmp, LIB)
+    # Test program of the C++ drop-in headers (run by tests/test_gpu_sah.py).
+    check_src = ROOT / "tests" / "cpp" / "sah_dropin_check.cpp"
+    if check_src.exists():
+        bindir = ROOT / "tests" / "_bin"
+        bindir.mkdir(exist_ok=True)
+        _run_all([["g++", "-O2", "-std=c++17", "-msse4.1", f"-I{ROOT / 'include' / 'rasterizer_amd'}",
+                   f"-I{ROOT / 'include'}", str(check_src), f"-L{LIBDIR}", "-lrasterizer_amd",
+                   "-Wl,-rpath,$ORIGIN/../../rasterizer_amd/_lib", "-Wl,-rpath,/opt/rocm/lib",
+                   "-o", str(bindir / "sah_dropin_check")]])
     return LIB
 
 

@@ -8,6 +8,7 @@
 #include "rasterizer_amd.h"
 #include "rasterizer_amd/Occluder.h"
 #include "rasterizer_amd/Rasterizer.h"
+#include "rasterizer_amd/SurfaceAreaHeuristic.h"
 
 namespace {
 
@@ -107,4 +108,35 @@ void Rasterizer::drawOccluders(const std::vector<const Occluder*>& occluders) {
   handles.reserve(occluders.size());
   for (const Occluder* o : occluders) handles.push_back(o->device(m_ctx));
   check(rz_draw_frame(m_ctx, uint32_t(handles.size()), handles.data(), nullptr), "drawOccluders");
+}
+
+// SurfaceAreaHeuristic::generateBatches (SurfaceAreaHeuristic.cpp:96-104) on
+// the GPU: a context of its own for the stream, the boxes uploaded once.
+std::vector<std::vector<uint32_t>> rz_dropin::sah_generate_batches(const float* boxes, size_t n, uint32_t targetSize,
+                                                                   uint32_t splitGranularity) {
+  if (n > 0xFFFFFFFFull) throw std::runtime_error("SurfaceAreaHeuristic::generateBatches: too many boxes");
+  std::vector<std::vector<uint32_t>> out;
+  rz_context* ctx = nullptr;
+  check(rz_create(8, 8, &ctx), "rz_create");
+  void* d_boxes = nullptr;
+  void* d_indices = nullptr;
+  int status = RZ_OK;
+  std::vector<uint32_t> offsets(n + 2, 0u), indices(n);
+  uint32_t nBatches = 0;
+  if (n) {
+    status = rz_device_alloc(ctx, n * 32, boxes, &d_boxes);
+    if (status == RZ_OK) status = rz_device_alloc(ctx, n * 4, nullptr, &d_indices);
+  }
+  if (status == RZ_OK)
+    status = rz_sah_batches(ctx, uint32_t(n), static_cast<const float*>(d_boxes), targetSize, splitGranularity,
+                            static_cast<uint32_t*>(d_indices), offsets.data(), &nBatches);
+  if (status == RZ_OK && n) status = rz_device_read(ctx, d_indices, n * 4, indices.data());
+  if (d_boxes) rz_device_free(d_boxes);
+  if (d_indices) rz_device_free(d_indices);
+  rz_destroy(ctx);
+  check(status, "SurfaceAreaHeuristic::generateBatches");
+  out.reserve(nBatches);
+  for (uint32_t b = 0; b < nBatches; ++b)
+    out.emplace_back(indices.begin() + offsets[b], indices.begin() + offsets[b + 1]);
+  return out;
 }

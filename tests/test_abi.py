@@ -58,7 +58,8 @@ def test_dropin_classes_exported():
                 "Rasterizer::clear()", "void Rasterizer::rasterize<true>(Occluder const&)",
                 "void Rasterizer::rasterize<false>(Occluder const&)", "Rasterizer::queryVisibility(",
                 "Rasterizer::query2D(unsigned int, unsigned int, unsigned int, unsigned int, unsigned int) const",
-                "Rasterizer::readBackDepth(void*) const", "Occluder::bake(", "Occluder::~Occluder()"):
+                "Rasterizer::readBackDepth(void*) const", "Occluder::bake(", "Occluder::~Occluder()",
+                "rz_dropin::sah_generate_batches("):
         assert sym in out, f"missing drop-in symbol {sym}"
 
 

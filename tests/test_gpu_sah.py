@@ -249,3 +249,39 @@ def test_one_million_quads(rast):
     ref = oracle.sah_reference()
     if ref is not None:
         assert _same(got, ref(mn, mx, 512, 8))
+
+
+@pytest.mark.parametrize("seed,n,target,gran,grid", [(31, 20000, 512, 8, 1000), (32, 300, 512, 8, 5)])
+def test_cpp_dropin_header(tmp_path, seed, n, target, gran, grid):
+    """SurfaceAreaHeuristic::generateBatches through the C++ drop-in header
+    (include/rasterizer_amd/SurfaceAreaHeuristic.h), called from a compiled
+    program as Main.cpp:107 calls it, against the restatement and, where it
+    is built, the reference's own generateBatches."""
+    import subprocess
+    from pathlib import Path
+
+    import oracle
+
+    exe = Path(__file__).resolve().parent / "_bin" / "sah_dropin_check"
+    if not exe.exists():
+        pytest.fail("tests/_bin/sah_dropin_check is not built (python -m rasterizer_amd.build)")
+    mn, mx = _random(seed, n, grid)
+    boxes = np.zeros((n, 8), np.float32)
+    boxes[:, 0:3], boxes[:, 4:7] = mn, mx
+    src, dst = tmp_path / "boxes.f32", tmp_path / "batches.u32"
+    boxes.tofile(src)
+    print(f"sah drop-in n={n}", flush=True)
+    r = subprocess.run([str(exe), str(src), str(dst), str(target), str(gran)], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stderr
+    words = np.fromfile(dst, np.uint32)
+    got, at = [], 1
+    for _ in range(int(words[0])):
+        size = int(words[at])
+        got.append(words[at + 1:at + 1 + size].astype(np.int64))
+        at += 1 + size
+    assert at == words.size
+    assert _same(got, scenes.sah_batches(mn, mx, target, gran))
+    ref = oracle.sah_reference()
+    if ref is not None:
+        assert _same(got, ref(mn, mx, target, gran))
