@@ -893,4 +893,23 @@ class Mlp:
         return float(loss[0])
 
 
-__all__ = ["Rasterizer", "Occluder", "DeviceBuffer", "Views", "Mlp", "RasterizerError", "lib", "LIB_PATH"]
+class SurfaceAreaHeuristic:
+    """SurfaceAreaHeuristic (SurfaceAreaHeuristic.h:5-11) with its static
+    generateBatches computed on the GPU (rz_sah_batches)."""
+
+    _rast = None
+
+    @staticmethod
+    def generateBatches(aabbs, targetSize: int, splitGranularity: int) -> list:
+        """aabbs: (n, 8) or (n, 2, 4) boxes (min xyzw, max xyzw), as
+        std::vector<Aabb> holds them; returns lists of quad indices in the
+        reference's batch order (SurfaceAreaHeuristic.cpp:96-104)."""
+        b = np.asarray(aabbs, np.float32).reshape(-1, 8)
+        if SurfaceAreaHeuristic._rast is None:
+            SurfaceAreaHeuristic._rast = Rasterizer(8, 8)
+        out = SurfaceAreaHeuristic._rast.sah_batches_device(b[:, 0:4], b[:, 4:8], targetSize, splitGranularity)
+        return [x.tolist() for x in out]
+
+
+__all__ = ["Rasterizer", "Occluder", "SurfaceAreaHeuristic", "DeviceBuffer", "Views", "Mlp", "RasterizerError", "lib",
+           "LIB_PATH"]

@@ -285,3 +285,16 @@ def test_cpp_dropin_header(tmp_path, seed, n, target, gran, grid):
     ref = oracle.sah_reference()
     if ref is not None:
         assert _same(got, ref(mn, mx, target, gran))
+
+
+def test_python_class_mirror():
+    """rasterizer_amd.SurfaceAreaHeuristic.generateBatches, named as the
+    reference's class, on Aabb-layout boxes."""
+    from rasterizer_amd import SurfaceAreaHeuristic
+
+    mn, mx = _random(41, 3000, 6)
+    boxes = np.zeros((3000, 2, 4), np.float32)
+    boxes[:, 0, :3], boxes[:, 1, :3] = mn, mx
+    got = SurfaceAreaHeuristic.generateBatches(boxes, 512, 8)
+    want = scenes.sah_batches(mn, mx, 512, 8)
+    assert [list(map(int, b)) for b in want] == got
