@@ -51,16 +51,24 @@ def algorithmic_bytes_per_frame(quads: int, width: int, height: int) -> int:
     return 16 * quads + 2 * width * height + 2 * (width // 8) * (height // 8) + 64
 
 
-def measured_traffic(kernel_profile: str):
-    """HBM bytes per launch of the frame kernel on this workload, from the
-    calibrated FETCH_SIZE/WRITE_SIZE runs recorded under profiles/ (rocprofv3
-    cannot run inside the timed process), or None."""
+def measured_traffic(kernel_profile: str, frames: int):
+    """HBM bytes per launch of the multi-view raster kernel for `frames`
+    frames in flight, from the calibrated FETCH_SIZE/WRITE_SIZE runs recorded
+    under profiles/ (rocprofv3 cannot run inside the timed process): the
+    entry measured at this frame count, else the nearest measured count
+    scaled per frame.  Returns (bytes, source) or (None, reason)."""
     path = os.path.join(ROOT, "profiles", kernel_profile)
     try:
         vals = dict(line.strip().split("=", 1) for line in open(path) if "=" in line and not line.startswith("#"))
-        return float(vals["traffic_bytes"])
-    except (OSError, KeyError, ValueError):
-        return None
+    except OSError:
+        return None, f"{kernel_profile} missing"
+    measured = {int(k[len("traffic_bytes_f"):]): float(v) for k, v in vals.items() if k.startswith("traffic_bytes_f")}
+    if not measured:
+        return None, f"{kernel_profile} has no traffic_bytes_f<frames> entry"
+    if frames in measured:
+        return measured[frames], f"measured at {frames} frames per launch ({kernel_profile})"
+    near = min(measured, key=lambda f: abs(f - frames))
+    return measured[near] * frames / near, f"measured at {near} frames per launch, scaled per frame ({kernel_profile})"
 
 
 def cpu_baseline(scene, orders, threads: int, frames: int, query_reps: int):
@@ -505,6 +513,7 @@ def main() -> int:
                  "frame": frame_ms, "query": prof["query_ms"] / max(prof["query_batches"], 1)}
     log("single-frame kernels: " + ", ".join(f"{k} {v:.4f} ms" for k, v in kernel_ms.items()))
 
+    traffic_bytes, traffic_source = measured_traffic("rz_views_kernel_city1080_traffic.txt", F)
     alg_frame = algorithmic_bytes_per_frame(scene.quad_count, W, H)
     alg_launch = F * alg_frame
     achieved = alg_launch / (raster_ms * 1e-3) / 1e9
@@ -543,7 +552,8 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": measured_traffic("rz_views_kernel_city1080_f128_traffic.txt") if F == 128 else None,
+            "traffic": traffic_bytes,
+            "traffic_source": traffic_source,
             "kernel": "rz_views_kernel (query-gated, frames in flight)",
             "algorithmic_bytes": alg_launch,
         },

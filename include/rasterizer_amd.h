@@ -93,6 +93,14 @@ int rz_rasterize(rz_context* ctx, const rz_occluder* occluder, int possiblyNearC
  * rasterize<false>, 2 drawn with rasterize<true>. */
 int rz_draw_frame(rz_context* ctx, uint32_t n, const rz_occluder* const* occluders, uint8_t* statusOut);
 
+/* Bound of each cross-tile wait of the query-gated draw, in ticks of the
+ * device's 100 MHz s_memrealtime clock (0 restores the default, 2 s).  No
+ * counterpart in the reference, whose loop is sequential.  A wait that runs
+ * out ends every other wait of the launch, every kernel exits normally, the
+ * frame's contents are unspecified and the next blocking call (rz_sync)
+ * returns RZ_ERR_TIMEOUT. */
+int rz_set_wait_timeout(rz_context* ctx, uint64_t ticks);
+
 /* Rasterizer::queryVisibility(__m128 boundsMin, __m128 boundsMax, bool& needsClipping)
  * (Rasterizer.h:22, Rasterizer.cpp:123-281).  Blocks.  *needsClipping is
  * written only when the box passes the frustum test, as in the reference. */
@@ -181,9 +189,14 @@ int rz_sah_batches(rz_context* ctx, uint32_t n, const float* d_boxes, uint32_t t
 /* Main.cpp:86-128 on the device, for occluder sets that live in HBM: pad to
  * whole 8-quad batches with copies of vertex 0 (:90-94), one Aabb per quad
  * (:96-105), SurfaceAreaHeuristic::generateBatches (:107, as
- * rz_sah_batches), the scene box over every vertex (:109-113) and
- * Occluder::bake of each batch against it (:115-128, as rz_bake_device).
- * d_vertices: nQuads * 4 vertices, xyzw floats, 16-byte aligned, device.
+ * rz_sah_batches), the scene box (:109-113) and Occluder::bake of each batch
+ * against it (:115-128, as rz_bake_device).  The input is the quads' own
+ * vertex list, so the scene box is taken over the quads' vertices; Main.cpp
+ * takes it over the whole indexed vertex buffer, which can hold vertices no
+ * quad references (a caller with such a mesh gets a different box there).
+ * d_vertices: nQuads * 4 vertices, xyzw floats, 16-byte aligned, device;
+ * x, y and z finite with |coordinate| <= 2^40 (else RZ_ERR_INVALID_ARGUMENT,
+ * as for rz_sah_batches' boxes).
  * Writes occluder i's packets, one word per vertex, to
  * d_packets[4 * quadOffsets[i] .. 4 * quadOffsets[i + 1]) (room for
  * 4 * nQuads rounded up to 32 words), its boundsMin, boundsMax and centre to
@@ -198,6 +211,20 @@ int rz_sah_batches(rz_context* ctx, uint32_t n, const float* d_boxes, uint32_t t
 int rz_prepare_scene_device(rz_context* ctx, uint32_t nQuads, const float* d_vertices, uint32_t targetSize,
                             uint32_t splitGranularity, uint32_t* d_packets, float* d_bounds, uint32_t* quadOffsets,
                             uint32_t* nOccluders, float refMinMax[8]);
+
+/* QuadDecomposition::decompose(indices, vertices) (QuadDecomposition.h:8,
+ * QuadDecomposition.cpp:311-445), host side, no device needed: pairs the
+ * triangles (indices: nIndices words, three per triangle; a trailing partial
+ * triangle is ignored) into quads along shared edges whose two triangles keep
+ * the far vertex within 0.5 units of each other's plane (RSQRTPS-normalised
+ * normals, as the reference), as a maximum matching grown greedily and then
+ * by augmenting paths; a triangle left single becomes the degenerate quad
+ * (i0, i2, i1, i0).  Writes 4 indices per quad to out (room for
+ * 4 * (nIndices / 3) words) and the number of words to *nOut; identical to
+ * the reference's output, quad for quad.  vertices: nVertices x 4 floats
+ * (xyzw); an index >= nVertices is RZ_ERR_INVALID_ARGUMENT. */
+int rz_quad_decompose(const uint32_t* indices, size_t nIndices, const float* vertices, size_t nVertices, uint32_t* out,
+                      size_t* nOut);
 
 /* Timing helpers on the context stream (HIP events): rz_timer_start records
  * a start event, rz_timer_stop a stop event and returns (after waiting for
@@ -316,6 +343,11 @@ int rz_views_read_status(rz_views* views, uint32_t view, uint8_t* status);
  * a batch order or a model (RZ_ERR_INVALID_ARGUMENT).  0 restores
  * classify-then-draw. */
 int rz_views_set_query_gated(rz_views* views, int gated);
+/* Bound of each cross-tile wait of query-gated steps, in 100 MHz
+ * s_memrealtime ticks (0 restores the default, 2 s); as rz_set_wait_timeout:
+ * a wait that runs out ends the launch's other waits, every kernel exits
+ * normally and rz_views_sync returns RZ_ERR_TIMEOUT. */
+int rz_views_set_wait_timeout(rz_views* views, uint64_t ticks);
 
 /* Observability of the last step (no counterpart in the reference): the
  * view's per-occluder prologue state (128 B records: the folded matrix T[4][4]

@@ -19,22 +19,20 @@ namespace rz_dropin {
 std::vector<std::vector<uint32_t>> sah_generate_batches(const float* boxes, size_t n, uint32_t targetSize,
                                                         uint32_t splitGranularity);
 
-// The class behind SurfaceAreaHeuristic.  A template on the box type, so that
-// generateBatches is compiled only where it is called, where the caller's
-// std::vector<Aabb> has made Aabb complete.
-template <typename AabbT = Aabb>
-class SurfaceAreaHeuristicT {
- public:
-  static std::vector<std::vector<uint32_t>> generateBatches(const std::vector<AabbT>& aabbs, uint32_t targetSize,
-                                                            uint32_t splitGranularity) {
-    static_assert(sizeof(AabbT) == 8 * sizeof(float), "Aabb is two __m128 (m_min, m_max)");
-    return sah_generate_batches(reinterpret_cast<const float*>(aabbs.data()), aabbs.size(), targetSize,
-                                splitGranularity);
-  }
-};
-
 }  // namespace rz_dropin
 
 // SurfaceAreaHeuristic::generateBatches(aabbs, targetSize, splitGranularity)
-// as the reference calls it (Main.cpp:107).
-using SurfaceAreaHeuristic = rz_dropin::SurfaceAreaHeuristicT<>;
+// as the reference calls it (Main.cpp:107).  A plain class, so that it can be
+// forward-declared as the reference's header allows; generateBatches is a
+// member template on the box type, compiled only where it is called, where
+// the caller's std::vector<Aabb> has made Aabb complete.
+class SurfaceAreaHeuristic {
+ public:
+  template <typename AabbT = Aabb>
+  static std::vector<std::vector<uint32_t>> generateBatches(const std::vector<AabbT>& aabbs, uint32_t targetSize,
+                                                            uint32_t splitGranularity) {
+    static_assert(sizeof(AabbT) == 8 * sizeof(float), "Aabb is two __m128 (m_min, m_max)");
+    return rz_dropin::sah_generate_batches(reinterpret_cast<const float*>(aabbs.data()), aabbs.size(), targetSize,
+                                           splitGranularity);
+  }
+};

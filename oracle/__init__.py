@@ -36,6 +36,7 @@ class _Occ(ctypes.Structure):
 
 REF_DIR = Path("/root/reference")
 SAH_REF_PATH = HERE / "_ref" / "libsah_ref.so"
+QUAD_REF_PATH = HERE / "_ref" / "libquad_ref.so"
 
 
 def build() -> Path:
@@ -46,6 +47,30 @@ def build() -> Path:
     if (REF_DIR / "SoftwareRasterizer" / "SurfaceAreaHeuristic.cpp").exists():
         subprocess.run(["make", "-s", "-C", str(HERE), "ref", f"REF={REF_DIR}"], check=True)
     return HERE / "_build" / "liboracle.so"
+
+
+def quad_reference():
+    """The reference's QuadDecomposition::decompose as built into oracle/_ref/
+    (None if that build is absent): a function (indices, vertices n x 4) ->
+    quad indices (4 per quad)."""
+    if not QUAD_REF_PATH.exists():
+        return None
+    lib = ctypes.CDLL(str(QUAD_REF_PATH))
+    fn = lib.quad_ref_decompose
+    fn.restype = ctypes.c_int
+    fn.argtypes = [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p,
+                   ctypes.POINTER(ctypes.c_size_t)]
+
+    def run(indices, vertices):
+        idx = np.ascontiguousarray(np.asarray(indices, np.uint32).reshape(-1))
+        v = np.ascontiguousarray(np.asarray(vertices, np.float32).reshape(-1, 4))
+        out = np.zeros(max(4 * (idx.size // 3), 1), np.uint32)
+        n = ctypes.c_size_t(0)
+        if fn(idx.ctypes.data, idx.size, v.ctypes.data, v.shape[0], out.ctypes.data, ctypes.byref(n)) != 0:
+            raise ValueError("quad_ref_decompose: index outside the vertex array")
+        return out[:n.value].copy()
+
+    return run
 
 
 def sah_reference():
@@ -108,6 +133,7 @@ def lib() -> ctypes.CDLL:
             "or_setup_dump": ([P, P, U, P, P, ctypes.c_int, P], ctypes.c_int),
             "or_prologue": ([P, P, P, P], None),
             "or_bake": ([P, U, P, P, P, P, P, P], ctypes.c_int),
+            "or_bake_order": ([P, U, P], ctypes.c_int),
             "or_x86_rcp": ([ctypes.c_float], ctypes.c_float),
             "or_x86_cvtt": ([ctypes.c_float], ctypes.c_int32),
             "or_bench_frames": ([U, U, U, P, U, P, P, U], ctypes.c_double),
@@ -288,6 +314,16 @@ def bake(vertices, ref_min, ref_max):
     if rc != 0:
         raise ValueError("oracle bake: vertex count must be a positive multiple of 32")
     return packets, bmin, bmax, cen
+
+
+def bake_order(vertices) -> np.ndarray:
+    """The quad order of Occluder::bake's k-means regroup (Occluder.cpp:11-93), restated (oracle.c)."""
+    v = np.ascontiguousarray(np.asarray(vertices, np.float32).reshape(-1, 4))
+    nq = v.shape[0] // 4
+    out = np.zeros(nq, np.uint32)
+    if nq == 0 or lib().or_bake_order(_ptr(v), nq, _ptr(out)) != 0:
+        raise ValueError("oracle bake_order: needs at least one quad")
+    return out
 
 
 def x86_rcp(x: float) -> float:

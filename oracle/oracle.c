@@ -931,15 +931,13 @@ static void v_normalize(float v[4]) {
   for (int j = 0; j < 4; ++j) v[j] = v[j] * rs;
 }
 
-int or_bake(const float* vertices, uint32_t n, const float refMin[4], const float refMax[4], uint32_t* packetsOut,
-            float boundsMin[4], float boundsMax[4], float center[4]) {
-  if (n == 0 || n % 32) return -1;
-  uint32_t nq = n / 4;
+/* Occluder.cpp:11-93: per-quad normals, the 6-centroid k-means by normal
+ * direction and the stable regroup; order[i] = the quad placed i-th. */
+static int bake_order(const float* vertices, uint32_t nq, uint32_t* order) {
   float* normals = (float*)malloc(sizeof(float) * 4 * nq);
   uint32_t* assign = (uint32_t*)calloc(nq, sizeof(uint32_t));
-  uint32_t* order = (uint32_t*)malloc(sizeof(uint32_t) * nq);
-  if (!normals || !assign || !order) {
-    free(normals); free(assign); free(order);
+  if (!normals || !assign) {
+    free(normals); free(assign);
     return -2;
   }
   for (uint32_t q = 0; q < nq; ++q) {
@@ -978,6 +976,26 @@ int or_bake(const float* vertices, uint32_t n, const float refMin[4], const floa
   for (uint32_t k = 0; k < 6; ++k)
     for (uint32_t q = 0; q < nq; ++q)
       if (assign[q] == k) order[cnt++] = q;
+  free(normals);
+  free(assign);
+  return 0;
+}
+
+int or_bake_order(const float* vertices, uint32_t nQuads, uint32_t* orderOut) {
+  if (nQuads == 0) return -1;
+  return bake_order(vertices, nQuads, orderOut);
+}
+
+int or_bake(const float* vertices, uint32_t n, const float refMin[4], const float refMax[4], uint32_t* packetsOut,
+            float boundsMin[4], float boundsMax[4], float center[4]) {
+  if (n == 0 || n % 32) return -1;
+  uint32_t nq = n / 4;
+  uint32_t* order = (uint32_t*)malloc(sizeof(uint32_t) * nq);
+  if (!order) return -2;
+  if (bake_order(vertices, nq, order) != 0) {
+    free(order);
+    return -2;
+  }
 
   float invExt[4];
   for (int j = 0; j < 4; ++j) invExt[j] = 1.0f / (refMax[j] - refMin[j]);
@@ -1007,8 +1025,6 @@ int or_bake(const float* vertices, uint32_t n, const float refMin[4], const floa
     boundsMax[j] = mx[j];
     center[j] = (mx[j] + mn[j]) * 0.5f;
   }
-  free(normals);
-  free(assign);
   free(order);
   return 0;
 }

@@ -117,6 +117,10 @@ void or_frame_labels(OrRaster* r, uint32_t n, const OrOccluder* occ, const uint3
 int or_bake(const float* vertices, uint32_t n, const float refMin[4], const float refMax[4],
             uint32_t* packetsOut, float boundsMin[4], float boundsMax[4], float center[4]);
 
+/* Occluder.cpp:11-93 alone: the order in which bake places the quads (the
+ * k-means groups by normal, each in quad order).  orderOut: nQuads words. */
+int or_bake_order(const float* vertices, uint32_t nQuads, uint32_t* orderOut);
+
 /* Host x86 primitives as used above (exposed for tests). */
 float or_x86_rcp(float x);
 int32_t or_x86_cvtt(float x);

@@ -73,6 +73,7 @@ _SIGNATURES = {
     "rz_occluder_destroy": ([_P], ctypes.c_int),
     "rz_rasterize": ([_P, _P, ctypes.c_int], ctypes.c_int),
     "rz_draw_frame": ([_P, _U32, _P, _P], ctypes.c_int),
+    "rz_set_wait_timeout": ([_P, ctypes.c_uint64], ctypes.c_int),
     "rz_query": ([_P, _P, _P, _I32P, _I32P], ctypes.c_int),
     "rz_query2d": ([_P, _U32, _U32, _U32, _U32, _U32, _I32P], ctypes.c_int),
     "rz_query_batch": ([_P, _U32, _P, _P, _P], ctypes.c_int),
@@ -134,9 +135,12 @@ _SIGNATURES = {
     "rz_views_tune": ([_P, _U32, ctypes.c_char_p, _P, _P], ctypes.c_int),
     "rz_views_load_config": ([_P, ctypes.c_char_p, _P], ctypes.c_int),
     "rz_views_set_query_gated": ([_P, ctypes.c_int], ctypes.c_int),
+    "rz_views_set_wait_timeout": ([_P, ctypes.c_uint64], ctypes.c_int),
     "rz_views_profile_enable": ([_P, ctypes.c_int], ctypes.c_int),
     "rz_views_profile_read": ([_P, _P], ctypes.c_int),
     "rz_views_read_items": ([_P, _U32, _P, _P], ctypes.c_int),
+    "rz_quad_decompose": ([_P, ctypes.c_size_t, _P, ctypes.c_size_t, _P, ctypes.POINTER(ctypes.c_size_t)],
+                          ctypes.c_int),
     "rz_write_occluders": ([ctypes.c_char_p, _U32, _P], ctypes.c_int),
     "rz_read_occluders": ([ctypes.c_char_p, _P, _P, _P, _P], ctypes.c_int),
     "rz_write_frame": ([ctypes.c_char_p, _U32, _U32, _P, _P, _U32, _P, _P], ctypes.c_int),
@@ -431,6 +435,10 @@ class Rasterizer:
     def draw_frame_handles(self, handles, n: int) -> None:
         """rz_draw_frame on a prepared ctypes array of device handles (no status)."""
         _check(lib().rz_draw_frame(self._ctx, n, ctypes.cast(handles, ctypes.c_void_p), None), "draw_frame")
+
+    def set_wait_timeout(self, ticks: int) -> None:
+        """Bound of each cross-tile wait of draw_frame in 100 MHz ticks (0: the default 2 s)."""
+        _check(lib().rz_set_wait_timeout(self._ctx, int(ticks)), "rz_set_wait_timeout")
 
     def handle_array(self, occluders):
         return (ctypes.c_void_p * max(len(occluders), 1))(*[o.device(self._ctx).value for o in occluders])
@@ -807,6 +815,10 @@ class Views:
         """Every view runs the reference's query-gated draw loop (as rz_draw_frame)."""
         _check(lib().rz_views_set_query_gated(self._v, 1 if gated else 0), "rz_views_set_query_gated")
 
+    def set_wait_timeout(self, ticks: int) -> None:
+        """Bound of each cross-tile wait of query-gated steps in 100 MHz ticks (0: the default 2 s)."""
+        _check(lib().rz_views_set_wait_timeout(self._v, int(ticks)), "rz_views_set_wait_timeout")
+
     def config(self) -> dict:
         c = ViewsConfig()
         _check(lib().rz_views_get_config(self._v, ctypes.byref(c)), "rz_views_get_config")
@@ -891,6 +903,25 @@ class Mlp:
         _check(lib().rz_mlp_train_step(self._m, x.shape[0], _ptr(x), pr.shape[0], _ptr(pr), ctypes.c_float(lr),
                                        _ptr(loss)), "rz_mlp_train_step")
         return float(loss[0])
+
+
+class QuadDecomposition:
+    """QuadDecomposition (QuadDecomposition.h:5-11) under the reference's name."""
+
+    @staticmethod
+    def decompose(indices, vertices) -> np.ndarray:
+        """Triangle indices + vertices (n x 4 or n x 3 floats) -> quad indices
+        (4 per quad), identical to the reference's QuadDecomposition::decompose."""
+        idx = np.ascontiguousarray(np.asarray(indices, dtype=np.uint32).reshape(-1))
+        v = np.asarray(vertices, dtype=np.float32)
+        if v.ndim == 2 and v.shape[1] == 3:
+            v = np.concatenate([v, np.ones((v.shape[0], 1), np.float32)], axis=1)
+        v = np.ascontiguousarray(v.reshape(-1, 4))
+        out = np.zeros(max(4 * (idx.size // 3), 1), np.uint32)
+        n = ctypes.c_size_t(0)
+        _check(lib().rz_quad_decompose(_ptr(idx), idx.size, _ptr(v), v.shape[0], _ptr(out), ctypes.byref(n)),
+               "QuadDecomposition.decompose")
+        return out[:n.value].copy()
 
 
 class SurfaceAreaHeuristic:

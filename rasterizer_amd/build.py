@@ -62,6 +62,7 @@ def build(verbose: bool = False) -> Path:
     objs = {
         "rz_host.o": ["g++", *HOST_FLAGS, "-c", str(CSRC / "rz_host.cpp")],
         "rz_io.o": ["g++", *HOST_FLAGS, "-c", str(CSRC / "rz_io.cpp")],
+        "rz_quad.o": ["g++", *HOST_FLAGS, "-O3", "-c", str(CSRC / "rz_quad.cpp")],
         "dropin.o": ["g++", *HOST_FLAGS, "-c", str(CSRC / "dropin.cpp")],
         "rz_kernels.o": [hipcc, *HIP_FLAGS, "-c", str(CSRC / "rz_kernels.hip")],
         "rz_api.o": [hipcc, *HIP_FLAGS, "-c", str(CSRC / "rz_api.cpp")],
@@ -82,15 +83,20 @@ def build(verbose: bool = False) -> Path:
             *[str(BUILD / n) for n in objs], "-lpthread"]
     _run_all([link])
     os.replace(tmp, LIB)
-    # Test program of the C++ drop-in headers (run by tests/test_gpu_sah.py).
-    check_src = ROOT / "tests" / "cpp" / "sah_dropin_check.cpp"
-    if check_src.exists():
-        bindir = ROOT / "tests" / "_bin"
-        bindir.mkdir(exist_ok=True)
-        _run_all([["g++", "-O2", "-std=c++17", "-msse4.1", f"-I{ROOT / 'include' / 'rasterizer_amd'}",
-                   f"-I{ROOT / 'include'}", str(check_src), f"-L{LIBDIR}", "-lrasterizer_amd",
-                   "-Wl,-rpath,$ORIGIN/../../rasterizer_amd/_lib", "-Wl,-rpath,/opt/rocm/lib",
-                   "-o", str(bindir / "sah_dropin_check")]])
+    # Test programs of the C++ drop-in headers (run by tests/test_gpu_sah.py
+    # and tests/test_gpu_dropin.py).
+    bindir = ROOT / "tests" / "_bin"
+    progs = []
+    for name in ("sah_dropin_check", "main_sequence_check"):
+        src = ROOT / "tests" / "cpp" / f"{name}.cpp"
+        if src.exists():
+            bindir.mkdir(exist_ok=True)
+            progs.append(["g++", "-O2", "-std=c++17", "-msse4.1", f"-I{ROOT / 'include' / 'rasterizer_amd'}",
+                          f"-I{ROOT / 'include'}", str(src), f"-L{LIBDIR}", "-lrasterizer_amd",
+                          "-Wl,-rpath,$ORIGIN/../../rasterizer_amd/_lib", "-Wl,-rpath,/opt/rocm/lib",
+                          "-o", str(bindir / name)])
+    if progs:
+        _run_all(progs)
     return LIB
 
 

@@ -7,6 +7,7 @@
 
 #include "rasterizer_amd.h"
 #include "rasterizer_amd/Occluder.h"
+#include "rasterizer_amd/QuadDecomposition.h"
 #include "rasterizer_amd/Rasterizer.h"
 #include "rasterizer_amd/SurfaceAreaHeuristic.h"
 
@@ -89,9 +90,12 @@ bool Rasterizer::queryVisibility(__m128 boundsMin, __m128 boundsMax, bool& needs
   float lo[4], hi[4];
   store4(boundsMin, lo);
   store4(boundsMax, hi);
-  int visible = 0, clip = needsClipping ? 1 : 0;
+  // needsClipping is only ever written (Rasterizer.cpp:164-167 leaves it
+  // unwritten for a frustum-culled box), so the caller's value is not read:
+  // rz_query replaces the sentinel only when it writes the flag.
+  int visible = 0, clip = -1;
   check(rz_query(m_ctx, lo, hi, &visible, &clip), "queryVisibility");
-  needsClipping = clip != 0;
+  if (clip != -1) needsClipping = clip != 0;
   return visible != 0;
 }
 
@@ -108,6 +112,20 @@ void Rasterizer::drawOccluders(const std::vector<const Occluder*>& occluders) {
   handles.reserve(occluders.size());
   for (const Occluder* o : occluders) handles.push_back(o->device(m_ctx));
   check(rz_draw_frame(m_ctx, uint32_t(handles.size()), handles.data(), nullptr), "drawOccluders");
+}
+
+// ------------------------------------------------------- QuadDecomposition
+
+std::vector<uint32_t> QuadDecomposition::decompose(const std::vector<uint32_t>& indices,
+                                                   const std::vector<__m128>& vertices) {
+  std::vector<float> flat(vertices.size() * 4);
+  for (size_t i = 0; i < vertices.size(); ++i) store4(vertices[i], &flat[4 * i]);
+  std::vector<uint32_t> out(4 * (indices.size() / 3));
+  size_t n = 0;
+  check(rz_quad_decompose(indices.data(), indices.size(), flat.data(), vertices.size(), out.data(), &n),
+        "QuadDecomposition::decompose");
+  out.resize(n);
+  return out;
 }
 
 // SurfaceAreaHeuristic::generateBatches (SurfaceAreaHeuristic.cpp:96-104) on

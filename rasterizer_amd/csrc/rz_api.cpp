@@ -100,6 +100,7 @@ struct rz_context {
   uint64_t* d_frameStats = nullptr;  // per-tile frame-kernel counters when enabled
   bool cooperative = true;           // RZ_FRAME_LAUNCH=plain at creation: plain dispatch of the same grid
   int waves = 8;                     // wave64s per frame-kernel tile
+  uint64_t timeoutTicks = kTimeoutTicks;  // bound of each cross-tile wait (rz_set_wait_timeout)
 };
 
 #define RZ_HIP(call)                           \
@@ -383,21 +384,121 @@ static int flush_pending_status(rz_context* c) {
   return RZ_OK;
 }
 
+// Device buffers one draw uses besides the context's depth, hiZ and tables.
+struct DrawBufs {
+  ItemDesc* items;
+  ItemState* state;
+  uint32_t* done;
+  uint32_t* visible;
+  QuadRec* recs;
+  uint64_t* bbox;
+  uint64_t* groupBox;
+};
+
+// Total quads of occ[0..n) (8 per batch of 4 packets); RZ_ERR_INVALID_ARGUMENT
+// on a null handle.
+static int count_quads(uint32_t n, const rz_occluder* const* occ, uint64_t* totalQuads) {
+  uint64_t t = 0;
+  for (uint32_t i = 0; i < n; ++i) {
+    if (!occ[i]) return RZ_ERR_INVALID_ARGUMENT;
+    t += uint64_t(occ[i]->packetCount / 4) * 8;
+  }
+  *totalQuads = t;
+  return RZ_OK;
+}
+
+// Item descriptors of occ[0..n) with their quads numbered in order.
+static void fill_items(ItemDesc* items, uint32_t n, const rz_occluder* const* occ, const uint32_t* kinds) {
+  uint32_t base = 0;
+  for (uint32_t i = 0; i < n; ++i) {
+    const rz_occluder* o = occ[i];
+    ItemDesc& d = items[i];
+    d.packets = o->d_packets;
+    d.quadBase = base;
+    d.quadCount = (o->packetCount / 4) * 8;
+    d.kind = kinds ? kinds[i] : uint32_t(kItemQuery);
+    d.view = 0;
+    std::memcpy(d.refMin, o->refMin, 16);
+    std::memcpy(d.refMax, o->refMax, 16);
+    std::memcpy(d.boundsMin, o->boundsMin, 16);
+    std::memcpy(d.boundsMax, o->boundsMax, 16);
+    base += d.quadCount;
+  }
+}
+
+// Enqueues the prep, setup and frame kernels of one draw whose item
+// descriptors are already in b.items (or enqueued before them).  prof:
+// optional event quadruple (prep, setup, frame, end).
+static int enqueue_kernels(rz_context* c, uint32_t n, uint32_t totalQuads, const DrawBufs& b, hipEvent_t* prof) {
+  if (prof) RZ_HIP(hipEventRecord(prof[0], c->stream));
+  PrepParams pp{};
+  pp.items = b.items;
+  pp.state = b.state;
+  pp.nItems = n;
+  pp.width = c->width;
+  pp.height = c->height;
+  pp.tileW = uint32_t(c->tileW);
+  pp.tileH = uint32_t(c->tileW);
+  pp.rcp = c->d_rcp;
+  pp.done = b.done;
+  pp.visible = b.visible;
+  pp.mvp = c->mvp;
+  RZ_HIP(launch_prep(pp, c->stream));
+  if (prof) RZ_HIP(hipEventRecord(prof[1], c->stream));
+
+  SetupParams sp{};
+  sp.items = b.items;
+  sp.state = b.state;
+  sp.nItems = n;
+  sp.nQuads = totalQuads;
+  sp.bX = c->bX;
+  sp.bY = c->bY;
+  sp.rcp = c->d_rcp;
+  sp.recs = b.recs;
+  sp.bbox = b.bbox;
+  sp.groupBox = b.groupBox;
+  RZ_HIP(launch_setup(sp, c->stream));
+  if (prof) RZ_HIP(hipEventRecord(prof[2], c->stream));
+
+  FrameParams fp{};
+  fp.items = b.items;
+  fp.state = b.state;
+  fp.nItems = n;
+  fp.bX = c->bX;
+  fp.bY = c->bY;
+  fp.tilesX = uint32_t(c->tilesX);
+  fp.tilesY = uint32_t(c->tilesY);
+  fp.recs = b.recs;
+  fp.bbox = b.bbox;
+  fp.groupBox = b.groupBox;
+  fp.done = b.done;
+  fp.visible = b.visible;
+  fp.depth = c->d_depth;
+  fp.hiz = c->d_hiz;
+  fp.cov = c->d_cov;
+  fp.err = c->d_err;
+  fp.timeoutTicks = c->timeoutTicks;
+  fp.tileMin = c->d_tileMin;
+  fp.pyrX = c->pyrX;
+  fp.stats = c->d_frameStats;
+  RZ_HIP(launch_frame(fp, c->tileW, c->stream, c->cooperative, c->waves));
+  if (prof) RZ_HIP(hipEventRecord(prof[3], c->stream));
+  return RZ_OK;
+}
+
 // Enqueues prep, setup and frame kernels for `n` items of the given kinds.
 static int enqueue_draw(rz_context* c, uint32_t n, const rz_occluder* const* occ, const uint32_t* kinds,
                         uint8_t* statusOut) {
   if (n == 0) return RZ_OK;
   uint64_t totalQuads = 0;
-  for (uint32_t i = 0; i < n; ++i) {
-    if (!occ[i]) return RZ_ERR_INVALID_ARGUMENT;
-    totalQuads += uint64_t(occ[i]->packetCount / 4) * 8;
-  }
+  int s = count_quads(n, occ, &totalQuads);
+  if (s != RZ_OK) return s;
   if (totalQuads >= (1ull << 31)) return RZ_ERR_TOO_LARGE;
   if (statusOut) {
-    int s = flush_pending_status(c);
+    s = flush_pending_status(c);
     if (s != RZ_OK) return s;
   }
-  int s = ensure_capacity(c, n, uint32_t(totalQuads));
+  s = ensure_capacity(c, n, uint32_t(totalQuads));
   if (s != RZ_OK) return s;
 
   // Host staging: wait until this ring slot's previous upload has completed.
@@ -414,78 +515,12 @@ static int enqueue_draw(rz_context* c, uint32_t n, const rz_occluder* const* occ
     c->h_itemsCap[slot] = cap;
   }
   ItemDesc* items = c->h_items[slot];
-  uint32_t base = 0;
-  for (uint32_t i = 0; i < n; ++i) {
-    const rz_occluder* o = occ[i];
-    ItemDesc& d = items[i];
-    d.packets = o->d_packets;
-    d.quadBase = base;
-    d.quadCount = (o->packetCount / 4) * 8;
-    d.kind = kinds[i];
-    d.view = 0;
-    std::memcpy(d.refMin, o->refMin, 16);
-    std::memcpy(d.refMax, o->refMax, 16);
-    std::memcpy(d.boundsMin, o->boundsMin, 16);
-    std::memcpy(d.boundsMax, o->boundsMax, 16);
-    base += d.quadCount;
-  }
+  fill_items(items, n, occ, kinds);
   RZ_HIP(hipMemcpyAsync(c->d_items, items, n * sizeof(ItemDesc), hipMemcpyHostToDevice, c->stream));
   RZ_HIP(hipEventRecord(c->ev_items[slot], c->stream));
-  hipEvent_t* prof = profile_slot(c, 0);
-  if (prof) RZ_HIP(hipEventRecord(prof[0], c->stream));
-
-  PrepParams pp{};
-  pp.items = c->d_items;
-  pp.state = c->d_state;
-  pp.nItems = n;
-  pp.width = c->width;
-  pp.height = c->height;
-  pp.tileW = uint32_t(c->tileW);
-  pp.tileH = uint32_t(c->tileW);
-  pp.rcp = c->d_rcp;
-  pp.done = c->d_done;
-  pp.visible = c->d_visible;
-  pp.mvp = c->mvp;
-  RZ_HIP(launch_prep(pp, c->stream));
-  if (prof) RZ_HIP(hipEventRecord(prof[1], c->stream));
-
-  SetupParams sp{};
-  sp.items = c->d_items;
-  sp.state = c->d_state;
-  sp.nItems = n;
-  sp.nQuads = uint32_t(totalQuads);
-  sp.bX = c->bX;
-  sp.bY = c->bY;
-  sp.rcp = c->d_rcp;
-  sp.recs = c->d_recs;
-  sp.bbox = c->d_bbox;
-  sp.groupBox = c->d_groupBox;
-  RZ_HIP(launch_setup(sp, c->stream));
-  if (prof) RZ_HIP(hipEventRecord(prof[2], c->stream));
-
-  FrameParams fp{};
-  fp.items = c->d_items;
-  fp.state = c->d_state;
-  fp.nItems = n;
-  fp.bX = c->bX;
-  fp.bY = c->bY;
-  fp.tilesX = uint32_t(c->tilesX);
-  fp.tilesY = uint32_t(c->tilesY);
-  fp.recs = c->d_recs;
-  fp.bbox = c->d_bbox;
-  fp.groupBox = c->d_groupBox;
-  fp.done = c->d_done;
-  fp.visible = c->d_visible;
-  fp.depth = c->d_depth;
-  fp.hiz = c->d_hiz;
-  fp.cov = c->d_cov;
-  fp.err = c->d_err;
-  fp.timeoutTicks = kTimeoutTicks;
-  fp.tileMin = c->d_tileMin;
-  fp.pyrX = c->pyrX;
-  fp.stats = c->d_frameStats;
-  RZ_HIP(launch_frame(fp, c->tileW, c->stream, c->cooperative, c->waves));
-  if (prof) RZ_HIP(hipEventRecord(prof[3], c->stream));
+  const DrawBufs bufs{c->d_items, c->d_state, c->d_done, c->d_visible, c->d_recs, c->d_bbox, c->d_groupBox};
+  s = enqueue_kernels(c, n, uint32_t(totalQuads), bufs, profile_slot(c, 0));
+  if (s != RZ_OK) return s;
 
   if (statusOut) {
     if (c->h_statusCap < n) {
@@ -512,8 +547,13 @@ extern "C" int rz_rasterize(rz_context* ctx, const rz_occluder* occluder, int po
 
 extern "C" int rz_draw_frame(rz_context* ctx, uint32_t n, const rz_occluder* const* occluders, uint8_t* statusOut) {
   if (!ctx || (n && !occluders)) return RZ_ERR_INVALID_ARGUMENT;
-  std::vector<uint32_t> kinds(n, kItemQuery);
-  return enqueue_draw(ctx, n, occluders, kinds.data(), statusOut);
+  return enqueue_draw(ctx, n, occluders, nullptr, statusOut);
+}
+
+extern "C" int rz_set_wait_timeout(rz_context* ctx, uint64_t ticks) {
+  if (!ctx) return RZ_ERR_INVALID_ARGUMENT;
+  ctx->timeoutTicks = ticks ? ticks : kTimeoutTicks;
+  return RZ_OK;
 }
 
 // Waits for the stream, then reports and clears the device error word.
@@ -775,15 +815,19 @@ extern "C" int rz_prepare_scene_device(rz_context* ctx, uint32_t nQuads, const f
   uint32_t* d_order = nullptr;
   uint32_t* d_keys = nullptr;
   int status = RZ_OK;
-  uint32_t keys[8] = {~0u, ~0u, ~0u, ~0u, 0u, 0u, 0u, 0u};
+  // Scene box as ordered-float keys (min xyzw, max xyzw), then a word that
+  // the boxes kernel sets when a vertex is not finite or exceeds 2^40.
+  uint32_t keys[9] = {~0u, ~0u, ~0u, ~0u, 0u, 0u, 0u, 0u, 0u};
   if (dev_alloc(&d_boxes, size_t(nPad) * 8) != hipSuccess || dev_alloc(&d_sorted, size_t(nPad) * 16) != hipSuccess ||
-      dev_alloc(&d_order, nPad) != hipSuccess || dev_alloc(&d_keys, 8) != hipSuccess) {
+      dev_alloc(&d_order, nPad) != hipSuccess || dev_alloc(&d_keys, 9) != hipSuccess) {
     status = RZ_ERR_OUT_OF_MEMORY;
   } else if (hipMemcpyAsync(d_keys, keys, sizeof keys, hipMemcpyHostToDevice, ctx->stream) != hipSuccess ||
              launch_scene_boxes(d_vertices, nQuads, nPad, d_boxes, d_keys, ctx->stream) != hipSuccess ||
              hipMemcpyAsync(keys, d_keys, sizeof keys, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
              hipStreamSynchronize(ctx->stream) != hipSuccess) {
     status = RZ_ERR_DEVICE;
+  } else if (keys[8] != 0u) {
+    status = RZ_ERR_INVALID_ARGUMENT;
   } else {
     for (int l = 0; l < 8; ++l) {
       const uint32_t k = keys[l];

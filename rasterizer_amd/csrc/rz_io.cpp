@@ -191,6 +191,9 @@ extern "C" int rz_read_frame(const char* path, uint32_t* width, uint32_t* height
   Reader r{payload};
   const uint32_t w = r.u32(), h = r.u32(), nq = r.u32();
   if (!r.ok || w == 0 || h == 0 || w % 8 || h % 8) return RZ_ERR_IO;
+  // The buffer sizes rz_create accepts; larger headers are damaged files, and
+  // bounding them first keeps the size arithmetic below far from wrapping.
+  if (w > 65528 || h > 65528 || uint64_t(w / 8) * (h / 8) >= (1u << 24)) return RZ_ERR_IO;
   const size_t nb = size_t(w / 8) * (h / 8);
   if (payload.size() != 12 + nb * 128 + (nb + 8) * 2 + size_t(nq) * 2) return RZ_ERR_IO;
   *width = w;

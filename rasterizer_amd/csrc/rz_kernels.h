@@ -277,7 +277,9 @@ int sah_batches_run(hipStream_t stream, uint32_t n, const float* d_boxes, uint32
                     uint32_t* d_indices, uint32_t* batchOffsets, uint32_t* nBatches);
 // Scene preparation around it (rz_sah.hip): boxes of nPad padded quads and
 // the scene box as ordered-float keys (min xyzw, max xyzw; the caller sets
-// them to ~0 and 0 first); the padded quads' vertices gathered in `order`.
+// them to ~0 and 0 first) and sceneKeys[8] set non-zero when a vertex has a
+// coordinate (x, y, z) that is not finite or exceeds 2^40 (zero it first);
+// the padded quads' vertices gathered in `order`.
 hipError_t launch_scene_boxes(const float* vertices, uint32_t nQuads, uint32_t nPad, float* boxes,
                               uint32_t* sceneKeys, hipStream_t stream);
 hipError_t launch_scene_gather(const float* vertices, uint32_t nQuads, uint32_t nPad, const uint32_t* order,

@@ -406,6 +406,7 @@ __global__ void __launch_bounds__(256) scene_boxes_kernel(const float4* __restri
   float4 mn = make_float4(INFINITY, INFINITY, INFINITY, INFINITY);
   float4 mx = make_float4(-INFINITY, -INFINITY, -INFINITY, -INFINITY);
   uint32_t kmn[4] = {~0u, ~0u, ~0u, ~0u}, kmx[4] = {0u, 0u, 0u, 0u};
+  bool bad = false;
   if (j < nPad) {
     // Main.cpp:96-105: the quad's Aabb over its four vertices (all lanes).
 #pragma unroll
@@ -414,6 +415,9 @@ __global__ void __launch_bounds__(256) scene_boxes_kernel(const float4* __restri
       mn = make_float4(fminf(mn.x, v.x), fminf(mn.y, v.y), fminf(mn.z, v.z), fminf(mn.w, v.w));
       mx = make_float4(fmaxf(mx.x, v.x), fmaxf(mx.y, v.y), fmaxf(mx.z, v.z), fmaxf(mx.w, v.w));
       const float c[4] = {v.x, v.y, v.z, v.w};
+      // The SAH costs need finite boxes of bounded size (as rz_sah_batches
+      // checks); NaN would also escape the fminf/fmaxf boxes.
+      bad = bad || !(fabsf(v.x) <= 0x1p40f) || !(fabsf(v.y) <= 0x1p40f) || !(fabsf(v.z) <= 0x1p40f);
 #pragma unroll
       for (int l = 0; l < 4; ++l) {
         kmn[l] = min(kmn[l], ordered_key(c[l]));
@@ -423,6 +427,7 @@ __global__ void __launch_bounds__(256) scene_boxes_kernel(const float4* __restri
     boxes[2 * size_t(j)] = mn;
     boxes[2 * size_t(j) + 1] = mx;
   }
+  if (__ballot(bad) != 0ull && (threadIdx.x & 63u) == 0u) atomicOr(&sceneKeys[8], 1u);
   // Main.cpp:109-113: the scene box over every vertex, as ordered keys.
 #pragma unroll
   for (int l = 0; l < 4; ++l) {

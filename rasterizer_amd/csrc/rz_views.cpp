@@ -61,6 +61,7 @@ hipError_t dev_alloc(T** p, size_t count) {
 
 struct rz_views {
   hipStream_t stream = nullptr;
+  uint64_t timeoutTicks = kGatedTimeoutTicks;  // bound of each cross-tile wait (rz_views_set_wait_timeout)
   std::vector<ViewHost> views;
   uint64_t* d_cov = nullptr;
   uint32_t* d_rcp = nullptr;
@@ -687,7 +688,7 @@ static int enqueue_step(rz_views* v, bool capturing) {
     RZ_HIP(launch_views_compact(rp, v->d_unitTmp, v->stream));
     rp.done = v->gated ? v->d_done : nullptr;
     rp.err = v->d_err;
-    rp.timeoutTicks = kGatedTimeoutTicks;
+    rp.timeoutTicks = v->timeoutTicks;
     if (prof) RZ_HIP(hipEventRecord(v->ph[1], v->stream));
     RZ_HIP(launch_views(rp, v->nGroups, v->stream, v->gated && v->cooperative));
     if (prof) RZ_HIP(hipEventRecord(v->ph[2], v->stream));
@@ -818,6 +819,12 @@ extern "C" int rz_views_set_query_gated(rz_views* v, int gated) {
   if (s != RZ_OK) return s;
   v->gated = gated != 0;
   v->dirty = true;
+  return RZ_OK;
+}
+
+extern "C" int rz_views_set_wait_timeout(rz_views* v, uint64_t ticks) {
+  if (!v) return RZ_ERR_INVALID_ARGUMENT;
+  v->timeoutTicks = ticks ? ticks : kGatedTimeoutTicks;
   return RZ_OK;
 }
 

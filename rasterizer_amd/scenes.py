@@ -209,10 +209,13 @@ class Scene:
     meta: dict = field(default_factory=dict)
 
 
-def bake_scene(quads: np.ndarray, real_quads: int | None = None, target: int = 512, gran: int = 8) -> Scene:
-    """Main.cpp:90-128: pad, SAH-batch and bake with the scene AABB."""
+def bake_scene(quads: np.ndarray, real_quads: int | None = None, target: int = 512, gran: int = 8,
+               vertex_buffer: np.ndarray | None = None) -> Scene:
+    """Main.cpp:90-128: pad, SAH-batch and bake with the scene AABB, taken
+    over vertex_buffer (Main.cpp:109-113 takes it over the whole vertex
+    buffer) or, without one, over the quads' vertices."""
     q = pad_to_batches(np.asarray(quads, np.float32))
-    verts = q.reshape(-1, 4)
+    verts = q.reshape(-1, 4) if vertex_buffer is None else np.asarray(vertex_buffer, np.float32).reshape(-1, 4)
     ref_min = verts.min(axis=0).astype(np.float32)
     ref_max = verts.max(axis=0).astype(np.float32)
     batches = sah_batches(q.min(axis=1), q.max(axis=1), target, gran)
@@ -307,36 +310,75 @@ class Heightfield:
         return (h[i, j] * (1 - tx) + h[i + 1, j] * tx) * (1 - tz) + (h[i, j + 1] * (1 - tx) + h[i + 1, j + 1] * tx) * tz
 
     def quads(self) -> np.ndarray:
-        """Triangulated grid (cell diagonal (i,j)-(i+1,j+1)) paired back into
-        quads where both triangles lie within 0.5 units of each other's plane
-        (the merge test of QuadDecomposition.cpp:326-343); other triangles
-        become degenerate quads (a, b, c, a).  Up-facing surfaces get positive
-        screen-space area."""
-        n, s = self.n, self.spacing
-        g = np.arange(n, dtype=np.float64) * s
-        P = np.zeros((n, n, 4))
-        P[..., 0] = g[:, None]
-        P[..., 2] = g[None, :]
-        P[..., 1] = self.h
-        P[..., 3] = 1.0
-        p00, p10 = P[:-1, :-1].reshape(-1, 4), P[1:, :-1].reshape(-1, 4)
-        p11, p01 = P[1:, 1:].reshape(-1, 4), P[:-1, 1:].reshape(-1, 4)
+        """The triangulated grid (heightfield_triangles) through the
+        reference's quad decomposition (QuadDecomposition.cpp:346-445, the
+        library's rz_quad_decompose): quads (m, 4, 4), up-facing surfaces
+        with positive screen-space area."""
+        tri, verts = heightfield_triangles(self)
+        return decompose_mesh(tri, verts)
 
-        def unit_normal(a, b, c):
-            v = np.cross(b[:, :3] - a[:, :3], c[:, :3] - a[:, :3])
-            return v / np.linalg.norm(v, axis=1, keepdims=True)
 
-        n0 = unit_normal(p00, p10, p11)
-        n2 = unit_normal(p11, p01, p00)
-        diag = p10[:, :3] - p01[:, :3]
-        planar = (np.abs((n0 * diag).sum(1)) <= 0.5) & (np.abs((n2 * diag).sum(1)) <= 0.5)
-        out = []
-        quad = np.stack([p00, p10, p11, p01], axis=1)
-        out.append(quad[planar])
-        np_ = ~planar
-        out.append(np.stack([p00[np_], p10[np_], p11[np_], p00[np_]], axis=1))
-        out.append(np.stack([p00[np_], p11[np_], p01[np_], p00[np_]], axis=1))
-        return np.concatenate(out, axis=0).astype(np.float32)
+def heightfield_triangles(hf: Heightfield) -> tuple[np.ndarray, np.ndarray]:
+    """Indexed triangle mesh of a heightfield: vertex i * n + j at
+    (i * spacing, h[i, j], j * spacing, 1); each cell split along its
+    (i, j)-(i+1, j+1) diagonal into two triangles, cell by cell.  Triangles
+    are wound as the reference's index buffers are: opposite to the quads
+    QuadDecomposition emits, which write a pair (i0, i1, i2) + apex as
+    (i0, i2, i1, apex) (QuadDecomposition.cpp:420-427)."""
+    n, s = hf.n, hf.spacing
+    g = np.arange(n, dtype=np.float64) * s
+    X, Z = np.meshgrid(g, g, indexing="ij")
+    verts = np.stack([X.ravel(), hf.h.astype(np.float64).ravel(), Z.ravel(), np.ones(n * n)], axis=1)
+    ij = np.arange(n * n, dtype=np.uint32).reshape(n, n)
+    p00, p10, p11, p01 = ij[:-1, :-1].ravel(), ij[1:, :-1].ravel(), ij[1:, 1:].ravel(), ij[:-1, 1:].ravel()
+    tri = np.stack([np.stack([p00, p11, p10], 1), np.stack([p00, p01, p11], 1)], 1).reshape(-1, 3)
+    return tri.astype(np.uint32), verts.astype(np.float32)
+
+
+def box_mesh(center, half) -> tuple[np.ndarray, np.ndarray]:
+    """An axis-aligned box as an indexed mesh: 8 vertices, 12 triangles wound
+    so that QuadDecomposition pairs each face's two triangles back into
+    box_quads' outward-facing quads."""
+    q = box_quads(center, half)
+    verts, index = [], {}
+    tri = []
+    for f in range(6):
+        ids = []
+        for k in range(4):
+            key = tuple(q[f, k, :3].tolist())
+            if key not in index:
+                index[key] = len(verts)
+                verts.append(q[f, k])
+            ids.append(index[key])
+        a, b, c, d = ids
+        tri += [(a, c, b), (a, d, c)]
+    return np.array(tri, np.uint32), np.array(verts, np.float32)
+
+
+def triangulated_boxes(rng: np.random.Generator, n: int) -> tuple[np.ndarray, np.ndarray]:
+    """n random boxes (centres U[-50,50]^3, half-extents U[1,5]) as one
+    indexed triangle mesh."""
+    tris, verts, base = [], [], 0
+    for _ in range(n):
+        t, v = box_mesh(rng.uniform(-50, 50, 3), rng.uniform(1, 5, 3))
+        tris.append(t + base)
+        verts.append(v)
+        base += v.shape[0]
+    return np.concatenate(tris).astype(np.uint32), np.concatenate(verts).astype(np.float32)
+
+
+def QuadDecomposition_indices(tri: np.ndarray, verts: np.ndarray) -> np.ndarray:
+    """QuadDecomposition::decompose (the library's rz_quad_decompose): the
+    quads' vertex indices, (m, 4)."""
+    from . import QuadDecomposition
+
+    return QuadDecomposition.decompose(tri, verts).reshape(-1, 4).astype(np.int64)
+
+
+def decompose_mesh(tri: np.ndarray, verts: np.ndarray) -> np.ndarray:
+    """QuadDecomposition::decompose of an indexed triangle mesh, as quads
+    (m, 4, 4) of the mesh's vertices."""
+    return np.ascontiguousarray(np.asarray(verts, np.float32)[QuadDecomposition_indices(tri, verts)])
 
 
 def catmull_rom(points: np.ndarray, samples: int):
@@ -356,7 +398,9 @@ def config2(seed: int = 1, frames: int = 256, n_queries: int = 65536, grid: int 
             width: int = 1920, height: int = 1080):
     """Procedural city on terrain: grid x grid box buildings (footprint 8x8 on
     a 16-unit pitch, heights log-U[5,80]) on a terrain_n^2 fBm heightfield
-    (amplitude 20, 4 octaves, 2-unit spacing), ~286k quads at the defaults; a
+    (amplitude 20, 4 octaves, 2-unit spacing), triangulated as one indexed
+    mesh and quad-decomposed (QuadDecomposition::decompose), ~286k quads at
+    the defaults, baked against the vertex buffer's box (Main.cpp:86-128); a
     seeded Catmull-Rom path along the streets at eye height 2 (near plane 1,
     so terrain quads cross it); query boxes are the building bounds plus
     small props on the terrain."""
@@ -366,7 +410,9 @@ def config2(seed: int = 1, frames: int = 256, n_queries: int = 65536, grid: int 
     extent = (terrain_n - 1) * spacing
     pitch = extent / grid
     heights = rng.log_uniform(grid * grid, 5.0, 80.0)
-    bq, bb = [], []
+    terrain_tri, terrain_v = heightfield_triangles(hf)
+    tris, verts, bb = [terrain_tri], [terrain_v], []
+    vbase = terrain_v.shape[0]
     for i in range(grid):
         for j in range(grid):
             cx, cz = (i + 0.5) * pitch, (j + 0.5) * pitch
@@ -376,11 +422,19 @@ def config2(seed: int = 1, frames: int = 256, n_queries: int = 65536, grid: int 
             hgt = float(heights[i * grid + j])
             c = np.array([cx, base + 0.5 * hgt, cz])
             h = np.array([4.0, 0.5 * hgt, 4.0])
-            bq.append(box_quads(c, h))
+            t, v = box_mesh(c, h)
+            tris.append(t + np.uint32(vbase))
+            verts.append(v)
+            vbase += v.shape[0]
             bb.append((c, h))
-    terrain = hf.quads()
-    quads = np.concatenate([terrain] + bq, axis=0)
-    scene = bake_scene(quads)
+    # The whole scene as one indexed triangle mesh, quad-decomposed and baked
+    # as Main.cpp:86-128 prepares its scene.
+    tri = np.concatenate(tris).astype(np.uint32)
+    vbuf = np.concatenate(verts).astype(np.float32)
+    qidx = QuadDecomposition_indices(tri, vbuf)
+    quads = vbuf[qidx]
+    terrain_quads = int((qidx < terrain_v.shape[0]).all(axis=1).sum())
+    scene = bake_scene(quads, vertex_buffer=vbuf)
 
     # Query boxes: building bounds, then props on the terrain.
     nb = len(bb)
@@ -416,7 +470,7 @@ def config2(seed: int = 1, frames: int = 256, n_queries: int = 65536, grid: int 
         d[1] = -0.08
         mvps[f] = view_projection(eyes[f], d, (0, 1, 0), math.pi / 3, width / height)
     scene.meta.update(width=width, height=height, eyes=eyes, mvps=mvps, queries=queries, seed=seed,
-                      terrain_quads=int(terrain.shape[0]), building_quads=int(6 * nb))
+                      terrain_quads=terrain_quads, building_quads=int(quads.shape[0] - terrain_quads))
     return scene
 
 

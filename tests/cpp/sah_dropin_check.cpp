@@ -13,6 +13,10 @@
 #include <exception>
 #include <vector>
 
+// A forward declaration, as code written against the reference's header may
+// carry: the drop-in's SurfaceAreaHeuristic is a class, so it must agree.
+class SurfaceAreaHeuristic;
+
 #include "SurfaceAreaHeuristic.h"
 
 struct Aabb {

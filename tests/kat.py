@@ -95,3 +95,51 @@ def unpack_block_linear(depth: np.ndarray, width: int, height: int) -> np.ndarra
 def pack_block_linear(img: np.ndarray) -> np.ndarray:
     h, w = img.shape
     return np.ascontiguousarray(img.reshape(h // 8, 8, w // 8, 8).transpose(0, 2, 1, 3).reshape(-1))
+
+
+def vertical_edge_rects(width: int, height: int, pixels, heights=(5.3, 12.7), dist: float = 100.0):
+    """Screen-facing rectangles much taller than the screen with one edge on
+    screen, exactly vertical in screen space (both of its vertices project to
+    the same x'), so that edge's normal has ny == 0 exactly.  For each pixel
+    column p in `pixels` and each height (in screen heights), one rectangle
+    whose right edge (normal +x) and one whose left edge (normal -x) crosses
+    column p; the other vertical edge is far off screen, so the on-screen
+    edge's offsets are folded across every block column from the buffer's
+    left border.  Returns a list of (quads (1, 4, 4), pixel, edge side)."""
+    half_w = dist * math.tan(math.pi / 6) * (width / height)
+    half_h = dist * math.tan(math.pi / 6)
+    px = 2.0 * half_w / (width - 8.0)  # one x' unit in world units at dist
+    out = []
+    for p in pixels:
+        x = -half_w + (p + 0.37) * px
+        for hm in heights:
+            y = hm * half_h
+            out.append((facing_rect(-4.0 * half_w, x, -y, y, dist), p, "right"))
+            out.append((facing_rect(x, 4.0 * half_w, -y, y, dist), p, "left"))
+    return out
+
+
+def vertical_edge_partials(o, bake_fn, width: int, height: int, pixels):
+    """For each rectangle of vertical_edge_rects: its on-screen vertical
+    edge's per-block offsets, folded from the rectangle's first block column
+    in float32 as the traversal folds them (Rasterizer.cpp:1136-1146: one add
+    of the edge normal's x per block), from the setup of the oracle
+    rasterizer `o` (which gets the camera's matrix), and the blocks where
+    they are partial (0 <= offset < 63).  Yields (occluder, slope row 2q,
+    [(block x, offset)])."""
+    mvp, _ = camera(width, height)
+    o.setModelViewProjection(mvp)
+    for quad, _, _ in vertical_edge_rects(width, height, pixels):
+        occ = occluder_from_quads(bake_fn, [quad])
+        d = o.setup_dump(occ, False)[0]
+        assert d[0] == 1 and d[1] == 6, "a screen-facing rectangle is a valid Convex quad"
+        min_x, range_x = int(d[2]), int(d[4])
+        off, nx, ny = d[8:12].astype(np.float32), d[12:16].astype(np.float32), d[16:20].astype(np.float32)
+        for e in range(4):
+            if ny[e] != 0 or range_x < 1:
+                continue
+            seq = np.add.accumulate(np.concatenate([[off[e]], np.full(range_x - 1, nx[e], np.float32)]),
+                                    dtype=np.float32)
+            part = np.flatnonzero((seq >= 0) & (seq < 63))
+            # nx * mul + add is 31.5 (+x) or 0.5 (-x) before truncation.
+            yield occ, (62 if nx[e] > 0 else 0), [(min_x + int(j), float(seq[j])) for j in part]

@@ -78,6 +78,36 @@ def test_damaged_and_foreign_files_are_rejected(tmp_path):
     assert e.value.status == -8
 
 
+def _fnv1a64(data: bytes) -> int:
+    h = 0xCBF29CE484222325
+    for byte in data:
+        h = ((h ^ byte) * 0x100000001B3) & 0xFFFFFFFFFFFFFFFF
+    return h
+
+
+def test_frame_header_sizes_are_bounded(tmp_path):
+    """A well-formed file (magic, version, size, checksum all valid) whose
+    header claims a 2400000008 x 3783947488 buffer and 4110202492 queries:
+    12 + blocks*128 + (blocks+8)*2 + queries*2 wraps in 64 bits to exactly the
+    76-byte payload.  It must be rejected before any size is trusted."""
+    import struct
+
+    w, h, nq = 8 * 300000001, 8 * 472993436, 4110202492
+    nb = (w // 8) * (h // 8)
+    payload = struct.pack("<III", w, h, nq) + bytes(64)
+    assert (12 + nb * 128 + (nb + 8) * 2 + nq * 2) % (1 << 64) == len(payload)
+    data = b"RZFRM\0\0\0" + struct.pack("<IIQ", 1, 0, len(payload)) + payload + struct.pack("<Q", _fnv1a64(payload))
+    p = tmp_path / "forged.rzf"
+    p.write_bytes(data)
+    with pytest.raises(RasterizerError) as e:
+        load_frame(p)
+    assert e.value.status == -8
+    # The same header with a real size is accepted (the checksum path is right).
+    good = struct.pack("<III", 8, 8, 0) + bytes(128 + 9 * 2)
+    p.write_bytes(b"RZFRM\0\0\0" + struct.pack("<IIQ", 1, 0, len(good)) + good + struct.pack("<Q", _fnv1a64(good)))
+    assert load_frame(p)["width"] == 8
+
+
 def test_golden_frame_matches_oracle():
     """The committed frame dump for this host's RCPPS, made by the oracle, is
     what the oracle computes here."""

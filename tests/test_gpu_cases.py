@@ -260,3 +260,33 @@ def test_degenerate_reference_box_matches_oracle(oracle_mod, report):
     assert valid.any() and np.all((rec["modeZ"][valid] >> 16) == 0)
     _, hz = V.raw(0)
     assert np.all(hz[:128] == 1)
+
+
+def test_vertical_edges_wide_buffer(oracle_mod, report):
+    """Exactly vertical edges (edge normal ny == 0) across a 32768x16 buffer,
+    where the float32 fold of their offsets over thousands of block columns
+    reaches the offset buckets at which the slope rule of Rasterizer.cpp:484
+    (ny <= 0) and the table build's (ny < 0) select different rows.  Each
+    rectangle is drawn on both sides from a cleared buffer; the sweep's
+    rectangles that reach such a bucket on this host (RCPPS sets the edge
+    normal's length) are counted and reported."""
+    w, h = 32768, 16
+    g, o = _pair(oracle_mod, w, h)
+    t = o.table().reshape(64, 64)
+    differ = {b for b in range(64) if t[62, b] != t[63, b] or t[0, b] != t[1, b]}
+    decisive = drawn = 0
+    for occ, _, parts in kat.vertical_edge_partials(o, oracle_mod.bake, w, h, range(w - 3000, w - 4, 25)):
+        hit = any(int(v) in differ for _, v in parts)
+        decisive += hit
+        if not hit and drawn % 8:
+            drawn += 1
+            continue
+        drawn += 1
+        prod = Occluder(occ.packets, occ.m_refMin, occ.m_refMax, occ.m_boundsMin, occ.m_boundsMax, occ.m_center)
+        for r in (g, o):
+            r.clear()
+        o.rasterize(occ, False)
+        g.rasterize(prod, False)
+        _same(g, o, report, f"vertical edge at blocks {[bx for bx, _ in parts]}")
+    print(f"vertical edges: {drawn} rectangles, {decisive} reach a bucket where the two slope rules differ",
+          flush=True)

@@ -133,6 +133,13 @@ def test_prepare_scene_arguments(rast):
         with pytest.raises(RasterizerError):
             rast.prepare_scene_device(quads, target, gran)
     assert rast.prepare_scene_device(quads[:0]) == []
+    # Vertices the SAH costs cannot take: NaN, infinite or beyond 2^40.
+    for bad in (np.nan, np.inf, -np.inf, 2.0 ** 41):
+        q = quads[:40].copy()
+        q[17, 2, 1] = bad
+        with pytest.raises(RasterizerError) as e:
+            rast.prepare_scene_device(q)
+        assert e.value.status == -1, bad
     # 5 quads pad to 8, one batch in input order on both sides.
     _same_occluders(rast.prepare_scene_device(quads[:5]), scenes.bake_scene(quads[:5]).occluders)
 

@@ -697,6 +697,91 @@ def test_bake_box_order_and_roundtrip(oracle_mod):
     assert np.array_equal(cen, (bmax + bmin) * np.float32(0.5))
 
 
+_CENTROIDS0 = np.array([[1, 0, 0], [0, 1, 0], [0, 0, 1], [0, -1, 0], [0, 0, -1], [-1, 0, 0]], np.float64)
+
+
+def _kmeans_f64(quads: np.ndarray, margin: float):
+    """Lloyd's iterations of Occluder.cpp:11-93 in float64 from the mathematics:
+    a quad's normal is the normalised sum of its triangles' normals (v1-v0)x(v2-v0)
+    and (v2-v0)x(v3-v0); start from the six axis centroids +x, +y, +z, -y, -z, -x;
+    assign each quad to the centroid of largest dot product; replace each
+    centroid by its members' normalised normal sum; stop after 10 rounds or a
+    round that changes nothing; then list the quads group by group, each group
+    in quad order.  An empty group's centroid is 0 * rsqrt(0) = NaN, and a
+    NaN dot product never passes COMISS's >= (unordered compares false, Intel
+    SDM), so it never wins a quad.  Returns (order, rounds, groups used), or None when some
+    assignment's best dot product beats the runner-up by less than `margin`,
+    where the float32 arithmetic (RSQRTPS: relative error <= 1.5 * 2^-12) could
+    decide otherwise."""
+    v = quads.astype(np.float64)
+    nrm = np.cross(v[:, 1, :3] - v[:, 0, :3], v[:, 2, :3] - v[:, 0, :3]) + \
+        np.cross(v[:, 2, :3] - v[:, 0, :3], v[:, 3, :3] - v[:, 0, :3])
+    nrm /= np.linalg.norm(nrm, axis=1, keepdims=True)
+    cent = _CENTROIDS0.copy()
+    assign = np.zeros(len(v), np.int64)
+    rounds = 0
+    for _ in range(10):
+        rounds += 1
+        dots = nrm @ cent.T  # (quads, 6); NaN rows of empty groups
+        dots = np.where(np.isnan(dots), -np.inf, dots)
+        new = dots.argmax(axis=1)
+        top = np.sort(dots, axis=1)
+        if np.any(top[:, -1] - top[:, -2] < margin):
+            return None
+        changed = np.any(new != assign)
+        assign = new
+        for k in range(6):
+            s = nrm[assign == k].sum(axis=0)
+            cent[k] = s / np.linalg.norm(s) if np.any(assign == k) else np.nan
+        if not changed:
+            break
+    order = np.concatenate([np.flatnonzero(assign == k) for k in range(6)])
+    return order, rounds, len(set(assign.tolist()))
+
+
+def test_bake_kmeans_general_normals(oracle_mod):
+    """Occluder::bake's group order (Occluder.cpp:11-93) for quads of random
+    orientation (not planar), against the float64 Lloyd's iterations above.  Cases where
+    any assignment is within 3e-3 of a tie are redrawn, so the order is the
+    mathematics' and not a rounding accident.  The accepted cases must include
+    ones that needed several rounds (the centroid updates matter) and ones
+    with empty groups."""
+    rng = np.random.default_rng(2024)
+    accepted = multi_round = empty = 0
+    tries = 0
+    while accepted < 300:
+        tries += 1
+        assert tries < 20000, "too few cases clear of the decision boundaries"
+        nq = int(rng.integers(1, 48))
+        # Planar quads around random centres, normals biased towards a few
+        # random directions so that groups drift away from the axes.
+        dirs = rng.normal(size=(3, 3))
+        n = dirs[rng.integers(0, 3, nq)] + 0.6 * rng.normal(size=(nq, 3))
+        n /= np.linalg.norm(n, axis=1, keepdims=True)
+        a = np.cross(n, rng.normal(size=(nq, 3)))
+        a /= np.linalg.norm(a, axis=1, keepdims=True)
+        b = np.cross(n, a)
+        c = rng.uniform(-20, 20, (nq, 3))
+        sa, sb = rng.uniform(0.5, 4, (nq, 1)), rng.uniform(0.5, 4, (nq, 1))
+        # The last corner leaves the plane, so that the two triangles'
+        # normals differ and their sum (not either one) decides the group.
+        lift = rng.uniform(-2.0, 2.0, (nq, 1)) * n
+        corners = [c - sa * a - sb * b, c + sa * a - sb * b, c + sa * a + sb * b, c - sa * a + sb * b + lift]
+        q = np.ones((nq, 4, 4), np.float32)
+        for k in range(4):
+            q[:, k, :3] = corners[k]
+        want = _kmeans_f64(q, 3e-3)
+        if want is None:
+            continue
+        order, rounds, used = want
+        got = oracle_mod.bake_order(q.reshape(-1, 4))
+        assert np.array_equal(got, order), f"case {accepted} ({nq} quads, {rounds} rounds): {got} vs {order}"
+        accepted += 1
+        multi_round += rounds >= 3
+        empty += used < 6
+    assert multi_round >= 30 and empty >= 30, (multi_round, empty)
+
+
 def test_bake_quantisation(oracle_mod):
     """Each vertex is stored as trunc((v - min) / extent * scale + 0.5) with
     scale 2047 / 2047 / 1023 (x biased by -1024), i.e. rounded to the nearest
@@ -762,3 +847,59 @@ def test_degenerate_reference_box_gives_x86_default_nan(oracle_mod):
     o.rasterize(occ, False)
     _, hz = o.raw()
     assert np.all(hz[:16 * 8] == 1)
+
+
+# Offset buckets (pixel distance from the block centre scaled by 70/8, plus
+# 32) at which an exactly vertical edge on the pixel grid meets its partial
+# block: 5.75 + 8.75 k.
+_VERTICAL_LATTICE = np.array([5.75 + 8.75 * k for k in range(7)])
+
+
+def test_vertical_edge_slope_rule(oracle_mod, table):
+    """Rasterizer.cpp:484: the raster's slope lookup sets the "ny negative"
+    bit of the table row for ny <= 0, where the table build (:472) uses
+    ny < 0.  Only an exactly vertical edge has ny == 0; it reads row 63 (normal
+    +x) or 1 (-x) where ny < 0 would read 62 or 0.  Those rows differ only at
+    a few offset buckets (found here from the table).  On the pixel grid a
+    vertical edge meets its partial block at offset 5.75 + 8.75 k, away from
+    all of them, and the float32 fold across the buffer moves it by less than
+    the distance to the nearest (1.75 buckets) for every buffer up to 7680
+    pixels wide: there the two rules give the same pixels, so the sub-rule
+    cannot change any result (the buffers of every workload here are at most
+    3840 wide).  Wider buffers accumulate more fold error; at 32768 pixels a
+    sweep reaches blocks where the rows differ, and there the oracle's pixels
+    must be row 2q + 1's (the reference text: nothing independent of it fixes
+    that case)."""
+    t = table.reshape(64, 64)
+    differ = [b for b in range(64) if t[62, b] != t[63, b] or t[0, b] != t[1, b]]
+    assert differ, "rows 2q and 2q + 1 differ somewhere"
+    gap = min(abs(lat - edge) for lat in _VERTICAL_LATTICE for b in differ for edge in (b, b + 1))
+    assert gap >= 1.75, gap
+    h = 16
+    for w in (256, 1920, 3840, 7680):
+        o = oracle_mod.OracleRasterizer(w, h)
+        n = 0
+        for _, _, parts in kat.vertical_edge_partials(o, oracle_mod.bake, w, h, range(4, w - 4, max(1, w // 500))):
+            for _, v in parts:
+                dev = float(np.abs(_VERTICAL_LATTICE - v).min())
+                assert dev < gap and int(v) not in differ, f"width {w}: partial offset {v}"
+                n += 1
+        assert n > 200, n
+    # Beyond the lattice margin: the oracle's own pixels at the decisive blocks.
+    w = 32768
+    o = oracle_mod.OracleRasterizer(w, h)
+    decisive = 0
+    for occ, row, parts in kat.vertical_edge_partials(o, oracle_mod.bake, w, h, range(w - 3000, w - 4, 5)):
+        hits = [(bx, v) for bx, v in parts if int(v) in differ]
+        if not hits:
+            continue
+        o.clear()
+        o.rasterize(occ, False)
+        img = kat.unpack_block_linear(o.raw()[0], w, h)
+        for bx, v in hits:
+            for by in range(h // 8):  # the horizontal edges are far: their entries are full
+                got = img[8 * by:8 * by + 8, 8 * bx:8 * bx + 8] != 0
+                assert np.array_equal(got, _mask_bits(int(t[row + 1, int(v)]))), f"block {bx}: ny <= 0 row"
+                assert not np.array_equal(got, _mask_bits(int(t[row, int(v)])))
+                decisive += 1
+    assert decisive > 0, "no vertical edge reached a bucket where the rows differ"
