@@ -101,6 +101,24 @@ int rz_draw_frame(rz_context* ctx, uint32_t n, const rz_occluder* const* occlude
  * returns RZ_ERR_TIMEOUT. */
 int rz_set_wait_timeout(rz_context* ctx, uint64_t ticks);
 
+/* rz_draw_frame without any allocation.  No counterpart in the reference:
+ * for callers that cannot allocate device or pinned memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * draw's working buffers (item descriptors and states, query done words,
+ * quad setup records and boxes) live in one static device array of the
+ * library, about 47 MB, that each GPU the library runs on holds until the
+ * process ends.  Limits: n <= RZ_NOALLOC_MAX_ITEMS occluders and at most
+ * RZ_NOALLOC_MAX_QUADS quads (8 per batch) in total, else RZ_ERR_TOO_LARGE
+ * and nothing is enqueued.  Every call on one GPU shares that array: calls
+ * whose draws overlap on one GPU (on different contexts or host threads)
+ * race, so order them on the device (one stream, or events).  d_statusOut
+ * (optional) is DEVICE memory of n bytes, written with rz_draw_frame's
+ * statuses when the stream reaches the draw.  Results are identical to
+ * rz_draw_frame's. */
+#define RZ_NOALLOC_MAX_ITEMS 4096u
+#define RZ_NOALLOC_MAX_QUADS 524288u
+int rz_draw_frame_noalloc(rz_context* ctx, uint32_t n, const rz_occluder* const* occluders, uint8_t* d_statusOut);
+
 /* Rasterizer::queryVisibility(__m128 boundsMin, __m128 boundsMax, bool& needsClipping)
  * (Rasterizer.h:22, Rasterizer.cpp:123-281).  Blocks.  *needsClipping is
  * written only when the box passes the frustum test, as in the reference. */

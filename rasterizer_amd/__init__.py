@@ -73,6 +73,7 @@ _SIGNATURES = {
     "rz_occluder_destroy": ([_P], ctypes.c_int),
     "rz_rasterize": ([_P, _P, ctypes.c_int], ctypes.c_int),
     "rz_draw_frame": ([_P, _U32, _P, _P], ctypes.c_int),
+    "rz_draw_frame_noalloc": ([_P, _U32, _P, _P], ctypes.c_int),
     "rz_set_wait_timeout": ([_P, ctypes.c_uint64], ctypes.c_int),
     "rz_query": ([_P, _P, _P, _I32P, _I32P], ctypes.c_int),
     "rz_query2d": ([_P, _U32, _U32, _U32, _U32, _U32, _I32P], ctypes.c_int),

@@ -157,6 +157,8 @@ static void free_context(rz_context* c) {
   delete c;
 }
 
+static int noalloc_init(int device);
+
 extern "C" int rz_create(uint32_t width, uint32_t height, rz_context** out) {
   if (!out) return RZ_ERR_INVALID_ARGUMENT;
   *out = nullptr;
@@ -179,6 +181,7 @@ extern "C" int rz_create(uint32_t width, uint32_t height, rz_context** out) {
     return code;
   };
   if (hipGetDevice(&c->device) != hipSuccess) return fail(RZ_ERR_DEVICE);
+  if (noalloc_init(c->device) != RZ_OK) return fail(RZ_ERR_DEVICE);
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return fail(RZ_ERR_DEVICE);
   const size_t nBlocks = size_t(c->bX) * c->bY;
   c->pyrX = (c->bX + 7) / 8;
@@ -548,6 +551,75 @@ extern "C" int rz_rasterize(rz_context* ctx, const rz_occluder* occluder, int po
 extern "C" int rz_draw_frame(rz_context* ctx, uint32_t n, const rz_occluder* const* occluders, uint8_t* statusOut) {
   if (!ctx || (n && !occluders)) return RZ_ERR_INVALID_ARGUMENT;
   return enqueue_draw(ctx, n, occluders, nullptr, statusOut);
+}
+
+// ---------------------------------------------------------------------------
+// rz_draw_frame without allocation.  The per-draw buffers (item descriptors
+// and states, done and visibility words, quad setup records, quad and group
+// boxes) live in `scratch`, a static device array that every GPU the library
+// runs on holds for the life of the process.  Its address on each device is
+// taken once, with hipGetSymbolAddress, when the first context is created on
+// that device; host code uses only that address, never the array's name.
+// ---------------------------------------------------------------------------
+namespace {
+
+constexpr size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
+constexpr size_t kNaItemsOff = 0;
+constexpr size_t kNaStateOff = kNaItemsOff + align256(sizeof(ItemDesc) * RZ_NOALLOC_MAX_ITEMS);
+constexpr size_t kNaDoneOff = kNaStateOff + align256(sizeof(ItemState) * RZ_NOALLOC_MAX_ITEMS);
+constexpr size_t kNaVisibleOff = kNaDoneOff + align256(sizeof(uint32_t) * RZ_NOALLOC_MAX_ITEMS);
+constexpr size_t kNaRecsOff = kNaVisibleOff + align256(sizeof(uint32_t) * RZ_NOALLOC_MAX_ITEMS);
+constexpr size_t kNaBboxOff = kNaRecsOff + align256(sizeof(QuadRec) * size_t(RZ_NOALLOC_MAX_QUADS));
+constexpr size_t kNaGroupOff = kNaBboxOff + align256(sizeof(uint64_t) * size_t(RZ_NOALLOC_MAX_QUADS));
+constexpr size_t kNaBytes = kNaGroupOff + align256(sizeof(uint64_t) * size_t(RZ_NOALLOC_MAX_QUADS / 64));
+constexpr int kNaMaxDevices = 64;
+
+std::mutex g_naMutex;
+void* g_naScratch[kNaMaxDevices] = {};  // scratch's address on each device, once known
+ItemDesc g_naItems[RZ_NOALLOC_MAX_ITEMS];  // host staging of the item descriptors (guarded by g_naMutex)
+
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[kNaBytes];
+
+// Takes scratch's address on the current device (idempotent per device).
+static int noalloc_init(int device) {
+  if (device < 0 || device >= kNaMaxDevices) return RZ_ERR_DEVICE;
+  std::lock_guard<std::mutex> lock(g_naMutex);
+  if (g_naScratch[device]) return RZ_OK;
+  void* ptr = nullptr;
+  if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr) return RZ_ERR_DEVICE;
+  g_naScratch[device] = ptr;
+  return RZ_OK;
+}
+
+extern "C" int rz_draw_frame_noalloc(rz_context* ctx, uint32_t n, const rz_occluder* const* occluders,
+                                     uint8_t* d_statusOut) {
+  if (!ctx || (n && !occluders)) return RZ_ERR_INVALID_ARGUMENT;
+  if (n == 0) return RZ_OK;
+  if (n > RZ_NOALLOC_MAX_ITEMS) return RZ_ERR_TOO_LARGE;
+  uint64_t totalQuads = 0;
+  int s = count_quads(n, occluders, &totalQuads);
+  if (s != RZ_OK) return s;
+  if (totalQuads > RZ_NOALLOC_MAX_QUADS) return RZ_ERR_TOO_LARGE;
+  if (ctx->device < 0 || ctx->device >= kNaMaxDevices) return RZ_ERR_DEVICE;
+  std::lock_guard<std::mutex> lock(g_naMutex);
+  unsigned char* base = static_cast<unsigned char*>(g_naScratch[ctx->device]);
+  if (!base) return RZ_ERR_DEVICE;  // set by rz_create on this device
+  const DrawBufs bufs{reinterpret_cast<ItemDesc*>(base + kNaItemsOff), reinterpret_cast<ItemState*>(base + kNaStateOff),
+                      reinterpret_cast<uint32_t*>(base + kNaDoneOff), reinterpret_cast<uint32_t*>(base + kNaVisibleOff),
+                      reinterpret_cast<QuadRec*>(base + kNaRecsOff), reinterpret_cast<uint64_t*>(base + kNaBboxOff),
+                      reinterpret_cast<uint64_t*>(base + kNaGroupOff)};
+  fill_items(g_naItems, n, occluders, nullptr);
+  // From pageable memory: the copy has consumed g_naItems when it returns.
+  RZ_HIP(hipMemcpyAsync(bufs.items, g_naItems, n * sizeof(ItemDesc), hipMemcpyHostToDevice, ctx->stream));
+  // The prep kernel writes every state, done and visibility word of items
+  // 0..n-1 and the setup kernel every record and box of their quads, so
+  // nothing left in scratch by an earlier call is read.
+  s = enqueue_kernels(ctx, n, uint32_t(totalQuads), bufs, nullptr);
+  if (s != RZ_OK) return s;
+  if (d_statusOut) RZ_HIP(launch_status(bufs.state, bufs.done, n, d_statusOut, ctx->stream));
+  return RZ_OK;
 }
 
 extern "C" int rz_set_wait_timeout(rz_context* ctx, uint64_t ticks) {
