@@ -175,16 +175,29 @@ def views_bench(steps: int, warmup: int, seed: int = 3) -> dict:
     t_b, gpu_b = time_batched("batched, occluder order")
     cnt_b = block_counters()
     # Learned order: the trained MLP scores every batch each step (features
-    # of this step's setup, labels of the previous step); occluders are drawn
-    # by their best batch's score.
+    # of this step's setup, with or without labels of the previous step);
+    # occluders are drawn by their best batch's score.  Two models: trained
+    # without the previous-label feature (so the draw tracks no labels), and
+    # with it.
     from rasterizer_amd import learn
 
-    model = learn.load()
-    V.set_model(model, by_occluder=True)
-    t_l, gpu_l = time_batched("batched, learned order")
-    cnt_l = block_counters()
-    log(f"reject rate: occluder order {cnt_b['reject_rate']:.4f}, learned {cnt_l['reject_rate']:.4f}")
-    V.set_model(None)
+    learned = {}
+    for name, path in (("without labels", learn.WEIGHTS.with_name("order_mlp_nolabels.npz")),
+                       ("with labels", learn.WEIGHTS)):
+        V.set_model(learn.load(path), by_occluder=True)
+        t_l, gpu_l = time_batched(f"batched, learned order ({name})")
+        cnt_l = block_counters()
+        log(f"reject rate: occluder order {cnt_b['reject_rate']:.4f}, learned ({name}) {cnt_l['reject_rate']:.4f}")
+        V.set_model(None)
+        learned[name] = {
+            "granularity": "occluder (best batch score)",
+            "weights": str(path.relative_to(ROOT)),
+            "quads_per_s": steps * total_quads / t_l,
+            "queries_per_s": steps * total_q / t_l,
+            "ms_per_step": t_l * 1e3 / steps,
+            "gpu_ms_per_step": gpu_l,
+            "block_counters": cnt_l,
+        }
 
     # Plain: one context per view, the single-view frame path.
     ctxs = [Rasterizer(*v["size"]) for v in views]
@@ -228,16 +241,8 @@ def views_bench(steps: int, warmup: int, seed: int = 3) -> dict:
         "steps": steps,
         "plain_steps": plain_steps,
         "launch_config": tuned,
-        "learned_order": {
-            "granularity": "occluder (best batch score)",
-            "weights": "rasterizer_amd/order_mlp.npz",
-            "quads_per_s": steps * total_quads / t_l,
-            "queries_per_s": steps * total_q / t_l,
-            "ms_per_step": t_l * 1e3 / steps,
-            "gpu_ms_per_step": gpu_l,
-            "block_counters": cnt_l,
-            "speedup_vs_plain": (steps * total_quads / t_l) / plain_qps,
-        },
+        "learned_order": {name: {**d, "speedup_vs_plain": d["quads_per_s"] / plain_qps}
+                          for name, d in learned.items()},
     }
 
 
@@ -288,9 +293,13 @@ def stress_bench(steps: int, warmup: int, seed: int = 5) -> dict:
         return t
 
     t_b = timed("batched, occluder order")
-    V.set_model(learn.load(), by_occluder=True)
-    t_l = timed("batched, learned order")
-    V.set_model(None)
+    t_learned = {}
+    for name, path in (("without labels", learn.WEIGHTS.with_name("order_mlp_nolabels.npz")),
+                       ("with labels", learn.WEIGHTS)):
+        V.set_model(learn.load(path), by_occluder=True)
+        t_learned[name] = timed(f"batched, learned order ({name})")
+        V.set_model(None)
+    t_l = t_learned["without labels"]
 
     handles = [ctx[sizes[k]].handle_array([sc.occluders[i] for i in orders[k]]) for k in range(nv)]
     dq = [(DeviceBuffer(ctx[sizes[k]], v["queries"].nbytes, v["queries"]),
@@ -325,6 +334,8 @@ def stress_bench(steps: int, warmup: int, seed: int = 5) -> dict:
         "batched_ms_per_step": t_b * 1e3 / steps,
         "learned_quads_per_s": steps * total_quads / t_l,
         "learned_ms_per_step": t_l * 1e3 / steps,
+        "learned_model": "rasterizer_amd/order_mlp_nolabels.npz",
+        "learned_with_labels_ms_per_step": t_learned["with labels"] * 1e3 / steps,
         "plain_quads_per_s": plain_qps,
         "plain_ms_per_step": t_p * 1e3 / plain_steps,
         "speedup_vs_plain": (steps * total_quads / t_b) / plain_qps,
@@ -333,6 +344,93 @@ def stress_bench(steps: int, warmup: int, seed: int = 5) -> dict:
         "steps": steps,
         "plain_steps": plain_steps,
         "launch_config": tuned,
+    }
+
+
+def learned_frames_in_flight(R, scene, orders, handles, plans, F: int, tuned: dict, t_plain: float,
+                             plain_value: float):
+    """The frames-in-flight launches again, each frame's occluders ordered by
+    the city-trained MLP (whole occluders by their best batch's score, label
+    tracking off) instead of front to back, still query-gated: the batched +
+    learned path at 1920x1080.  Frames 0 and F-1 are checked bit for bit
+    against the single-view path drawing the occluders in the order the
+    device chose.  Reports quads/s (submitted real quads), occluders drawn
+    per frame against the front-to-back launches, and the time."""
+    from rasterizer_amd import learn
+
+    path = learn.WEIGHTS.with_name("order_mlp_city.npz")
+    if not path.exists():
+        return None
+    model = learn.load(path)
+    m = scene.meta
+    W, H = m["width"], m["height"]
+    nf = m["mvps"].shape[0]
+    n_occ = len(scene.occluders)
+    K = F * len(plans)
+
+    def drawn(V):
+        return sum(int((V.status(v) > 0).sum()) for v in range(F))
+
+    plain_drawn = [drawn(Vj) for Vj in plans]
+    for Vj in plans:
+        Vj.set_model(model, by_occluder=True)
+        Vj.set_config(tuned["chosen"]["waves"], tuned["chosen"]["groupsPerCU"], tuned["chosen"]["tileW"])
+        Vj.run(graph=False)
+        Vj.sync()
+    # Check the first launch's first and last frames (each drawn from the
+    # buffer its previous launch left, which the single-view path is given).
+    nb = (W // 8) * (H // 8)
+    V0 = plans[0]
+    before = [V0.raw(v) for v in sorted({0, F - 1})]
+    V0.run(graph=False)
+    V0.sync()
+    for (d0, h0), v in zip(before, sorted({0, F - 1})):
+        order = V0.order(v)
+        f = v % nf
+        occ = [scene.occluders[i] for i in orders[f]]
+        R.upload_raw(d0, h0)
+        R.clear()
+        R.setModelViewProjection(m["mvps"][f])
+        st = R.draw_frame([occ[i] for i in order])
+        want = np.empty_like(st)
+        want[order] = st
+        d, h = R.raw()
+        dv, hv = V0.raw(v)
+        if not (np.array_equal(d, dv) and np.array_equal(h, hv) and np.array_equal(V0.status(v), want)):
+            raise RuntimeError(f"batched + learned: frame {f} differs from the single-view path in the learned order")
+    for Vj in plans:
+        Vj.run(graph=False)
+    for Vj in plans:
+        Vj.sync()
+    t0 = time.perf_counter()
+    for Vj in plans:
+        Vj.run(graph=False)
+    for Vj in plans:
+        Vj.sync()
+    t = time.perf_counter() - t0
+    learned_drawn = [drawn(Vj) for Vj in plans]
+    phases = []
+    for Vj in plans:
+        Vj.profile(True)
+        Vj.run(graph=False)
+        phases.append(Vj.profile_read())
+        Vj.profile(False)
+    for Vj in plans:
+        Vj.set_model(None)
+    value = K * scene.quad_count / t
+    log(f"batched + learned (city model, gated, 1080p): {K} frames {t * 1e3:.3f} ms ({value / 1e6:.1f} M quads/s); "
+        f"occluders drawn {sum(learned_drawn)} vs front to back {sum(plain_drawn)}")
+    return {
+        "model": str(path.relative_to(ROOT)),
+        "quads_per_s": value,
+        "ms_per_frame": t * 1e3 / K,
+        "front_to_back_quads_per_s": plain_value,
+        "front_to_back_ms_per_frame": t_plain * 1e3 / K,
+        "occluders_drawn_per_frame": sum(learned_drawn) / K,
+        "front_to_back_occluders_drawn_per_frame": sum(plain_drawn) / K,
+        "occluders_per_frame": n_occ,
+        "raster_kernel_ms_per_launch": float(np.mean([p["raster_ms"] for p in phases])),
+        "setup_phase_ms_per_launch": float(np.mean([p["setup_ms"] for p in phases])),
     }
 
 
@@ -356,6 +454,7 @@ def main() -> int:
     ap.add_argument("--views-steps", type=int, default=32)
     ap.add_argument("--frames-in-flight", type=int, default=128, help="frames per persistent launch (at most)")
     ap.add_argument("--no-stress", action="store_true", help="skip the stress configuration")
+    ap.add_argument("--no-learned", action="store_true", help="skip the batched + learned frames-in-flight leg")
     ap.add_argument("--stress-steps", type=int, default=8)
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="first copy this many MiB device-to-device 3 times (a kernel of known traffic for "
@@ -451,7 +550,7 @@ def main() -> int:
     fif_tuned = plans[0].autotune(reps=2)
     log(f"frames in flight autotune: {fif_tuned}")
     for Vj in plans[1:]:
-        Vj.set_config(fif_tuned["chosen"]["waves"], fif_tuned["chosen"]["groupsPerCU"])
+        Vj.set_config(fif_tuned["chosen"]["waves"], fif_tuned["chosen"]["groupsPerCU"], fif_tuned["chosen"]["tileW"])
         Vj.run(graph=False)
         Vj.sync()
     for Vj in plans:  # warm-up launch of each plan (untimed)
@@ -476,6 +575,9 @@ def main() -> int:
     value = K * scene.quad_count / t_fif
     log(f"frames in flight: {K} frames in {launches} launch(es) of {F}: {t_fif * 1e3:.3f} ms "
         f"({value / 1e6:.1f} M quads/s); per launch: raster kernel {raster_ms:.3f} ms, setup phase {setup_ms:.3f} ms")
+
+    learned_fif = (None if args.no_learned else
+                   learned_frames_in_flight(R, scene, orders, handles, plans, F, fif_tuned, t_fif, value))
 
     # --- Single frame at a time (the plain single-view path).
     def raster_step(k: int) -> None:
@@ -546,6 +648,7 @@ def main() -> int:
                          "kernel_ms": kernel_ms,
                          "frame_kernel_hbm_gbs": alg_frame / (frame_ms * 1e-3) / 1e9},
         "query_ms_per_batch": t_query * 1e3 / K,
+        "batched_learned_1080p": learned_fif,
         "roofline": {
             "bound": "hbm",
             "achieved": achieved,

@@ -355,10 +355,15 @@ int rz_views_read_status(rz_views* views, uint32_t view, uint8_t* status);
  * Each view equals rz_draw_frame on a Rasterizer whose buffer holds what the
  * view's buffer held (clear leaves depth stale, and query2D reads stale depth
  * in partly covered cleared blocks, Rasterizer.cpp:283-349).  Tiles of a view wait for
- * each other's query answers, so a step needs every view's tiles (8x8-block
- * tiles) to fit co-resident (else run returns RZ_ERR_TOO_LARGE), is launched
- * cooperatively and is not captured into a graph; it cannot be combined with
- * a batch order or a model (RZ_ERR_INVALID_ARGUMENT).  0 restores
+ * each other's query answers, so a step needs every view's tiles to fit
+ * co-resident (a view too large for 8x8-block tiles moves the step to
+ * 16x16-block tiles; beyond those run returns RZ_ERR_TOO_LARGE), is launched
+ * cooperatively and is not captured into a graph.  It combines with a
+ * front-to-back order (rz_views_set_eye) and with a model that orders whole
+ * occluders (rz_views_set_model with byOccluder != 0): the query-gated loop
+ * then runs over the occluders in that order, one query per occluder as in
+ * the reference; batch orders, explicit or learned, would query per batch and
+ * are refused (RZ_ERR_INVALID_ARGUMENT at the next run).  0 restores
  * classify-then-draw. */
 int rz_views_set_query_gated(rz_views* views, int gated);
 /* Bound of each cross-tile wait of query-gated steps, in 100 MHz
@@ -387,23 +392,29 @@ int rz_views_profile_read(rz_views* views, float ms[3]);
 int rz_views_get_info(rz_views* views, uint32_t info[8]);
 
 /* Launch configuration of the multi-view raster kernel (no counterpart in the
- * reference): wave64s per (view, tile) work-group, 4 or 8, and persistent
- * work-groups per CU (0: as many as fit). */
+ * reference): the tile (bin) edge in 8x8 blocks, 8 (64-pixel bins) or 16
+ * (128-pixel bins), wave64s per (view, tile) work-group, 4 or 8, and
+ * persistent work-groups per CU (0: as many as fit).  Labelled steps (labels
+ * on, or a model that reads them) draw with 8x8-block tiles whatever the
+ * setting.  A query-gated step whose views' 8x8-block tiles cannot all be
+ * co-resident (2048^2, 3840x2160) draws with 16x16-block tiles by itself. */
 typedef struct rz_views_config {
+  uint32_t tileW;
   uint32_t waves;
   uint32_t groupsPerCU;
   float ms; /* rz_views_tune: median step time; -1 if it does not fit */
 } rz_views_config;
 int rz_views_get_config(rz_views* views, rz_views_config* config);
-int rz_views_set_config(rz_views* views, uint32_t waves, uint32_t groupsPerCU);
-/* Autotuner over waves {4, 8} x groupsPerCU {1, all that fit}: runs the
- * current step (matrices, occluders, queries, order) under each, requires
- * identical buffers, query bits and labels from all (else
+int rz_views_set_config(rz_views* views, uint32_t tileW, uint32_t waves, uint32_t groupsPerCU);
+/* Autotuner over tileW {8, 16} x waves {4, 8} x groupsPerCU {1, all that
+ * fit}: runs the current step (matrices, occluders, queries, order) under
+ * each, requires identical buffers, query bits and labels from all (else
  * RZ_ERR_NOT_IDENTICAL, configuration unchanged; configurations whose grid
- * cannot hold a query-gated view's tiles are skipped), times `reps` steps each,
- * applies the fastest and records it in cachePath (if not NULL) for this
- * device and set of view sizes.  Blocks. */
-int rz_views_tune(rz_views* views, uint32_t reps, const char* cachePath, rz_views_config candidates[4],
+ * cannot hold a query-gated view's tiles, and 16x16 tiles in labelled steps,
+ * are skipped), times `reps` steps each, applies the fastest and records it
+ * in cachePath (if not NULL) for this device and set of view sizes.
+ * candidates (optional) receives up to 8 entries.  Blocks. */
+int rz_views_tune(rz_views* views, uint32_t reps, const char* cachePath, rz_views_config candidates[8],
                   uint32_t* nCandidates);
 /* As rz_load_config; rz_views_create does it itself with RZ_TUNE_CACHE. */
 int rz_views_load_config(rz_views* views, const char* cachePath, int* found);
@@ -434,6 +445,11 @@ int rz_mlp_create(uint32_t seed, rz_mlp** out);
 int rz_mlp_destroy(rz_mlp* mlp);
 int rz_mlp_set_params(rz_mlp* mlp, const float* params, const float* norm);
 int rz_mlp_get_params(rz_mlp* mlp, float* params, float* norm);
+/* Whether the model reads feature 10, the batch's label from the previous
+ * step (default 1).  A model trained with that feature held at 0 is set to 0:
+ * multi-view steps ordered by it then feed 0 there and draw without tracking
+ * labels, which saves the labelled raster variant's cost. */
+int rz_mlp_set_uses_labels(rz_mlp* mlp, int usesLabels);
 /* Scores of n samples (16 floats each); host arrays. */
 int rz_mlp_forward(rz_mlp* mlp, uint32_t n, const float* x, float* scores);
 /* Mean pairwise ranking loss softplus(s_j - s_i) over pairs (i, j) (sample i

@@ -117,6 +117,7 @@ _SIGNATURES = {
     "rz_mlp_create": ([_U32, ctypes.POINTER(_P)], ctypes.c_int),
     "rz_mlp_destroy": ([_P], ctypes.c_int),
     "rz_mlp_set_params": ([_P, _P, _P], ctypes.c_int),
+    "rz_mlp_set_uses_labels": ([_P, ctypes.c_int], ctypes.c_int),
     "rz_mlp_get_params": ([_P, _P, _P], ctypes.c_int),
     "rz_mlp_forward": ([_P, _U32, _P, _P], ctypes.c_int),
     "rz_mlp_gradients": ([_P, _U32, _P, _U32, _P, _P, _P], ctypes.c_int),
@@ -132,7 +133,7 @@ _SIGNATURES = {
     "rz_tune_frame": ([_P, _U32, _P, _U32, ctypes.c_char_p, _P, _P], ctypes.c_int),
     "rz_load_config": ([_P, ctypes.c_char_p, _P], ctypes.c_int),
     "rz_views_get_config": ([_P, _P], ctypes.c_int),
-    "rz_views_set_config": ([_P, _U32, _U32], ctypes.c_int),
+    "rz_views_set_config": ([_P, _U32, _U32, _U32], ctypes.c_int),
     "rz_views_tune": ([_P, _U32, ctypes.c_char_p, _P, _P], ctypes.c_int),
     "rz_views_load_config": ([_P, ctypes.c_char_p, _P], ctypes.c_int),
     "rz_views_set_query_gated": ([_P, ctypes.c_int], ctypes.c_int),
@@ -162,8 +163,10 @@ class FrameConfig(ctypes.Structure):
 
 
 class ViewsConfig(ctypes.Structure):
-    """rz_views_config: wave64s per work-group, work-groups per CU (0: all that fit), tuned ms."""
-    _fields_ = [("waves", ctypes.c_uint32), ("groupsPerCU", ctypes.c_uint32), ("ms", ctypes.c_float)]
+    """rz_views_config: tile edge in 8x8 blocks, wave64s per work-group, work-groups per CU (0: all that
+    fit), tuned ms."""
+    _fields_ = [("tileW", ctypes.c_uint32), ("waves", ctypes.c_uint32), ("groupsPerCU", ctypes.c_uint32),
+                ("ms", ctypes.c_float)]
 
 
 def _cfg_dict(c) -> dict:
@@ -823,15 +826,15 @@ class Views:
     def config(self) -> dict:
         c = ViewsConfig()
         _check(lib().rz_views_get_config(self._v, ctypes.byref(c)), "rz_views_get_config")
-        return {"waves": c.waves, "groupsPerCU": c.groupsPerCU}
+        return {"tileW": c.tileW, "waves": c.waves, "groupsPerCU": c.groupsPerCU}
 
-    def set_config(self, waves: int, groups_per_cu: int = 0) -> None:
-        _check(lib().rz_views_set_config(self._v, waves, groups_per_cu), "rz_views_set_config")
+    def set_config(self, waves: int, groups_per_cu: int = 0, tile_w: int = 8) -> None:
+        _check(lib().rz_views_set_config(self._v, tile_w, waves, groups_per_cu), "rz_views_set_config")
 
     def autotune(self, reps: int = 5, cache: "str | os.PathLike | None" = None) -> dict:
         """rz_views_tune: times the current step under every configuration
         (identical bits required) and keeps the fastest."""
-        cands = (ViewsConfig * 4)()
+        cands = (ViewsConfig * 8)()
         k = ctypes.c_uint32(0)
         _check(lib().rz_views_tune(self._v, reps, os.fsencode(cache) if cache is not None else None, cands,
                                    ctypes.byref(k)), "rz_views_tune")
@@ -857,6 +860,7 @@ class Mlp:
     def __init__(self, seed: int = 1):
         self._m = ctypes.c_void_p()
         _check(lib().rz_mlp_create(seed, ctypes.byref(self._m)), "rz_mlp_create")
+        self.uses_labels = True
         _live[1].add(self)
 
     def close(self) -> None:
@@ -881,6 +885,12 @@ class Mlp:
         n = None if norm is None else np.ascontiguousarray(norm, np.float32).reshape(32)
         _check(lib().rz_mlp_set_params(self._m, _ptr(w) if w is not None else None,
                                        _ptr(n) if n is not None else None), "rz_mlp_set_params")
+
+    def set_uses_labels(self, uses: bool) -> None:
+        """False for a model trained with feature 10 (the previous step's
+        label) held at 0: ordered steps then skip the label tracking."""
+        self.uses_labels = bool(uses)
+        _check(lib().rz_mlp_set_uses_labels(self._m, 1 if uses else 0), "rz_mlp_set_uses_labels")
 
     def forward(self, x) -> np.ndarray:
         x = np.ascontiguousarray(x, np.float32).reshape(-1, 16)

@@ -35,6 +35,11 @@ hipError_t dev_alloc(T** p, size_t count) {
 
 namespace rzi {
 
+void report_hip_error(hipError_t error, const char* call, const char* file, int line) {
+  static const bool on = std::getenv("RZ_DEBUG") != nullptr;
+  if (on) std::fprintf(stderr, "rasterizer_amd: %s failed: %s (%s:%d)\n", call, hipGetErrorString(error), file, line);
+}
+
 // Both tables depend only on this host (libm, RCPPS); build them once.
 const HostTables& host_tables() {
   static HostTables t;
@@ -102,11 +107,6 @@ struct rz_context {
   int waves = 8;                     // wave64s per frame-kernel tile
   uint64_t timeoutTicks = kTimeoutTicks;  // bound of each cross-tile wait (rz_set_wait_timeout)
 };
-
-#define RZ_HIP(call)                           \
-  do {                                         \
-    if ((call) != hipSuccess) return RZ_ERR_DEVICE; \
-  } while (0)
 
 extern "C" const char* rz_status_string(int s) {
   switch (s) {

@@ -9,6 +9,15 @@
 
 #include "rasterizer_amd.h"
 
+#define RZ_HIP(call)                                                \
+  do {                                                              \
+    const hipError_t rz_hip_error_ = (call);                        \
+    if (rz_hip_error_ != hipSuccess) {                              \
+      rzi::report_hip_error(rz_hip_error_, #call, __FILE__, __LINE__); \
+      return RZ_ERR_DEVICE;                                         \
+    }                                                               \
+  } while (0)
+
 // A baked occluder's device copy: Occluder::m_vertexData (Occluder.h:17-18)
 // plus its reference box and bounds.
 struct rz_occluder {
@@ -34,9 +43,17 @@ struct rz_mlp {
   uint32_t capPairs = 0;
   float* d_loss = nullptr;
   uint32_t step = 0;
+  // The model reads feature 10, the previous step's label of the batch; when
+  // it does not (trained with that feature held at 0), multi-view steps
+  // ordered by it skip the label tracking of the draw.
+  bool usesLabels = true;
 };
 
 namespace rzi {
+
+// Prints a failed HIP call to stderr when the environment variable RZ_DEBUG
+// is set (the C ABI itself only returns RZ_ERR_DEVICE).
+void report_hip_error(hipError_t error, const char* call, const char* file, int line);
 
 // The coverage table and this host's RCPPS table, built once per process.
 struct HostTables {

@@ -53,6 +53,7 @@ struct ViewsParams {
   // [views[v].batchBase, + nBatches) of items/state.
   uint32_t batchItems;
   uint32_t waves;  // wave64s per work-group: 4 or 8
+  uint32_t tileB;  // tile edge in 8x8 blocks of every view of the launch: 8, or 16 (no labels)
   // Optional block-loop counters of the launch (visited, rejected by hiZ or
   // the edge tests, written), added to stats[0..2].
   unsigned long long* stats;
@@ -172,7 +173,7 @@ hipError_t launch_fill_u16(uint16_t* p, uint32_t n, uint16_t v, hipStream_t stre
 // Multi-view: grid of `groups` persistent work-groups over p.nUnits units of
 // 8x8-block tiles.
 hipError_t launch_views(const ViewsParams& p, uint32_t groups, hipStream_t stream, bool cooperative = false);
-hipError_t views_max_groups(int waves, int* maxGroups);
+hipError_t views_max_groups(int tileB, int waves, int* maxGroups);
 // Builds p.lists / p.listLen from p.state (after prep and setup); tileBox:
 // nItems u64 of scratch (the items of p.items / p.state, batch items
 // included).

@@ -224,7 +224,7 @@ __global__ void __launch_bounds__(256) rz_prep_kernel(PrepParams p) {
     M = &V.mvp;
     width = V.width;
     height = V.height;
-    tileW = tileH = 8;
+    tileW = tileH = V.tileB;
   }
   if (i == 0 && p.queue) *p.queue = 0u;
   const float* P = M->prebaked;
@@ -1845,19 +1845,21 @@ __global__ void __launch_bounds__(256) rz_item_tiles_kernel(ViewsParams p, uint3
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const ItemState& S = p.state[i];
+  // Tiles of tileB blocks: block >> sh, pixel >> (sh + 3).
+  const uint32_t sh = p.views[p.items[i].view].tileB == 16u ? 4u : 3u;
   uint32_t x0 = 0xFFFFu, x1 = 0u, y0 = 0xFFFFu, y1 = 0u;
   if (S.status != kStatusSkip) {
     if (S.fp[0] <= S.fp[1]) {
-      x0 = S.fp[0] >> 3;
-      x1 = S.fp[1] >> 3;
-      y0 = S.fp[2] >> 3;
-      y1 = S.fp[3] >> 3;
+      x0 = S.fp[0] >> sh;
+      x1 = S.fp[1] >> sh;
+      y0 = S.fp[2] >> sh;
+      y1 = S.fp[3] >> sh;
     }
     if (S.status == kStatusTest) {
-      x0 = min(x0, S.qMinX >> 6);
-      x1 = max(x1, S.qMaxX >> 6);
-      y0 = min(y0, S.qMinY >> 6);
-      y1 = max(y1, S.qMaxY >> 6);
+      x0 = min(x0, S.qMinX >> (sh + 3u));
+      x1 = max(x1, S.qMaxX >> (sh + 3u));
+      y0 = min(y0, S.qMinY >> (sh + 3u));
+      y1 = max(y1, S.qMaxY >> (sh + 3u));
     }
   }
   out[i] = uint64_t(min(x0, 0xFFFFu)) | uint64_t(min(x1, 0xFFFFu)) << 16 | uint64_t(min(y0, 0xFFFFu)) << 32 |
@@ -1884,8 +1886,6 @@ __global__ void __launch_bounds__(256) rz_views_bin_kernel(ViewsParams p, const 
     const ViewDesc& V = p.views[lo];
     const uint32_t t = unit - V.unitBase;
     const uint32_t tX = t % V.tilesX, tY = t / V.tilesX;
-    const int tbx0 = int(tX) * 8, tby0 = int(tY) * 8;
-    const int tbx1 = min(tbx0 + 8, int(V.bX)) - 1, tby1 = min(tby0 + 8, int(V.bY)) - 1;
     const uint32_t begin = p.batchItems ? V.batchBase : V.itemBase;
     const uint32_t end = begin + (p.batchItems ? V.nBatches : V.nItems);
     uint32_t* out = p.lists + V.listBase + size_t(t) * (end - begin);
@@ -2318,6 +2318,23 @@ static const void* frame_kernel_fn(int tileW, int waves, bool stats) {
   return nullptr;
 }
 
+// Launches a grid that the host has sized to be co-resident, cooperatively
+// when asked.  The runtime can refuse a cooperative launch while another
+// process on the device holds its cooperative (global wave sync) resources;
+// the grid is then dispatched plainly, which the kernels allow: they never
+// synchronise the whole grid, and each of their cross-tile waits is bounded
+// in time (a grid that another process kept from being co-resident reports
+// RZ_ERR_TIMEOUT instead of hanging).
+static hipError_t launch_coresident(const void* fn, dim3 grid, dim3 block, void** args, hipStream_t stream,
+                                    bool cooperative) {
+  if (cooperative) {
+    const hipError_t e = hipLaunchCooperativeKernel(fn, grid, block, args, 0, stream);
+    if (e != hipErrorIllegalState && e != hipErrorCooperativeLaunchTooLarge) return e;
+    (void)hipGetLastError();  // the refusal is handled: clear it
+  }
+  return hipLaunchKernel(fn, grid, block, args, 0, stream);
+}
+
 hipError_t launch_frame(const FrameParams& p, int tileW, hipStream_t stream, bool cooperative, int waves) {
   if (p.nItems == 0) return hipSuccess;
   FrameParams local = p;
@@ -2325,8 +2342,7 @@ hipError_t launch_frame(const FrameParams& p, int tileW, hipStream_t stream, boo
   const dim3 grid(p.tilesX * p.tilesY), block(64 * waves);
   const void* fn = frame_kernel_fn(tileW, waves, p.stats != nullptr);
   if (!fn) return hipErrorInvalidValue;
-  if (!cooperative) return hipLaunchKernel(fn, grid, block, args, 0, stream);
-  return hipLaunchCooperativeKernel(fn, grid, block, args, 0, stream);
+  return launch_coresident(fn, grid, block, args, stream, cooperative);
 }
 
 hipError_t frame_max_groups(int tileW, int waves, int* maxGroups) {
@@ -2387,8 +2403,13 @@ hipError_t launch_fill_u16(uint16_t* p, uint32_t n, uint16_t v, hipStream_t stre
   return hipGetLastError();
 }
 
+// Labelled draws need 8x8-block tiles (the block-major draw); 16x16-block
+// tiles have no labelled variant.
 template <int NW>
-static const void* views_fn(bool stats, bool labels) {
+static const void* views_fn(int tileB, bool stats, bool labels) {
+  if (tileB == 16)
+    return stats ? reinterpret_cast<const void*>(&rz_views_kernel<16, 16, NW, true, false>)
+                 : reinterpret_cast<const void*>(&rz_views_kernel<16, 16, NW, false, false>);
   if (labels)
     return stats ? reinterpret_cast<const void*>(&rz_views_kernel<8, 8, NW, true, true>)
                  : reinterpret_cast<const void*>(&rz_views_kernel<8, 8, NW, false, true>);
@@ -2399,25 +2420,26 @@ static const void* views_fn(bool stats, bool labels) {
 hipError_t launch_views(const ViewsParams& p, uint32_t nGroups, hipStream_t stream, bool cooperative) {
   if (p.nUnits == 0 || nGroups == 0) return hipSuccess;
   const bool st = p.stats != nullptr, lab = p.labels != nullptr;
-  const void* fn = p.waves == 8 ? views_fn<8>(st, lab) : views_fn<4>(st, lab);
+  if (p.tileB != 8 && (p.tileB != 16 || lab)) return hipErrorInvalidValue;
+  const int tb = int(p.tileB);
+  const void* fn = p.waves == 8 ? views_fn<8>(tb, st, lab) : views_fn<4>(tb, st, lab);
   ViewsParams local = p;
   void* args[] = {&local};
   const dim3 grid(nGroups), block(64 * (p.waves == 8 ? 8 : 4));
-  if (cooperative) return hipLaunchCooperativeKernel(fn, grid, block, args, 0, stream);
-  return hipLaunchKernel(fn, grid, block, args, 0, stream);
+  return launch_coresident(fn, grid, block, args, stream, cooperative);
 }
 
-hipError_t views_max_groups(int waves, int* maxGroups) {
+hipError_t views_max_groups(int tileB, int waves, int* maxGroups) {
   int dev = 0, cus = 0;
   hipError_t e = hipGetDevice(&dev);
   if (e != hipSuccess) return e;
   e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
   if (e != hipSuccess) return e;
   int perCU = 1 << 30;
-  for (int variant = 0; variant < 4; ++variant) {
+  for (int variant = 0; variant < (tileB == 16 ? 2 : 4); ++variant) {
     int n = 0;
     const bool stats = variant & 1, labels = variant & 2;
-    const void* fn = waves == 8 ? views_fn<8>(stats, labels) : views_fn<4>(stats, labels);
+    const void* fn = waves == 8 ? views_fn<8>(tileB, stats, labels) : views_fn<4>(tileB, stats, labels);
     e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fn, 64 * (waves == 8 ? 8 : 4), 0);
     if (e != hipSuccess) return e;
     perCU = n < perCU ? n : perCU;

@@ -15,11 +15,6 @@
 
 using namespace rzd;
 
-#define RZ_HIP(call)                                \
-  do {                                              \
-    if ((call) != hipSuccess) return RZ_ERR_DEVICE; \
-  } while (0)
-
 namespace {
 
 // SplitMix64 (Steele, Lea, Flood 2014) for deterministic initial weights.
@@ -151,6 +146,12 @@ extern "C" int rz_mlp_create(uint32_t seed, rz_mlp** out) {
 extern "C" int rz_mlp_destroy(rz_mlp* m) {
   if (!m) return RZ_ERR_INVALID_ARGUMENT;
   mlp_free(m);
+  return RZ_OK;
+}
+
+extern "C" int rz_mlp_set_uses_labels(rz_mlp* m, int usesLabels) {
+  if (!m) return RZ_ERR_INVALID_ARGUMENT;
+  m->usesLabels = usesLabels != 0;
   return RZ_OK;
 }
 

@@ -17,7 +17,9 @@
 // would emit them, so the batches are the final segments, left to right.
 #include <hip/hip_runtime.h>
 
-#include <hipcub/hipcub.hpp>
+#include <cstring>
+#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan_by_key.hpp>
 
 #include <algorithm>
 #include <cstdint>
@@ -295,10 +297,10 @@ int sah_batches_run(hipStream_t stream, uint32_t n, const float* d_boxes, uint32
       alloc(&B.rev, nChunks) != hipSuccess || alloc(&B.revScan, nChunks) != hipSuccess)
     return RZ_ERR_OUT_OF_MEMORY;
   size_t sortBytes = 0, scanBytes = 0;
-  SAH_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, sortBytes, B.keys[0], B.keys[1], B.idx[0], B.idx[1], n, 0, 64,
-                                             stream));
-  SAH_HIP(hipcub::DeviceScan::InclusiveScanByKey(nullptr, scanBytes, B.chunkSeg, B.fwd, B.fwdScan, MergeExt(),
-                                                 nChunks, hipcub::Equality(), stream));
+  SAH_HIP(rocprim::radix_sort_pairs(nullptr, sortBytes, B.keys[0], B.keys[1], B.idx[0], B.idx[1], n, 0u, 64u,
+                                    stream));
+  SAH_HIP(rocprim::inclusive_scan_by_key(nullptr, scanBytes, B.chunkSeg, B.fwd, B.fwdScan, size_t(nChunks), MergeExt(),
+                                         rocprim::equal_to<uint32_t>(), stream));
   size_t tempBytes = std::max(sortBytes, scanBytes);
   if (alloc(reinterpret_cast<uint8_t**>(&B.temp), tempBytes) != hipSuccess) return RZ_ERR_OUT_OF_MEMORY;
 
@@ -326,19 +328,19 @@ int sah_batches_run(hipStream_t stream, uint32_t n, const float* d_boxes, uint32
                                                       axis < 3 ? axis : -1, n, B.keys[0]);
       SAH_HIP(hipGetLastError());
       size_t bytes = tempBytes;
-      SAH_HIP(hipcub::DeviceRadixSort::SortPairs(B.temp, bytes, B.keys[0], B.keys[1], B.idx[cur], B.idx[cur ^ 1], n,
-                                                 0, int(32 + bits), stream));
+      SAH_HIP(rocprim::radix_sort_pairs(B.temp, bytes, B.keys[0], B.keys[1], B.idx[cur], B.idx[cur ^ 1], n, 0u,
+                                        32u + bits, stream));
       cur ^= 1;
       if (axis == 3) break;
       sah_chunks_kernel<<<grid_of(nChunks), 256, 0, stream>>>(d_boxes, B.idx[cur], B.segOf, n, gran, nChunks,
                                                               B.fwd, B.rev, B.chunkSeg, B.revSeg);
       SAH_HIP(hipGetLastError());
       bytes = tempBytes;
-      SAH_HIP(hipcub::DeviceScan::InclusiveScanByKey(B.temp, bytes, B.chunkSeg, B.fwd, B.fwdScan, MergeExt(), nChunks,
-                                                     hipcub::Equality(), stream));
+      SAH_HIP(rocprim::inclusive_scan_by_key(B.temp, bytes, B.chunkSeg, B.fwd, B.fwdScan, size_t(nChunks), MergeExt(),
+                                             rocprim::equal_to<uint32_t>(), stream));
       bytes = tempBytes;
-      SAH_HIP(hipcub::DeviceScan::InclusiveScanByKey(B.temp, bytes, B.revSeg, B.rev, B.revScan, MergeExt(), nChunks,
-                                                     hipcub::Equality(), stream));
+      SAH_HIP(rocprim::inclusive_scan_by_key(B.temp, bytes, B.revSeg, B.rev, B.revScan, size_t(nChunks), MergeExt(),
+                                             rocprim::equal_to<uint32_t>(), stream));
       sah_cost_kernel<<<grid_of(nChunks), 256, 0, stream>>>(B.fwdScan, B.revScan, B.chunkSeg, B.segStart, B.segLen,
                                                             B.segOpen, nChunks, gran, uint32_t(axis), B.segBest);
       SAH_HIP(hipGetLastError());

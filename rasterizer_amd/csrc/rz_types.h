@@ -79,7 +79,7 @@ struct ViewDesc {
   uint8_t* queryClip;        // nQueries
   uint32_t* winner;          // per pixel, for draw-order labels (null when labels are off)
   uint32_t listBase;         // first entry of the view's per-tile item lists
-  uint32_t pad;
+  uint32_t tileB;            // raster tile edge in 8x8 blocks (8 or 16); tilesX/tilesY count these tiles
   Mvp mvp;
 };
 static_assert(sizeof(ViewDesc) == 256, "ViewDesc layout");

@@ -11,7 +11,8 @@
 // pinned host table that the graph copies to the device at the start of each
 // replay, so changing them does not need a new capture.
 #include <hip/hip_runtime.h>
-#include <hipcub/hipcub.hpp>
+#include <cstring>
+#include <rocprim/device/device_radix_sort.hpp>
 
 #include <algorithm>
 #include <cstdio>
@@ -35,7 +36,7 @@ using namespace rzd;
 namespace {
 
 struct ViewHost {
-  uint32_t width = 0, height = 0, bX = 0, bY = 0, tilesX = 0, tilesY = 0, pyrX = 0, pyrY = 0;
+  uint32_t width = 0, height = 0, bX = 0, bY = 0, pyrX = 0, pyrY = 0;  // pyr: the 8x8-block tile grid of tileMin
   uint16_t* d_depth = nullptr;
   uint16_t* d_hiz = nullptr;
   uint16_t* d_tileMin = nullptr;
@@ -108,6 +109,8 @@ struct rz_views {
   void* d_sortTemp = nullptr;
   size_t sortTempBytes = 0;
   int waves = 4;  // wave64s per raster work-group
+  int tileB = 8;  // raster tile edge in 8x8 blocks (8 or 16; labelled steps use 8)
+  int planTileB = 8;  // the tile edge of the current plan
   int groupsPerCU = 0;  // persistent raster work-groups per CU (0: as many as fit)
   uint32_t* d_lists = nullptr;    // per-tile item lists (null: tiles walk their view's item range)
   uint32_t* d_listLen = nullptr;  // per (view, tile) unit
@@ -131,11 +134,6 @@ struct rz_views {
   bool phRecorded = false;
   bool ran = false;
 };
-
-#define RZ_HIP(call)                                \
-  do {                                              \
-    if ((call) != hipSuccess) return RZ_ERR_DEVICE; \
-  } while (0)
 
 static void free_plan(rz_views* v) {
   if (v->exec) (void)hipGraphExecDestroy(v->exec);
@@ -271,10 +269,8 @@ extern "C" int rz_views_create(uint32_t nViews, const uint32_t* widths, const ui
     h.height = heights[i];
     h.bX = h.width / 8;
     h.bY = h.height / 8;
-    h.tilesX = (h.bX + 7) / 8;
-    h.tilesY = (h.bY + 7) / 8;
-    h.pyrX = h.tilesX;
-    h.pyrY = h.tilesY;
+    h.pyrX = (h.bX + 7) / 8;
+    h.pyrY = (h.bY + 7) / 8;
     const size_t nb = size_t(h.bX) * h.bY;
     if (dev_alloc(&h.d_depth, nb * 64) != hipSuccess || dev_alloc(&h.d_hiz, nb + 8) != hipSuccess ||
         dev_alloc(&h.d_tileMin, size_t(h.pyrX) * h.pyrY) != hipSuccess)
@@ -348,6 +344,17 @@ extern "C" int rz_views_set_queries(rz_views* v, uint32_t view, uint32_t n, cons
 static int build_plan(rz_views* v) {
   free_plan(v);
   const uint32_t nv = uint32_t(v->views.size());
+  // Labelled draws need 8x8-block tiles; a query-gated view whose 8x8-block
+  // tiles cannot all be co-resident moves the launch to 16x16-block tiles.
+  const bool labelled = v->labels || (v->model && v->model->usesLabels);
+  int tileB = labelled ? 8 : v->tileB;
+  if (v->gated && tileB == 8 && !labelled) {
+    int maxGroups = 0;
+    RZ_HIP(views_max_groups(8, v->waves, &maxGroups));
+    for (const ViewHost& h : v->views)
+      if (((h.bX + 7) / 8) * ((h.bY + 7) / 8) > uint32_t(maxGroups)) tileB = 16;
+  }
+  v->planTileB = tileB;
   std::vector<ItemDesc> items;
   uint64_t quads = 0;
   uint32_t units = 0, queries = 0, maxClear = 0;
@@ -362,14 +369,15 @@ static int build_plan(rz_views* v) {
     d.height = h.height;
     d.bX = h.bX;
     d.bY = h.bY;
-    d.tilesX = h.tilesX;
-    d.tilesY = h.tilesY;
+    d.tilesX = (h.bX + uint32_t(tileB) - 1) / uint32_t(tileB);
+    d.tilesY = (h.bY + uint32_t(tileB) - 1) / uint32_t(tileB);
+    d.tileB = uint32_t(tileB);
     d.pyrX = h.pyrX;
     d.pyrY = h.pyrY;
     d.itemBase = uint32_t(items.size());
     d.nItems = uint32_t(h.occ.size());
     d.unitBase = units;
-    d.nUnits = h.tilesX * h.tilesY;
+    d.nUnits = d.tilesX * d.tilesY;
     d.queryBase = queries;
     d.nQueries = h.nQueries;
     d.batchBase = uint32_t(quads / 8);
@@ -410,7 +418,7 @@ static int build_plan(rz_views* v) {
   v->nQueries = queries;
   v->maxClear = maxClear;
   int maxGroups = 0;
-  RZ_HIP(views_max_groups(v->waves, &maxGroups));
+  RZ_HIP(views_max_groups(tileB, v->waves, &maxGroups));
   if (v->groupsPerCU > 0) {
     int dev = 0, cus = 0;
     RZ_HIP(hipGetDevice(&dev));
@@ -421,12 +429,13 @@ static int build_plan(rz_views* v) {
   if (v->gated) {
     // Every wait of a gated view ends only if all of the view's tiles can run
     // at once (nGroups <= units, and units count every view's tiles); batch
-    // orders would query per batch, which the reference's loop does not.
-    bool order = v->model != nullptr;
+    // orders would query per batch, which the reference's loop does not (a
+    // model that orders whole occluders keeps the loop's per-occluder query).
+    bool order = v->model != nullptr && !v->byOccluder;
     for (const ViewHost& h : v->views) order |= !h.order.empty();
     if (order) return RZ_ERR_INVALID_ARGUMENT;
-    for (const ViewHost& h : v->views)
-      if (h.tilesX * h.tilesY > v->nGroups) return RZ_ERR_TOO_LARGE;
+    for (uint32_t i = 0; i < nv; ++i)
+      if (v->h_views[i].nUnits > v->nGroups) return RZ_ERR_TOO_LARGE;
   }
   const size_t ni = std::max<size_t>(items.size(), 1);
   if (dev_alloc(&v->d_items, ni) != hipSuccess || dev_alloc(&v->d_state, ni) != hipSuccess ||
@@ -516,8 +525,8 @@ static int build_plan(rz_views* v) {
     if (dev_alloc(&v->d_sortKeys, std::max<size_t>(v->nItems, 1)) != hipSuccess ||
         dev_alloc(&v->d_sortKeysOut, std::max<size_t>(v->nItems, 1)) != hipSuccess)
       return RZ_ERR_OUT_OF_MEMORY;
-    RZ_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, v->d_sortKeys, v->d_sortKeysOut, v->d_itemIota,
-                                              v->d_itemOrder, int(v->nItems), 0, v->sortEndBit, v->stream));
+    RZ_HIP(rocprim::radix_sort_pairs(nullptr, bytes, v->d_sortKeys, v->d_sortKeysOut, v->d_itemIota,
+                                              v->d_itemOrder, uint32_t(v->nItems), 0u, unsigned(v->sortEndBit), v->stream));
     if (hipMalloc(&v->d_sortTemp, std::max<size_t>(bytes, 16)) != hipSuccess) return RZ_ERR_OUT_OF_MEMORY;
     v->sortTempBytes = bytes;
   }
@@ -532,11 +541,11 @@ static int build_plan(rz_views* v) {
     if (v->nBatches) RZ_HIP(hipMemcpy(v->d_iota, iota.data(), iota.size() * 4, hipMemcpyHostToDevice));
     // One device-wide stable radix sort on (view, descending score) keys.
     size_t bytes = 0, itemBytes = 0;
-    RZ_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, v->d_sortKeys, v->d_sortKeysOut, v->d_iota, v->d_iota,
-                                              int(v->nBatches), 0, v->sortEndBit, v->stream));
+    RZ_HIP(rocprim::radix_sort_pairs(nullptr, bytes, v->d_sortKeys, v->d_sortKeysOut, v->d_iota, v->d_iota,
+                                              uint32_t(v->nBatches), 0u, unsigned(v->sortEndBit), v->stream));
     if (v->byOccluder) {
-      RZ_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, itemBytes, v->d_sortKeys, v->d_sortKeysOut, v->d_itemIota,
-                                                v->d_itemOrder, int(v->nItems), 0, v->sortEndBit, v->stream));
+      RZ_HIP(rocprim::radix_sort_pairs(nullptr, itemBytes, v->d_sortKeys, v->d_sortKeysOut, v->d_itemIota,
+                                                v->d_itemOrder, uint32_t(v->nItems), 0u, unsigned(v->sortEndBit), v->stream));
       bytes = std::max(bytes, itemBytes);
     }
     if (hipMalloc(&v->d_sortTemp, std::max<size_t>(bytes, 16)) != hipSuccess) return RZ_ERR_OUT_OF_MEMORY;
@@ -639,14 +648,14 @@ static int enqueue_step(rz_views* v, bool capturing) {
         // Whole occluders by their best batch score, stable within a view.
         RZ_HIP(launch_item_scores(lp, v->d_scores, v->d_itemScores, v->stream));
         RZ_HIP(launch_sort_keys(v->d_views, nv, v->d_itemScores, v->nItems, true, v->d_sortKeys, v->stream));
-        RZ_HIP(hipcub::DeviceRadixSort::SortPairs(v->d_sortTemp, bytes, v->d_sortKeys, v->d_sortKeysOut,
-                                                  v->d_itemIota, v->d_itemOrder, int(v->nItems), 0, v->sortEndBit,
+        RZ_HIP(rocprim::radix_sort_pairs(v->d_sortTemp, bytes, v->d_sortKeys, v->d_sortKeysOut,
+                                                  v->d_itemIota, v->d_itemOrder, uint32_t(v->nItems), 0u, unsigned(v->sortEndBit),
                                                   v->stream));
         RZ_HIP(launch_item_reorder(lp, v->d_itemOrder, v->d_ritems, v->d_rstate, v->stream));
       } else {
         RZ_HIP(launch_sort_keys(v->d_views, nv, v->d_scores, v->nBatches, false, v->d_sortKeys, v->stream));
-        RZ_HIP(hipcub::DeviceRadixSort::SortPairs(v->d_sortTemp, bytes, v->d_sortKeys, v->d_sortKeysOut, v->d_iota,
-                                                  v->d_order, int(v->nBatches), 0, v->sortEndBit, v->stream));
+        RZ_HIP(rocprim::radix_sort_pairs(v->d_sortTemp, bytes, v->d_sortKeys, v->d_sortKeysOut, v->d_iota,
+                                                  v->d_order, uint32_t(v->nBatches), 0u, unsigned(v->sortEndBit), v->stream));
       }
     }
     if (v->f2b) {
@@ -654,8 +663,8 @@ static int enqueue_step(rz_views* v, bool capturing) {
       size_t bytes = v->sortTempBytes;
       RZ_HIP(hipMemcpyAsync(v->d_eyes, v->h_eyes, nv * 4 * sizeof(float), hipMemcpyHostToDevice, v->stream));
       RZ_HIP(launch_f2b_keys(v->d_items, v->nItems, v->d_eyes, v->d_sortKeys, v->stream));
-      RZ_HIP(hipcub::DeviceRadixSort::SortPairs(v->d_sortTemp, bytes, v->d_sortKeys, v->d_sortKeysOut, v->d_itemIota,
-                                                v->d_itemOrder, int(v->nItems), 0, v->sortEndBit, v->stream));
+      RZ_HIP(rocprim::radix_sort_pairs(v->d_sortTemp, bytes, v->d_sortKeys, v->d_sortKeysOut, v->d_itemIota,
+                                                v->d_itemOrder, uint32_t(v->nItems), 0u, unsigned(v->sortEndBit), v->stream));
       RZ_HIP(launch_item_reorder(lp, v->d_itemOrder, v->d_ritems, v->d_rstate, v->stream));
     }
     if (v->batchOrder) RZ_HIP(launch_batch_items(lp, v->d_order, v->nBatches, v->d_bitems, v->d_bstate, v->stream));
@@ -672,12 +681,15 @@ static int enqueue_step(rz_views* v, bool capturing) {
     rp.cov = v->d_cov;
     rp.queue = v->d_queue;
     rp.waves = uint32_t(v->waves);
+    rp.tileB = uint32_t(v->planTileB);
     rp.stats = v->stats ? v->d_stats : nullptr;
     if (v->stats)
       RZ_HIP(launch_fill_u32(reinterpret_cast<uint32_t*>(v->d_stats), 2 * (3 + kFrameStatWords), 0u, v->stream));
     // Labels are counted by the draw itself (after this step's features
     // have read the previous step's).
-    rp.labels = (v->labels || v->model) ? v->d_labels : nullptr;
+    // A model that does not read the previous labels leaves them at 0 (the
+    // plan zeroes them) and the draw without label tracking.
+    rp.labels = (v->labels || (v->model && v->model->usesLabels)) ? v->d_labels : nullptr;
     if (rp.labels) RZ_HIP(launch_fill_u32(v->d_labels, v->nBatches, 0u, v->stream));
     rp.lists = v->d_lists;
     rp.listLen = v->d_listLen;
@@ -797,7 +809,7 @@ extern "C" int rz_views_read_status(rz_views* v, uint32_t view, uint8_t* status)
   std::vector<uint32_t> done(v->gated ? d.nItems : 0);
   if (v->gated && d.nItems) {
     RZ_HIP(hipMemcpy(done.data(), v->d_done + d.itemBase, d.nItems * sizeof(uint32_t), hipMemcpyDeviceToHost));
-    if (v->f2b) {
+    if (v->f2b || (v->model && v->byOccluder)) {
       // The launch drew the items in sorted order: done word k is item order[k]'s.
       std::vector<uint32_t> order(d.nItems), byItem(d.nItems);
       RZ_HIP(hipMemcpy(order.data(), v->d_itemOrder + d.itemBase, d.nItems * 4, hipMemcpyDeviceToHost));
@@ -850,7 +862,7 @@ extern "C" int rz_views_set_model(rz_views* v, rz_mlp* model, int byOccluder) {
   if (!v) return RZ_ERR_INVALID_ARGUMENT;
   int s = wait_last_step(v);
   if (s != RZ_OK) return s;
-  if (model && !v->labels) {
+  if (model && model->usesLabels && !v->labels) {
     s = rz_views_enable_labels(v, 1);  // features read the previous step's labels
     if (s != RZ_OK) return s;
   }
@@ -964,16 +976,19 @@ static std::string views_key(const rz_views* v) {
 
 extern "C" int rz_views_get_config(rz_views* v, rz_views_config* config) {
   if (!v || !config) return RZ_ERR_INVALID_ARGUMENT;
+  config->tileW = uint32_t(v->tileB);
   config->waves = uint32_t(v->waves);
   config->groupsPerCU = uint32_t(v->groupsPerCU);
   config->ms = -1.0f;
   return RZ_OK;
 }
 
-extern "C" int rz_views_set_config(rz_views* v, uint32_t waves, uint32_t groupsPerCU) {
-  if (!v || (waves != 4 && waves != 8) || groupsPerCU > 64) return RZ_ERR_INVALID_ARGUMENT;
+extern "C" int rz_views_set_config(rz_views* v, uint32_t tileW, uint32_t waves, uint32_t groupsPerCU) {
+  if (!v || (tileW != 8 && tileW != 16) || (waves != 4 && waves != 8) || groupsPerCU > 64)
+    return RZ_ERR_INVALID_ARGUMENT;
   int s = wait_last_step(v);
   if (s != RZ_OK) return s;
+  v->tileB = int(tileW);
   v->waves = int(waves);
   v->groupsPerCU = int(groupsPerCU);
   v->dirty = true;
@@ -985,9 +1000,9 @@ extern "C" int rz_views_load_config(rz_views* v, const char* cachePath, int* fou
   *found = 0;
   std::string val;
   if (!rzi::tune_lookup(cachePath, views_key(v), val)) return RZ_OK;
-  unsigned waves = 0, gpc = 0;
-  if (std::sscanf(val.c_str(), "waves=%u groupsPerCU=%u", &waves, &gpc) != 2) return RZ_OK;
-  const int s = rz_views_set_config(v, waves, gpc);
+  unsigned tileW = 0, waves = 0, gpc = 0;
+  if (std::sscanf(val.c_str(), "tileW=%u waves=%u groupsPerCU=%u", &tileW, &waves, &gpc) != 3) return RZ_OK;
+  const int s = rz_views_set_config(v, tileW, waves, gpc);
   if (s == RZ_OK) *found = 1;
   return s == RZ_ERR_INVALID_ARGUMENT ? RZ_OK : s;
 }
@@ -1017,59 +1032,66 @@ static int step_outputs(rz_views* v, std::vector<uint8_t>& out) {
   return RZ_OK;
 }
 
-extern "C" int rz_views_tune(rz_views* v, uint32_t reps, const char* cachePath, rz_views_config candidates[4],
+extern "C" int rz_views_tune(rz_views* v, uint32_t reps, const char* cachePath, rz_views_config candidates[8],
                              uint32_t* nCandidates) {
   if (!v || reps == 0) return RZ_ERR_INVALID_ARGUMENT;
-  const int oldWaves = v->waves, oldGpc = v->groupsPerCU;
+  const int oldTile = v->tileB, oldWaves = v->waves, oldGpc = v->groupsPerCU;
   auto restore = [&](int code) {
-    (void)rz_views_set_config(v, uint32_t(oldWaves), uint32_t(oldGpc));
+    (void)rz_views_set_config(v, uint32_t(oldTile), uint32_t(oldWaves), uint32_t(oldGpc));
     return code;
   };
+  // Labelled steps draw with 8x8-block tiles only.
+  const bool labelled = v->labels || (v->model && v->model->usesLabels);
   std::vector<uint8_t> ref, got;
-  rz_views_config all[4];
+  rz_views_config all[8];
   uint32_t k = 0;
   int best = -1;
-  for (uint32_t waves : {4u, 8u}) {
-    int maxGroups = 0, dev = 0, cus = 1;
-    RZ_HIP(views_max_groups(int(waves), &maxGroups));
-    RZ_HIP(hipGetDevice(&dev));
-    RZ_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-    const uint32_t perCU = uint32_t(std::max(maxGroups / std::max(cus, 1), 1));
-    for (uint32_t gpc : {1u, 0u}) {
-      rz_views_config& cf = all[k];
-      cf.waves = waves;
-      cf.groupsPerCU = gpc;
-      cf.ms = -1.0f;
-      const uint32_t idx = k++;
-      if (gpc == 0 && perCU == 1) continue;  // same launch as one group per CU
-      // Each step is preceded by one with the same inputs, so that every
-      // configuration's step starts from the same buffers (and labels).
-      int s = rz_views_set_config(v, waves, gpc);
-      float ms = 0.0f;
-      for (int warm = 0; warm < 2 && s == RZ_OK; ++warm) {
-        s = rz_views_run(v, 0);
-        if (s == RZ_OK) s = rz_views_sync(v, &ms);
+  for (uint32_t tileW : {8u, 16u}) {
+    for (uint32_t waves : {4u, 8u}) {
+      int maxGroups = 0, dev = 0, cus = 1;
+      RZ_HIP(views_max_groups(int(tileW), int(waves), &maxGroups));
+      RZ_HIP(hipGetDevice(&dev));
+      RZ_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+      const uint32_t perCU = uint32_t(std::max(maxGroups / std::max(cus, 1), 1));
+      for (uint32_t gpc : {1u, 0u}) {
+        rz_views_config& cf = all[k];
+        cf.tileW = tileW;
+        cf.waves = waves;
+        cf.groupsPerCU = gpc;
+        cf.ms = -1.0f;
+        const uint32_t idx = k++;
+        if (gpc == 0 && perCU == 1) continue;  // same launch as one group per CU
+        if (tileW == 16 && labelled) continue;
+        // Each step is preceded by one with the same inputs, so that every
+        // configuration's step starts from the same buffers (and labels).
+        int s = rz_views_set_config(v, tileW, waves, gpc);
+        float ms = 0.0f;
+        for (int warm = 0; warm < 2 && s == RZ_OK; ++warm) {
+          s = rz_views_run(v, 0);
+          if (s == RZ_OK) s = rz_views_sync(v, &ms);
+        }
+        if (s == RZ_ERR_TOO_LARGE) continue;  // a gated view's tiles do not fit this grid
+        if (s == RZ_OK && v->planTileB != int(tileW)) continue;  // moved to 16x16 tiles: measured as such
+        if (s == RZ_OK) s = step_outputs(v, got);
+        if (s != RZ_OK) return restore(s);
+        if (ref.empty()) ref.swap(got);
+        else if (got != ref) return restore(RZ_ERR_NOT_IDENTICAL);
+        std::vector<float> t(reps);
+        for (uint32_t r = 0; r < reps; ++r) {
+          if ((s = rz_views_run(v, 1)) != RZ_OK || (s = rz_views_sync(v, &t[r])) != RZ_OK) return restore(s);
+        }
+        std::sort(t.begin(), t.end());
+        cf.ms = t[reps / 2];
+        if (best < 0 || cf.ms < all[best].ms) best = int(idx);
       }
-      if (s == RZ_ERR_TOO_LARGE) continue;  // a gated view's tiles do not fit this grid
-      if (s == RZ_OK) s = step_outputs(v, got);
-      if (s != RZ_OK) return restore(s);
-      if (ref.empty()) ref.swap(got);
-      else if (got != ref) return restore(RZ_ERR_NOT_IDENTICAL);
-      std::vector<float> t(reps);
-      for (uint32_t r = 0; r < reps; ++r) {
-        if ((s = rz_views_run(v, 1)) != RZ_OK || (s = rz_views_sync(v, &t[r])) != RZ_OK) return restore(s);
-      }
-      std::sort(t.begin(), t.end());
-      cf.ms = t[reps / 2];
-      if (best < 0 || cf.ms < all[best].ms) best = int(idx);
     }
   }
   if (best < 0) return restore(RZ_ERR_DEVICE);
-  int s = rz_views_set_config(v, all[best].waves, all[best].groupsPerCU);
+  int s = rz_views_set_config(v, all[best].tileW, all[best].waves, all[best].groupsPerCU);
   if (s != RZ_OK) return restore(s);
   if (cachePath) {
-    const std::string val =
-        "waves=" + std::to_string(all[best].waves) + " groupsPerCU=" + std::to_string(all[best].groupsPerCU);
+    const std::string val = "tileW=" + std::to_string(all[best].tileW) + " waves=" + std::to_string(all[best].waves) +
+                            " groupsPerCU=" + std::to_string(all[best].groupsPerCU);
     if (!rzi::tune_store(cachePath, views_key(v), val)) return RZ_ERR_INVALID_ARGUMENT;
   }
   if (candidates)

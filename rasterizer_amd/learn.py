@@ -33,10 +33,28 @@ def random_camera(rng: np.random.Generator, half: float = 50.0):
     return eye, d.astype(np.float32), up
 
 
+def street_camera(scene, rng: np.random.Generator):
+    """A street-level camera in the city scene (scenes.config2): eye 2 units
+    above the terrain at a random street intersection or along a street, a
+    random horizontal direction tilted down as the scene's camera path is."""
+    hf, pitch, grid = scene.meta["heightfield"], scene.meta["pitch"], scene.meta["grid"]
+    along = rng.uniform(1.0, grid - 1.0)
+    cross = float(rng.integers(1, grid))
+    x, z = (along, cross) if rng.random() < 0.5 else (cross, along)
+    x, z = x * pitch, z * pitch
+    eye = np.array([x, float(hf.height(x, z)) + 2.0, z], np.float32)
+    ang = rng.uniform(0.0, 2.0 * math.pi)
+    d = np.array([math.cos(ang), -0.08, math.sin(ang)])
+    d /= np.linalg.norm(d)
+    return eye, d.astype(np.float32), (0.0, 1.0, 0.0)
+
+
 def generate(scene, frames: int, seed: int, width: int = 1280, height: int = 720, views_per_launch: int = 64,
-             log=print):
-    """Features (list per frame of (nBatches, 16)) and labels (per frame)."""
+             log=print, camera=None):
+    """Features (list per frame of (nBatches, 16)) and labels (per frame).
+    camera(rng) -> (eye, direction, up); default: random_camera."""
     rng = np.random.default_rng(seed)
+    camera = camera or random_camera
     owner = Rasterizer(64, 64)
     V = Views(owner, [(width, height)] * views_per_launch)
     V.enable_labels(True)
@@ -45,7 +63,7 @@ def generate(scene, frames: int, seed: int, width: int = 1280, height: int = 720
     done = 0
     while done < frames:
         k = min(views_per_launch, frames - done)
-        cams = [random_camera(rng) for _ in range(views_per_launch)]
+        cams = [camera(rng) for _ in range(views_per_launch)]
         for v in range(views_per_launch):
             eye, d, up = cams[v]
             occ = [scene.occluders[i] for i in scenes.front_to_back(scene.occluders, eye)]
@@ -78,6 +96,20 @@ def frame_pairs(labels: np.ndarray, rng: np.random.Generator, count: int) -> np.
     return np.stack([hi, lo], 1)
 
 
+LABEL_FEATURE = 10  # log2(1 + previous step's label)
+
+
+def without_labels(feats):
+    """Features with the previous-label feature held at 0 (for models that
+    run without label tracking)."""
+    out = []
+    for f in feats:
+        g = np.array(f, np.float32, copy=True)
+        g[:, LABEL_FEATURE] = 0.0
+        out.append(g)
+    return out
+
+
 def normalisation(feats) -> np.ndarray:
     x = np.concatenate(feats)
     mean = x.mean(0)
@@ -87,9 +119,14 @@ def normalisation(feats) -> np.ndarray:
 
 
 def train(feats, labels, steps: int = 400, frames_per_step: int = 32, pairs_per_frame: int = 256, lr: float = 2e-3,
-          seed: int = 1, log=print):
+          seed: int = 1, log=print, use_labels: bool = True):
+    """use_labels=False trains without the previous-label feature (held at 0),
+    for ordering without label tracking."""
     rng = np.random.default_rng(seed)
+    if not use_labels:
+        feats = without_labels(feats)
     m = Mlp(seed=seed)
+    m.set_uses_labels(use_labels)
     norm = normalisation(feats)
     m.set_params(norm=norm)
     losses = []
@@ -116,6 +153,8 @@ def pair_agreement(m: Mlp, feats, labels, seed: int = 5, pairs_per_frame: int = 
     """Fraction of held-out within-frame pairs with different labels that the
     scores rank the same way."""
     rng = np.random.default_rng(seed)
+    if not getattr(m, "uses_labels", True):
+        feats = without_labels(feats)
     ok = tot = 0
     for f, lab in zip(feats, labels):
         pr = frame_pairs(lab, rng, pairs_per_frame)
@@ -129,33 +168,46 @@ def pair_agreement(m: Mlp, feats, labels, seed: int = 5, pairs_per_frame: int = 
 
 def save(m: Mlp, path: Path = WEIGHTS, **meta) -> None:
     w, norm = m.params()
-    np.savez(path, params=w, norm=norm, **{k: np.asarray(v) for k, v in meta.items()})
+    np.savez(path, params=w, norm=norm, uses_labels=np.asarray(int(getattr(m, "uses_labels", True))),
+             **{k: np.asarray(v) for k, v in meta.items()})
 
 
 def load(path: Path = WEIGHTS) -> Mlp:
     d = np.load(path, allow_pickle=False)
     m = Mlp(seed=0)
     m.set_params(d["params"], d["norm"])
+    m.set_uses_labels(bool(int(d["uses_labels"])) if "uses_labels" in d.files else True)
     return m
 
 
 def main() -> int:
     import argparse
 
-    ap = argparse.ArgumentParser(description="train the batch-ordering MLP on the 1,024-box scene")
+    ap = argparse.ArgumentParser(description="train the batch-ordering MLP")
+    ap.add_argument("--scene", choices=["boxes", "city"], default="boxes",
+                    help="boxes: the 1,024-box scene, random cameras in its cube (1280x720); "
+                         "city: the city scene, street-level cameras (1920x1080)")
     ap.add_argument("--frames", type=int, default=10000)
     ap.add_argument("--steps", type=int, default=400)
+    ap.add_argument("--no-labels", action="store_true",
+                    help="train without the previous-label feature, for ordering without label tracking")
     ap.add_argument("--out", default=str(WEIGHTS))
     args = ap.parse_args()
-    scene = scenes.config1(seed=1)
     n_test = max(args.frames // 10, 64)
-    feats, labels = generate(scene, args.frames + n_test, seed=10)
+    if args.scene == "boxes":
+        scene = scenes.config1(seed=1)
+        feats, labels = generate(scene, args.frames + n_test, seed=10)
+    else:
+        scene = scenes.config2(seed=1, frames=8, n_queries=16)
+        feats, labels = generate(scene, args.frames + n_test, seed=12, width=1920, height=1080, views_per_launch=32,
+                                 camera=lambda rng: street_camera(scene, rng))
     tr_f, tr_l = feats[:args.frames], labels[:args.frames]
     te_f, te_l = feats[args.frames:], labels[args.frames:]
-    m, norm, losses = train(tr_f, tr_l, steps=args.steps)
+    m, norm, losses = train(tr_f, tr_l, steps=args.steps, use_labels=not args.no_labels)
     agree = pair_agreement(m, te_f, te_l)
     print(f"held-out pair agreement {agree:.3f} over {len(te_f)} frames")
-    save(m, Path(args.out), agreement=agree, frames=args.frames, steps=args.steps, final_loss=np.mean(losses[-20:]))
+    save(m, Path(args.out), agreement=agree, frames=args.frames, steps=args.steps, final_loss=np.mean(losses[-20:]),
+         scene=args.scene)
     return 0
 
 

@@ -470,6 +470,7 @@ def config2(seed: int = 1, frames: int = 256, n_queries: int = 65536, grid: int 
         d[1] = -0.08
         mvps[f] = view_projection(eyes[f], d, (0, 1, 0), math.pi / 3, width / height)
     scene.meta.update(width=width, height=height, eyes=eyes, mvps=mvps, queries=queries, seed=seed,
+                      heightfield=hf, pitch=pitch, grid=grid,
                       terrain_quads=terrain_quads, building_quads=int(quads.shape[0] - terrain_quads))
     return scene
 

@@ -82,7 +82,7 @@ def test_null_arguments_rejected(rzlib):
     assert rzlib.rz_tune_frame(None, 0, None, 1, None, None, None) == -1
     assert rzlib.rz_load_config(None, None, None) == -1
     assert rzlib.rz_views_get_config(None, None) == -1
-    assert rzlib.rz_views_set_config(None, 4, 0) == -1
+    assert rzlib.rz_views_set_config(None, 8, 4, 0) == -1
     assert rzlib.rz_views_tune(None, 1, None, None, None) == -1
     assert rzlib.rz_views_load_config(None, None, None) == -1
     assert rzlib.rz_views_read_tile_stats(None, None) == -1

@@ -116,7 +116,9 @@ def test_views_tuner(oracle_mod, scene, report, tmp_path):
     cands = [c for c in res["candidates"] if c["ms"] > 0]
     assert len(cands) >= 3
     best = min(cands, key=lambda c: c["ms"])
-    assert res["chosen"] == {"waves": best["waves"], "groupsPerCU": best["groupsPerCU"]}
+    assert res["chosen"] == {"tileW": best["tileW"], "waves": best["waves"], "groupsPerCU": best["groupsPerCU"]}
+    # Labelled steps draw with 8x8-block tiles: the 16x16 candidates are not timed.
+    assert all(c["ms"] < 0 for c in res["candidates"] if c["tileW"] == 16)
     V.run(graph=True)
     V.sync()
     for k, (w, h) in enumerate(sizes):
@@ -137,3 +139,40 @@ def test_views_tuner(oracle_mod, scene, report, tmp_path):
         assert np.array_equal(V.labels(k)[0], lab)
     V2 = Views(Rasterizer(64, 64), sizes)
     assert V2.load_config(cache) and V2.config() == res["chosen"]
+
+
+def test_views_tuner_tile_sizes(oracle_mod, scene, report):
+    """Without labels the tuner also times 16x16-block (128-pixel) tiles; every
+    candidate gives the same bits, and the chosen configuration's frames equal
+    the oracle's."""
+    sizes = [(1280, 720), (512, 512), (200, 104)]
+    V = Views(Rasterizer(64, 64), sizes)
+    rng = np.random.default_rng(18)
+    setup = []
+    for k, (w, h) in enumerate(sizes):
+        eye = rng.uniform(-40, 40, 3).astype(np.float32)
+        mvp = scenes.view_projection(eye, (0, 0, 1), (0, 1, 0), np.pi / 3, w / h)
+        occ = [scene.occluders[i] for i in scenes.front_to_back(scene.occluders, eye)]
+        V.set_mvp(k, mvp)
+        V.set_occluders(k, occ)
+        V.set_queries(k, scene.meta["queries"][:300])
+        setup.append((mvp, occ))
+    res = V.autotune(reps=2)
+    print(res, flush=True)
+    timed = {(c["tileW"], c["waves"], c["groupsPerCU"]) for c in res["candidates"] if c["ms"] > 0}
+    assert any(t[0] == 16 for t in timed) and any(t[0] == 8 for t in timed)
+    for tile_w in (8, 16):
+        V.set_config(8, 0, tile_w)
+        V.run(graph=True)
+        V.sync()
+        for k, (w, h) in enumerate(sizes):
+            mvp, occ = setup[k]
+            o = oracle_mod.OracleRasterizer(w, h)
+            o.clear()
+            o.setModelViewProjection(mvp)
+            assert np.array_equal(V.status(k), o.draw_frame(occ, gated=False))
+            dg, hg = V.raw(k)
+            do, ho = o.raw()
+            written = np.repeat(ho[: (w // 8) * (h // 8)] != 1, 64)
+            assert np.array_equal(hg, ho), f"tile {tile_w} view {k}: hiZ {report(hg, ho)}"
+            assert np.array_equal(dg[written], do[written]), f"tile {tile_w} view {k}: depth"

@@ -180,12 +180,14 @@ def test_views_query_gated_limits(scene):
     from rasterizer_amd import Mlp, RasterizerError
 
     owner = Rasterizer(64, 64)
-    # 32 x 32 tiles cannot all be co-resident: refused, not run.
-    V = Views(owner, [(2048, 2048)])
+    # 8192x4096: even its 16x16-block tiles (64 x 32) cannot all be
+    # co-resident: refused, not run.
+    V = Views(owner, [(8192, 4096)])
     V.set_query_gated(True)
     V.set_occluders(0, scene.occluders[:10])
     with pytest.raises(RasterizerError):
         V.run(graph=False)
+    # A batch-granular model would query per batch: refused.
     V2 = Views(owner, [(320, 192)])
     V2.set_occluders(0, scene.occluders[:10])
     V2.set_query_gated(True)
@@ -195,6 +197,47 @@ def test_views_query_gated_limits(scene):
     V2.set_model(None)
     V2.run(graph=False)
     V2.sync()
+
+
+def test_views_query_gated_large_views(oracle_mod, scene, report):
+    """Query-gated 2048^2 and 3840x2160 views, whose 8x8-block tiles cannot
+    all be co-resident: the step moves to 16x16-block tiles and each view
+    still equals the reference's frame loop (the oracle's draw_frame), over
+    two steps (the second from the buffers the first left)."""
+    sizes = [(2048, 2048), (3840, 2160), (1920, 1080)]
+    owner = Rasterizer(64, 64)
+    V = Views(owner, sizes)
+    V.set_query_gated(True)
+    oracles = [oracle_mod.OracleRasterizer(w, h) for w, h in sizes]
+    rng = np.random.default_rng(31)
+    drawn = skipped = 0
+    for step in range(2):
+        want = []
+        for k, (w, h) in enumerate(sizes):
+            eye = rng.uniform(-40, 40, 3).astype(np.float32)
+            d = rng.normal(size=3)
+            d /= np.linalg.norm(d)
+            up = (0, 1, 0) if abs(d[1]) < 0.95 else (1, 0, 0)
+            mvp = scenes.view_projection(eye, d, up, np.pi / 3, w / h)
+            occ = [scene.occluders[i] for i in scenes.front_to_back(scene.occluders, eye)]
+            V.set_mvp(k, mvp)
+            V.set_occluders(k, occ)
+            oracles[k].clear()
+            oracles[k].setModelViewProjection(mvp)
+            st_o = oracles[k].draw_frame(occ, gated=True)
+            want.append(st_o)
+            drawn += int((st_o > 0).sum())
+            skipped += int((st_o == 0).sum())
+        print(f"gated large views step {step}: {V.info()}", flush=True)
+        V.run(graph=False)
+        V.sync()
+        for k, (w, h) in enumerate(sizes):
+            assert np.array_equal(V.status(k), want[k]), f"step {step} view {k} {w}x{h}: statuses"
+            dg, hg = V.raw(k)
+            do, ho = oracles[k].raw()
+            assert np.array_equal(hg, ho), f"step {step} view {k} {w}x{h}: hiZ {report(hg, ho)}"
+            assert np.array_equal(dg, do), f"step {step} view {k} {w}x{h}: depth {report(dg, do)}"
+    assert drawn > 20 and skipped > 20
 
 
 def test_stress_scene_views_match_oracle(oracle_mod, report):
