@@ -71,8 +71,10 @@ struct ViewsParams {
   // lists[views[v].listBase + t * (items in view) ...], listLen[unit] long.
   uint32_t* lists;
   uint32_t* listLen;
-  // Optional (with lists): the units that have items, in unit order, as
-  // view << 20 | tile, and their count; the queue hands out only these.
+  // Optional (with lists): the units that have items, as view << 20 | tile,
+  // in view order and within a view by descending item count (log2
+  // buckets, tile order within a bucket), and their count; the queue hands
+  // out only these.
   uint32_t* unitList;
   uint32_t* nActive;
   uint32_t* unitCounts;  // scratch: per view

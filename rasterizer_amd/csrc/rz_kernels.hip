@@ -1804,23 +1804,31 @@ __global__ void __launch_bounds__(256) rz_views_compact_kernel(ViewsParams p, ui
   const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
   const uint32_t v = blockIdx.x;
   const ViewDesc& V = p.views[v];
+  // Units with more items first (buckets of floor(log2(items)), 8 and up
+  // together at the top), tile order within a bucket: the tiles that the
+  // others' query waits depend on most start first.  Any order within a view
+  // keeps every wait finite (all of the earliest view's tiles run).
   uint32_t n = 0;
-  for (uint32_t base = 0; base < V.nUnits; base += 256u) {
-    const uint32_t t = base + tid;
-    const bool has = t < V.nUnits && p.listLen[V.unitBase + t] != 0u;
-    const uint64_t m = __ballot(has);
-    const uint32_t pre = __builtin_amdgcn_mbcnt_hi(uint32_t(m >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(m), 0u));
-    if (lane == 0) wcnt[wave] = uint32_t(__popcll(m));
-    __syncthreads();
-    uint32_t off = n, total = 0;
+  for (int bucket = 7; bucket >= 0; --bucket) {
+    for (uint32_t base = 0; base < V.nUnits; base += 256u) {
+      const uint32_t t = base + tid;
+      const uint32_t len = t < V.nUnits ? p.listLen[V.unitBase + t] : 0u;
+      const int lg = len ? min(31 - __builtin_clz(len), 7) : -1;
+      const bool has = lg == bucket;
+      const uint64_t m = __ballot(has);
+      const uint32_t pre = __builtin_amdgcn_mbcnt_hi(uint32_t(m >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(m), 0u));
+      if (lane == 0) wcnt[wave] = uint32_t(__popcll(m));
+      __syncthreads();
+      uint32_t off = n, total = 0;
 #pragma unroll
-    for (uint32_t w = 0; w < 4; ++w) {
-      off += w < wave ? wcnt[w] : 0u;
-      total += wcnt[w];
+      for (uint32_t w = 0; w < 4; ++w) {
+        off += w < wave ? wcnt[w] : 0u;
+        total += wcnt[w];
+      }
+      if (has) unitTmp[V.unitBase + off + pre] = (v << 20) | t;
+      n += total;
+      __syncthreads();
     }
-    if (has) unitTmp[V.unitBase + off + pre] = (v << 20) | t;
-    n += total;
-    __syncthreads();
   }
   if (tid == 0) p.unitCounts[v] = n;
 }
