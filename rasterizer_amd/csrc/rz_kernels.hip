@@ -841,31 +841,24 @@ __device__ __forceinline__ void raster_quad_wave(const QuadRec& R, TileSmem<TBX,
   }
 }
 
-// One block of one quad with the block's hiZ and depth in registers (the
-// block-major draw), in two parts.  sample_block_reg: what the block loop
-// computes for the quad alone (Rasterizer.cpp:1148-1290: the four edge
-// lookups and the coverage mask, Convex blocks with an offset >= 63 and other
-// modes' empty masks marked `skip`, depth rows 0/1 and 8/9 from the plane,
-// packed averages between, 0 outside the mask), with the same arithmetic as
-// raster_block with g = 0: lanes 0-3 of `s` hold the folded edge offsets,
-// lanes 4-11 the depth samples.  It depends on nothing the previous quads
-// wrote.  merge_block_reg: the hiZ reject and the merge, in submission order
-// (the caller tests hiZ before sampling as well, since most visits end
-// there).
-//
-// LABELS: `win` is the lane's pixel's winner, the batch whose write produced
-// its depth: the first write of a cleared block (hz == 1) sets it (none where
-// the written depth is 0), later writes only where they raise the depth.
-__device__ __forceinline__ void sample_block_reg(const uint64_t* __restrict__ cov, float s, uint32_t lane,
-                                                 int kLane, const QuadConst& Q, uint32_t& dOut, bool& skip) {
+// One block of one quad in the block-major draw, the part that depends on
+// nothing the previous quads wrote (Rasterizer.cpp:1148-1290): the four edge
+// lookups and the coverage mask (Convex blocks with an offset >= 63 and other
+// modes' empty masks set `skip`, the same in every lane), depth rows 0/1 and
+// 8/9 from the plane, packed averages between, 0 outside the mask.  `o` holds
+// the block's four folded edge offsets (the same values in every lane) and
+// `v` the lane's row-0/1 depth sample (sample kLane of the eight), as
+// raster_block reads them from its lane groups.  The hiZ reject and the merge
+// are the caller's, in submission order.
+__device__ __forceinline__ void sample_block(const uint64_t* __restrict__ cov, const float (&o)[4], float v,
+                                             uint32_t lane, const QuadConst& Q, uint32_t& dOut, bool& skip) {
   const bool convex = Q.mode == kModeConvex;
   bool outside = false;
   uint32_t idx[4];
 #pragma unroll
   for (int e = 0; e < 4; ++e) {
-    const float o = readlane_f(s, e);
-    const int32_t t = cvtt_x86(o);
-    outside |= o >= 63.0f;
+    const int32_t t = cvtt_x86(o[e]);
+    outside |= o[e] >= 63.0f;
     const uint32_t tc = convex ? uint32_t(max(t, 0)) : uint32_t(min(max(t, 0), 63));
     idx[e] = (Q.sl[e] | tc) & 4095u;
   }
@@ -879,8 +872,9 @@ __device__ __forceinline__ void sample_block_reg(const uint64_t* __restrict__ co
                         : Q.mode == kModeConcaveRight ? mRight
                         : Q.mode == kModeConcaveCenter ? mCenter
                                                        : mLeft;
-  skip = convex ? outside : mask == 0ull;
-  float v = __shfl(s, 4 + kLane);
+  // Every lane computes the same test from the same values; the ballot
+  // makes it a wave-uniform value for the caller's branch.
+  skip = __ballot(convex ? outside : mask == 0ull) != 0ull;
   const uint32_t px = lane & 7u, py = lane >> 3;
   if (px >= 4u) v = fmaf(Q.dDx, 0.5f, v);
   const uint32_t lo = pack_depth(v);
@@ -894,21 +888,7 @@ __device__ __forceinline__ void sample_block_reg(const uint64_t* __restrict__ co
   dOut = ((mask >> maskBit) & 1ull) ? d : 0u;
 }
 
-template <bool STATS, bool LABELS>
-__device__ __forceinline__ void merge_block_reg(uint32_t d, bool skip, uint32_t maxZ, uint32_t& hz, uint32_t& dval,
-                                                bool& dirty, BlockCounters& bc, uint32_t& win, uint32_t batch) {
-  if (STATS) bc.visited++;
-  if (hz >= maxZ || skip) {
-    if (STATS) bc.rejected++;
-    return;
-  }
-  if (STATS) bc.written++;
-  if (LABELS) win = hz == 1u ? (d != 0u ? batch : kNoWinner) : (d > dval ? batch : win);
-  if (hz != 1u) d = max(d, dval);
-  dval = d;
-  hz = __ockl_wfred_min_u32(d);
-  dirty = true;
-}
+__device__ __forceinline__ uint32_t readlane_u(uint32_t v, uint32_t l) { return __builtin_amdgcn_readlane(v, l); }
 
 // s + step, n times (0 <= n <= 7), each add rounded, without branches.
 __device__ __forceinline__ float fold_upto7(float s, float step, int n) {
@@ -969,6 +949,9 @@ template <int TBX, int TBY, int NW, bool STATS, bool LABELS>
 __device__ uint64_t draw_chunk_block_major(TileSmem<TBX, TBY>& sm, uint32_t n, int tbx0, int tbx1, int tby0, int tby1,
                                            int bX, float invBX, uint32_t wave, uint32_t lane, BlockCounters& bc,
                                            uint32_t* __restrict__ winner) {
+  // The wave index as a scalar, so that block indices and fold counts
+  // derived from it are wave-uniform.
+  wave = __builtin_amdgcn_readfirstlane(wave);
   const uint32_t g = lane / 12u, e = lane - 12u * g;
   for (uint32_t q = wave; q < n; q += NW) {
     const QuadRec& R = sm.recs[q];
@@ -1011,21 +994,36 @@ __device__ uint64_t draw_chunk_block_major(TileSmem<TBX, TBY>& sm, uint32_t n, i
 
   const int kLane = int(((lane & 7u) & 3u) + 4u * ((lane >> 3) & 1u));
   const bool valid = lane < n;
+  // Lane q holds staged quad q's constants for phase B (read with readlane):
+  // its rectangle in the tile, wrap flag, mode and packed maximum depth,
+  // slope lookups and depth gradients.
   const uint32_t rect = valid ? sm.qinfo[lane][0] : 0xFFu;
-  const uint32_t maxZq = valid ? (sm.recs[lane].modeZ >> 16) : 0u;
+  const uint32_t wrapq = valid ? sm.qinfo[lane][3] : 0u;
+  const uint32_t modeZq = valid ? sm.recs[lane].modeZ : 0u;
+  const uint32_t maxZq = modeZq >> 16;
+  const uint32_t sl01q = valid ? sm.recs[lane].slope01 : 0u, sl23q = valid ? sm.recs[lane].slope23 : 0u;
+  const float dDxq = valid ? sm.recs[lane].dDx : 0.0f, dDyq = valid ? sm.recs[lane].dDy : 0.0f;
   const int qx0 = int(rect & 0xFFu), qx1 = int((rect >> 8) & 0xFFu);
   const int qy0 = int((rect >> 16) & 0xFFu), qy1 = int(rect >> 24);
   const uint32_t ce = lane < 12u ? lane : 0u;
   // The wave's blocks: (lbx + lby) & (NW - 1) == wave, TBX / NW per row.
   static_assert(TBX % NW == 0, "diagonal block ownership");
   constexpr int kPerRow = TBX / NW, kOwned = TBX * TBY / NW;
-  // Two blocks are drawn at a time: their candidate loops are independent
-  // chains, interleaved so that one's latencies overlap the other's.
+  // Two blocks are drawn at a time, two candidates per step: the samples
+  // depend on nothing the merges change, so they are computed together and
+  // only the merges run in submission order.
+  //
+  // A block's state: until its first write in this chunk, the hiZ it was
+  // opened with (which the reject test and the fast-clear test read); after
+  // it, hiZ is the minimum of the depth in registers, so the reject test is a
+  // ballot over the depth (hiZ >= maxZ exactly when no pixel is below maxZ)
+  // and the fast-clear test (hiZ == 1) a flag kept by two ballots.  The
+  // minimum itself is taken once, when the block is closed.
   struct Slot {
-    int lb;  // lbx = lb % TBX, lby = lb / TBX
-    uint32_t hz, dval, win;
-    bool dirty;
-    uint64_t m;  // candidates left, in submission order
+    int lb = 0;  // lbx = lb % TBX, lby = lb / TBX
+    uint32_t hz = 0, dval = 0, win = kNoWinner;
+    bool dirty = false, one = false;
+    uint64_t m = 0;  // candidates left, in submission order
   };
   int nextK = 0;
   // Next owned block with candidates into S (false when none is left).
@@ -1050,6 +1048,7 @@ __device__ uint64_t draw_chunk_block_major(TileSmem<TBX, TBY>& sm, uint32_t n, i
       S.m = m;
       S.dval = sm.depth[lb * 64 + int(lane)];
       S.dirty = false;
+      S.one = false;
       S.win = kNoWinner;
       if (LABELS && hz != 1u) S.win = winner[(size_t(tby0 + lby) * size_t(bX) + size_t(tbx0 + lbx)) * 64u + lane];
       return true;
@@ -1059,67 +1058,136 @@ __device__ uint64_t draw_chunk_block_major(TileSmem<TBX, TBY>& sm, uint32_t n, i
   auto close = [&](const Slot& S) {
     if (!S.dirty) return;
     sm.depth[S.lb * 64 + int(lane)] = uint16_t(S.dval);
-    if (lane == 0) sm.hiz[S.lb] = uint16_t(S.hz);
+    const uint32_t mn = __ockl_wfred_min_u32(S.dval);
+    if (lane == 0) sm.hiz[S.lb] = uint16_t(mn);
     if (LABELS)
       winner[(size_t(tby0 + S.lb / TBX) * size_t(bX) + size_t(tbx0 + S.lb % TBX)) * 64u + lane] = S.win;
   };
-  // The block's twelve components of staged quad q (lane ce holds component
-  // ce): false when a quad with a wrapped first-block index does not cover
-  // the block.
-  auto component = [&](const Slot& S, uint32_t q, float& sv) -> bool {
-    const QuadRec& R = sm.recs[q];
-    if (sm.qinfo[q][3] & 1u) {
-      // Wrapped first-block index: this block's (row, column) in the quad's
-      // block numbering, folded from the origin.
-      const int d = (tby0 + S.lb / TBX) * bX + tbx0 + S.lb % TBX - int(R.fb);
-      const int rangeX = int(R.range & 0xFFFFu), rangeY = int(R.range >> 16);
-      if (d < 0) return false;
-      const int r = div_bx(d, bX, invBX), c = d - r * bX;
-      if (r >= rangeY || c >= rangeX) return false;
-      float base, stepY, stepX;
-      quad_component(R, ce, base, stepY, stepX);
-      sv = fold_n(fold_n(base, stepY, r), stepX, c);
+  auto qconst = [&](uint32_t q) -> QuadConst {
+    QuadConst Q;
+    const uint32_t mz = readlane_u(modeZq, q), s01 = readlane_u(sl01q, q), s23 = readlane_u(sl23q, q);
+    Q.mode = mz & 0xFFu;
+    Q.maxZ = mz >> 16;
+    Q.sl[0] = s01 & 0xFFFFu;
+    Q.sl[1] = s01 >> 16;
+    Q.sl[2] = s23 & 0xFFFFu;
+    Q.sl[3] = s23 >> 16;
+    Q.dDx = readlane_f(dDxq, q);
+    Q.dDy = readlane_f(dDyq, q);
+    return Q;
+  };
+  // Staged quad q's sample for block lb when its first-block index did not
+  // wrap: phase A's components at the rectangle's first column in this row,
+  // folded with the column steps left; every lane folds the four edge
+  // offsets and its own depth sample.
+  auto sample_fast = [&](int lb, uint32_t q, uint32_t& d, bool& skip) {
+    const QuadConst Q = qconst(q);
+    const int c = lb % TBX - int(readlane_u(rect, q) & 0xFFu);
+    const float* E = &sm.entry[(q * TBY + uint32_t(lb / TBX)) * 12u];
+    const float* nx = sm.recs[q].nx;
+    const float n0 = nx[0], n1 = nx[1], n2 = nx[2], n3 = nx[3];
+    float o[4] = {E[0], E[1], E[2], E[3]};
+    float v = E[4 + kLane];
+    // c is wave-uniform (0..7): a scalar loop, five independent adds a step.
+    for (int k = 0; k < c; ++k) {
+      o[0] = o[0] + n0;
+      o[1] = o[1] + n1;
+      o[2] = o[2] + n2;
+      o[3] = o[3] + n3;
+      v = v + Q.dDx;
+    }
+    sample_block(sm.cov, o, v, lane, Q, d, skip);
+  };
+  // Any staged quad: false when a quad with a wrapped first-block index does
+  // not cover the block; such a quad's block (row, column) in its block
+  // numbering is found and its twelve components folded from the origin.
+  auto sample_any = [&](int lb, uint32_t q, uint32_t& d, bool& skip) -> bool {
+    if (!(readlane_u(wrapq, q) & 1u)) {
+      sample_fast(lb, q, d, skip);
       return true;
     }
-    const int x0 = int(sm.qinfo[q][0] & 0xFFu);
-    const float stepX = ce < 4u ? (ce == 0 ? R.nx[0] : ce == 1 ? R.nx[1] : ce == 2 ? R.nx[2] : R.nx[3]) : R.dDx;
-    sv = fold_upto7(sm.entry[(q * TBY + uint32_t(S.lb / TBX)) * 12u + ce], stepX, S.lb % TBX - x0);
+    const QuadRec& R = sm.recs[q];
+    const int dd = (tby0 + lb / TBX) * bX + tbx0 + lb % TBX - int(R.fb);
+    const int rangeX = int(R.range & 0xFFFFu), rangeY = int(R.range >> 16);
+    if (dd < 0) return false;
+    const int r = div_bx(dd, bX, invBX), c = dd - r * bX;
+    if (r >= rangeY || c >= rangeX) return false;
+    float base, stepY, stepX;
+    quad_component(R, ce, base, stepY, stepX);
+    const float s = fold_n(fold_n(base, stepY, r), stepX, c);
+    float o[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) o[k] = readlane_f(s, k);
+    const float v = __shfl(s, 4 + kLane);
+    sample_block(sm.cov, o, v, lane, qconst(q), d, skip);
     return true;
   };
-  // One candidate visit of slot S: the hiZ test first (most visits end
-  // there), then the block's coverage and depth and the merge.
+  // Candidate q of slot S, in submission order: the hiZ reject, then the
+  // merge (max unless the block is fast-cleared), Rasterizer.cpp:1148-1290.
+  //
+  // LABELS: `win` is the lane's pixel's winner, the batch whose write
+  // produced its depth: the first write of a cleared block (hiZ == 1) sets it
+  // (none where the written depth is 0), later writes only where they raise
+  // the depth.
+  auto visit = [&](Slot& S, uint32_t q, uint32_t d, bool skip) {
+    const uint32_t maxZ = readlane_u(maxZq, q);
+    if (STATS) bc.visited++;
+    const bool rejected = S.dirty ? __ballot(S.dval < maxZ) == 0ull : S.hz >= maxZ;
+    if (rejected || skip) {
+      if (STATS) bc.rejected++;
+      return;
+    }
+    if (STATS) bc.written++;
+    const bool cleared = S.dirty ? S.one : S.hz == 1u;
+    if (LABELS) {
+      const uint32_t batch = sm.qid[q] >> 3;
+      S.win = cleared ? (d != 0u ? batch : kNoWinner) : (d > S.dval ? batch : S.win);
+    }
+    if (!cleared) d = max(d, S.dval);
+    S.dval = d;
+    // hiZ == 1 after this write: no pixel at 0 and one at 1.
+    S.one = __ballot(d == 0u) == 0ull && __ballot(d == 1u) != 0ull;
+    S.dirty = true;
+  };
+  // Two samples per step: the next candidates of both open blocks, or the
+  // next two of the only open one (kept in A).
   Slot A, B;
   bool aOn = open(A), bOn = open(B);
   while (aOn || bOn) {
-    const uint32_t qa = aOn ? uint32_t(__builtin_ctzll(A.m)) : 0u;
-    const uint32_t qb = bOn ? uint32_t(__builtin_ctzll(B.m)) : 0u;
-    if (aOn) A.m &= A.m - 1ull;
-    if (bOn) B.m &= B.m - 1ull;
-    float sa = 0.0f, sb = 0.0f;
-    bool va = aOn && component(A, qa, sa);
-    bool vb = bOn && component(B, qb, sb);
-    const QuadConst Qa = quad_const(sm.recs[qa]), Qb = quad_const(sm.recs[qb]);
-    if (va && A.hz >= Qa.maxZ) {
-      va = false;
-      if (STATS) {
-        bc.visited++;
-        bc.rejected++;
-      }
+    if (!aOn) {
+      A = B;
+      aOn = true;
+      bOn = false;
     }
-    if (vb && B.hz >= Qb.maxZ) {
-      vb = false;
-      if (STATS) {
-        bc.visited++;
-        bc.rejected++;
-      }
+    // An open slot has at least one candidate left.
+    const uint32_t qa = uint32_t(__builtin_ctzll(A.m));
+    A.m &= A.m - 1ull;
+    uint32_t qb = 0;
+    const bool second = bOn || A.m != 0ull;
+    if (bOn) {
+      qb = uint32_t(__builtin_ctzll(B.m));
+      B.m &= B.m - 1ull;
+    } else if (second) {
+      qb = uint32_t(__builtin_ctzll(A.m));
+      A.m &= A.m - 1ull;
     }
+    const int lbb = bOn ? B.lb : A.lb;
     uint32_t da = 0, db = 0;
-    bool ka = true, kb = true;
-    if (va) sample_block_reg(sm.cov, sa, lane, kLane, Qa, da, ka);
-    if (vb) sample_block_reg(sm.cov, sb, lane, kLane, Qb, db, kb);
-    if (va) merge_block_reg<STATS, LABELS>(da, ka, Qa.maxZ, A.hz, A.dval, A.dirty, bc, A.win, LABELS ? sm.qid[qa] >> 3 : 0u);
-    if (vb) merge_block_reg<STATS, LABELS>(db, kb, Qb.maxZ, B.hz, B.dval, B.dirty, bc, B.win, LABELS ? sm.qid[qb] >> 3 : 0u);
-    if (aOn && A.m == 0ull) {
+    bool ka = true, kb = true, ca = true, cb = second;
+    if (!((readlane_u(wrapq, qa) | readlane_u(wrapq, qb)) & 1u)) {
+      // Without a second candidate, quad 0 is sampled and not used.
+      sample_fast(A.lb, qa, da, ka);
+      sample_fast(lbb, qb, db, kb);
+    } else {
+      ca = sample_any(A.lb, qa, da, ka);
+      cb = cb && sample_any(lbb, qb, db, kb);
+    }
+    if (ca) visit(A, qa, da, ka);
+    if (cb) {
+      if (bOn) visit(B, qb, db, kb);
+      else visit(A, qb, db, kb);
+    }
+    if (A.m == 0ull) {
       close(A);
       aOn = open(A);
     }
