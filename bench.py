@@ -6,15 +6,17 @@ fBm heightfield, 285,697 quads in 878 SAH-batched occluders -- rendered with
 the reference's frame sequence (Main.cpp:180-206: clear,
 setModelViewProjection, query-gated rasterize of every occluder in
 front-to-back order) for K frames of a 256-frame camera path, then 65,536
-AABB queries against a frame's buffer.  A step is one frame.  The draw order
-of each frame is computed before timing; inputs are resident in HBM.
+AABB queries against each frame's buffer.  A step is one frame.  The draw
+order of each frame is computed before timing; inputs are resident in HBM.
 
   value         = real submitted quads per second over the K frames, with
                   the frames in flight together: F frames per persistent
-                  launch, each frame query-gated on its own buffer (bit-exact
-                  with the single-view path from the same buffer)
-  single_frame  = the same frames one at a time through rz_draw_frame
-  queries_per_s = AABB queries per second of the query phase
+                  launch, each frame query-gated on its own buffer and then
+                  queried (bit-exact with the single-view path from the same
+                  buffer)
+  queries_per_s = AABB queries per second of the same timed launches
+  single_frame  = the same frames and queries one at a time through
+                  rz_draw_frame and rz_query_batch_device
 
 Also reports the roofline of the dominant kernel (the multi-view raster
 kernel of the frames-in-flight launch), timed with HIP events on its own
@@ -354,8 +356,9 @@ def learned_frames_in_flight(R, scene, orders, handles, plans, F: int, tuned: di
     tracking off) instead of front to back, still query-gated: the batched +
     learned path at 1920x1080.  Frames 0 and F-1 are checked bit for bit
     against the single-view path drawing the occluders in the order the
-    device chose.  Reports quads/s (submitted real quads), occluders drawn
-    per frame against the front-to-back launches, and the time."""
+    device chose, queries included.  Reports quads/s (submitted real quads),
+    occluders drawn per frame against the front-to-back launches, and the
+    time."""
     from rasterizer_amd import learn
 
     path = learn.WEIGHTS.with_name("order_mlp_city.npz")
@@ -396,7 +399,10 @@ def learned_frames_in_flight(R, scene, orders, handles, plans, F: int, tuned: di
         want[order] = st
         d, h = R.raw()
         dv, hv = V0.raw(v)
-        if not (np.array_equal(d, dv) and np.array_equal(h, hv) and np.array_equal(V0.status(v), want)):
+        vis, clip = R.query_batch(m["queries"])
+        vv, cv = V0.queries(v)
+        if not (np.array_equal(d, dv) and np.array_equal(h, hv) and np.array_equal(V0.status(v), want) and
+                np.array_equal(vis, vv) and np.array_equal(clip, cv)):
             raise RuntimeError(f"batched + learned: frame {f} differs from the single-view path in the learned order")
     for Vj in plans:
         Vj.run(graph=False)
@@ -423,6 +429,7 @@ def learned_frames_in_flight(R, scene, orders, handles, plans, F: int, tuned: di
     return {
         "model": str(path.relative_to(ROOT)),
         "quads_per_s": value,
+        "queries_per_s": K * m["queries"].shape[0] / t,
         "ms_per_frame": t * 1e3 / K,
         "front_to_back_quads_per_s": plain_value,
         "front_to_back_ms_per_frame": t_plain * 1e3 / K,
@@ -508,6 +515,7 @@ def main() -> int:
         for v in range(F):
             f = (j * F + v) % nf
             V.set_occluders(v, [scene.occluders[i] for i in orders[f]])
+            V.set_queries(v, q)
             V.set_mvp(v, m["mvps"][f])
 
     plans = []
@@ -527,6 +535,7 @@ def main() -> int:
             for v in range(F):
                 f = (j * F + v) % nf
                 Vj.set_occluders(v, [scene.occluders[i] for i in orders[f]])
+                Vj.set_queries(v, q)
                 Vj.set_mvp(v, m["mvps"][f])
             Vj.run(graph=False)
             Vj.sync()
@@ -543,7 +552,10 @@ def main() -> int:
         R.draw_frame_handles(handles[f], n_occ)
         d, h = R.raw()
         dv, hv = plans[0].raw(v)
-        if not (np.array_equal(d, dv) and np.array_equal(h, hv)):
+        vis, clip = R.query_batch(q)
+        vv, cv = plans[0].queries(v)
+        if not (np.array_equal(d, dv) and np.array_equal(h, hv) and np.array_equal(vis, vv) and
+                np.array_equal(clip, cv)):
             raise RuntimeError(f"frames in flight: frame {f} differs from the single-view path")
     # Launch configuration by the autotuner on the first plan (identical bits
     # required; untimed), applied to every plan.
@@ -572,9 +584,11 @@ def main() -> int:
         Vj.profile(False)
     raster_ms = float(np.mean([p["raster_ms"] for p in phases]))
     setup_ms = float(np.mean([p["setup_ms"] for p in phases]))
+    query_ms = float(np.mean([p["query_ms"] for p in phases]))
     value = K * scene.quad_count / t_fif
     log(f"frames in flight: {K} frames in {launches} launch(es) of {F}: {t_fif * 1e3:.3f} ms "
-        f"({value / 1e6:.1f} M quads/s); per launch: raster kernel {raster_ms:.3f} ms, setup phase {setup_ms:.3f} ms")
+        f"({value / 1e6:.1f} M quads/s, {K * nq / t_fif / 1e9:.2f} G queries/s); per launch: raster kernel "
+        f"{raster_ms:.3f} ms, setup phase {setup_ms:.3f} ms, queries {query_ms:.3f} ms")
 
     learned_fif = (None if args.no_learned else
                    learned_frames_in_flight(R, scene, orders, handles, plans, F, fif_tuned, t_fif, value))
@@ -619,7 +633,7 @@ def main() -> int:
     alg_frame = algorithmic_bytes_per_frame(scene.quad_count, W, H)
     alg_launch = F * alg_frame
     achieved = alg_launch / (raster_ms * 1e-3) / 1e9
-    queries_per_s = K * nq / t_query
+    queries_per_s = K * nq / t_fif
     out = {
         "metric": METRIC,
         "value": value,
@@ -635,19 +649,20 @@ def main() -> int:
         "data": "synthetic",
         "config": {
             "workload": (f"city+terrain {W}x{H}: {scene.quad_count:,} quads in {n_occ} occluders, frames 0-{K - 1} "
-                         f"of a {nf}-frame street-level camera path, query-gated draw in front-to-back order; "
-                         f"{nq:,} AABB queries per frame"),
+                         f"of a {nf}-frame street-level camera path, query-gated draw in front-to-back order, "
+                         f"then {nq:,} AABB queries against each frame's buffer, in the same timed launches"),
             "mode": f"plain method, {F} frames in flight per persistent launch (query-gated views)",
             "seed": args.seed,
         },
         "queries_per_s": queries_per_s,
         "frames_in_flight": {"frames_per_launch": F, "launches": launches, "wall_ms": t_fif * 1e3,
                              "launch_config": fif_tuned,
-                             "raster_kernel_ms_per_launch": raster_ms, "setup_phase_ms_per_launch": setup_ms},
+                             "raster_kernel_ms_per_launch": raster_ms, "setup_phase_ms_per_launch": setup_ms,
+                             "query_kernel_ms_per_launch": query_ms},
         "single_frame": {"quads_per_s": K * scene.quad_count / t_raster, "ms_per_frame": t_raster * 1e3 / K,
                          "kernel_ms": kernel_ms,
-                         "frame_kernel_hbm_gbs": alg_frame / (frame_ms * 1e-3) / 1e9},
-        "query_ms_per_batch": t_query * 1e3 / K,
+                         "frame_kernel_hbm_gbs": alg_frame / (frame_ms * 1e-3) / 1e9,
+                         "queries_per_s": K * nq / t_query, "query_ms_per_batch": t_query * 1e3 / K},
         "batched_learned_1080p": learned_fif,
         "roofline": {
             "bound": "hbm",
