@@ -890,6 +890,50 @@ __device__ __forceinline__ void sample_block(const uint64_t* __restrict__ cov, c
 
 __device__ __forceinline__ uint32_t readlane_u(uint32_t v, uint32_t l) { return __builtin_amdgcn_readlane(v, l); }
 
+__device__ __forceinline__ uint64_t readlane_u64(uint64_t v, uint32_t l) {
+  return uint64_t(readlane_u(uint32_t(v), l)) | uint64_t(readlane_u(uint32_t(v >> 32), l)) << 32;
+}
+
+// sample_block with the four edge lookups spread over the lanes: lane L
+// holds edge (L & 3)'s folded offset in `oe` and looks up its coverage word;
+// lanes 0-3's words come back to the wave as scalars, where the mode's mask
+// logic runs.  `sl01` / `sl23` are the quad's packed slope lookups.  The
+// same values as sample_block, with one LDS lookup instead of four and the
+// per-edge index arithmetic done once for all four edges.
+__device__ __forceinline__ void sample_block_lanes(const uint64_t* __restrict__ cov, float oe, float v, uint32_t lane,
+                                                   const QuadConst& Q, uint32_t sl01, uint32_t sl23, uint32_t& dOut,
+                                                   bool& skip) {
+  const bool convex = Q.mode == kModeConvex;
+  const uint32_t e = lane & 3u;
+  const uint32_t slw = (e & 2u) ? sl23 : sl01;
+  const uint32_t sl = (e & 1u) ? (slw >> 16) : (slw & 0xFFFFu);
+  const int32_t t = cvtt_x86(oe);
+  const uint32_t tc = convex ? uint32_t(max(t, 0)) : uint32_t(min(max(t, 0), 63));
+  const uint64_t word = cov[(sl | tc) & 4095u];
+  const uint64_t A = readlane_u64(word, 0), B = readlane_u64(word, 1), C = readlane_u64(word, 2),
+                 D = readlane_u64(word, 3);
+  const bool outside = (__ballot(oe >= 63.0f) & 0xFull) != 0ull;
+  uint64_t mask;
+  if (convex) mask = (A & B) & (C & D);
+  else if (Q.mode == kModeTriangle0) mask = A & B & C;
+  else if (Q.mode == kModeTriangle1) mask = A & C & D;
+  else if (Q.mode == kModeConcaveRight) mask = (A | D) & (B & C);
+  else if (Q.mode == kModeConcaveCenter) mask = (A & B) | (C & D);
+  else mask = (A & D) & (B | C);
+  skip = convex ? outside : mask == 0ull;
+  const uint32_t px = lane & 7u, py = lane >> 3;
+  if (px >= 4u) v = fmaf(Q.dDx, 0.5f, v);
+  const uint32_t lo = pack_depth(v);
+  const uint32_t hi = pack_depth(Q.dDy + v);
+  const uint32_t mid = avg_u16(lo, hi);
+  const uint32_t q1 = avg_u16(lo, mid), q3 = avg_u16(mid, hi);
+  const uint32_t j = py >> 1;
+  const uint32_t a01 = (j & 1u) ? q1 : lo, a23 = (j & 1u) ? q3 : mid;
+  const uint32_t d = (j & 2u) ? a23 : a01;
+  const uint32_t maskBit = 8u * px + ((py & 1u) ? (py >> 1) : 4u + (py >> 1));
+  dOut = ((mask >> maskBit) & 1ull) ? d : 0u;
+}
+
 // s + step, n times (0 <= n <= 7), each add rounded, without branches.
 __device__ __forceinline__ float fold_upto7(float s, float step, int n) {
 #pragma unroll
@@ -1084,19 +1128,16 @@ __device__ uint64_t draw_chunk_block_major(TileSmem<TBX, TBY>& sm, uint32_t n, i
     const QuadConst Q = qconst(q);
     const int c = lb % TBX - int(readlane_u(rect, q) & 0xFFu);
     const float* E = &sm.entry[(q * TBY + uint32_t(lb / TBX)) * 12u];
-    const float* nx = sm.recs[q].nx;
-    const float n0 = nx[0], n1 = nx[1], n2 = nx[2], n3 = nx[3];
-    float o[4] = {E[0], E[1], E[2], E[3]};
+    const float step = sm.recs[q].nx[lane & 3u];
+    float oe = E[lane & 3u];
     float v = E[4 + kLane];
-    // c is wave-uniform (0..7): a scalar loop, five independent adds a step.
+    // c is wave-uniform (0..7): a scalar loop of two adds a step, the lane's
+    // edge offset and its depth sample.
     for (int k = 0; k < c; ++k) {
-      o[0] = o[0] + n0;
-      o[1] = o[1] + n1;
-      o[2] = o[2] + n2;
-      o[3] = o[3] + n3;
+      oe = oe + step;
       v = v + Q.dDx;
     }
-    sample_block(sm.cov, o, v, lane, Q, d, skip);
+    sample_block_lanes(sm.cov, oe, v, lane, Q, readlane_u(sl01q, q), readlane_u(sl23q, q), d, skip);
   };
   // Any staged quad: false when a quad with a wrapped first-block index does
   // not cover the block; such a quad's block (row, column) in its block
