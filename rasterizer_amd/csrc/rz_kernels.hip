@@ -610,6 +610,12 @@ __device__ __forceinline__ float readlane_f(float v, int l) {
   return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
 }
 
+__device__ __forceinline__ uint32_t readlane_u(uint32_t v, uint32_t l) { return __builtin_amdgcn_readlane(v, l); }
+
+__device__ __forceinline__ uint64_t readlane_u64(uint64_t v, uint32_t l) {
+  return uint64_t(readlane_u(uint32_t(v), l)) | uint64_t(readlane_u(uint32_t(v >> 32), l)) << 32;
+}
+
 __device__ __forceinline__ int div_bx(int d, int bX, float invBX) {
   int q = int(float(d) * invBX);
   if (q * bX > d) --q;
@@ -704,25 +710,21 @@ __device__ __forceinline__ void raster_block(TileSmem<TBX, TBY>& sm, int lb, flo
     if (STATS) bc.rejected++;
     return;
   }
-  float o[4];
-  uint32_t idx[4];
-  bool outside = false;
-#pragma unroll
-  for (int e = 0; e < 4; ++e) {
-    o[e] = readlane_f(s, 12 * g + e);
-    const int32_t t = cvtt_x86(o[e]);
-    if (Q.mode == kModeConvex) {
-      outside |= o[e] >= 63.0f;
-      idx[e] = (Q.sl[e] | uint32_t(max(t, 0))) & 4095u;
-    } else {
-      idx[e] = (Q.sl[e] | uint32_t(min(max(t, 0), 63))) & 4095u;
-    }
-  }
-  if (outside) {
+  // Lane L looks up edge (L & 3): its folded offset from lane 12g + (L & 3),
+  // one coverage read, and lanes 0-3's words back to the wave as scalars.
+  const uint32_t e = lane & 3u;
+  const float oe = __shfl(s, 12 * g + int(e));
+  const bool convex = Q.mode == kModeConvex;
+  if (convex && (__ballot(oe >= 63.0f) & 0xFull) != 0ull) {
     if (STATS) bc.rejected++;
     return;
   }
-  const uint64_t A = sm.cov[idx[0]], B = sm.cov[idx[1]], C = sm.cov[idx[2]], D = sm.cov[idx[3]];
+  const uint32_t sl = e == 0u ? Q.sl[0] : e == 1u ? Q.sl[1] : e == 2u ? Q.sl[2] : Q.sl[3];
+  const int32_t t = cvtt_x86(oe);
+  const uint32_t tc = convex ? uint32_t(max(t, 0)) : uint32_t(min(max(t, 0), 63));
+  const uint64_t word = sm.cov[(sl | tc) & 4095u];
+  const uint64_t A = readlane_u64(word, 0), B = readlane_u64(word, 1), C = readlane_u64(word, 2),
+                 D = readlane_u64(word, 3);
   uint64_t mask;
   if (Q.mode == kModeConvex) mask = (A & B) & (C & D);
   else if (Q.mode == kModeTriangle0) mask = A & B & C;
@@ -888,11 +890,6 @@ __device__ __forceinline__ void sample_block(const uint64_t* __restrict__ cov, c
   dOut = ((mask >> maskBit) & 1ull) ? d : 0u;
 }
 
-__device__ __forceinline__ uint32_t readlane_u(uint32_t v, uint32_t l) { return __builtin_amdgcn_readlane(v, l); }
-
-__device__ __forceinline__ uint64_t readlane_u64(uint64_t v, uint32_t l) {
-  return uint64_t(readlane_u(uint32_t(v), l)) | uint64_t(readlane_u(uint32_t(v >> 32), l)) << 32;
-}
 
 // sample_block with the four edge lookups spread over the lanes: lane L
 // holds edge (L & 3)'s folded offset in `oe` and looks up its coverage word;
