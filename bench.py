@@ -184,8 +184,11 @@ def views_bench(steps: int, warmup: int, seed: int = 3) -> dict:
     from rasterizer_amd import learn
 
     learned = {}
-    for name, path in (("without labels", learn.WEIGHTS.with_name("order_mlp_nolabels.npz")),
-                       ("with labels", learn.WEIGHTS)):
+    models = [("without labels", learn.WEIGHTS.with_name("order_mlp_nolabels.npz")), ("with labels", learn.WEIGHTS)]
+    scene_model = learn.WEIGHTS.with_name("order_mlp_boxes3.npz")  # trained on this workload's box scene
+    if scene_model.exists():
+        models.insert(0, ("trained on this scene, without labels", scene_model))
+    for name, path in models:
         V.set_model(learn.load(path), by_occluder=True)
         t_l, gpu_l = time_batched(f"batched, learned order ({name})")
         cnt_l = block_counters()

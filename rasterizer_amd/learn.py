@@ -184,8 +184,10 @@ def main() -> int:
     import argparse
 
     ap = argparse.ArgumentParser(description="train the batch-ordering MLP")
-    ap.add_argument("--scene", choices=["boxes", "city"], default="boxes",
+    ap.add_argument("--scene", choices=["boxes", "boxes3", "city"], default="boxes",
                     help="boxes: the 1,024-box scene, random cameras in its cube (1280x720); "
+                         "boxes3: the 40,000-box scene of the batched multi-view workload, cameras inside its "
+                         "cube as its views are placed (1024x1024); "
                          "city: the city scene, street-level cameras (1920x1080)")
     ap.add_argument("--frames", type=int, default=10000)
     ap.add_argument("--steps", type=int, default=400)
@@ -197,6 +199,10 @@ def main() -> int:
     if args.scene == "boxes":
         scene = scenes.config1(seed=1)
         feats, labels = generate(scene, args.frames + n_test, seed=10)
+    elif args.scene == "boxes3":
+        scene = scenes.config3(seed=3, sizes=[])
+        feats, labels = generate(scene, args.frames + n_test, seed=13, width=1024, height=1024, views_per_launch=32,
+                                 camera=lambda rng: random_camera(rng, half=0.8 * 171.0))
     else:
         scene = scenes.config2(seed=1, frames=8, n_queries=16)
         feats, labels = generate(scene, args.frames + n_test, seed=12, width=1920, height=1080, views_per_launch=32,
