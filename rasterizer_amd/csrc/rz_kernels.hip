@@ -192,12 +192,26 @@ __device__ bool query2d_wave(uint32_t minX, uint32_t maxX, uint32_t minY, uint32
           if (startY == 0 && endY == 7 && startX == 0 && endX == 7) {
             v = true;
           } else {
+            // The rectangle's rows of the block, four rows' loads issued
+            // before their tests (a row-by-row loop with an early exit would
+            // wait for each load in turn).
             const uint4* rows = reinterpret_cast<const uint4*>(depth + size_t(b) * 64);
-            for (uint32_t y = startY; y <= endY && !v; ++y) {
-              const uint4 r = rows[y];
-              const uint32_t w[4] = {r.x, r.y, r.z, r.w};
-              for (uint32_t x = startX; x <= endX; ++x)
-                v |= ((w[x >> 1] >> (16 * (x & 1))) & 0xFFFFu) < maxZ;
+            for (uint32_t h = startY & ~3u; h <= endY && !v; h += 4) {
+              uint4 r[4];
+#pragma unroll
+              for (uint32_t j = 0; j < 4; ++j) {
+                r[j] = make_uint4(0, 0, 0, 0);
+                if (h + j >= startY && h + j <= endY) r[j] = rows[h + j];
+              }
+#pragma unroll
+              for (uint32_t j = 0; j < 4; ++j) {
+                const uint32_t y = h + j;
+                const uint32_t w[4] = {r[j].x, r[j].y, r[j].z, r[j].w};
+#pragma unroll
+                for (uint32_t x = 0; x < 8; ++x)
+                  v |= y >= startY && y <= endY && x >= startX && x <= endX &&
+                       ((w[x >> 1] >> (16 * (x & 1))) & 0xFFFFu) < maxZ;
+              }
             }
           }
         }
