@@ -280,7 +280,7 @@ int rz_frame_stats_read(rz_context* ctx, uint64_t* out, uint32_t* count);
  * ------------------------------------------------------------------------- */
 typedef struct rz_frame_config {
   uint32_t tileW; /* tile edge in 8x8 blocks: 8 (64-pixel bins) or 16 (128-pixel bins) */
-  uint32_t waves; /* wave64s per tile work-group: 4 or 8 */
+  uint32_t waves; /* wave64s per tile work-group: 2, 4 or 8 */
   float ms;       /* rz_tune_frame: median time of one rz_draw_frame; -1 if the grid cannot be co-resident */
 } rz_frame_config;
 
@@ -288,16 +288,17 @@ int rz_get_config(rz_context* ctx, rz_frame_config* config);
 /* RZ_ERR_TOO_LARGE (configuration unchanged) when the tile grid cannot be
  * co-resident on the device. */
 int rz_set_config(rz_context* ctx, uint32_t tileW, uint32_t waves);
-/* Autotuner: for every configuration (tileW x waves) that fits, clears and
+/* Autotuner: for every configuration (tileW {8, 16} x waves {2, 4, 8}) that
+ * fits, clears and
  * draws the frame of occ (rz_draw_frame with the context's matrix), requires
  * identical depth, hiZ and statuses from all of them (else
  * RZ_ERR_NOT_IDENTICAL, configuration unchanged), and times `reps` draws
  * (HIP events, median).  Applies the fastest and, with a non-NULL cachePath,
- * records it there for this device and buffer size.  candidates (optional, 4
+ * records it there for this device and buffer size.  candidates (optional, 6
  * entries) receives every configuration tried.  Leaves the drawn frame in
  * the buffer.  Blocks. */
 int rz_tune_frame(rz_context* ctx, uint32_t n, const rz_occluder* const* occ, uint32_t reps, const char* cachePath,
-                  rz_frame_config candidates[4], uint32_t* nCandidates);
+                  rz_frame_config candidates[6], uint32_t* nCandidates);
 /* Applies the configuration cachePath records for this device and buffer
  * size; *found = 0 when it has none.  rz_create does this itself when the
  * environment variable RZ_TUNE_CACHE names a cache file. */
@@ -393,7 +394,7 @@ int rz_views_get_info(rz_views* views, uint32_t info[8]);
 
 /* Launch configuration of the multi-view raster kernel (no counterpart in the
  * reference): the tile (bin) edge in 8x8 blocks, 8 (64-pixel bins) or 16
- * (128-pixel bins), wave64s per (view, tile) work-group, 4 or 8, and
+ * (128-pixel bins), wave64s per (view, tile) work-group, 2, 4 or 8, and
  * persistent work-groups per CU (0: as many as fit).  Labelled steps (labels
  * on, or a model that reads them) draw with 8x8-block tiles whatever the
  * setting.  A query-gated step whose views' 8x8-block tiles cannot all be
@@ -406,15 +407,15 @@ typedef struct rz_views_config {
 } rz_views_config;
 int rz_views_get_config(rz_views* views, rz_views_config* config);
 int rz_views_set_config(rz_views* views, uint32_t tileW, uint32_t waves, uint32_t groupsPerCU);
-/* Autotuner over tileW {8, 16} x waves {4, 8} x groupsPerCU {1, all that
+/* Autotuner over tileW {8, 16} x waves {2, 4, 8} x groupsPerCU {1, all that
  * fit}: runs the current step (matrices, occluders, queries, order) under
  * each, requires identical buffers, query bits and labels from all (else
  * RZ_ERR_NOT_IDENTICAL, configuration unchanged; configurations whose grid
  * cannot hold a query-gated view's tiles, and 16x16 tiles in labelled steps,
  * are skipped), times `reps` steps each, applies the fastest and records it
  * in cachePath (if not NULL) for this device and set of view sizes.
- * candidates (optional) receives up to 8 entries.  Blocks. */
-int rz_views_tune(rz_views* views, uint32_t reps, const char* cachePath, rz_views_config candidates[8],
+ * candidates (optional) receives up to 12 entries.  Blocks. */
+int rz_views_tune(rz_views* views, uint32_t reps, const char* cachePath, rz_views_config candidates[12],
                   uint32_t* nCandidates);
 /* As rz_load_config; rz_views_create does it itself with RZ_TUNE_CACHE. */
 int rz_views_load_config(rz_views* views, const char* cachePath, int* found);

@@ -460,7 +460,7 @@ class Rasterizer:
         (with identical bits required) and keeps the fastest."""
         n = len(occluders)
         handles = self.handle_array(occluders)
-        cands = (FrameConfig * 4)()
+        cands = (FrameConfig * 6)()
         k = ctypes.c_uint32(0)
         _check(lib().rz_tune_frame(self._ctx, n, ctypes.cast(handles, ctypes.c_void_p), reps,
                                    os.fsencode(cache) if cache is not None else None, cands, ctypes.byref(k)),
@@ -834,7 +834,7 @@ class Views:
     def autotune(self, reps: int = 5, cache: "str | os.PathLike | None" = None) -> dict:
         """rz_views_tune: times the current step under every configuration
         (identical bits required) and keeps the fastest."""
-        cands = (ViewsConfig * 8)()
+        cands = (ViewsConfig * 12)()
         k = ctypes.c_uint32(0)
         _check(lib().rz_views_tune(self._v, reps, os.fsencode(cache) if cache is not None else None, cands,
                                    ctypes.byref(k)), "rz_views_tune")

@@ -1036,7 +1036,7 @@ static std::string frame_key(const rz_context* c) {
 // Switches the frame kernel to (tileW, waves) if its tile grid can be
 // co-resident; reallocates the per-tile counters when the tile count changes.
 static int apply_config(rz_context* c, int tw, int waves) {
-  if ((tw != 8 && tw != 16) || (waves != 4 && waves != 8)) return RZ_ERR_INVALID_ARGUMENT;
+  if ((tw != 8 && tw != 16) || (waves != 2 && waves != 4 && waves != 8)) return RZ_ERR_INVALID_ARGUMENT;
   int maxGroups = 0;
   RZ_HIP(frame_max_groups(tw, waves, &maxGroups));
   const int tx = int((c->bX + tw - 1) / tw), ty = int((c->bY + tw - 1) / tw);
@@ -1080,7 +1080,7 @@ extern "C" int rz_load_config(rz_context* ctx, const char* cachePath, int* found
 }
 
 extern "C" int rz_tune_frame(rz_context* ctx, uint32_t n, const rz_occluder* const* occ, uint32_t reps,
-                             const char* cachePath, rz_frame_config candidates[4], uint32_t* nCandidates) {
+                             const char* cachePath, rz_frame_config candidates[6], uint32_t* nCandidates) {
   if (!ctx || (n && !occ) || reps == 0) return RZ_ERR_INVALID_ARGUMENT;
   rz_context* c = ctx;
   const int oldTw = c->tileW, oldWaves = c->waves;
@@ -1091,11 +1091,11 @@ extern "C" int rz_tune_frame(rz_context* ctx, uint32_t n, const rz_occluder* con
   const size_t nb = size_t(c->bX) * c->bY;
   std::vector<uint16_t> d(nb * 64), h(nb + 8), refD, refH;
   std::vector<uint8_t> st(std::max<uint32_t>(n, 1)), refS;
-  rz_frame_config all[4];
+  rz_frame_config all[6];
   uint32_t k = 0;
   int best = -1;
   for (int tw : {8, 16})
-    for (int waves : {4, 8}) {
+    for (int waves : {2, 4, 8}) {
       rz_frame_config& cf = all[k];
       cf.tileW = uint32_t(tw);
       cf.waves = uint32_t(waves);

@@ -1459,6 +1459,7 @@ __device__ __forceinline__ void process_tile(const TileJob& J, TileSmem<TBX, TBY
 
   int involved = 0;
   constexpr uint32_t NT = 64u * NW;
+  static_assert(NT >= kItemWindow, "the item window gathers one item per thread");
   if (J.list) involved = nList > 0u;
   else
     for (uint32_t i = J.itemBegin + tid; i < J.itemEnd && !involved; i += NT) involved = relevance(i) != 0u;
@@ -1782,8 +1783,8 @@ __device__ __forceinline__ void process_tile(const TileJob& J, TileSmem<TBX, TBY
   // Min hiZ of each 8x8-block sub-tile for the query kernel (blocks outside
   // the buffer hold 0xFFFF and do not lower it).
   constexpr int kSubX = TBX / 8, kSubY = TBY / 8;
-  if (int(wave) < kSubX * kSubY) {
-    const int sx = int(wave) % kSubX, sy = int(wave) / kSubX;
+  for (int sub = int(wave); sub < kSubX * kSubY; sub += NW) {
+    const int sx = sub % kSubX, sy = sub / kSubX;
     const int lb = (sy * 8 + int(lane >> 3)) * TBX + sx * 8 + int(lane & 7u);
     const uint32_t mn = __ockl_wfred_min_u32(uint32_t(sm.hiz[lb]));
     const uint32_t px = tX * kSubX + uint32_t(sx), py = tY * kSubY + uint32_t(sy);
@@ -1838,7 +1839,8 @@ __global__ void __launch_bounds__(64 * NW, NW == 8 ? 4 : 2) rz_frame_kernel(Fram
 // view always has all its tiles running, so every wait ends (and each is
 // bounded by p.timeoutTicks in any case).
 // Two work-groups per CU (the LDS allows two): with eight waves each, every
-// SIMD must hold four waves, i.e. at most 128 VGPRs per lane.
+// SIMD must hold four waves, i.e. at most 128 VGPRs per lane.  NW is 2, 4 or
+// 8 (the item window gathers one item per thread of its first 128).
 template <int TBX, int TBY, int NW, bool STATS, bool LABELS>
 __global__ void __launch_bounds__(64 * NW, NW == 8 ? 4 : 2) rz_views_kernel(ViewsParams p) {
   __shared__ TileSmem<TBX, TBY> sm;
@@ -2441,8 +2443,16 @@ static const void* frame_fn(bool stats) {
 }
 
 static const void* frame_kernel_fn(int tileW, int waves, bool stats) {
-  if (tileW == 8) return waves == 8 ? frame_fn<8, 8>(stats) : waves == 4 ? frame_fn<8, 4>(stats) : nullptr;
-  if (tileW == 16) return waves == 8 ? frame_fn<16, 8>(stats) : waves == 4 ? frame_fn<16, 4>(stats) : nullptr;
+  if (tileW == 8)
+    return waves == 8 ? frame_fn<8, 8>(stats)
+           : waves == 4 ? frame_fn<8, 4>(stats)
+           : waves == 2 ? frame_fn<8, 2>(stats)
+                        : nullptr;
+  if (tileW == 16)
+    return waves == 8 ? frame_fn<16, 8>(stats)
+           : waves == 4 ? frame_fn<16, 4>(stats)
+           : waves == 2 ? frame_fn<16, 2>(stats)
+                        : nullptr;
   return nullptr;
 }
 
@@ -2550,10 +2560,12 @@ hipError_t launch_views(const ViewsParams& p, uint32_t nGroups, hipStream_t stre
   const bool st = p.stats != nullptr, lab = p.labels != nullptr;
   if (p.tileB != 8 && (p.tileB != 16 || lab)) return hipErrorInvalidValue;
   const int tb = int(p.tileB);
-  const void* fn = p.waves == 8 ? views_fn<8>(tb, st, lab) : views_fn<4>(tb, st, lab);
+  if (p.waves != 2 && p.waves != 4 && p.waves != 8) return hipErrorInvalidValue;
+  const void* fn = p.waves == 8 ? views_fn<8>(tb, st, lab) : p.waves == 4 ? views_fn<4>(tb, st, lab)
+                                                                          : views_fn<2>(tb, st, lab);
   ViewsParams local = p;
   void* args[] = {&local};
-  const dim3 grid(nGroups), block(64 * (p.waves == 8 ? 8 : 4));
+  const dim3 grid(nGroups), block(64 * p.waves);
   return launch_coresident(fn, grid, block, args, stream, cooperative);
 }
 
@@ -2567,8 +2579,11 @@ hipError_t views_max_groups(int tileB, int waves, int* maxGroups) {
   for (int variant = 0; variant < (tileB == 16 ? 2 : 4); ++variant) {
     int n = 0;
     const bool stats = variant & 1, labels = variant & 2;
-    const void* fn = waves == 8 ? views_fn<8>(tileB, stats, labels) : views_fn<4>(tileB, stats, labels);
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fn, 64 * (waves == 8 ? 8 : 4), 0);
+    if (waves != 2 && waves != 4 && waves != 8) return hipErrorInvalidValue;
+    const void* fn = waves == 8 ? views_fn<8>(tileB, stats, labels)
+                     : waves == 4 ? views_fn<4>(tileB, stats, labels)
+                                  : views_fn<2>(tileB, stats, labels);
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fn, 64 * waves, 0);
     if (e != hipSuccess) return e;
     perCU = n < perCU ? n : perCU;
   }

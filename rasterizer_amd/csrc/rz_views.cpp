@@ -984,7 +984,7 @@ extern "C" int rz_views_get_config(rz_views* v, rz_views_config* config) {
 }
 
 extern "C" int rz_views_set_config(rz_views* v, uint32_t tileW, uint32_t waves, uint32_t groupsPerCU) {
-  if (!v || (tileW != 8 && tileW != 16) || (waves != 4 && waves != 8) || groupsPerCU > 64)
+  if (!v || (tileW != 8 && tileW != 16) || (waves != 2 && waves != 4 && waves != 8) || groupsPerCU > 64)
     return RZ_ERR_INVALID_ARGUMENT;
   int s = wait_last_step(v);
   if (s != RZ_OK) return s;
@@ -1032,7 +1032,7 @@ static int step_outputs(rz_views* v, std::vector<uint8_t>& out) {
   return RZ_OK;
 }
 
-extern "C" int rz_views_tune(rz_views* v, uint32_t reps, const char* cachePath, rz_views_config candidates[8],
+extern "C" int rz_views_tune(rz_views* v, uint32_t reps, const char* cachePath, rz_views_config candidates[12],
                              uint32_t* nCandidates) {
   if (!v || reps == 0) return RZ_ERR_INVALID_ARGUMENT;
   const int oldTile = v->tileB, oldWaves = v->waves, oldGpc = v->groupsPerCU;
@@ -1043,11 +1043,11 @@ extern "C" int rz_views_tune(rz_views* v, uint32_t reps, const char* cachePath, 
   // Labelled steps draw with 8x8-block tiles only.
   const bool labelled = v->labels || (v->model && v->model->usesLabels);
   std::vector<uint8_t> ref, got;
-  rz_views_config all[8];
+  rz_views_config all[12];
   uint32_t k = 0;
   int best = -1;
   for (uint32_t tileW : {8u, 16u}) {
-    for (uint32_t waves : {4u, 8u}) {
+    for (uint32_t waves : {2u, 4u, 8u}) {
       int maxGroups = 0, dev = 0, cus = 1;
       RZ_HIP(views_max_groups(int(tileW), int(waves), &maxGroups));
       RZ_HIP(hipGetDevice(&dev));
