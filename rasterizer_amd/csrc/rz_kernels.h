@@ -35,7 +35,12 @@ struct SetupParams {
   uint64_t* groupBox;  // box of each aligned group of 64 quads (union of valid quads)
   const ViewDesc* views;  // multi-view launches: bX, bY from views[item.view]
   const uint32_t* groupItem;  // optional: per group of 64 quads, the item of its first quad
+  // Optional (learned-order steps): per quad of a drawn item, its record's
+  // modeZ | kFeatValid when the quad is drawn (non-empty rectangle), else 0:
+  // the features' per-quad input in 4 bytes instead of the 80-byte record.
+  uint32_t* featZ;
 };
+constexpr uint32_t kFeatValid = 0x100u;  // bits 8-15 of modeZ are unused
 
 // Persistent multi-view raster launch: work units are (view, tile) pairs taken
 // from an atomic queue in view order; units wait only on tiles of their own
@@ -197,7 +202,7 @@ struct LearnParams {
   uint32_t nItems;
   const ViewDesc* views;
   uint32_t nViews;
-  const QuadRec* recs;
+  const uint32_t* featZ;  // per quad, as SetupParams::featZ
   const uint64_t* bbox;
   uint32_t nQuads;
   const uint64_t* cov;

@@ -554,6 +554,7 @@ __global__ void __launch_bounds__(256) rz_setup_kernel(SetupParams p) {
   uint32_t item = 0;
   bool valid = false;
   uint64_t box = kEmptyBox;
+  uint32_t fz = 0u;
   if (inRange) {
     item = item_of_quad(p.items, p.nItems, g, p.groupItem);
     const ItemDesc& D = p.items[item];
@@ -574,6 +575,7 @@ __global__ void __launch_bounds__(256) rz_setup_kernel(SetupParams p) {
         valid = true;
         p.recs[g] = o.rec;
         box = quad_box(o.rec.fb, o.rec.range & 0xFFFFu, o.rec.range >> 16, bX, bY);
+        if ((o.rec.range & 0xFFFFu) != 0u && (o.rec.range >> 16) != 0u) fz = o.rec.modeZ | kFeatValid;
       }
     }
     // Records and boxes of items that are not drawn are never read (tiles,
@@ -582,6 +584,7 @@ __global__ void __launch_bounds__(256) rz_setup_kernel(SetupParams p) {
     if (st != kStatusSkip) {
       if (!valid) p.recs[g].range = 0u;
       p.bbox[g] = box;
+      if (p.featZ) p.featZ[g] = fz;
     }
   }
   // Footprint of each item's valid quads: one atomic per wave when the whole
@@ -2253,9 +2256,8 @@ __global__ void __launch_bounds__(256) rz_features_kernel(LearnParams p, uint32_
       for (int a = 0; a < 3; ++a) cen[a] += P[j][a] * (1.0f / 32.0f);
     }
     if (S.status == kStatusSkip) continue;  // not drawn: no valid quads (records not written)
-    const QuadRec& R = p.recs[g];
-    const uint32_t rx = R.range & 0xFFFFu, ry = R.range >> 16;
-    if (rx == 0u || ry == 0u) continue;
+    const uint32_t fz = p.featZ[g];
+    if (!(fz & kFeatValid)) continue;  // empty rectangle
     ++n;
     const uint64_t bb = p.bbox[g];
     const uint32_t bc0 = uint32_t(bb & 0xFFFF), bc1 = uint32_t((bb >> 16) & 0xFFFF);
@@ -2265,11 +2267,11 @@ __global__ void __launch_bounds__(256) rz_features_kernel(LearnParams p, uint32_
     r0 = min(r0, br0);
     r1 = max(r1, br1);
     areaSum += float((bc1 - bc0 + 1) * (br1 - br0 + 1));
-    const float z = float(R.modeZ >> 16) * (1.0f / 65535.0f);
+    const float z = float(fz >> 16) * (1.0f / 65535.0f);
     zmin = fminf(zmin, z);
     zmax = fmaxf(zmax, z);
     zsum += z;
-    convex += (R.modeZ & 0xFFu) == kModeConvex ? 1u : 0u;
+    convex += (fz & 0xFFu) == kModeConvex ? 1u : 0u;
     float e1[3], e2[3], e3[3], nrm[3];
 #pragma unroll
     for (int a = 0; a < 3; ++a) {

@@ -74,6 +74,7 @@ struct rz_views {
   uint32_t* d_done = nullptr;
   uint32_t* d_visible = nullptr;
   QuadRec* d_recs = nullptr;
+  uint32_t* d_featZ = nullptr;  // per quad: modeZ | kFeatValid of drawn quads (learned-order steps)
   uint64_t* d_bbox = nullptr;
   uint64_t* d_groupBox = nullptr;
   uint32_t* d_groupItem = nullptr;  // per group of 64 quads: the item of its first quad
@@ -145,6 +146,8 @@ static void free_plan(rz_views* v) {
   hipFree(v->d_done);
   hipFree(v->d_visible);
   hipFree(v->d_recs);
+  hipFree(v->d_featZ);
+  v->d_featZ = nullptr;
   hipFree(v->d_bbox);
   hipFree(v->d_groupBox);
   hipFree(v->d_groupItem);
@@ -441,6 +444,7 @@ static int build_plan(rz_views* v) {
   if (dev_alloc(&v->d_items, ni) != hipSuccess || dev_alloc(&v->d_state, ni) != hipSuccess ||
       dev_alloc(&v->d_done, ni) != hipSuccess || dev_alloc(&v->d_visible, ni) != hipSuccess ||
       dev_alloc(&v->d_recs, std::max<size_t>(quads, 1)) != hipSuccess ||
+      dev_alloc(&v->d_featZ, std::max<size_t>(quads, 1)) != hipSuccess ||
       dev_alloc(&v->d_bbox, std::max<size_t>(quads, 1)) != hipSuccess ||
       dev_alloc(&v->d_groupBox, std::max<size_t>((quads + 63) / 64, 1)) != hipSuccess ||
       dev_alloc(&v->d_groupItem, std::max<size_t>((quads + 63) / 64, 1)) != hipSuccess ||
@@ -578,7 +582,7 @@ static LearnParams learn_params(rz_views* v) {
   lp.nItems = v->nItems;
   lp.views = v->d_views;
   lp.nViews = uint32_t(v->views.size());
-  lp.recs = v->d_recs;
+  lp.featZ = v->d_featZ;
   lp.bbox = v->d_bbox;
   lp.nQuads = v->nQuads;
   lp.cov = v->d_cov;
@@ -625,6 +629,7 @@ static int enqueue_step(rz_views* v, bool capturing) {
     sp.groupBox = v->d_groupBox;
     sp.groupItem = v->d_groupItem;
     sp.views = v->d_views;
+    sp.featZ = (v->labels || v->model) ? v->d_featZ : nullptr;
     RZ_HIP(launch_setup(sp, v->stream));
     const LearnParams lp = learn_params(v);
     if (lp.nDrawn) RZ_HIP(launch_fill_u32(lp.nDrawn, 1, 0u, v->stream));
