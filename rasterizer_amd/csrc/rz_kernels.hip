@@ -2208,9 +2208,8 @@ __global__ void rz_fill_u16_kernel(uint16_t* p, uint32_t n, uint16_t v) {
 //  7, 8 union box centre / buffer size (x, y)   9 log2(1 + |W| of the batch centroid)
 // 10 log2(1 + previous step's label)  11-13 mean, min, max of (unit quad normal . view direction)
 // 14 position of the batch in the view's plain order   15 fraction of Convex quads
-__global__ void __launch_bounds__(256) rz_features_kernel(LearnParams p, uint32_t nBatches) {
-  const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= nBatches) return;
+// Writes batch b's features; returns whether its occluder is drawn.
+__device__ bool batch_features(const LearnParams& p, uint32_t b) {
   const uint32_t item = item_of_quad(p.items, p.nItems, b * 8u, p.groupItem);
   const ItemDesc& D = p.items[item];
   const ItemState& S = p.state[item];
@@ -2311,19 +2310,28 @@ __global__ void __launch_bounds__(256) rz_features_kernel(LearnParams p, uint32_
   f[14] = V.nBatches > 1 ? float(b - V.batchBase) / float(V.nBatches - 1) : 0.0f;
 #pragma unroll
   for (int k = 0; k < kFeatures; ++k) p.features[size_t(b) * kFeatures + k] = f[k];
-  if (p.drawn) {
-    // Append the batch to the drawn list, one atomic per wave.
-    const bool drawn = S.status != kStatusSkip;
-    const uint64_t m = __ballot(drawn);
-    if (m != 0ull) {
-      const uint32_t lane = threadIdx.x & 63u, leader = uint32_t(__builtin_ctzll(m));
-      uint32_t base = 0;
-      if (lane == leader) base = atomicAdd(p.nDrawn, uint32_t(__popcll(m)));
-      base = uint32_t(__builtin_amdgcn_readlane(int(base), int(leader)));
-      const uint32_t pre = __builtin_amdgcn_mbcnt_hi(uint32_t(m >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(m), 0u));
-      if (drawn) p.drawn[base + pre] = b;
-    }
-  }
+  return S.status != kStatusSkip;
+}
+
+__global__ void __launch_bounds__(256) rz_features_kernel(LearnParams p, uint32_t nBatches) {
+  __shared__ uint32_t wgCount, wgBase;
+  const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool drawn = b < nBatches && batch_features(p, b);
+  if (!p.drawn) return;
+  // Append the work-group's drawn batches to the list with one atomic on the
+  // list's count (the count is a single word: one atomic per wave serialised
+  // there).
+  if (threadIdx.x == 0) wgCount = 0;
+  __syncthreads();
+  const uint64_t m = __ballot(drawn);
+  uint32_t waveOff = 0;
+  if ((threadIdx.x & 63u) == 0 && m != 0ull) waveOff = atomicAdd(&wgCount, uint32_t(__popcll(m)));
+  waveOff = uint32_t(__builtin_amdgcn_readlane(int(waveOff), 0));
+  __syncthreads();
+  if (threadIdx.x == 0) wgBase = wgCount != 0 ? atomicAdd(p.nDrawn, wgCount) : 0u;
+  __syncthreads();
+  const uint32_t pre = __builtin_amdgcn_mbcnt_hi(uint32_t(m >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(m), 0u));
+  if (drawn) p.drawn[wgBase + waveOff + pre] = b;
 }
 
 // Item k of a reordered draw: batch order[k] alone, with its occluder's

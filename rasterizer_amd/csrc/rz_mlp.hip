@@ -43,7 +43,19 @@ struct MlpWeights {
   __bf16 w2n[64 * kPadK64];  // [in][out]  (dH1 = dZ2 W2^T: B[n = in][k = out])
 };
 
-// One wave's 32-sample working set.
+// The forward pass's weights and one wave's forward working set (the
+// scoring kernel: 14 KB + 4 x 7 KB of LDS, so that three work-groups share a
+// CU).
+struct MlpWeightsFwd {
+  __bf16 w1t[64 * kPadK32];
+  __bf16 w2t[64 * kPadK64];
+};
+struct MlpWaveFwd {
+  __bf16 xn[32 * kPadK32];
+  __bf16 h1[32 * kPadK64];
+};
+
+// One wave's 32-sample working set (training).
 struct MlpWave {
   __bf16 xn[32 * kPadK32];    // [sample][feature], features 16-31 zero
   __bf16 xnt[16 * kPadK32];   // [feature][sample]
@@ -54,7 +66,8 @@ struct MlpWave {
   __bf16 dz1t[64 * kPadK32];  // [hidden][sample]
 };
 
-__device__ void stage_weights(MlpWeights& W, const float* __restrict__ w) {
+template <class WeightsT>
+__device__ void stage_weights(WeightsT& W, const float* __restrict__ w) {
   for (int i = int(threadIdx.x); i < 64 * 32; i += 256) {
     const int o = i >> 5, k = i & 31;
     W.w1t[o * kPadK32 + k] = __bf16(k < 16 ? w[kMlpW1 + k * 64 + o] : 0.0f);
@@ -62,7 +75,8 @@ __device__ void stage_weights(MlpWeights& W, const float* __restrict__ w) {
   for (int i = int(threadIdx.x); i < 64 * 64; i += 256) {
     const int a = i >> 6, b = i & 63;
     W.w2t[a * kPadK64 + b] = __bf16(w[kMlpW2 + b * 64 + a]);  // a = out, b = in
-    W.w2n[a * kPadK64 + b] = __bf16(w[kMlpW2 + a * 64 + b]);  // a = in, b = out
+    if constexpr (sizeof(WeightsT) == sizeof(MlpWeights))
+      W.w2n[a * kPadK64 + b] = __bf16(w[kMlpW2 + a * 64 + b]);  // a = in, b = out
   }
 }
 
@@ -70,8 +84,8 @@ __device__ void stage_weights(MlpWeights& W, const float* __restrict__ w) {
 // for the backward pass (TRANSPOSED), xnt.  Sample s of the wave is row
 // base + s of x, or row rows[base + s] when a row list is given (n = its
 // length).
-template <bool TRANSPOSED>
-__device__ void stage_inputs(MlpWave& S, const MlpParams& p, uint32_t base, uint32_t n, uint32_t lane) {
+template <bool TRANSPOSED, class WaveT>
+__device__ void stage_inputs(WaveT& S, const MlpParams& p, uint32_t base, uint32_t n, uint32_t lane) {
   const uint32_t s = lane >> 1, f0 = (lane & 1u) * 8u;
   const bool in = base + s < n;
   const uint32_t g = !in ? 0u : p.rows ? p.rows[base + s] : base + s;
@@ -82,7 +96,7 @@ __device__ void stage_inputs(MlpWave& S, const MlpParams& p, uint32_t base, uint
     const __bf16 b = __bf16(x);
     S.xn[s * kPadK32 + f] = b;
     S.xn[s * kPadK32 + 16 + f] = __bf16(0.0f);
-    if (TRANSPOSED) S.xnt[f * kPadK32 + s] = b;
+    if constexpr (TRANSPOSED) S.xnt[f * kPadK32 + s] = b;
   }
 }
 
@@ -94,8 +108,8 @@ struct Acts {
 // Forward of the wave's 32 samples: z1, z2 (pre-activations), h2 in
 // registers, h1 (BF16) in LDS, and for the backward pass (TRANSPOSED) also
 // transposed.
-template <bool TRANSPOSED>
-__device__ void forward_wave(const MlpWeights& W, MlpWave& S, const float* __restrict__ w, uint32_t lane, Acts& a) {
+template <bool TRANSPOSED, class WeightsT, class WaveT>
+__device__ void forward_wave(const WeightsT& W, WaveT& S, const float* __restrict__ w, uint32_t lane, Acts& a) {
   const uint32_t col = lane & 15u, rq = (lane >> 4) * 4u;
 #pragma unroll
   for (int rt = 0; rt < 2; ++rt)
@@ -111,7 +125,7 @@ __device__ void forward_wave(const MlpWeights& W, MlpWave& S, const float* __res
         const __bf16 h = __bf16(gelu(z));
         const uint32_t row = 16u * rt + rq + i, hid = 16u * t + col;
         S.h1[row * kPadK64 + hid] = h;
-        if (TRANSPOSED) S.h1t[hid * kPadK32 + row] = h;
+        if constexpr (TRANSPOSED) S.h1t[hid * kPadK32 + row] = h;
       }
     }
   __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0): the wave's LDS writes are done
@@ -147,10 +161,10 @@ __device__ __forceinline__ float sum16(float v) {
 }  // namespace
 
 __global__ void __launch_bounds__(256) rz_mlp_forward_kernel(MlpParams p) {
-  __shared__ __attribute__((aligned(16))) MlpWeights W;
-  __shared__ __attribute__((aligned(16))) MlpWave Sw[4];
+  __shared__ __attribute__((aligned(16))) MlpWeightsFwd W;
+  __shared__ __attribute__((aligned(16))) MlpWaveFwd Sw[4];
   const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
-  MlpWave& S = Sw[wave];
+  MlpWaveFwd& S = Sw[wave];
   stage_weights(W, p.w);
   __syncthreads();
   const uint32_t col = lane & 15u, rq = (lane >> 4) * 4u;
