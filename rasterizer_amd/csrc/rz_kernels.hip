@@ -1489,8 +1489,8 @@ __device__ __forceinline__ void process_tile(const TileJob& J, TileSmem<TBX, TBY
 
   uint32_t qPar = 0;
   BlockCounters bc;
-  // Early "not visible" answers for later query items of the window (see
-  // spec_query_wave), in query-gated launches only.
+  // Early "not visible" answers for later query items of the window, tried
+  // while the tile waits (see spec_query_wave), in query-gated launches only.
   const bool gated = J.done != nullptr;
   uint32_t drawVersion = 1, sPar = 0, wPar = 0, nRelW = 0;
   auto answered_early = [&](uint32_t k) -> bool { return (sm.spec[k >> 5] >> (k & 31u)) & 1u; };
@@ -1573,19 +1573,9 @@ __device__ __forceinline__ void process_tile(const TileJob& J, TileSmem<TBX, TBY
     }
     __syncthreads();
     nRelW = nRel;
-    if (gated && nRel > 0) {
-      // Answer what the tile can already prove; skipped while no block of
-      // the tile has hiZ >= 2 (nothing can be proven then).
-      int written = 0;
-      for (int lb = int(tid); lb < NB && !written; lb += int(NT)) {
-        const int gx = tbx0 + lb % TBX, gy = tby0 + lb / TBX;
-        written = gx <= tbx1 && gy <= tby1 && sm.hiz[lb] > 1u;
-      }
-      if (__syncthreads_or(written)) {
-        for (uint32_t c = next_early(0); c < nRel; c = next_early(c + 1)) try_early(c);
-        __syncthreads();
-      }
-    }
+    // Later query items are answered early only while the tile waits: a
+    // pass over the window's query items at its start delays the tiles with
+    // the longest lists, which other tiles' waits depend on.
 
     for (uint32_t k = 0; k < nRel; ++k) {
       const ItemLocal& L = sm.items[k];
