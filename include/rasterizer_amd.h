@@ -243,6 +243,19 @@ int rz_prepare_scene_device(rz_context* ctx, uint32_t nQuads, const float* d_ver
  * (xyzw); an index >= nVertices is RZ_ERR_INVALID_ARGUMENT. */
 int rz_quad_decompose(const uint32_t* indices, size_t nIndices, const float* vertices, size_t nVertices, uint32_t* out,
                       size_t* nOut);
+/* QuadDecomposition::decompose (QuadDecomposition.cpp:311-445) for a mesh in
+ * HBM: the same quads as rz_quad_decompose, quad for quad.  The candidate
+ * graph (half-edges grouped by a device radix sort, the 0.5-unit plane tests
+ * with the host CPU's RSQRTPS, the adjacency lists in the reference's link
+ * order) and the quads are built on the device; the maximum matching, a
+ * sequential augmenting-path search, runs on the host over the adjacency
+ * lists copied back.  d_indices: 3 * nTriangles words (device); d_vertices:
+ * nVertices x 4 floats (xyzw, device, 16-byte aligned); an index >=
+ * nVertices is RZ_ERR_INVALID_ARGUMENT.  Writes 4 indices per quad to d_out
+ * (device, room for 4 * nTriangles words) and the number of words to *nOut.
+ * Blocks. */
+int rz_quad_decompose_device(rz_context* ctx, uint32_t nTriangles, const uint32_t* d_indices, const float* d_vertices,
+                             uint32_t nVertices, uint32_t* d_out, uint32_t* nOut);
 
 /* Timing helpers on the context stream (HIP events): rz_timer_start records
  * a start event, rz_timer_stop a stop event and returns (after waiting for

@@ -143,6 +143,7 @@ _SIGNATURES = {
     "rz_views_read_items": ([_P, _U32, _P, _P], ctypes.c_int),
     "rz_quad_decompose": ([_P, ctypes.c_size_t, _P, ctypes.c_size_t, _P, ctypes.POINTER(ctypes.c_size_t)],
                           ctypes.c_int),
+    "rz_quad_decompose_device": ([_P, _U32, _P, _P, _U32, _P, ctypes.POINTER(_U32)], ctypes.c_int),
     "rz_write_occluders": ([ctypes.c_char_p, _U32, _P], ctypes.c_int),
     "rz_read_occluders": ([ctypes.c_char_p, _P, _P, _P, _P], ctypes.c_int),
     "rz_write_frame": ([ctypes.c_char_p, _U32, _U32, _P, _P, _U32, _P, _P], ctypes.c_int),
@@ -576,6 +577,35 @@ class Rasterizer:
                 b.free()
         return [Occluder(packets[4 * offsets[i]:4 * offsets[i + 1]].copy(), ref[0:4], ref[4:8], bounds[i, 0:4],
                          bounds[i, 4:8], bounds[i, 8:12]) for i in range(k)]
+
+    def quad_decompose_device(self, indices, vertices) -> np.ndarray:
+        """rz_quad_decompose_device: QuadDecomposition::decompose with the
+        candidate graph and the quads built on the GPU (the matching on the
+        host) from a mesh uploaded once; the same quads as
+        QuadDecomposition.decompose."""
+        idx = np.ascontiguousarray(np.asarray(indices, dtype=np.uint32).reshape(-1))
+        n_tri = idx.size // 3
+        idx = idx[:3 * n_tri]
+        v = np.asarray(vertices, dtype=np.float32)
+        if v.ndim == 2 and v.shape[1] == 3:
+            v = np.concatenate([v, np.ones((v.shape[0], 1), np.float32)], axis=1)
+        v = np.ascontiguousarray(v.reshape(-1, 4))
+        n = _U32(0)
+        if n_tri == 0:
+            _check(lib().rz_quad_decompose_device(self.ctx, 0, None, None, v.shape[0], None, ctypes.byref(n)),
+                   "rz_quad_decompose_device")
+            return np.zeros(0, np.uint32)
+        d_i = DeviceBuffer(self, idx.nbytes, idx)
+        d_v = DeviceBuffer(self, max(v.nbytes, 16), v if v.nbytes else None)
+        d_o = DeviceBuffer(self, 16 * n_tri)
+        try:
+            _check(lib().rz_quad_decompose_device(self.ctx, n_tri, d_i.ptr, d_v.ptr, v.shape[0], d_o.ptr,
+                                                  ctypes.byref(n)), "rz_quad_decompose_device")
+            out = d_o.read(np.uint32, n.value) if n.value else np.zeros(0, np.uint32)
+        finally:
+            for b in (d_i, d_v, d_o):
+                b.free()
+        return out
 
     def query_batch_device(self, n: int, d_bounds: "DeviceBuffer", d_visible: "DeviceBuffer",
                            d_clip: "DeviceBuffer | None" = None) -> None:

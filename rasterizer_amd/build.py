@@ -71,6 +71,7 @@ def build(verbose: bool = False) -> Path:
         "rz_mlp.o": [hipcc, *HIP_FLAGS, "-c", str(CSRC / "rz_mlp.hip")],
         "rz_bake.o": [hipcc, *HIP_FLAGS, "-c", str(CSRC / "rz_bake.hip")],
         "rz_sah.o": [hipcc, *HIP_FLAGS, "-c", str(CSRC / "rz_sah.hip")],
+        "rz_quad_dev.o": [hipcc, *HIP_FLAGS, "-c", str(CSRC / "rz_quad.hip")],
         "rz_tune.o": [hipcc, *HIP_FLAGS, "-c", str(CSRC / "rz_tune.cpp")],
     }
     cmds = [cmd + ["-o", str(BUILD / name)] for name, cmd in objs.items()]

@@ -858,6 +858,18 @@ extern "C" int rz_bake_device(rz_context* ctx, uint32_t n, const uint32_t* verte
   return status;
 }
 
+extern "C" int rz_quad_decompose_device(rz_context* ctx, uint32_t nTriangles, const uint32_t* d_indices,
+                                        const float* d_vertices, uint32_t nVertices, uint32_t* d_out,
+                                        uint32_t* nOut) {
+  if (!ctx || !nOut || (nTriangles && (!d_indices || !d_vertices || !d_out))) return RZ_ERR_INVALID_ARGUMENT;
+  if (reinterpret_cast<uintptr_t>(d_vertices) % 16u != 0u) return RZ_ERR_INVALID_ARGUMENT;  // read as float4
+  *nOut = 0;
+  if (nTriangles == 0) return RZ_OK;
+  const rzi::HostTables& tabs = rzi::host_tables();
+  if (tabs.rsqrtStatus != RZ_OK) return tabs.rsqrtStatus;
+  return quad_decompose_run(ctx->stream, nTriangles, d_indices, d_vertices, nVertices, tabs.rsqrt, d_out, nOut);
+}
+
 extern "C" int rz_sah_batches(rz_context* ctx, uint32_t n, const float* d_boxes, uint32_t targetSize,
                               uint32_t splitGranularity, uint32_t* d_indices, uint32_t* batchOffsets,
                               uint32_t* nBatches) {

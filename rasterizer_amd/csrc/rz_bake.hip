@@ -14,29 +14,6 @@ namespace rzd {
 
 namespace {
 
-struct V3 {
-  float x, y, z;
-};
-
-__device__ __forceinline__ V3 sub3(float4 a, float4 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
-
-// cross3 of the reference's shuffle form: (a * b.yzx - a.yzx * b).yzx.
-__device__ __forceinline__ V3 cross3(V3 a, V3 b) {
-  return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
-}
-
-// _mm_dp_ps(a, b, 0x7F): (x + y) + (z + the masked-out lane's +0).
-__device__ __forceinline__ float dp3(V3 a, V3 b) {
-  const float p0 = a.x * b.x, p1 = a.y * b.y, p2 = a.z * b.z;
-  return (p0 + p1) + (p2 + 0.0f);
-}
-
-// normalize (VectorMath.h:20-23): v * RSQRTPS(v . v).
-__device__ __forceinline__ V3 approx_normalize(V3 v, const uint32_t* __restrict__ rsq) {
-  const float r = rsqrt_x86(dp3(v, v), rsq);
-  return {v.x * r, v.y * r, v.z * r};
-}
-
 // First occurrence of the smallest (largest) value in vertex order, as the
 // reference's sequential _mm_min_ps(v, acc) (_mm_max_ps) keeps it: a value
 // replaces the accumulator only when strictly smaller; NaNs never do.

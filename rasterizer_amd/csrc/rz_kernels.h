@@ -283,6 +283,12 @@ hipError_t launch_bake(const BakeParams& p, uint32_t nOccluders, hipStream_t str
 // SAH batching of n quad boxes on the stream (rz_sah.hip); an RZ_* status.
 int sah_batches_run(hipStream_t stream, uint32_t n, const float* d_boxes, uint32_t target, uint32_t gran,
                     uint32_t* d_indices, uint32_t* batchOffsets, uint32_t* nBatches);
+// QuadDecomposition of nTri device triangles on the stream (rz_quad.hip): the
+// candidate graph and the quads on the device, the matching on the host;
+// writes the quads' indices to d_out and their word count to *nOut.  An
+// RZ_* status.
+int quad_decompose_run(hipStream_t stream, uint32_t nTri, const uint32_t* d_idx, const float* d_vertices,
+                       uint32_t nVertices, const uint32_t* hostRsqrt, uint32_t* d_out, uint32_t* nOut);
 // Scene preparation around it (rz_sah.hip): boxes of nPad padded quads and
 // the scene box as ordered-float keys (min xyzw, max xyzw; the caller sets
 // them to ~0 and 0 first) and sceneKeys[8] set non-zero when a vertex has a

@@ -18,6 +18,7 @@
 #include <vector>
 
 #include "rasterizer_amd.h"
+#include "rz_quad.h"
 
 namespace {
 
@@ -70,8 +71,8 @@ inline uint64_t edge_key(uint32_t a, uint32_t b) { return (uint64_t(a) << 32) | 
 // --- maximum matching (QuadDecomposition.cpp:28-309) ------------------------
 class Matcher {
  public:
-  Matcher(const std::vector<uint32_t>& adjStart, const std::vector<int>& adj)
-      : start_(adjStart), adj_(adj), n_(int(adjStart.size()) - 1), mate_(size_t(n_), -1), bridge_(size_t(n_)),
+  Matcher(int n, const uint32_t* adjStart, const int* adj)
+      : start_(adjStart), adj_(adj), n_(n), mate_(size_t(n_), -1), bridge_(size_t(n_)),
         depth_(size_t(n_), 0), parent_(size_t(n_), -1), blossom_(size_t(n_), 0), token_(size_t(n_), 0) {}
 
   // Greedy maximal matching in vertex and adjacency order (:34-54), then an
@@ -230,8 +231,8 @@ class Matcher {
     path.push_back(s);
   }
 
-  const std::vector<uint32_t>& start_;
-  const std::vector<int>& adj_;
+  const uint32_t* start_;
+  const int* adj_;
   int n_;
   std::vector<int> mate_;
   std::vector<std::pair<int, int>> bridge_;
@@ -241,6 +242,16 @@ class Matcher {
 };
 
 }  // namespace
+
+namespace rzq {
+
+void match(uint32_t n, const uint32_t* start, const int* adj, int* mate) {
+  Matcher m(int(n), start, adj);
+  m.run();
+  for (uint32_t v = 0; v < n; ++v) mate[v] = m.mate(int(v));
+}
+
+}  // namespace rzq
 
 extern "C" int rz_quad_decompose(const uint32_t* indices, size_t nIndices, const float* vertices, size_t nVertices,
                                  uint32_t* out, size_t* nOut) {
@@ -305,7 +316,7 @@ extern "C" int rz_quad_decompose(const uint32_t* indices, size_t nIndices, const
     std::vector<uint32_t>().swap(linkA);
     std::vector<uint32_t>().swap(linkB);
 
-    Matcher m(start, adj);
+    Matcher m(int(T), start.data(), adj.data());
     m.run();
 
     // Quads in triangle order (:392-441): a single triangle becomes the

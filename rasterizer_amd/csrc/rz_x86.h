@@ -60,6 +60,31 @@ __device__ __forceinline__ float rsqrt_x86(float x, const uint32_t* __restrict__
   return bitsf((uint32_t(int((t >> 23) & 0xFFu) - k) << 23) | (t & 0x7FFFFFu));
 }
 
+// The xyz lanes of an SSE vector (the w lane never reaches a result below:
+// DPPS with mask 0x7F leaves it out).
+struct V3 {
+  float x, y, z;
+};
+
+__device__ __forceinline__ V3 sub3(float4 a, float4 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+
+// cross3 of the reference's shuffle form: (a * b.yzx - a.yzx * b).yzx.
+__device__ __forceinline__ V3 cross3(V3 a, V3 b) {
+  return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+
+// _mm_dp_ps(a, b, 0x7F): (x + y) + (z + the masked-out lane's +0).
+__device__ __forceinline__ float dp3(V3 a, V3 b) {
+  const float p0 = a.x * b.x, p1 = a.y * b.y, p2 = a.z * b.z;
+  return (p0 + p1) + (p2 + 0.0f);
+}
+
+// normalize (VectorMath.h:20-23): v * RSQRTPS(v . v).
+__device__ __forceinline__ V3 approx_normalize(V3 v, const uint32_t* __restrict__ rsq) {
+  const float r = rsqrt_x86(dp3(v, v), rsq);
+  return {v.x * r, v.y * r, v.z * r};
+}
+
 __device__ __forceinline__ int32_t cvtt_x86(float x) {
   return (__builtin_fabsf(x) < 2147483648.0f) ? int32_t(x) : int32_t(0x80000000u);
 }
