@@ -229,6 +229,22 @@ int rz_sah_batches(rz_context* ctx, uint32_t n, const float* d_boxes, uint32_t t
 int rz_prepare_scene_device(rz_context* ctx, uint32_t nQuads, const float* d_vertices, uint32_t targetSize,
                             uint32_t splitGranularity, uint32_t* d_packets, float* d_bounds, uint32_t* quadOffsets,
                             uint32_t* nOccluders, float refMinMax[8]);
+/* Main.cpp:86-128 on the device from the indexed triangle mesh, as Main.cpp
+ * loads it: QuadDecomposition::decompose (as rz_quad_decompose_device), then
+ * the padding, quad boxes, SAH batches and bake of rz_prepare_scene_device,
+ * with the scene box taken over the whole vertex buffer (:109-113), which
+ * can hold vertices no quad references.  d_indices: 3 * nTriangles words;
+ * d_vertices: nVertices x 4 floats (xyzw, 16-byte aligned), every vertex's
+ * x, y and z finite with |coordinate| <= 2^40 (else
+ * RZ_ERR_INVALID_ARGUMENT); both device.  Outputs as rz_prepare_scene_device
+ * for at most nTriangles quads: room for 4 * nTriangles rounded up to 32
+ * words of packets, 12 * (nTriangles + 7) / 8 floats of bounds and
+ * (nTriangles + 7) / 8 + 1 host words of quadOffsets.  Bit-identical to
+ * rz_quad_decompose followed by the host preparation with the vertex
+ * buffer's box.  Blocks. */
+int rz_prepare_mesh_device(rz_context* ctx, uint32_t nTriangles, const uint32_t* d_indices, const float* d_vertices,
+                           uint32_t nVertices, uint32_t targetSize, uint32_t splitGranularity, uint32_t* d_packets,
+                           float* d_bounds, uint32_t* quadOffsets, uint32_t* nOccluders, float refMinMax[8]);
 
 /* QuadDecomposition::decompose(indices, vertices) (QuadDecomposition.h:8,
  * QuadDecomposition.cpp:311-445), host side, no device needed: pairs the

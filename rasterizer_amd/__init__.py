@@ -144,6 +144,8 @@ _SIGNATURES = {
     "rz_quad_decompose": ([_P, ctypes.c_size_t, _P, ctypes.c_size_t, _P, ctypes.POINTER(ctypes.c_size_t)],
                           ctypes.c_int),
     "rz_quad_decompose_device": ([_P, _U32, _P, _P, _U32, _P, ctypes.POINTER(_U32)], ctypes.c_int),
+    "rz_prepare_mesh_device": ([_P, _U32, _P, _P, _U32, _U32, _U32, _P, _P, _P, ctypes.POINTER(_U32), _P],
+                               ctypes.c_int),
     "rz_write_occluders": ([ctypes.c_char_p, _U32, _P], ctypes.c_int),
     "rz_read_occluders": ([ctypes.c_char_p, _P, _P, _P, _P], ctypes.c_int),
     "rz_write_frame": ([ctypes.c_char_p, _U32, _U32, _P, _P, _U32, _P, _P], ctypes.c_int),
@@ -574,6 +576,45 @@ class Rasterizer:
             bounds = d_b.read(np.float32, max_occ * 12).reshape(max_occ, 12)
         finally:
             for b in (d_v, d_p, d_b):
+                b.free()
+        return [Occluder(packets[4 * offsets[i]:4 * offsets[i + 1]].copy(), ref[0:4], ref[4:8], bounds[i, 0:4],
+                         bounds[i, 4:8], bounds[i, 8:12]) for i in range(k)]
+
+    def prepare_mesh_device(self, indices, vertices, target: int = 512, gran: int = 8) -> list:
+        """rz_prepare_mesh_device: Main.cpp:86-128 on the device from an
+        indexed triangle mesh uploaded once (QuadDecomposition, padding, SAH
+        batching, the vertex buffer's box, bake).  Returns the Occluders,
+        identical to scenes.bake_scene(scenes.decompose_mesh(indices,
+        vertices), vertex_buffer=vertices).occluders."""
+        idx = np.ascontiguousarray(np.asarray(indices, dtype=np.uint32).reshape(-1))
+        n_tri = idx.size // 3
+        idx = np.ascontiguousarray(idx[:3 * n_tri])
+        v = np.asarray(vertices, dtype=np.float32)
+        if v.ndim == 2 and v.shape[1] == 3:
+            v = np.concatenate([v, np.ones((v.shape[0], 1), np.float32)], axis=1)
+        v = np.ascontiguousarray(v.reshape(-1, 4))
+        max_occ = (n_tri + 7) // 8
+        offsets = np.zeros(max_occ + 1, np.uint32)
+        ref = np.zeros(8, np.float32)
+        count = _U32(0)
+        if n_tri == 0:
+            _check(lib().rz_prepare_mesh_device(self.ctx, 0, None, None, v.shape[0], target, gran, None, None,
+                                                _ptr(offsets), ctypes.byref(count), _ptr(ref)),
+                   "rz_prepare_mesh_device")
+            return []
+        d_i = DeviceBuffer(self, idx.nbytes, idx)
+        d_v = DeviceBuffer(self, max(v.nbytes, 16), v if v.nbytes else None)
+        d_p = DeviceBuffer(self, max_occ * 8 * 16)
+        d_b = DeviceBuffer(self, max_occ * 48)
+        try:
+            _check(lib().rz_prepare_mesh_device(self.ctx, n_tri, d_i.ptr, d_v.ptr, v.shape[0], target, gran, d_p.ptr,
+                                                d_b.ptr, _ptr(offsets), ctypes.byref(count), _ptr(ref)),
+                   "rz_prepare_mesh_device")
+            k = count.value
+            packets = d_p.read(np.uint32, max_occ * 32)
+            bounds = d_b.read(np.float32, max_occ * 12).reshape(max_occ, 12)
+        finally:
+            for b in (d_i, d_v, d_p, d_b):
                 b.free()
         return [Occluder(packets[4 * offsets[i]:4 * offsets[i + 1]].copy(), ref[0:4], ref[4:8], bounds[i, 0:4],
                          bounds[i, 4:8], bounds[i, 8:12]) for i in range(k)]

@@ -878,21 +878,14 @@ extern "C" int rz_sah_batches(rz_context* ctx, uint32_t n, const float* d_boxes,
   return sah_batches_run(ctx->stream, n, d_boxes, targetSize, splitGranularity, d_indices, batchOffsets, nBatches);
 }
 
-extern "C" int rz_prepare_scene_device(rz_context* ctx, uint32_t nQuads, const float* d_vertices,
-                                       uint32_t targetSize, uint32_t splitGranularity, uint32_t* d_packets,
-                                       float* d_bounds, uint32_t* quadOffsets, uint32_t* nOccluders,
-                                       float refMinMax[8]) {
-  if (!ctx || !quadOffsets || !nOccluders || !refMinMax || (nQuads && (!d_vertices || !d_packets || !d_bounds)))
-    return RZ_ERR_INVALID_ARGUMENT;
-  // Occluder::bake takes whole 8-quad batches, so every split must fall on one.
-  if (splitGranularity == 0 || splitGranularity % 8u != 0u) return RZ_ERR_INVALID_ARGUMENT;
-  if (reinterpret_cast<uintptr_t>(d_vertices) % 16u != 0u) return RZ_ERR_INVALID_ARGUMENT;
-  quadOffsets[0] = 0;
-  *nOccluders = 0;
-  if (nQuads == 0) return RZ_OK;
-  if (nQuads > (1u << 28)) return RZ_ERR_TOO_LARGE;
-  const rzi::HostTables& tabs = rzi::host_tables();
-  if (tabs.rsqrtStatus != RZ_OK) return tabs.rsqrtStatus;
+namespace {
+
+// Main.cpp:90-128 on the device for nQuads quads (4 xyzw vertices each, in
+// HBM), after the arguments are checked: the scene box over d_boxVertices
+// (nBox vertices) when given, else over the quads' vertices.
+int prepare_scene(rz_context* ctx, uint32_t nQuads, const float* d_vertices, const float* d_boxVertices,
+                  uint32_t nBox, uint32_t targetSize, uint32_t splitGranularity, uint32_t* d_packets,
+                  float* d_bounds, uint32_t* quadOffsets, uint32_t* nOccluders, float refMinMax[8]) {
   const uint32_t nPad = (nQuads + 7u) & ~7u;  // Main.cpp:90-94
   float* d_boxes = nullptr;
   float* d_sorted = nullptr;
@@ -907,6 +900,7 @@ extern "C" int rz_prepare_scene_device(rz_context* ctx, uint32_t nQuads, const f
     status = RZ_ERR_OUT_OF_MEMORY;
   } else if (hipMemcpyAsync(d_keys, keys, sizeof keys, hipMemcpyHostToDevice, ctx->stream) != hipSuccess ||
              launch_scene_boxes(d_vertices, nQuads, nPad, d_boxes, d_keys, ctx->stream) != hipSuccess ||
+             (d_boxVertices && launch_vertex_keys(d_boxVertices, nBox, d_keys, ctx->stream) != hipSuccess) ||
              hipMemcpyAsync(keys, d_keys, sizeof keys, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
              hipStreamSynchronize(ctx->stream) != hipSuccess) {
     status = RZ_ERR_DEVICE;
@@ -938,6 +932,66 @@ extern "C" int rz_prepare_scene_device(rz_context* ctx, uint32_t nQuads, const f
   hipFree(d_sorted);
   hipFree(d_order);
   hipFree(d_keys);
+  return status;
+}
+
+}  // namespace
+
+extern "C" int rz_prepare_scene_device(rz_context* ctx, uint32_t nQuads, const float* d_vertices,
+                                       uint32_t targetSize, uint32_t splitGranularity, uint32_t* d_packets,
+                                       float* d_bounds, uint32_t* quadOffsets, uint32_t* nOccluders,
+                                       float refMinMax[8]) {
+  if (!ctx || !quadOffsets || !nOccluders || !refMinMax || (nQuads && (!d_vertices || !d_packets || !d_bounds)))
+    return RZ_ERR_INVALID_ARGUMENT;
+  // Occluder::bake takes whole 8-quad batches, so every split must fall on one.
+  if (splitGranularity == 0 || splitGranularity % 8u != 0u) return RZ_ERR_INVALID_ARGUMENT;
+  if (reinterpret_cast<uintptr_t>(d_vertices) % 16u != 0u) return RZ_ERR_INVALID_ARGUMENT;
+  quadOffsets[0] = 0;
+  *nOccluders = 0;
+  if (nQuads == 0) return RZ_OK;
+  if (nQuads > (1u << 28)) return RZ_ERR_TOO_LARGE;
+  const rzi::HostTables& tabs = rzi::host_tables();
+  if (tabs.rsqrtStatus != RZ_OK) return tabs.rsqrtStatus;
+  return prepare_scene(ctx, nQuads, d_vertices, nullptr, 0, targetSize, splitGranularity, d_packets, d_bounds,
+                       quadOffsets, nOccluders, refMinMax);
+}
+
+extern "C" int rz_prepare_mesh_device(rz_context* ctx, uint32_t nTriangles, const uint32_t* d_indices,
+                                      const float* d_vertices, uint32_t nVertices, uint32_t targetSize,
+                                      uint32_t splitGranularity, uint32_t* d_packets, float* d_bounds,
+                                      uint32_t* quadOffsets, uint32_t* nOccluders, float refMinMax[8]) {
+  if (!ctx || !quadOffsets || !nOccluders || !refMinMax ||
+      (nTriangles && (!d_indices || !d_vertices || !d_packets || !d_bounds)))
+    return RZ_ERR_INVALID_ARGUMENT;
+  if (splitGranularity == 0 || splitGranularity % 8u != 0u) return RZ_ERR_INVALID_ARGUMENT;
+  if (reinterpret_cast<uintptr_t>(d_vertices) % 16u != 0u) return RZ_ERR_INVALID_ARGUMENT;
+  quadOffsets[0] = 0;
+  *nOccluders = 0;
+  if (nTriangles == 0) return RZ_OK;
+  if (nTriangles > (1u << 28)) return RZ_ERR_TOO_LARGE;
+  const rzi::HostTables& tabs = rzi::host_tables();
+  if (tabs.rsqrtStatus != RZ_OK) return tabs.rsqrtStatus;
+  uint32_t* d_quadIdx = nullptr;
+  float* d_quadVertices = nullptr;
+  uint32_t nWords = 0;
+  int status = RZ_OK;
+  // Main.cpp:86: the quads' indices, then their vertices in quad order.
+  if (dev_alloc(&d_quadIdx, size_t(nTriangles) * 4) != hipSuccess) status = RZ_ERR_OUT_OF_MEMORY;
+  else
+    status = quad_decompose_run(ctx->stream, nTriangles, d_indices, d_vertices, nVertices, tabs.rsqrt, d_quadIdx,
+                                &nWords);
+  if (status == RZ_OK && nWords > 0) {
+    if (dev_alloc(&d_quadVertices, size_t(nWords) * 4) != hipSuccess)
+      status = RZ_ERR_OUT_OF_MEMORY;
+    else if (launch_quad_gather(d_vertices, d_quadIdx, nWords, d_quadVertices, ctx->stream) != hipSuccess)
+      status = RZ_ERR_DEVICE;
+    else
+      status = prepare_scene(ctx, nWords / 4u, d_quadVertices, d_vertices, nVertices, targetSize, splitGranularity,
+                             d_packets, d_bounds, quadOffsets, nOccluders, refMinMax);
+  }
+  (void)hipStreamSynchronize(ctx->stream);
+  hipFree(d_quadIdx);
+  hipFree(d_quadVertices);
   return status;
 }
 

@@ -298,6 +298,12 @@ hipError_t launch_scene_boxes(const float* vertices, uint32_t nQuads, uint32_t n
                               uint32_t* sceneKeys, hipStream_t stream);
 hipError_t launch_scene_gather(const float* vertices, uint32_t nQuads, uint32_t nPad, const uint32_t* order,
                                float* out, hipStream_t stream);
+// The scene box's ordered keys (and the bad-vertex word) over a whole vertex
+// buffer of n xyzw vertices, folded into sceneKeys.
+hipError_t launch_vertex_keys(const float* vertices, uint32_t n, uint32_t* sceneKeys, hipStream_t stream);
+// Quad q's four vertices, vertices[quadIdx[4q + j]], for nWords / 4 quads.
+hipError_t launch_quad_gather(const float* vertices, const uint32_t* quadIdx, uint32_t nWords, float* out,
+                              hipStream_t stream);
 hipError_t launch_mlp_backward(const MlpParams& p, hipStream_t stream);
 // Pairwise ranking loss over pairs (i, j) whose sample i should rank above j:
 // mean softplus(s_j - s_i); adds its gradient to dscore and its value to *loss.

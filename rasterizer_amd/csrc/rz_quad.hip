@@ -182,6 +182,13 @@ __global__ void __launch_bounds__(kQdBlock) qd_emit_kernel(QdParams p, const int
   atomicOr(p.bad, 2u);  // a matched pair shares no edge: not reachable from the links
 }
 
+__global__ void __launch_bounds__(kQdBlock) qd_gather_kernel(const float4* __restrict__ vertices,
+                                                             const uint32_t* __restrict__ quadIdx, uint32_t nWords,
+                                                             float4* __restrict__ out) {
+  const uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w < nWords) out[w] = vertices[quadIdx[w]];
+}
+
 template <typename T>
 hipError_t qd_alloc(T** p, size_t count) {
   *p = nullptr;
@@ -222,6 +229,14 @@ struct QdBuffers {
   do {                                              \
     if ((call) != hipSuccess) return RZ_ERR_DEVICE; \
   } while (0)
+
+hipError_t launch_quad_gather(const float* vertices, const uint32_t* quadIdx, uint32_t nWords, float* out,
+                              hipStream_t stream) {
+  if (nWords == 0) return hipSuccess;
+  qd_gather_kernel<<<qd_grid(nWords), kQdBlock, 0, stream>>>(reinterpret_cast<const float4*>(vertices), quadIdx,
+                                                             nWords, reinterpret_cast<float4*>(out));
+  return hipGetLastError();
+}
 
 int quad_decompose_run(hipStream_t stream, uint32_t nTri, const uint32_t* d_idx, const float* d_vertices,
                        uint32_t nVertices, const uint32_t* hostRsqrt, uint32_t* d_out, uint32_t* nOut) {

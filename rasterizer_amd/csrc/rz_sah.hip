@@ -441,6 +441,32 @@ __global__ void __launch_bounds__(256) scene_boxes_kernel(const float4* __restri
   }
 }
 
+// Main.cpp:109-113 over a whole vertex buffer (which can hold vertices no
+// quad references): the scene box as ordered keys, and sceneKeys[8] set when
+// a vertex's x, y or z is not finite or exceeds 2^40.
+__global__ void __launch_bounds__(256) vertex_keys_kernel(const float4* __restrict__ vertices, uint32_t n,
+                                                          uint32_t* __restrict__ sceneKeys) {
+  const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t kmn[4] = {~0u, ~0u, ~0u, ~0u}, kmx[4] = {0u, 0u, 0u, 0u};
+  bool bad = false;
+  if (j < n) {
+    const float4 v = vertices[j];
+    const float c[4] = {v.x, v.y, v.z, v.w};
+    bad = !(fabsf(v.x) <= 0x1p40f) || !(fabsf(v.y) <= 0x1p40f) || !(fabsf(v.z) <= 0x1p40f);
+#pragma unroll
+    for (int l = 0; l < 4; ++l) kmn[l] = kmx[l] = ordered_key(c[l]);
+  }
+  if (__ballot(bad) != 0ull && (threadIdx.x & 63u) == 0u) atomicOr(&sceneKeys[8], 1u);
+#pragma unroll
+  for (int l = 0; l < 4; ++l) {
+    const uint32_t a = __ockl_wfred_min_u32(kmn[l]), b = __ockl_wfred_max_u32(kmx[l]);
+    if ((threadIdx.x & 63u) == 0u) {
+      atomicMin(&sceneKeys[l], a);
+      atomicMax(&sceneKeys[4 + l], b);
+    }
+  }
+}
+
 __global__ void __launch_bounds__(256) scene_gather_kernel(const float4* __restrict__ vertices, uint32_t nQuads,
                                                            uint32_t nPad, const uint32_t* __restrict__ order,
                                                            float4* __restrict__ out) {
@@ -455,6 +481,12 @@ hipError_t launch_scene_boxes(const float* vertices, uint32_t nQuads, uint32_t n
                               uint32_t* sceneKeys, hipStream_t stream) {
   scene_boxes_kernel<<<grid_of(nPad), 256, 0, stream>>>(reinterpret_cast<const float4*>(vertices), nQuads, nPad,
                                                         reinterpret_cast<float4*>(boxes), sceneKeys);
+  return hipGetLastError();
+}
+
+hipError_t launch_vertex_keys(const float* vertices, uint32_t n, uint32_t* sceneKeys, hipStream_t stream) {
+  if (n == 0) return hipSuccess;
+  vertex_keys_kernel<<<grid_of(n), 256, 0, stream>>>(reinterpret_cast<const float4*>(vertices), n, sceneKeys);
   return hipGetLastError();
 }
 

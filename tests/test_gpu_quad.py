@@ -111,3 +111,36 @@ def test_city_terrain_512(ref, rast):
     assert np.array_equal(host, want)
     print(f"QuadDecomposition 512^2 terrain ({tri.shape[0]:,} triangles -> {want.size // 4:,} quads): "
           f"device {t_dev * 1e3:.1f} ms, host {t_host * 1e3:.1f} ms, reference build {t_ref * 1e3:.1f} ms", flush=True)
+
+
+@pytest.mark.parametrize("which", ["terrain", "boxes"])
+def test_mesh_prepared_on_device(rast, which):
+    """rz_prepare_mesh_device (Main.cpp:86-128 from the triangle mesh: the
+    decomposition, padding, SAH batches, the vertex buffer's box and the bake,
+    all in HBM) against the host chain (rz_quad_decompose, then
+    scenes.bake_scene with the vertex buffer's box), bit for bit.  An extra
+    vertex that no triangle references widens the scene box, as Main.cpp:
+    109-113 takes the box over the whole vertex buffer."""
+    from test_gpu_sah import _same_occluders
+
+    if which == "terrain":
+        tri, V = scenes.heightfield_triangles(scenes.Heightfield(97, 4.0, 20.0, seed=3))
+    else:
+        tri, V = scenes.triangulated_boxes(np.random.default_rng(5), 300)
+    V = np.concatenate([V, np.array([[500.0, -300.0, 250.0, 1.0]], np.float32)])
+    want = scenes.bake_scene(scenes.decompose_mesh(tri, V), vertex_buffer=V).occluders
+    got = rast.prepare_mesh_device(tri, V)
+    _same_occluders(got, want)
+    assert float(want[0].m_refMax[0]) == 500.0 and float(want[0].m_refMin[1]) == -300.0
+
+
+def test_prepare_mesh_arguments(rast):
+    tri, V = scenes.triangulated_boxes(np.random.default_rng(6), 4)
+    assert rast.prepare_mesh_device(tri[:0], V) == []
+    with pytest.raises(RasterizerError):
+        rast.prepare_mesh_device(tri, V, 512, 4)  # granularity not a multiple of 8
+    bad = np.concatenate([V, np.array([[np.nan, 0.0, 0.0, 1.0]], np.float32)])  # not referenced, still in the box
+    with pytest.raises(RasterizerError):
+        rast.prepare_mesh_device(tri, bad)
+    with pytest.raises(RasterizerError):
+        rast.prepare_mesh_device(np.array([[0, 1, V.shape[0]]], np.uint32), V)
