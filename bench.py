@@ -73,13 +73,17 @@ def measured_traffic(kernel_profile: str, frames: int):
     return measured[near] * frames / near, f"measured at {near} frames per launch, scaled per frame ({kernel_profile})"
 
 
-def cpu_baseline(scene, orders, threads: int, frames: int, query_reps: int):
+def cpu_baseline(scene, orders, threads: int, frames: int, query_reps: int, path_frames: int):
+    """The oracle on the host cores over the frames the timed GPU launch
+    draws (frames 0 .. path_frames - 1 of the camera path, cycled to
+    `frames` frames), one oracle instance per thread."""
     import oracle
 
     m = scene.meta
     w, h = m["width"], m["height"]
     nf = m["mvps"].shape[0]
-    sel = np.arange(frames) % nf
+    span = max(1, min(path_frames, nf))
+    sel = np.arange(frames) % span
     mvps = m["mvps"][sel]
     ords = orders[sel]
     t_mt = oracle.bench_frames(w, h, scene.occluders, mvps, ords, threads)
@@ -99,7 +103,8 @@ def cpu_baseline(scene, orders, threads: int, frames: int, query_reps: int):
         "unit": "quads/s",
         "cores": threads,
         "kind": "port",
-        "sample": (f"{frames} frames of the 256-frame path ({frames * q:,} quads) on {threads} threads, one oracle "
+        "sample": (f"frames 0-{span - 1} of the 256-frame path (the frames the timed launch draws), cycled to "
+                   f"{frames} frames ({frames * q:,} quads) on {threads} threads, one oracle "
                    f"instance per thread: {t_mt:.2f} s; single thread {single_frames} frames: "
                    f"{single_frames * q / t_st:.3e} quads/s; queries: {query_reps}x{nq:,} on {threads} threads "
                    f"= {query_reps * nq / t_q:.3e} queries/s"),
@@ -687,7 +692,7 @@ def main() -> int:
     if not args.no_cpu:
         threads = min(len(os.sched_getaffinity(0)), 16)
         t0 = time.perf_counter()
-        out["cpu_baseline"] = cpu_baseline(scene, orders, threads, args.cpu_frames, 4)
+        out["cpu_baseline"] = cpu_baseline(scene, orders, threads, args.cpu_frames, 4, K)
         log(f"cpu baseline {time.perf_counter() - t0:.1f} s")
     print(json.dumps(out), flush=True)
     return 0
