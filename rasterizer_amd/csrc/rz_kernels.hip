@@ -677,6 +677,7 @@ struct TileSmem {
   uint32_t tried[kItemWindow];
   uint32_t spart[2][8];
   uint32_t wstate[2];
+  uint32_t wnext;          // candidate position where the next item window starts
   uint32_t blockStats[4];  // observability builds: visited, rejected, written, fold steps
 };
 
@@ -1527,14 +1528,18 @@ __device__ __forceinline__ void process_tile(const TileJob& J, TileSmem<TBX, TBY
     if (tid == 0) ctr[kStatLoadTicks] += stamp() - tLoad;
   }
   const uint64_t tLoop = stamp();
-  for (uint32_t wbase = 0; wbase < nList; wbase += kItemWindow) {
-    // Gather this window's items that touch the tile, in draw order.
-    const bool inWin = tid < kItemWindow && wbase + tid < nList;
+  for (uint32_t wbase = 0; wbase < nList;) {
+    // Gather this window's items that touch the tile, in draw order: every
+    // thread tests one candidate, the first kItemWindow relevant ones are
+    // kept, and the next window starts at the first relevant candidate not
+    // kept (or after the last candidate tested).
+    const bool inWin = wbase + tid < nList;
     const uint32_t i = inWin ? item_at(wbase + tid) : 0u;
     const uint32_t rel = inWin ? relevance(i) : 0u;
     const uint64_t m = __ballot(rel != 0u);
     const uint32_t pre = __builtin_amdgcn_mbcnt_hi(uint32_t(m >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(m), 0u));
     if (lane == 0) sm.wcnt[wave] = uint32_t(__popcll(m));
+    if (tid == 0) sm.wnext = min(wbase + NT, nList);
     __syncthreads();
     uint32_t off = 0, nRel = 0;
 #pragma unroll
@@ -1542,7 +1547,9 @@ __device__ __forceinline__ void process_tile(const TileJob& J, TileSmem<TBX, TBY
       off += w < wave ? sm.wcnt[w] : 0u;
       nRel += sm.wcnt[w];
     }
-    if (rel) {
+    if (rel && off + pre == kItemWindow) sm.wnext = wbase + tid;
+    nRel = min(nRel, kItemWindow);
+    if (rel && off + pre < kItemWindow) {
       const ItemState& S = J.state[i];
       ItemLocal& L = sm.items[off + pre];
       L.index = i;
@@ -1573,6 +1580,7 @@ __device__ __forceinline__ void process_tile(const TileJob& J, TileSmem<TBX, TBY
     }
     __syncthreads();
     nRelW = nRel;
+    const uint32_t wnext = sm.wnext;
     // Later query items are answered early only while the tile waits: a
     // pass over the window's query items at its start delays the tiles with
     // the longest lists, which other tiles' waits depend on.
@@ -1724,6 +1732,7 @@ __device__ __forceinline__ void process_tile(const TileJob& J, TileSmem<TBX, TBY
       ++drawVersion;  // the tile's state changed: early answers that failed may now succeed
     }
     __syncthreads();
+    wbase = wnext;
   }
 
   if (STATS) {
