@@ -262,6 +262,10 @@ struct MlpParams {
   float* grad;          // backward output, accumulated (zero it first)
 };
 hipError_t launch_mlp_forward(const MlpParams& p, hipStream_t stream);
+// Work-groups of the scoring kernel resident on the current device at once
+// (queried once per process; call outside stream capture before the first
+// captured launch).
+uint32_t mlp_forward_max_groups();
 
 // Occluder::bake on the device (rz_bake.hip): occluder o's vertices are
 // vertices[offsets[o] .. offsets[o + 1]) (4 floats each, a multiple of 32),

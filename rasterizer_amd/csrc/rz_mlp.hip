@@ -342,9 +342,26 @@ __global__ void rz_fill_f32_kernel(float* p, uint32_t n, float v) {
 
 static inline uint32_t groups_of(uint32_t n, uint32_t per) { return (n + per - 1) / per; }
 
+uint32_t mlp_forward_max_groups() {
+  // The scoring kernel's work-groups that fit the device at once (3 per CU
+  // on gfx950): a grid of exactly that many runs its grid-stride loop in
+  // one round, where a larger grid would leave a partial second round.
+  static const uint32_t cap = [] {
+    int dev = 0, cus = 0, per = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, rz_mlp_forward_kernel, 256, 0) != hipSuccess ||
+        cus <= 0 || per <= 0)
+      return 1024u;
+    return uint32_t(cus) * uint32_t(per);
+  }();
+  return cap;
+}
+
 hipError_t launch_mlp_forward(const MlpParams& p, hipStream_t stream) {
   if (p.n == 0) return hipSuccess;
-  const uint32_t g = groups_of(p.n, 128) < 1024 ? groups_of(p.n, 128) : 1024;
+  const uint32_t cap = mlp_forward_max_groups();
+  const uint32_t g = groups_of(p.n, 128) < cap ? groups_of(p.n, 128) : cap;
   hipLaunchKernelGGL(rz_mlp_forward_kernel, dim3(g), dim3(256), 0, stream, p);
   return hipGetLastError();
 }

@@ -871,6 +871,7 @@ extern "C" int rz_views_set_model(rz_views* v, rz_mlp* model, int byOccluder) {
     s = rz_views_enable_labels(v, 1);  // features read the previous step's labels
     if (s != RZ_OK) return s;
   }
+  if (model) (void)mlp_forward_max_groups();  // the occupancy query, before any captured step
   v->model = model;
   v->byOccluder = byOccluder != 0;
   v->dirty = true;
