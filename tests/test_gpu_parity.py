@@ -134,3 +134,65 @@ def test_counting_variant_matches_oracle(oracle_mod, scene1, report):
     print(f"visited {s[:, 13].sum()}, written {s[:, 15].sum()}, early tried {s[:, 20].sum()} "
           f"answered {s[:, 21].sum()}", flush=True)
     g.frame_stats(False)
+
+
+def test_crowded_tile_item_windows(oracle_mod, report):
+    """Hundreds of small occluders crowded into one or two screen tiles, so a
+    tile's item windows overflow: more than 128 of a window's candidates are
+    relevant, the first 128 are kept and the next window starts at the first
+    one not kept.  The single-view frame (all items as candidates), gated
+    views (binned lists, every candidate relevant) and classify-then-draw
+    views, against the oracle bit for bit."""
+    from rasterizer_amd import Views
+
+    rng = scenes.SplitMix64(21)
+    n = 1200
+    c, h = scenes.random_boxes(rng, n, -2.5, 2.5, 0.05, 0.3)
+    c[:, 2] += 100.0  # a cluster about 40 px across at 1280x720
+    bg_c, bg_h = scenes.random_boxes(rng, 200, -50.0, 50.0, 0.5, 5.0)
+    bg_c[:, 2] = np.abs(bg_c[:, 2]) + 60.0
+    quads = np.concatenate([scenes.box_quads(c[i], h[i]) for i in range(n)] +
+                           [scenes.box_quads(bg_c[i], bg_h[i]) for i in range(200)], axis=0)
+    scene = scenes.bake_scene(quads, target=24, gran=8)  # one to three batches per occluder
+    w, h = 1280, 720
+    eye = np.zeros(3, np.float32)
+    mvp = scenes.view_projection(eye, (0, 0, 1), (0, 1, 0), np.pi / 3, w / h)
+    occ = [scene.occluders[i] for i in scenes.front_to_back(scene.occluders, eye)]
+    qrng = scenes.SplitMix64(22)
+    qc, qh = scenes.random_boxes(qrng, 2048, -3.0, 3.0, 0.02, 0.2)
+    qc[:, 2] += 101.0
+    queries = scenes.aabb_bounds(qc, qh)
+    print(f"crowded tile: {len(occ)} occluders, {scene.quad_count} quads", flush=True)
+    g, o = _pair(oracle_mod, w, h, mvp)
+    st_o = o.draw_frame(occ)
+    st_g = g.draw_frame(occ)
+    assert np.array_equal(st_g, st_o), f"statuses differ at {np.nonzero(st_g != st_o)[0][:10]}"
+    _assert_same_buffers(g, o, report, "crowded tile frame")
+    vg, cg = g.query_batch(queries)
+    vo, co = o.query_batch(queries)
+    assert np.array_equal(vg, vo) and np.array_equal(cg, co)
+    # The cluster (|x|, |y| < 3 at z ~ 100) spans about 34 x 34 px around the
+    # screen centre, so its occluders all touch tiles (9, 5) and (10, 5): far
+    # more relevant items than one window keeps.
+    ctr = np.array([(o.m_boundsMin[:3] + o.m_boundsMax[:3]) / 2 for o in occ])
+    assert int(((np.abs(ctr[:, 0]) < 3) & (np.abs(ctr[:, 1]) < 3) & (np.abs(ctr[:, 2] - 100) < 3)).sum()) > 300
+    for gated in (True, False):
+        V = Views(Rasterizer(64, 64), [(w, h)])
+        V.set_query_gated(gated)
+        V.set_mvp(0, mvp)
+        V.set_occluders(0, occ)
+        V.set_queries(0, queries)
+        V.run(graph=False)
+        V.sync()
+        ov = oracle_mod.OracleRasterizer(w, h)
+        ov.clear()
+        ov.setModelViewProjection(mvp)
+        st_v = ov.draw_frame(occ, gated=gated)
+        assert np.array_equal(V.status(0), st_v), f"views gated={gated}: statuses differ"
+        dg, hg = V.raw(0)
+        do, ho = ov.raw()
+        assert np.array_equal(hg, ho), f"views gated={gated}: hiZ {report(hg, ho)}"
+        assert np.array_equal(dg, do), f"views gated={gated}: depth {report(dg, do)}"
+        vq, cq = V.queries(0)
+        oq, ocq = ov.query_batch(queries)
+        assert np.array_equal(vq, oq) and np.array_equal(cq, ocq)
