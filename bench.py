@@ -386,14 +386,14 @@ def learned_frames_in_flight(R, scene, orders, handles, plans, F: int, tuned: di
     for Vj in plans:
         Vj.set_model(model, by_occluder=True)
         Vj.set_config(tuned["chosen"]["waves"], tuned["chosen"]["groupsPerCU"], tuned["chosen"]["tileW"])
-        Vj.run(graph=False)
+        Vj.run(graph=True)
         Vj.sync()
     # Check the first launch's first and last frames (each drawn from the
     # buffer its previous launch left, which the single-view path is given).
     nb = (W // 8) * (H // 8)
     V0 = plans[0]
     before = [V0.raw(v) for v in sorted({0, F - 1})]
-    V0.run(graph=False)
+    V0.run(graph=True)
     V0.sync()
     for (d0, h0), v in zip(before, sorted({0, F - 1})):
         order = V0.order(v)
@@ -413,12 +413,12 @@ def learned_frames_in_flight(R, scene, orders, handles, plans, F: int, tuned: di
                 np.array_equal(vis, vv) and np.array_equal(clip, cv)):
             raise RuntimeError(f"batched + learned: frame {f} differs from the single-view path in the learned order")
     for Vj in plans:
-        Vj.run(graph=False)
+        Vj.run(graph=True)
     for Vj in plans:
         Vj.sync()
     t0 = time.perf_counter()
     for Vj in plans:
-        Vj.run(graph=False)
+        Vj.run(graph=True)
     for Vj in plans:
         Vj.sync()
     t = time.perf_counter() - t0
@@ -530,7 +530,7 @@ def main() -> int:
     if launches == 1:
         load_launch(0)
         t0 = time.perf_counter()
-        V.run(graph=False)
+        V.run(graph=True)
         V.sync()
         log(f"frames in flight: {F} frames per launch, plan built and run ({time.perf_counter() - t0:.1f} s), "
             f"{V.info()}")
@@ -545,7 +545,7 @@ def main() -> int:
                 Vj.set_occluders(v, [scene.occluders[i] for i in orders[f]])
                 Vj.set_queries(v, q)
                 Vj.set_mvp(v, m["mvps"][f])
-            Vj.run(graph=False)
+            Vj.run(graph=True)
             Vj.sync()
             plans.append(Vj)
     plans = plans or [V]
@@ -571,15 +571,15 @@ def main() -> int:
     log(f"frames in flight autotune: {fif_tuned}")
     for Vj in plans[1:]:
         Vj.set_config(fif_tuned["chosen"]["waves"], fif_tuned["chosen"]["groupsPerCU"], fif_tuned["chosen"]["tileW"])
-        Vj.run(graph=False)
+        Vj.run(graph=True)
         Vj.sync()
     for Vj in plans:  # warm-up launch of each plan (untimed)
-        Vj.run(graph=False)
+        Vj.run(graph=True)
     for Vj in plans:
         Vj.sync()
     t0 = time.perf_counter()
     for Vj in plans:
-        Vj.run(graph=False)
+        Vj.run(graph=True)
     for Vj in plans:
         Vj.sync()
     t_fif = time.perf_counter() - t0

@@ -127,6 +127,12 @@ struct rz_views {
   bool stats = false;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  // Cooperative (gated) steps: the launches before the raster launch and the
+  // queries after it, captured as two graphs around the directly enqueued
+  // raster launch, whose parameters are kept in rp.
+  hipGraph_t graphPre = nullptr, graphPost = nullptr;
+  hipGraphExec_t execPre = nullptr, execPost = nullptr;
+  ViewsParams rp{};
   hipEvent_t t0 = nullptr, t1 = nullptr;
   // Phase timing of direct (not captured) steps, when on: events before the
   // clear, before the raster launch, after it and after the queries.
@@ -141,6 +147,14 @@ static void free_plan(rz_views* v) {
   if (v->graph) (void)hipGraphDestroy(v->graph);
   v->exec = nullptr;
   v->graph = nullptr;
+  for (hipGraphExec_t* e : {&v->execPre, &v->execPost}) {
+    if (*e) (void)hipGraphExecDestroy(*e);
+    *e = nullptr;
+  }
+  for (hipGraph_t* g : {&v->graphPre, &v->graphPost}) {
+    if (*g) (void)hipGraphDestroy(*g);
+    *g = nullptr;
+  }
   hipFree(v->d_items);
   hipFree(v->d_state);
   hipFree(v->d_done);
@@ -596,120 +610,134 @@ static LearnParams learn_params(rz_views* v) {
   return lp;
 }
 
-// The launches of one step, on v->stream.
-static int enqueue_step(rz_views* v, bool capturing) {
+// Parts of a step: the launches before the raster launch, the raster
+// launch, and the queries after it.
+constexpr int kStepPre = 1, kStepRaster = 2, kStepPost = 4, kStepAll = 7;
+
+// The launches of one step (or of the given parts of it), on v->stream.
+static int enqueue_step(rz_views* v, bool capturing, int parts = kStepAll) {
   const uint32_t nv = uint32_t(v->views.size());
   const bool prof = v->profile && !capturing;
-  RZ_HIP(hipMemcpyAsync(v->d_views, v->h_views, nv * sizeof(ViewDesc), hipMemcpyHostToDevice, v->stream));
-  if (prof) RZ_HIP(hipEventRecord(v->ph[0], v->stream));
-  RZ_HIP(launch_views_clear(v->d_views, nv, v->maxClear, v->stream));
-  PrepParams pp{};
-  pp.items = v->d_items;
-  pp.state = v->d_state;
-  pp.nItems = v->nItems;
-  pp.tileW = pp.tileH = 8;
-  pp.rcp = v->d_rcp;
-  pp.done = v->d_done;
-  pp.visible = v->d_visible;
-  pp.views = v->d_views;
-  pp.queue = v->d_queue;
-  if (v->nItems == 0) {
-    // Nothing to draw: the prep launch that resets the queue is skipped, and
-    // so is the raster launch that reads it.
-  } else {
-    RZ_HIP(launch_prep(pp, v->stream));
-    SetupParams sp{};
-    sp.items = v->d_items;
-    sp.state = v->d_state;
-    sp.nItems = v->nItems;
-    sp.nQuads = v->nQuads;
-    sp.rcp = v->d_rcp;
-    sp.recs = v->d_recs;
-    sp.bbox = v->d_bbox;
-    sp.groupBox = v->d_groupBox;
-    sp.groupItem = v->d_groupItem;
-    sp.views = v->d_views;
-    sp.featZ = (v->labels || v->model) ? v->d_featZ : nullptr;
-    RZ_HIP(launch_setup(sp, v->stream));
-    const LearnParams lp = learn_params(v);
-    if (lp.nDrawn) RZ_HIP(launch_fill_u32(lp.nDrawn, 1, 0u, v->stream));
-    if (v->labels || v->model) RZ_HIP(launch_features(lp, v->nBatches, v->stream));
-    if (v->model && v->nBatches) {
-      // Score the batches of drawn occluders (the others score -inf), then
-      // order each view's batches by descending score; the radix sort is
-      // stable, so ties keep occluder order.
-      RZ_HIP(launch_fill_f32(v->d_scores, v->nBatches, -std::numeric_limits<float>::infinity(), v->stream));
-      MlpParams mp{};
-      mp.w = v->model->d_w;
-      mp.norm = v->model->d_norm;
-      mp.x = v->d_features;
-      mp.n = v->nBatches;
-      mp.rows = lp.drawn;
-      mp.nRows = lp.nDrawn;
-      mp.scores = v->d_scores;
-      RZ_HIP(launch_mlp_forward(mp, v->stream));
-      size_t bytes = v->sortTempBytes;
-      if (v->byOccluder) {
-        // Whole occluders by their best batch score, stable within a view.
-        RZ_HIP(launch_item_scores(lp, v->d_scores, v->d_itemScores, v->stream));
-        RZ_HIP(launch_sort_keys(v->d_views, nv, v->d_itemScores, v->nItems, true, v->d_sortKeys, v->stream));
-        RZ_HIP(rocprim::radix_sort_pairs(v->d_sortTemp, bytes, v->d_sortKeys, v->d_sortKeysOut,
-                                                  v->d_itemIota, v->d_itemOrder, uint32_t(v->nItems), 0u, unsigned(v->sortEndBit),
-                                                  v->stream));
+  if (parts == kStepRaster) {
+    if (v->nItems) RZ_HIP(launch_views(v->rp, v->nGroups, v->stream, v->gated && v->cooperative));
+    return RZ_OK;
+  }
+  if (parts & kStepPre) {
+    RZ_HIP(hipMemcpyAsync(v->d_views, v->h_views, nv * sizeof(ViewDesc), hipMemcpyHostToDevice, v->stream));
+    if (prof) RZ_HIP(hipEventRecord(v->ph[0], v->stream));
+    RZ_HIP(launch_views_clear(v->d_views, nv, v->maxClear, v->stream));
+    PrepParams pp{};
+    pp.items = v->d_items;
+    pp.state = v->d_state;
+    pp.nItems = v->nItems;
+    pp.tileW = pp.tileH = 8;
+    pp.rcp = v->d_rcp;
+    pp.done = v->d_done;
+    pp.visible = v->d_visible;
+    pp.views = v->d_views;
+    pp.queue = v->d_queue;
+    if (v->nItems == 0) {
+      // Nothing to draw: the prep launch that resets the queue is skipped, and
+      // so is the raster launch that reads it.
+    } else {
+      RZ_HIP(launch_prep(pp, v->stream));
+      SetupParams sp{};
+      sp.items = v->d_items;
+      sp.state = v->d_state;
+      sp.nItems = v->nItems;
+      sp.nQuads = v->nQuads;
+      sp.rcp = v->d_rcp;
+      sp.recs = v->d_recs;
+      sp.bbox = v->d_bbox;
+      sp.groupBox = v->d_groupBox;
+      sp.groupItem = v->d_groupItem;
+      sp.views = v->d_views;
+      sp.featZ = (v->labels || v->model) ? v->d_featZ : nullptr;
+      RZ_HIP(launch_setup(sp, v->stream));
+      const LearnParams lp = learn_params(v);
+      if (lp.nDrawn) RZ_HIP(launch_fill_u32(lp.nDrawn, 1, 0u, v->stream));
+      if (v->labels || v->model) RZ_HIP(launch_features(lp, v->nBatches, v->stream));
+      if (v->model && v->nBatches) {
+        // Score the batches of drawn occluders (the others score -inf), then
+        // order each view's batches by descending score; the radix sort is
+        // stable, so ties keep occluder order.
+        RZ_HIP(launch_fill_f32(v->d_scores, v->nBatches, -std::numeric_limits<float>::infinity(), v->stream));
+        MlpParams mp{};
+        mp.w = v->model->d_w;
+        mp.norm = v->model->d_norm;
+        mp.x = v->d_features;
+        mp.n = v->nBatches;
+        mp.rows = lp.drawn;
+        mp.nRows = lp.nDrawn;
+        mp.scores = v->d_scores;
+        RZ_HIP(launch_mlp_forward(mp, v->stream));
+        size_t bytes = v->sortTempBytes;
+        if (v->byOccluder) {
+          // Whole occluders by their best batch score, stable within a view.
+          RZ_HIP(launch_item_scores(lp, v->d_scores, v->d_itemScores, v->stream));
+          RZ_HIP(launch_sort_keys(v->d_views, nv, v->d_itemScores, v->nItems, true, v->d_sortKeys, v->stream));
+          RZ_HIP(rocprim::radix_sort_pairs(v->d_sortTemp, bytes, v->d_sortKeys, v->d_sortKeysOut,
+                                                    v->d_itemIota, v->d_itemOrder, uint32_t(v->nItems), 0u, unsigned(v->sortEndBit),
+                                                    v->stream));
+          RZ_HIP(launch_item_reorder(lp, v->d_itemOrder, v->d_ritems, v->d_rstate, v->stream));
+        } else {
+          RZ_HIP(launch_sort_keys(v->d_views, nv, v->d_scores, v->nBatches, false, v->d_sortKeys, v->stream));
+          RZ_HIP(rocprim::radix_sort_pairs(v->d_sortTemp, bytes, v->d_sortKeys, v->d_sortKeysOut, v->d_iota,
+                                                    v->d_order, uint32_t(v->nBatches), 0u, unsigned(v->sortEndBit), v->stream));
+        }
+      }
+      if (v->f2b) {
+        // Front to back (Main.cpp:184-190), stable: ties keep submission order.
+        size_t bytes = v->sortTempBytes;
+        RZ_HIP(hipMemcpyAsync(v->d_eyes, v->h_eyes, nv * 4 * sizeof(float), hipMemcpyHostToDevice, v->stream));
+        RZ_HIP(launch_f2b_keys(v->d_items, v->nItems, v->d_eyes, v->d_sortKeys, v->stream));
+        RZ_HIP(rocprim::radix_sort_pairs(v->d_sortTemp, bytes, v->d_sortKeys, v->d_sortKeysOut, v->d_itemIota,
+                                                  v->d_itemOrder, uint32_t(v->nItems), 0u, unsigned(v->sortEndBit), v->stream));
         RZ_HIP(launch_item_reorder(lp, v->d_itemOrder, v->d_ritems, v->d_rstate, v->stream));
-      } else {
-        RZ_HIP(launch_sort_keys(v->d_views, nv, v->d_scores, v->nBatches, false, v->d_sortKeys, v->stream));
-        RZ_HIP(rocprim::radix_sort_pairs(v->d_sortTemp, bytes, v->d_sortKeys, v->d_sortKeysOut, v->d_iota,
-                                                  v->d_order, uint32_t(v->nBatches), 0u, unsigned(v->sortEndBit), v->stream));
+      }
+      if (v->batchOrder) RZ_HIP(launch_batch_items(lp, v->d_order, v->nBatches, v->d_bitems, v->d_bstate, v->stream));
+      ViewsParams rp{};
+      const bool reordered = ((v->model && v->byOccluder) || v->f2b) && v->nItems;
+      rp.items = v->batchOrder ? v->d_bitems : (reordered ? v->d_ritems : v->d_items);
+      rp.state = v->batchOrder ? v->d_bstate : (reordered ? v->d_rstate : v->d_state);
+      rp.batchItems = v->batchOrder ? 1u : 0u;
+      rp.views = v->d_views;
+      rp.nViews = nv;
+      rp.nUnits = v->nUnits;
+      rp.recs = v->d_recs;
+      rp.groupBox = v->d_groupBox;
+      rp.cov = v->d_cov;
+      rp.queue = v->d_queue;
+      rp.waves = uint32_t(v->waves);
+      rp.tileB = uint32_t(v->planTileB);
+      rp.stats = v->stats ? v->d_stats : nullptr;
+      if (v->stats)
+        RZ_HIP(launch_fill_u32(reinterpret_cast<uint32_t*>(v->d_stats), 2 * (3 + kFrameStatWords), 0u, v->stream));
+      // Labels are counted by the draw itself (after this step's features
+      // have read the previous step's).
+      // A model that does not read the previous labels leaves them at 0 (the
+      // plan zeroes them) and the draw without label tracking.
+      rp.labels = (v->labels || (v->model && v->model->usesLabels)) ? v->d_labels : nullptr;
+      if (rp.labels) RZ_HIP(launch_fill_u32(v->d_labels, v->nBatches, 0u, v->stream));
+      rp.lists = v->d_lists;
+      rp.listLen = v->d_listLen;
+      RZ_HIP(launch_views_bin(rp, v->batchOrder ? v->nBatches : v->nItems, v->d_tileBox, v->stream));
+      rp.unitList = v->d_unitList;
+      rp.nActive = v->d_nActive;
+      rp.unitCounts = v->d_unitCounts;
+      RZ_HIP(launch_views_compact(rp, v->d_unitTmp, v->stream));
+      rp.done = v->gated ? v->d_done : nullptr;
+      rp.err = v->d_err;
+      rp.timeoutTicks = v->timeoutTicks;
+      v->rp = rp;
+      if (parts & kStepRaster) {
+        if (prof) RZ_HIP(hipEventRecord(v->ph[1], v->stream));
+        RZ_HIP(launch_views(rp, v->nGroups, v->stream, v->gated && v->cooperative));
+        if (prof) RZ_HIP(hipEventRecord(v->ph[2], v->stream));
       }
     }
-    if (v->f2b) {
-      // Front to back (Main.cpp:184-190), stable: ties keep submission order.
-      size_t bytes = v->sortTempBytes;
-      RZ_HIP(hipMemcpyAsync(v->d_eyes, v->h_eyes, nv * 4 * sizeof(float), hipMemcpyHostToDevice, v->stream));
-      RZ_HIP(launch_f2b_keys(v->d_items, v->nItems, v->d_eyes, v->d_sortKeys, v->stream));
-      RZ_HIP(rocprim::radix_sort_pairs(v->d_sortTemp, bytes, v->d_sortKeys, v->d_sortKeysOut, v->d_itemIota,
-                                                v->d_itemOrder, uint32_t(v->nItems), 0u, unsigned(v->sortEndBit), v->stream));
-      RZ_HIP(launch_item_reorder(lp, v->d_itemOrder, v->d_ritems, v->d_rstate, v->stream));
-    }
-    if (v->batchOrder) RZ_HIP(launch_batch_items(lp, v->d_order, v->nBatches, v->d_bitems, v->d_bstate, v->stream));
-    ViewsParams rp{};
-    const bool reordered = ((v->model && v->byOccluder) || v->f2b) && v->nItems;
-    rp.items = v->batchOrder ? v->d_bitems : (reordered ? v->d_ritems : v->d_items);
-    rp.state = v->batchOrder ? v->d_bstate : (reordered ? v->d_rstate : v->d_state);
-    rp.batchItems = v->batchOrder ? 1u : 0u;
-    rp.views = v->d_views;
-    rp.nViews = nv;
-    rp.nUnits = v->nUnits;
-    rp.recs = v->d_recs;
-    rp.groupBox = v->d_groupBox;
-    rp.cov = v->d_cov;
-    rp.queue = v->d_queue;
-    rp.waves = uint32_t(v->waves);
-    rp.tileB = uint32_t(v->planTileB);
-    rp.stats = v->stats ? v->d_stats : nullptr;
-    if (v->stats)
-      RZ_HIP(launch_fill_u32(reinterpret_cast<uint32_t*>(v->d_stats), 2 * (3 + kFrameStatWords), 0u, v->stream));
-    // Labels are counted by the draw itself (after this step's features
-    // have read the previous step's).
-    // A model that does not read the previous labels leaves them at 0 (the
-    // plan zeroes them) and the draw without label tracking.
-    rp.labels = (v->labels || (v->model && v->model->usesLabels)) ? v->d_labels : nullptr;
-    if (rp.labels) RZ_HIP(launch_fill_u32(v->d_labels, v->nBatches, 0u, v->stream));
-    rp.lists = v->d_lists;
-    rp.listLen = v->d_listLen;
-    RZ_HIP(launch_views_bin(rp, v->batchOrder ? v->nBatches : v->nItems, v->d_tileBox, v->stream));
-    rp.unitList = v->d_unitList;
-    rp.nActive = v->d_nActive;
-    rp.unitCounts = v->d_unitCounts;
-    RZ_HIP(launch_views_compact(rp, v->d_unitTmp, v->stream));
-    rp.done = v->gated ? v->d_done : nullptr;
-    rp.err = v->d_err;
-    rp.timeoutTicks = v->timeoutTicks;
-    if (prof) RZ_HIP(hipEventRecord(v->ph[1], v->stream));
-    RZ_HIP(launch_views(rp, v->nGroups, v->stream, v->gated && v->cooperative));
-    if (prof) RZ_HIP(hipEventRecord(v->ph[2], v->stream));
   }
+  if (!(parts & kStepPost)) return RZ_OK;
   ViewsQueryParams qp{};
   qp.views = v->d_views;
   qp.nViews = nv;
@@ -740,9 +768,35 @@ extern "C" int rz_views_run(rz_views* v, int useGraph) {
     v->h_views[i].mvp = v->views[i].mvp;
     if (v->h_eyes) std::memcpy(v->h_eyes + 4 * i, v->views[i].eye, 4 * sizeof(float));
   }
-  // Gated steps launch cooperatively (the grid must be co-resident), which
-  // is not captured: they are enqueued directly.
-  if ((v->gated && v->cooperative) || v->profile) useGraph = 0;
+  if (v->profile) useGraph = 0;
+  // Gated steps launch the raster kernel cooperatively (its grid must be
+  // co-resident), which is not captured: the launches before it and the
+  // queries after it are replayed as two graphs around the direct launch.
+  const bool split = useGraph && v->gated && v->cooperative;
+  if (split) {
+    if (!v->execPre) {
+      for (int part : {kStepPre, kStepPost}) {
+        RZ_HIP(hipStreamBeginCapture(v->stream, hipStreamCaptureModeThreadLocal));
+        s = enqueue_step(v, true, part);
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(v->stream, &g);
+        if (s != RZ_OK || e != hipSuccess) {
+          if (g) (void)hipGraphDestroy(g);
+          return s != RZ_OK ? s : RZ_ERR_DEVICE;
+        }
+        (part == kStepPre ? v->graphPre : v->graphPost) = g;
+        RZ_HIP(hipGraphInstantiate(part == kStepPre ? &v->execPre : &v->execPost, g, nullptr, nullptr, 0));
+      }
+    }
+    RZ_HIP(hipEventRecord(v->t0, v->stream));
+    RZ_HIP(hipGraphLaunch(v->execPre, v->stream));
+    s = enqueue_step(v, false, kStepRaster);
+    if (s != RZ_OK) return s;
+    RZ_HIP(hipGraphLaunch(v->execPost, v->stream));
+    RZ_HIP(hipEventRecord(v->t1, v->stream));
+    v->ran = true;
+    return RZ_OK;
+  }
   if (useGraph && !v->exec) {
     RZ_HIP(hipStreamBeginCapture(v->stream, hipStreamCaptureModeThreadLocal));
     s = enqueue_step(v, true);
@@ -964,7 +1018,7 @@ extern "C" int rz_views_get_info(rz_views* v, uint32_t info[8]) {
   info[3] = v->nUnits;
   info[4] = v->nQueries;
   info[5] = v->nGroups;
-  info[6] = v->exec ? 1u : 0u;
+  info[6] = (v->exec || v->execPre) ? 1u : 0u;
   info[7] = v->dirty ? 1u : 0u;
   return RZ_OK;
 }

@@ -124,6 +124,55 @@ def test_views_graph_replays_match_direct(oracle_mod, scene, report):
                         o)
 
 
+def test_views_gated_graph_replays(oracle_mod, scene, report):
+    """Query-gated steps replayed from their captured graphs (the launches
+    before the cooperative raster launch and the queries after it) with new
+    matrices each time, with direct launches between them: every replay
+    equals the direct launch of the same step and the oracle's gated frame
+    loop."""
+    sizes = [s for s in SIZES[:6] if (s[0] // 64 + (s[0] % 64 > 0)) * (s[1] // 64 + (s[1] % 64 > 0)) <= 510]
+    owner = Rasterizer(64, 64)
+    V = Views(owner, sizes)
+    V.set_query_gated(True)
+    setup = _views_setup(scene, 9)
+    for k in range(len(sizes)):
+        V.set_occluders(k, setup[k][1])
+        V.set_queries(k, setup[k][2])
+    rng = np.random.default_rng(6)
+    # Clear leaves depth stale, so each view's oracle is reused across steps
+    # and mirrors both launches of a repetition.
+    oracles = [oracle_mod.OracleRasterizer(w, h) for w, h in sizes]
+    for rep in range(3):
+        mvps = []
+        for k, (w, h) in enumerate(sizes):
+            eye = rng.uniform(-40, 40, 3).astype(np.float32)
+            mvps.append(scenes.view_projection(eye, (0, 0, 1), (0, 1, 0), np.pi / 3, w / h))
+            V.set_mvp(k, mvps[k])
+        V.run(graph=True)
+        V.sync()
+        assert V.info()["graph"] == 1
+        replay = [V.raw(k) + V.queries(k) + (V.status(k),) for k in range(len(sizes))]
+        V.run(graph=False)
+        V.sync()
+        for k, (w, h) in enumerate(sizes):
+            direct = V.raw(k) + V.queries(k) + (V.status(k),)
+            for a, b in zip(replay[k], direct):
+                assert np.array_equal(a, b), f"gated replay {rep} view {k} differs from the direct launch"
+            o = oracles[k]
+            for _ in range(2):
+                o.clear()
+                o.setModelViewProjection(mvps[k])
+                st_o = o.draw_frame(setup[k][1], gated=True)
+            assert np.array_equal(V.status(k), st_o), f"gated replay {rep} view {k}: statuses"
+            dg, hg = V.raw(k)
+            do, ho = o.raw()
+            assert np.array_equal(hg, ho), f"gated replay {rep} view {k}: hiZ {report(hg, ho)}"
+            assert np.array_equal(dg, do), f"gated replay {rep} view {k}: depth {report(dg, do)}"
+            vg, cg = V.queries(k)
+            vo, co = o.query_batch(setup[k][2])
+            assert np.array_equal(vg, vo) and np.array_equal(cg, co)
+
+
 def test_views_empty_lists():
     owner = Rasterizer(64, 64)
     V = Views(owner, [(64, 64), (32, 16)])
