@@ -2002,13 +2002,14 @@ __global__ void __launch_bounds__(256) rz_item_tiles_kernel(ViewsParams p, uint3
 // indices of the view's items whose tile box (rz_item_tiles_kernel) holds the
 // tile, in submission order, at lists[views[v].listBase + tile * (items in
 // view)], and their count in listLen[unit].  A box can hold a tile the item is
-// not relevant to; the raster kernel's exact test drops those.  One
-// work-group per unit tests the items 256 at a time and compacts them with
-// wave ballots.
+// not relevant to; the raster kernel's exact test drops those.  One wave per
+// unit tests the items 64 at a time and compacts them with ballots (no
+// barriers: a view's items are about a hundred, so a work-group per unit
+// spent its time being scheduled).
 __global__ void __launch_bounds__(256) rz_views_bin_kernel(ViewsParams p, const uint64_t* __restrict__ tileBox) {
-  __shared__ uint32_t wcnt[4];
-  const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
-  for (uint32_t unit = blockIdx.x; unit < p.nUnits; unit += gridDim.x) {
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t wave = uint32_t(__builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6)));
+  for (uint32_t unit = blockIdx.x * 4u + wave; unit < p.nUnits; unit += gridDim.x * 4u) {
     uint32_t lo = 0, hi = p.nViews - 1;
     while (lo < hi) {
       const uint32_t mid = (lo + hi + 1) >> 1;
@@ -2022,8 +2023,8 @@ __global__ void __launch_bounds__(256) rz_views_bin_kernel(ViewsParams p, const 
     const uint32_t end = begin + (p.batchItems ? V.nBatches : V.nItems);
     uint32_t* out = p.lists + V.listBase + size_t(t) * (end - begin);
     uint32_t n = 0;
-    for (uint32_t base = begin; base < end; base += 256u) {
-      const uint32_t i = base + tid;
+    for (uint32_t base = begin; base < end; base += 64u) {
+      const uint32_t i = base + lane;
       bool rel = false;
       if (i < end) {
         const uint64_t b = tileBox[i];
@@ -2032,19 +2033,10 @@ __global__ void __launch_bounds__(256) rz_views_bin_kernel(ViewsParams p, const 
       }
       const uint64_t m = __ballot(rel);
       const uint32_t pre = __builtin_amdgcn_mbcnt_hi(uint32_t(m >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(m), 0u));
-      if (lane == 0) wcnt[wave] = uint32_t(__popcll(m));
-      __syncthreads();
-      uint32_t off = n, total = 0;
-#pragma unroll
-      for (uint32_t w = 0; w < 4; ++w) {
-        off += w < wave ? wcnt[w] : 0u;
-        total += wcnt[w];
-      }
-      if (rel) out[off + pre] = i;
-      n += total;
-      __syncthreads();
+      if (rel) out[n + pre] = i;
+      n += uint32_t(__popcll(m));
     }
-    if (tid == 0) p.listLen[unit] = n;
+    if (lane == 0) p.listLen[unit] = n;
   }
 }
 
@@ -2053,8 +2045,8 @@ hipError_t launch_views_bin(const ViewsParams& p, uint32_t nItems, uint64_t* til
   if (nItems) hipLaunchKernelGGL(rz_item_tiles_kernel, dim3((nItems + 255) / 256), dim3(256), 0, stream, p, nItems, tileBox);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(rz_views_bin_kernel, dim3(std::min<uint32_t>(p.nUnits, 65535u)), dim3(256), 0, stream, p,
-                     tileBox);
+  hipLaunchKernelGGL(rz_views_bin_kernel, dim3(std::min<uint32_t>((p.nUnits + 3u) / 4u, 65535u)), dim3(256), 0,
+                     stream, p, tileBox);
   return hipGetLastError();
 }
 
