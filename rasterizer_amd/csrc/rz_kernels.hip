@@ -2138,10 +2138,13 @@ __global__ void __launch_bounds__(256) rz_views_query_kernel(ViewsQueryParams p)
 
 // clear (Rasterizer.cpp:107-121) of every view: hiZ := 1 in whole 8-entry
 // groups, and the per-tile min hiZ := 1.  blockIdx.y is the view.
+// Clears every view's hiZ to 1 (the cleared mark) eight entries per 16-byte
+// store (each view's hiZ buffer is its own allocation, holding whole groups of
+// eight), and its per-tile minima.
 __global__ void rz_views_clear_kernel(const ViewDesc* __restrict__ views) {
   const ViewDesc& V = views[blockIdx.y];
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < ((V.bX * V.bY + 8u) / 8u) * 8u) V.hiz[i] = 1;
+  if (i < (V.bX * V.bY + 8u) / 8u) reinterpret_cast<uint4*>(V.hiz)[i] = make_uint4(0x10001u, 0x10001u, 0x10001u, 0x10001u);
   if (i < V.pyrX * V.pyrY) V.tileMin[i] = 1;
 }
 

@@ -407,7 +407,7 @@ static int build_plan(rz_views* v) {
     d.mvp = h.mvp;
     units += d.nUnits;
     queries += d.nQueries;
-    maxClear = std::max(maxClear, std::max(((h.bX * h.bY + 8) / 8) * 8, h.pyrX * h.pyrY));
+    maxClear = std::max(maxClear, std::max((h.bX * h.bY + 8) / 8, h.pyrX * h.pyrY));  // 16-byte hiZ stores
     for (const rz_occluder* o : h.occ) {
       ItemDesc it{};
       it.packets = o->d_packets;
