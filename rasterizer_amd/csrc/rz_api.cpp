@@ -77,8 +77,10 @@ struct rz_context {
   QuadRec* d_recs = nullptr;
   uint64_t* d_bbox = nullptr;
   uint64_t* d_groupBox = nullptr;
-  // host staging ring for item tables
+  // host staging ring for item tables: pinned, coherent host memory that
+  // the prep kernel reads through d_hItems (its device mapping)
   ItemDesc* h_items[kStagingSlots] = {};
+  const ItemDesc* d_hItems[kStagingSlots] = {};
   uint32_t h_itemsCap[kStagingSlots] = {};
   hipEvent_t ev_items[kStagingSlots] = {};
   int slot = 0;
@@ -245,8 +247,8 @@ extern "C" int rz_clear(rz_context* ctx) {
   if (!ctx) return RZ_ERR_INVALID_ARGUMENT;
   // Whole 8-entry groups only (Rasterizer.cpp:115-120): the last
   // (blocks + 8) % 8 padding entries keep their value.
-  RZ_HIP(launch_fill_u16(ctx->d_hiz, ((ctx->bX * ctx->bY + 8) / 8) * 8, 1, ctx->stream));
-  RZ_HIP(launch_fill_u16(ctx->d_tileMin, ctx->pyrX * ctx->pyrY, 1, ctx->stream));
+  RZ_HIP(launch_clear(ctx->d_hiz, ((ctx->bX * ctx->bY + 8) / 8) * 8, ctx->d_tileMin, ctx->pyrX * ctx->pyrY,
+                      ctx->stream));
   return RZ_OK;
 }
 
@@ -430,9 +432,12 @@ static void fill_items(ItemDesc* items, uint32_t n, const rz_occluder* const* oc
 }
 
 // Enqueues the prep, setup and frame kernels of one draw whose item
-// descriptors are already in b.items (or enqueued before them).  prof:
-// optional event quadruple (prep, setup, frame, end).
-static int enqueue_kernels(rz_context* c, uint32_t n, uint32_t totalQuads, const DrawBufs& b, hipEvent_t* prof) {
+// descriptors are already in b.items (or enqueued before them), or, with
+// hostItems, in mapped host memory from which prep copies them to b.items;
+// afterPrep (optional) is recorded once prep has read them.  prof: optional
+// event quadruple (prep, setup, frame, end).
+static int enqueue_kernels(rz_context* c, uint32_t n, uint32_t totalQuads, const DrawBufs& b, hipEvent_t* prof,
+                           const ItemDesc* hostItems = nullptr, hipEvent_t afterPrep = nullptr) {
   if (prof) RZ_HIP(hipEventRecord(prof[0], c->stream));
   PrepParams pp{};
   pp.items = b.items;
@@ -446,7 +451,10 @@ static int enqueue_kernels(rz_context* c, uint32_t n, uint32_t totalQuads, const
   pp.done = b.done;
   pp.visible = b.visible;
   pp.mvp = c->mvp;
+  pp.hostItems = hostItems;
+  pp.itemsOut = b.items;
   RZ_HIP(launch_prep(pp, c->stream));
+  if (afterPrep) RZ_HIP(hipEventRecord(afterPrep, c->stream));
   if (prof) RZ_HIP(hipEventRecord(prof[1], c->stream));
 
   SetupParams sp{};
@@ -511,18 +519,22 @@ static int enqueue_draw(rz_context* c, uint32_t n, const rz_occluder* const* occ
   if (c->h_itemsCap[slot] < n) {
     if (c->h_items[slot]) (void)hipHostFree(c->h_items[slot]);
     c->h_items[slot] = nullptr;
+    c->d_hItems[slot] = nullptr;
     c->h_itemsCap[slot] = 0;
     const uint32_t cap = std::max<uint32_t>(n, 64);
-    if (hipHostMalloc(reinterpret_cast<void**>(&c->h_items[slot]), cap * sizeof(ItemDesc)) != hipSuccess)
+    // Coherent (fine-grained): the prep kernel reads each descriptor once,
+    // over the bus, with nothing of an earlier use of the slot in a cache.
+    if (hipHostMalloc(reinterpret_cast<void**>(&c->h_items[slot]), cap * sizeof(ItemDesc),
+                      hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess)
       return RZ_ERR_OUT_OF_MEMORY;
+    void* mapped = nullptr;
+    RZ_HIP(hipHostGetDevicePointer(&mapped, c->h_items[slot], 0));
+    c->d_hItems[slot] = static_cast<const ItemDesc*>(mapped);
     c->h_itemsCap[slot] = cap;
   }
-  ItemDesc* items = c->h_items[slot];
-  fill_items(items, n, occ, kinds);
-  RZ_HIP(hipMemcpyAsync(c->d_items, items, n * sizeof(ItemDesc), hipMemcpyHostToDevice, c->stream));
-  RZ_HIP(hipEventRecord(c->ev_items[slot], c->stream));
+  fill_items(c->h_items[slot], n, occ, kinds);
   const DrawBufs bufs{c->d_items, c->d_state, c->d_done, c->d_visible, c->d_recs, c->d_bbox, c->d_groupBox};
-  s = enqueue_kernels(c, n, uint32_t(totalQuads), bufs, profile_slot(c, 0));
+  s = enqueue_kernels(c, n, uint32_t(totalQuads), bufs, profile_slot(c, 0), c->d_hItems[slot], c->ev_items[slot]);
   if (s != RZ_OK) return s;
 
   if (statusOut) {

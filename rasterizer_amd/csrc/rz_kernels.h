@@ -22,6 +22,12 @@ struct PrepParams {
   // from views[item.view]) and the work-queue counter that prep resets.
   const ViewDesc* views;
   uint32_t* queue;
+  // Single-view draws: the descriptors as the host wrote them, in pinned
+  // host memory mapped for the device.  Prep reads each there and copies it
+  // to itemsOut, the device table the later kernels read (one transfer
+  // fewer than a copy before the launch).  Null: `items` holds them.
+  const ItemDesc* hostItems;
+  ItemDesc* itemsOut;
 };
 
 struct SetupParams {
@@ -176,7 +182,7 @@ hipError_t launch_tile_min(const uint16_t* hiz, uint32_t bX, uint32_t bY, uint16
                            uint32_t pyrY, hipStream_t stream);
 hipError_t launch_readback(const uint16_t* depth, const uint16_t* hiz, uint32_t width, uint32_t height, uint32_t bX,
                            uint32_t* rgba, hipStream_t stream);
-hipError_t launch_fill_u16(uint16_t* p, uint32_t n, uint16_t v, hipStream_t stream);
+hipError_t launch_clear(uint16_t* hiz, uint32_t nHiz, uint16_t* tileMin, uint32_t nMin, hipStream_t stream);
 // Multi-view: grid of `groups` persistent work-groups over p.nUnits units of
 // 8x8-block tiles.
 hipError_t launch_views(const ViewsParams& p, uint32_t groups, hipStream_t stream, bool cooperative = false);

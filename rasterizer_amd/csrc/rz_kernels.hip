@@ -229,7 +229,13 @@ __device__ bool query2d_wave(uint32_t minX, uint32_t maxX, uint32_t minY, uint32
 __global__ void __launch_bounds__(256) rz_prep_kernel(PrepParams p) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= p.nItems) return;
-  const ItemDesc& D = p.items[i];
+  ItemDesc D;
+  if (p.hostItems) {
+    D = p.hostItems[i];
+    p.itemsOut[i] = D;
+  } else {
+    D = p.items[i];
+  }
   ItemState& S = p.state[i];
   const Mvp* M = &p.mvp;
   uint32_t width = p.width, height = p.height, tileW = p.tileW, tileH = p.tileH;
@@ -2202,9 +2208,12 @@ __global__ void __launch_bounds__(256) rz_readback_kernel(const uint16_t* __rest
   rgba[size_t(py) * width + px] = out;
 }
 
-__global__ void rz_fill_u16_kernel(uint16_t* p, uint32_t n, uint16_t v) {
+// clear (Rasterizer.cpp:107-121) of a single view: hiZ and the per-tile min
+// hiZ := 1 (the cleared mark), both arrays in one launch.
+__global__ void rz_clear_kernel(uint16_t* hiz, uint32_t nHiz, uint16_t* tileMin, uint32_t nMin) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = v;
+  if (i < nHiz) hiz[i] = 1;
+  else if (i - nHiz < nMin) tileMin[i - nHiz] = 1;
 }
 
 // ---------------------------------------------------------------------------
@@ -2558,9 +2567,10 @@ hipError_t launch_readback(const uint16_t* depth, const uint16_t* hiz, uint32_t 
   return hipGetLastError();
 }
 
-hipError_t launch_fill_u16(uint16_t* p, uint32_t n, uint16_t v, hipStream_t stream) {
+hipError_t launch_clear(uint16_t* hiz, uint32_t nHiz, uint16_t* tileMin, uint32_t nMin, hipStream_t stream) {
+  const uint32_t n = nHiz + nMin;
   if (n == 0) return hipSuccess;
-  hipLaunchKernelGGL(rz_fill_u16_kernel, dim3(groups(n, 256)), dim3(256), 0, stream, p, n, v);
+  hipLaunchKernelGGL(rz_clear_kernel, dim3(groups(n, 256)), dim3(256), 0, stream, hiz, nHiz, tileMin, nMin);
   return hipGetLastError();
 }
 
