@@ -77,6 +77,7 @@ struct rz_context {
   QuadRec* d_recs = nullptr;
   uint64_t* d_bbox = nullptr;
   uint64_t* d_groupBox = nullptr;
+  uint32_t* d_groupItem = nullptr;  // per group of 64 quads: the item of its first quad (written by prep)
   // host staging ring for item tables: pinned, coherent host memory that
   // the prep kernel reads through d_hItems (its device mapping)
   ItemDesc* h_items[kStagingSlots] = {};
@@ -142,6 +143,7 @@ static void free_context(rz_context* c) {
   hipFree(c->d_recs);
   hipFree(c->d_bbox);
   hipFree(c->d_groupBox);
+  hipFree(c->d_groupItem);
   hipFree(c->d_qbounds);
   hipFree(c->d_qvis);
   hipFree(c->d_qclip);
@@ -371,9 +373,11 @@ static int ensure_capacity(rz_context* c, uint32_t nItems, uint32_t nQuads) {
     hipFree(c->d_recs);
     hipFree(c->d_bbox);
     hipFree(c->d_groupBox);
+  hipFree(c->d_groupItem);
     c->capQuads = 0;
     if (dev_alloc(&c->d_recs, cap) != hipSuccess || dev_alloc(&c->d_bbox, cap) != hipSuccess ||
-        dev_alloc(&c->d_groupBox, (cap + 63) / 64) != hipSuccess)
+        dev_alloc(&c->d_groupBox, (cap + 63) / 64) != hipSuccess ||
+        dev_alloc(&c->d_groupItem, (cap + 63) / 64) != hipSuccess)
       return RZ_ERR_OUT_OF_MEMORY;
     c->capQuads = cap;
   }
@@ -398,6 +402,7 @@ struct DrawBufs {
   QuadRec* recs;
   uint64_t* bbox;
   uint64_t* groupBox;
+  uint32_t* groupItem;  // optional: prep writes it, setup starts its item search there
 };
 
 // Total quads of occ[0..n) (8 per batch of 4 packets); RZ_ERR_INVALID_ARGUMENT
@@ -453,6 +458,7 @@ static int enqueue_kernels(rz_context* c, uint32_t n, uint32_t totalQuads, const
   pp.mvp = c->mvp;
   pp.hostItems = hostItems;
   pp.itemsOut = b.items;
+  pp.groupItem = b.groupItem;
   RZ_HIP(launch_prep(pp, c->stream));
   if (afterPrep) RZ_HIP(hipEventRecord(afterPrep, c->stream));
   if (prof) RZ_HIP(hipEventRecord(prof[1], c->stream));
@@ -468,6 +474,7 @@ static int enqueue_kernels(rz_context* c, uint32_t n, uint32_t totalQuads, const
   sp.recs = b.recs;
   sp.bbox = b.bbox;
   sp.groupBox = b.groupBox;
+  sp.groupItem = b.groupItem;
   RZ_HIP(launch_setup(sp, c->stream));
   if (prof) RZ_HIP(hipEventRecord(prof[2], c->stream));
 
@@ -533,7 +540,8 @@ static int enqueue_draw(rz_context* c, uint32_t n, const rz_occluder* const* occ
     c->h_itemsCap[slot] = cap;
   }
   fill_items(c->h_items[slot], n, occ, kinds);
-  const DrawBufs bufs{c->d_items, c->d_state, c->d_done, c->d_visible, c->d_recs, c->d_bbox, c->d_groupBox};
+  const DrawBufs bufs{c->d_items, c->d_state,    c->d_done,     c->d_visible,
+                      c->d_recs,  c->d_bbox,     c->d_groupBox, c->d_groupItem};
   s = enqueue_kernels(c, n, uint32_t(totalQuads), bufs, profile_slot(c, 0), c->d_hItems[slot], c->ev_items[slot]);
   if (s != RZ_OK) return s;
 
@@ -621,7 +629,7 @@ extern "C" int rz_draw_frame_noalloc(rz_context* ctx, uint32_t n, const rz_occlu
   const DrawBufs bufs{reinterpret_cast<ItemDesc*>(base + kNaItemsOff), reinterpret_cast<ItemState*>(base + kNaStateOff),
                       reinterpret_cast<uint32_t*>(base + kNaDoneOff), reinterpret_cast<uint32_t*>(base + kNaVisibleOff),
                       reinterpret_cast<QuadRec*>(base + kNaRecsOff), reinterpret_cast<uint64_t*>(base + kNaBboxOff),
-                      reinterpret_cast<uint64_t*>(base + kNaGroupOff)};
+                      reinterpret_cast<uint64_t*>(base + kNaGroupOff), nullptr};
   fill_items(g_naItems, n, occluders, nullptr);
   // From pageable memory: the copy has consumed g_naItems when it returns.
   RZ_HIP(hipMemcpyAsync(bufs.items, g_naItems, n * sizeof(ItemDesc), hipMemcpyHostToDevice, ctx->stream));

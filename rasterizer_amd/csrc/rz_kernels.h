@@ -28,6 +28,9 @@ struct PrepParams {
   // fewer than a copy before the launch).  Null: `items` holds them.
   const ItemDesc* hostItems;
   ItemDesc* itemsOut;
+  // Optional: for every group of 64 quads, the item that holds its first
+  // quad (the setup kernel's item search starts there).
+  uint32_t* groupItem;
 };
 
 struct SetupParams {

@@ -236,6 +236,12 @@ __global__ void __launch_bounds__(256) rz_prep_kernel(PrepParams p) {
   } else {
     D = p.items[i];
   }
+  if (p.groupItem && D.quadCount != 0u) {
+    // Items cover the quads 0..nQuads-1 in order, so each group's first
+    // quad lies in exactly one non-empty item.
+    const uint32_t last = (D.quadBase + D.quadCount - 1u) >> 6;
+    for (uint32_t k = (D.quadBase + 63u) >> 6; k <= last; ++k) p.groupItem[k] = i;
+  }
   ItemState& S = p.state[i];
   const Mvp* M = &p.mvp;
   uint32_t width = p.width, height = p.height, tileW = p.tileW, tileH = p.tileH;
@@ -541,7 +547,7 @@ __device__ uint64_t quad_box(uint32_t fb, uint32_t rangeX, uint32_t rangeY, uint
 __device__ __forceinline__ uint32_t item_of_quad(const ItemDesc* __restrict__ items, uint32_t nItems, uint32_t g,
                                                  const uint32_t* __restrict__ groupItem = nullptr) {
   if (groupItem) {
-    uint32_t i = groupItem[g >> 6];
+    uint32_t i = min(groupItem[g >> 6], nItems - 1u);
     while (i + 1 < nItems && items[i + 1].quadBase <= g) ++i;
     return i;
   }
@@ -2456,7 +2462,10 @@ static inline uint32_t groups(uint32_t n, uint32_t per) { return (n + per - 1) /
 
 hipError_t launch_prep(const PrepParams& p, hipStream_t stream) {
   if (p.nItems == 0) return hipSuccess;
-  hipLaunchKernelGGL(rz_prep_kernel, dim3(groups(p.nItems, 256)), dim3(256), 0, stream, p);
+  // One wave per work-group: a frame's few hundred items then spread over as
+  // many CUs, each with its own reads in flight (from host memory, for
+  // single-view draws).
+  hipLaunchKernelGGL(rz_prep_kernel, dim3(groups(p.nItems, 64)), dim3(64), 0, stream, p);
   return hipGetLastError();
 }
 
