@@ -20,7 +20,11 @@ using namespace rzd;
 
 namespace {
 
-constexpr int kStagingSlots = 8;
+// Host staging ring of item tables: draw d uses slot d % kStagingSlots.  An
+// event is recorded after every kDrawsPerEvent-th draw, alternating between
+// two events.
+constexpr int kStagingSlots = 16;
+constexpr uint64_t kDrawsPerEvent = 8;
 constexpr int kProfSlots = 256;
 constexpr uint64_t kTimeoutTicks = 200000000ull;  // 2 s of the 100 MHz s_memrealtime clock
 
@@ -83,8 +87,9 @@ struct rz_context {
   ItemDesc* h_items[kStagingSlots] = {};
   const ItemDesc* d_hItems[kStagingSlots] = {};
   uint32_t h_itemsCap[kStagingSlots] = {};
-  hipEvent_t ev_items[kStagingSlots] = {};
-  int slot = 0;
+  hipEvent_t ev_items[2] = {};
+  uint64_t draws = 0;        // draws staged through the ring
+  bool stagingSync = false;  // a draw's staging did not complete: synchronise before the next
   // query staging
   float* d_qbounds = nullptr;
   uint8_t* d_qvis = nullptr;
@@ -149,10 +154,10 @@ static void free_context(rz_context* c) {
   hipFree(c->d_qclip);
   hipFree(c->d_rgba);
   hipFree(c->d_frameStats);
-  for (int i = 0; i < kStagingSlots; ++i) {
+  for (int i = 0; i < kStagingSlots; ++i)
     if (c->h_items[i]) (void)hipHostFree(c->h_items[i]);
-    if (c->ev_items[i]) (void)hipEventDestroy(c->ev_items[i]);
-  }
+  for (hipEvent_t e : c->ev_items)
+    if (e) (void)hipEventDestroy(e);
   if (c->h_status) (void)hipHostFree(c->h_status);
   for (hipEvent_t e : c->profEvents) (void)hipEventDestroy(e);
   if (c->t0) (void)hipEventDestroy(c->t0);
@@ -218,8 +223,8 @@ extern "C" int rz_create(uint32_t width, uint32_t height, rz_context** out) {
     }
   }
   if (!placed) return fail(RZ_ERR_TOO_LARGE);
-  for (int i = 0; i < kStagingSlots; ++i)
-    if (hipEventCreateWithFlags(&c->ev_items[i], hipEventDisableTiming) != hipSuccess) return fail(RZ_ERR_DEVICE);
+  for (hipEvent_t& e : c->ev_items)
+    if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return fail(RZ_ERR_DEVICE);
   if (hipEventCreate(&c->t0) != hipSuccess || hipEventCreate(&c->t1) != hipSuccess) return fail(RZ_ERR_DEVICE);
   if (const char* mode = std::getenv("RZ_FRAME_LAUNCH")) c->cooperative = std::strcmp(mode, "plain") != 0;
   if (const char* cache = rzi::tune_cache_env()) {
@@ -438,11 +443,10 @@ static void fill_items(ItemDesc* items, uint32_t n, const rz_occluder* const* oc
 
 // Enqueues the prep, setup and frame kernels of one draw whose item
 // descriptors are already in b.items (or enqueued before them), or, with
-// hostItems, in mapped host memory from which prep copies them to b.items;
-// afterPrep (optional) is recorded once prep has read them.  prof: optional
-// event quadruple (prep, setup, frame, end).
+// hostItems, in mapped host memory from which prep copies them to b.items.
+// prof: optional event quadruple (prep, setup, frame, end).
 static int enqueue_kernels(rz_context* c, uint32_t n, uint32_t totalQuads, const DrawBufs& b, hipEvent_t* prof,
-                           const ItemDesc* hostItems = nullptr, hipEvent_t afterPrep = nullptr) {
+                           const ItemDesc* hostItems = nullptr) {
   if (prof) RZ_HIP(hipEventRecord(prof[0], c->stream));
   PrepParams pp{};
   pp.items = b.items;
@@ -460,7 +464,6 @@ static int enqueue_kernels(rz_context* c, uint32_t n, uint32_t totalQuads, const
   pp.itemsOut = b.items;
   pp.groupItem = b.groupItem;
   RZ_HIP(launch_prep(pp, c->stream));
-  if (afterPrep) RZ_HIP(hipEventRecord(afterPrep, c->stream));
   if (prof) RZ_HIP(hipEventRecord(prof[1], c->stream));
 
   SetupParams sp{};
@@ -519,10 +522,26 @@ static int enqueue_draw(rz_context* c, uint32_t n, const rz_occluder* const* occ
   s = ensure_capacity(c, n, uint32_t(totalQuads));
   if (s != RZ_OK) return s;
 
-  // Host staging: wait until this ring slot's previous upload has completed.
-  const int slot = c->slot;
-  c->slot = (c->slot + 1) % kStagingSlots;
-  RZ_HIP(hipEventSynchronize(c->ev_items[slot]));
+  // Host staging: draw d reuses the slot of draw d - kStagingSlots, whose
+  // prep must have read it.  The last event recorded at or after that draw
+  // follows draw e = (d - kStagingSlots) | 7, between d - 16 and d - 9, and
+  // still holds that record: the same event is recorded next after draw
+  // e + 16 >= d, and draw d records after this wait.  Waiting for it waits
+  // for a frame kernel after draw d - 16's prep.  (An event per draw put a
+  // marker between two kernels of every draw.)
+  static_assert(kStagingSlots == 2 * kDrawsPerEvent && kDrawsPerEvent == 8, "ring and event spacing");
+  const uint64_t d = c->draws++;
+  const int slot = int(d % kStagingSlots);
+  // Until this draw's kernels and event are enqueued, a failure leaves the
+  // ring's events behind its draws: the next draw then synchronises.
+  const bool syncAll = c->stagingSync;
+  c->stagingSync = true;
+  if (syncAll) {
+    RZ_HIP(hipStreamSynchronize(c->stream));
+  } else if (d >= uint64_t(kStagingSlots)) {
+    const uint64_t e = (d - kStagingSlots) | (kDrawsPerEvent - 1);
+    RZ_HIP(hipEventSynchronize(c->ev_items[(e / kDrawsPerEvent) & 1]));
+  }
   if (c->h_itemsCap[slot] < n) {
     if (c->h_items[slot]) (void)hipHostFree(c->h_items[slot]);
     c->h_items[slot] = nullptr;
@@ -542,8 +561,11 @@ static int enqueue_draw(rz_context* c, uint32_t n, const rz_occluder* const* occ
   fill_items(c->h_items[slot], n, occ, kinds);
   const DrawBufs bufs{c->d_items, c->d_state,    c->d_done,     c->d_visible,
                       c->d_recs,  c->d_bbox,     c->d_groupBox, c->d_groupItem};
-  s = enqueue_kernels(c, n, uint32_t(totalQuads), bufs, profile_slot(c, 0), c->d_hItems[slot], c->ev_items[slot]);
+  s = enqueue_kernels(c, n, uint32_t(totalQuads), bufs, profile_slot(c, 0), c->d_hItems[slot]);
   if (s != RZ_OK) return s;
+  if (d % kDrawsPerEvent == kDrawsPerEvent - 1)
+    RZ_HIP(hipEventRecord(c->ev_items[(d / kDrawsPerEvent) & 1], c->stream));
+  c->stagingSync = false;
 
   if (statusOut) {
     if (c->h_statusCap < n) {
