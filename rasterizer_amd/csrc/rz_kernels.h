@@ -199,7 +199,8 @@ hipError_t launch_views_bin(const ViewsParams& p, uint32_t nItems, uint64_t* til
 hipError_t launch_views_compact(const ViewsParams& p, uint32_t* unitTmp, hipStream_t stream);
 hipError_t launch_views_query(const ViewsQueryParams& p, hipStream_t stream);
 // hiZ := 1 (whole 8-entry groups) and tile min := 1 for every view.
-hipError_t launch_views_clear(const ViewDesc* views, uint32_t nViews, uint32_t maxEntries, hipStream_t stream);
+hipError_t launch_views_clear(const ViewDesc* src, ViewDesc* views, uint32_t nViews, uint32_t maxEntries,
+                              hipStream_t stream);
 
 // ---------------------------------------------------------------------------
 // Draw-order learning (rz_learn.hip).  Batches are numbered over the launch:

@@ -2169,9 +2169,15 @@ __global__ void __launch_bounds__(256) rz_views_query_kernel(ViewsQueryParams p)
 // clear (Rasterizer.cpp:107-121) of every view (blockIdx.y): hiZ := 1, the
 // cleared mark, eight entries per 16-byte store (each view's hiZ buffer is its
 // own allocation, holding whole groups of eight), and the per-tile min hiZ
-// := 1.
-__global__ void rz_views_clear_kernel(const ViewDesc* __restrict__ views) {
-  const ViewDesc& V = views[blockIdx.y];
+// := 1.  The view table comes from `src`, the host's pinned copy with this
+// step's matrices, read through its device mapping; the first work-group of
+// each view copies the view's entry to `views`, which every later kernel of
+// the step reads (no copy before the launch).
+__global__ void rz_views_clear_kernel(const ViewDesc* __restrict__ src, ViewDesc* __restrict__ views) {
+  const ViewDesc& V = src[blockIdx.y];
+  if (blockIdx.x == 0 && threadIdx.x < sizeof(ViewDesc) / 16)
+    reinterpret_cast<uint4*>(views + blockIdx.y)[threadIdx.x] =
+        reinterpret_cast<const uint4*>(src + blockIdx.y)[threadIdx.x];
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < (V.bX * V.bY + 8u) / 8u) reinterpret_cast<uint4*>(V.hiz)[i] = make_uint4(0x10001u, 0x10001u, 0x10001u, 0x10001u);
   if (i < V.pyrX * V.pyrY) V.tileMin[i] = 1;
@@ -2689,9 +2695,11 @@ hipError_t launch_fill_u32(uint32_t* p, uint32_t n, uint32_t v, hipStream_t stre
   return hipGetLastError();
 }
 
-hipError_t launch_views_clear(const ViewDesc* views, uint32_t nViews, uint32_t maxEntries, hipStream_t stream) {
-  if (nViews == 0 || maxEntries == 0) return hipSuccess;
-  hipLaunchKernelGGL(rz_views_clear_kernel, dim3(groups(maxEntries, 256), nViews), dim3(256), 0, stream, views);
+hipError_t launch_views_clear(const ViewDesc* src, ViewDesc* views, uint32_t nViews, uint32_t maxEntries,
+                              hipStream_t stream) {
+  if (nViews == 0) return hipSuccess;
+  hipLaunchKernelGGL(rz_views_clear_kernel, dim3(groups(std::max(maxEntries, 1u), 256), nViews), dim3(256), 0, stream,
+                     src, views);
   return hipGetLastError();
 }
 

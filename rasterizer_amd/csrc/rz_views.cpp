@@ -79,7 +79,8 @@ struct rz_views {
   uint64_t* d_groupBox = nullptr;
   uint32_t* d_groupItem = nullptr;  // per group of 64 quads: the item of its first quad
   ViewDesc* d_views = nullptr;
-  ViewDesc* h_views = nullptr;  // pinned; copied to d_views by every step
+  ViewDesc* h_views = nullptr;  // pinned, coherent; the clear kernel copies it to d_views every step
+  const ViewDesc* d_hViews = nullptr;  // h_views' device mapping
   float* h_eyes = nullptr;      // pinned, 4 floats per view; copied to d_eyes by front-to-back steps
   float* d_eyes = nullptr;
   bool f2b = false;             // some view is drawn front to back (order computed on the device)
@@ -219,6 +220,7 @@ static void free_plan(rz_views* v) {
   v->d_bbox = v->d_groupBox = nullptr;
   v->d_views = nullptr;
   v->h_views = nullptr;
+  v->d_hViews = nullptr;
 }
 
 static void free_views(rz_views* v) {
@@ -376,8 +378,14 @@ static int build_plan(rz_views* v) {
   uint64_t quads = 0;
   uint32_t units = 0, queries = 0, maxClear = 0;
   v->maxBlocks = 0;
-  if (hipHostMalloc(reinterpret_cast<void**>(&v->h_views), nv * sizeof(ViewDesc)) != hipSuccess)
+  if (hipHostMalloc(reinterpret_cast<void**>(&v->h_views), nv * sizeof(ViewDesc),
+                    hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess)
     return RZ_ERR_OUT_OF_MEMORY;
+  {
+    void* mapped = nullptr;
+    RZ_HIP(hipHostGetDevicePointer(&mapped, v->h_views, 0));
+    v->d_hViews = static_cast<const ViewDesc*>(mapped);
+  }
   for (uint32_t i = 0; i < nv; ++i) {
     ViewHost& h = v->views[i];
     ViewDesc& d = v->h_views[i];
@@ -623,9 +631,8 @@ static int enqueue_step(rz_views* v, bool capturing, int parts = kStepAll) {
     return RZ_OK;
   }
   if (parts & kStepPre) {
-    RZ_HIP(hipMemcpyAsync(v->d_views, v->h_views, nv * sizeof(ViewDesc), hipMemcpyHostToDevice, v->stream));
     if (prof) RZ_HIP(hipEventRecord(v->ph[0], v->stream));
-    RZ_HIP(launch_views_clear(v->d_views, nv, v->maxClear, v->stream));
+    RZ_HIP(launch_views_clear(v->d_hViews, v->d_views, nv, v->maxClear, v->stream));
     PrepParams pp{};
     pp.items = v->d_items;
     pp.state = v->d_state;
