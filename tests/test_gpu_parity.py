@@ -49,6 +49,28 @@ def test_config1_frame(oracle_mod, scene1, report):
     assert np.array_equal(cg, co)
 
 
+def test_many_draws_without_sync(oracle_mod, scene1, report):
+    """Forty query-gated draws into one buffer with no sync in between, each
+    of a random run of the front-to-back order (1 to all occluders): the
+    host's item-table slots are reused, and regrown, while earlier draws are
+    still in flight.  The oracle draws the same sequence."""
+    m = scene1.meta
+    g, o = _pair(oracle_mod, m["width"], m["height"], m["mvp"])
+    order = scenes.front_to_back(scene1.occluders, m["eye"])
+    occ = [scene1.occluders[i] for i in order]
+    rng = np.random.default_rng(17)
+    sizes = []
+    for _ in range(40):
+        n = int(rng.integers(1, len(occ) + 1))
+        start = int(rng.integers(0, len(occ) - n + 1))
+        run = occ[start:start + n]
+        g.draw_frame_handles(g.handle_array(run), n)
+        o.draw_frame(run)
+        sizes.append(n)
+    print(f"draw sizes {sizes}", flush=True)
+    _assert_same_buffers(g, o, report, "forty draws")
+
+
 def test_config1_dropin_calls(oracle_mod, scene1, report):
     """Main.cpp:192-206 through the per-call API: queryVisibility (blocking),
     then rasterize<needsClipping> (enqueued)."""
