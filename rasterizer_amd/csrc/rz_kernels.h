@@ -100,6 +100,12 @@ struct ViewsQueryParams {
   uint32_t nViews, nQueries;
   uint32_t maxPerView;  // most queries of any view (grid x extent, 256 per work-group)
   const uint32_t* rcp;
+  // Optional: the step's error word, copied by the first thread to errOut
+  // (host memory mapped for the device), so that the host reads it after
+  // the step without a copy of its own.  The kernel is then launched even
+  // without queries.
+  const uint32_t* err;
+  uint32_t* errOut;
 };
 
 struct FrameParams {

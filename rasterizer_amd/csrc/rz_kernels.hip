@@ -2134,6 +2134,8 @@ __global__ void __launch_bounds__(256) rz_query_kernel(QueryParams p) {
 // single-view kernel gives each wave four boxes: its batches are too small
 // to fill the GPU with 64 boxes per wave.)
 __global__ void __launch_bounds__(256) rz_views_query_kernel(ViewsQueryParams p) {
+  // The raster launch that sets the error word has completed.
+  if (p.errOut && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) *p.errOut = *p.err;
   const uint32_t lane = threadIdx.x & 63u;
   const ViewDesc& V = p.views[blockIdx.y];
   const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;  // 64 boxes per wave
@@ -2640,8 +2642,9 @@ hipError_t views_max_groups(int tileB, int waves, int* maxGroups) {
 }
 
 hipError_t launch_views_query(const ViewsQueryParams& p, hipStream_t stream) {
-  if (p.nQueries == 0) return hipSuccess;
-  hipLaunchKernelGGL(rz_views_query_kernel, dim3(groups(p.maxPerView, 256), p.nViews), dim3(256), 0, stream, p);
+  if (p.nViews == 0 || (p.nQueries == 0 && !p.errOut)) return hipSuccess;
+  hipLaunchKernelGGL(rz_views_query_kernel, dim3(std::max(groups(p.maxPerView, 256), 1u), p.nViews), dim3(256), 0,
+                     stream, p);
   return hipGetLastError();
 }
 

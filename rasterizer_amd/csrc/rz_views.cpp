@@ -124,6 +124,8 @@ struct rz_views {
   bool gated = false;   // query-gated draw loop per view (Main.cpp:192-206) instead of classify-then-draw
   bool cooperative = true;  // gated launches: cooperative dispatch (RZ_FRAME_LAUNCH=plain: plain)
   uint32_t* d_err = nullptr;  // device error word of gated waits
+  uint32_t* h_err = nullptr;  // pinned, coherent: d_err as the last step's query kernel left it
+  uint32_t* d_hErr = nullptr;  // h_err's device mapping
   unsigned long long* d_stats = nullptr;  // block-loop counters of the last step, when on
   bool stats = false;
   hipGraph_t graph = nullptr;
@@ -229,6 +231,9 @@ static void free_views(rz_views* v) {
   free_plan(v);
   hipFree(v->d_err);
   v->d_err = nullptr;
+  if (v->h_err) (void)hipHostFree(v->h_err);
+  v->h_err = nullptr;
+  v->d_hErr = nullptr;
   for (ViewHost& h : v->views) {
     hipFree(h.d_depth);
     hipFree(h.d_hiz);
@@ -276,6 +281,12 @@ extern "C" int rz_views_create(uint32_t nViews, const uint32_t* widths, const ui
       dev_alloc(&v->d_queue, 1) != hipSuccess || dev_alloc(&v->d_err, 1) != hipSuccess)
     return fail(RZ_ERR_OUT_OF_MEMORY);
   if (hipMemset(v->d_err, 0, sizeof(uint32_t)) != hipSuccess) return fail(RZ_ERR_DEVICE);
+  if (hipHostMalloc(reinterpret_cast<void**>(&v->h_err), sizeof(uint32_t), hipHostMallocMapped | hipHostMallocCoherent) !=
+      hipSuccess)
+    return fail(RZ_ERR_OUT_OF_MEMORY);
+  *v->h_err = 0;
+  if (hipHostGetDevicePointer(reinterpret_cast<void**>(&v->d_hErr), v->h_err, 0) != hipSuccess)
+    return fail(RZ_ERR_DEVICE);
   if (const char* mode = std::getenv("RZ_FRAME_LAUNCH")) v->cooperative = std::strcmp(mode, "plain") != 0;
   if (hipMemcpy(v->d_cov, tabs.cov, sizeof tabs.cov, hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(v->d_rcp, tabs.rcp, sizeof tabs.rcp, hipMemcpyHostToDevice) != hipSuccess)
@@ -751,6 +762,8 @@ static int enqueue_step(rz_views* v, bool capturing, int parts = kStepAll) {
   qp.nQueries = v->nQueries;
   for (const ViewHost& h : v->views) qp.maxPerView = std::max(qp.maxPerView, h.nQueries);
   qp.rcp = v->d_rcp;
+  qp.err = v->d_err;
+  qp.errOut = v->d_hErr;
   RZ_HIP(launch_views_query(qp, v->stream));
   if (prof) {
     if (v->nItems == 0) {
@@ -835,10 +848,13 @@ extern "C" int rz_views_sync(rz_views* v, float* ms) {
     *ms = 0.0f;
     if (v->ran) RZ_HIP(hipEventElapsedTime(ms, v->t0, v->t1));
   }
-  uint32_t err = 0;
-  RZ_HIP(hipMemcpy(&err, v->d_err, sizeof err, hipMemcpyDeviceToHost));
+  // Every step ends with the query kernel, which copies the error word to
+  // h_err: no copy here.  (Only steps set the word, and before any step
+  // both words are 0.)
+  const uint32_t err = *static_cast<volatile uint32_t*>(v->h_err);
   if (err) {
     RZ_HIP(hipMemset(v->d_err, 0, sizeof(uint32_t)));
+    *static_cast<volatile uint32_t*>(v->h_err) = 0;
     return (err & kErrTimeout) ? RZ_ERR_TIMEOUT : RZ_ERR_DEVICE;
   }
   return RZ_OK;
