@@ -2223,11 +2223,12 @@ __global__ void __launch_bounds__(256) rz_readback_kernel(const uint16_t* __rest
 }
 
 // clear (Rasterizer.cpp:107-121) of a single view: hiZ and the per-tile min
-// hiZ := 1 (the cleared mark), both arrays in one launch.
-__global__ void rz_clear_kernel(uint16_t* hiz, uint32_t nHiz, uint16_t* tileMin, uint32_t nMin) {
+// hiZ := 1 (the cleared mark), both arrays in one launch; hiZ (whole groups
+// of eight entries, in its own allocation) eight entries per 16-byte store.
+__global__ void rz_clear_kernel(uint4* hiz, uint32_t nHiz8, uint16_t* tileMin, uint32_t nMin) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < nHiz) hiz[i] = 1;
-  else if (i - nHiz < nMin) tileMin[i - nHiz] = 1;
+  if (i < nHiz8) hiz[i] = make_uint4(0x10001u, 0x10001u, 0x10001u, 0x10001u);
+  else if (i - nHiz8 < nMin) tileMin[i - nHiz8] = 1;
 }
 
 // ---------------------------------------------------------------------------
@@ -2585,9 +2586,11 @@ hipError_t launch_readback(const uint16_t* depth, const uint16_t* hiz, uint32_t 
 }
 
 hipError_t launch_clear(uint16_t* hiz, uint32_t nHiz, uint16_t* tileMin, uint32_t nMin, hipStream_t stream) {
-  const uint32_t n = nHiz + nMin;
+  if (nHiz % 8 != 0 || reinterpret_cast<uintptr_t>(hiz) % 16 != 0) return hipErrorInvalidValue;
+  const uint32_t n = nHiz / 8 + nMin;
   if (n == 0) return hipSuccess;
-  hipLaunchKernelGGL(rz_clear_kernel, dim3(groups(n, 256)), dim3(256), 0, stream, hiz, nHiz, tileMin, nMin);
+  hipLaunchKernelGGL(rz_clear_kernel, dim3(groups(n, 256)), dim3(256), 0, stream, reinterpret_cast<uint4*>(hiz),
+                     nHiz / 8, tileMin, nMin);
   return hipGetLastError();
 }
 
